@@ -1,0 +1,263 @@
+/*
+ * multem_hip.h -- C ABI of the MI355X multislice engine.
+ *
+ * The engine replaces the multislice path reached through MULTEM's
+ * `ilc_multem` mex entry (mex_files_multem/ilc_multem.cu:520-548) and the C++
+ * driver `mt::Multislice<T,dev>` (src/tem_simulation.cuh:51-97).  The input
+ * structure carries the same fields, with the same names and units, as the
+ * MATLAB struct that `read_input_multislice` (ilc_multem.cu:37-379) parses:
+ * angles in mrad or degrees, Cs in mm, lengths in Angstrom, 1-based output
+ * area.  The engine applies the reader's unit conversions and
+ * `Input_Multislice::validate_parameters` (src/input_multislice.cuh:622-796).
+ *
+ * Arrays are column-major as in MATLAB: a ny x nx image stores pixel (ix, iy)
+ * at ix*ny + iy.  Complex arrays are interleaved (re, im).  All outputs are
+ * caller-owned and written in double precision (the mex layer hands MATLAB
+ * doubles; ilc_multem.cu:381-481).
+ *
+ * Threading: one host thread per engine call.  Calls on the same GPU are
+ * serialised by the library.  Errors are returned as mt_status codes; the
+ * message of the last error of the calling thread is read with
+ * mt_hip_last_error().
+ */
+#ifndef MULTEM_HIP_H
+#define MULTEM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MT_ABI_VERSION 1
+
+typedef enum {
+    MT_OK = 0,
+    MT_ERR_INVALID_ARGUMENT = 1,   /* an input field is out of range */
+    MT_ERR_UNSUPPORTED = 2,        /* a valid MULTEM option this engine does not provide */
+    MT_ERR_OUT_OF_MEMORY = 3,      /* device or host allocation failed / budget too small */
+    MT_ERR_DEVICE = 4,             /* HIP runtime error */
+    MT_ERR_STOPPED = 5,            /* mt_hip_request_stop() ended the run early */
+    MT_ERR_NO_DEVICE = 6           /* no GPU visible */
+} mt_status;
+
+/* system_conf (src/types.cuh:4440-4512; matlab_mex.cuh:263-288) */
+typedef struct {
+    int precision;     /* 1: float, 2: double */
+    int device;        /* 1: CPU, 2: GPU */
+    int cpu_nthread;
+    int gpu_device;
+} mt_system_conf;
+
+/* Lens fields of the input struct, cond_lens_* / obj_lens_* (ilc_multem.cu:147-259).
+   Units as in MATLAB: c_30, c_50 in mm; phi_* in degrees; apertures in mrad. */
+typedef struct {
+    double m;
+    double c_10, c_12, phi_12;
+    double c_21, phi_21, c_23, phi_23;
+    double c_30, c_32, phi_32, c_34, phi_34;
+    double c_41, phi_41, c_43, phi_43, c_45, phi_45;
+    double c_50, c_52, phi_52, c_54, phi_54, c_56, phi_56;
+    double inner_aper_ang, outer_aper_ang;
+    double ti_a, ti_sigma, ti_beta;
+    int ti_npts;
+    double si_a, si_sigma, si_beta;
+    int si_rad_npts, si_azm_npts;
+    int zero_defocus_type;        /* 1 first, 2 middle, 3 last, 4 user defined */
+    double zero_defocus_plane;
+} mt_lens;
+
+/* detector.cir(i) (ilc_multem.cu:272-290), angles in mrad */
+typedef struct {
+    double inner_ang;
+    double outer_ang;
+} mt_detector_cir;
+
+/* The ilc_multem input struct (MATLAB multem_input.parameters, parameters.m:6-239). */
+typedef struct {
+    int simulation_type;       /* 11 STEM, 21 CBED, 51 EWFS, 52 EWRS, ... (types.cuh:193-205) */
+    int interaction_model;     /* 1 multislice, 2 phase object, 3 weak phase object */
+    int potential_type;        /* 6 Lobato (the only parameterisation provided) */
+    int operation_mode;
+    int reverse_multislice;
+
+    int pn_model;              /* 1 still atom, 2 absorptive, 3 frozen phonon */
+    int pn_coh_contrib;
+    int pn_single_conf;
+    int pn_nconf;
+    int pn_dim;                /* 110, 111, ... */
+    int pn_seed;
+
+    const double *spec_atoms;  /* column-major n_atoms x spec_atoms_cols: Z x y z sigma occ region charge */
+    int spec_atoms_rows;
+    int spec_atoms_cols;
+    double spec_lx, spec_ly, spec_lz, spec_dz;
+
+    double spec_rot_theta;     /* degrees; only 0 is accepted */
+
+    int thick_type;            /* 1 whole specimen, 2 through thickness, 3 through slices */
+    const double *thick;
+    int n_thick;
+
+    int potential_slicing;     /* 1 planes, 2 dz_Proj, 3 dz_Sub, 4 auto */
+
+    int nx, ny;
+    int bwl;
+
+    int iw_type;               /* 1 plane, 2 convergent, 3 user defined, 4 auto */
+    const double *iw_psi;      /* interleaved complex ny x nx, iw_type 3 only */
+    const double *iw_x;
+    const double *iw_y;
+    int n_iw_xy;
+
+    double E_0;                /* keV */
+    double theta, phi;         /* degrees */
+
+    int illumination_model;    /* 1 coherent, 2 partial coherent, 3 TCC, 4 full integration */
+    int temporal_spatial_incoh;
+
+    mt_lens cond_lens;
+    mt_lens obj_lens;
+
+    int scanning_type;         /* 1 line, 2 area */
+    int scanning_periodic;
+    int scanning_square_pxs;
+    int scanning_ns;
+    double scanning_x0, scanning_y0, scanning_xe, scanning_ye;
+
+    int detector_type;         /* 1 circular */
+    int n_detector;
+    const mt_detector_cir *detector_cir;
+
+    int output_area_ix_0, output_area_iy_0, output_area_ix_e, output_area_iy_e;  /* 1-based as in MATLAB */
+} mt_input_multislice;
+
+/* Output kinds (eTEM_Output_Type, types.cuh:209-214) */
+enum {
+    MT_OT_IMAGE_TOT_COH = 1, MT_OT_IMAGE_TOT = 2,
+    MT_OT_M2PSI_TOT_COH = 3, MT_OT_M2PSI_TOT = 4,
+    MT_OT_M2PSI_TOT_PSI_COH = 5, MT_OT_PSI_COH = 6,
+    MT_OT_PSI_0 = 7, MT_OT_V = 8, MT_OT_TRANS = 9
+};
+
+/* Output_Multislice (src/output_multislice.hpp:1165-1199), flattened.
+   mt_hip_output_dims() fills the scalar fields; the caller then provides
+   each array that output_type needs (others may be NULL):
+     image_tot / image_coh : [n_thk][n_det][ny*nx]      (STEM, scan grid)
+     m2psi_tot / m2psi_coh : [n_thk][ny*nx]
+     psi_coh               : [n_thk][ny*nx] complex
+   x (nx), y (ny) and thick (n_thk) are optional. */
+typedef struct {
+    int output_type;
+    int nx, ny;
+    int n_thk;
+    int n_det;
+    double dx, dy;
+    double *x, *y;
+    double *thick;
+    double *image_tot;
+    double *image_coh;
+    double *m2psi_tot;
+    double *m2psi_coh;
+    double *psi_coh;
+} mt_output_multislice;
+
+/* One slice of the specimen as found by Slicing (src/slicing.hpp:339-403) */
+typedef struct {
+    double z_0, z_e;
+    double z_int_0, z_int_e;
+    int iatom_0, iatom_e;
+    int ithk;
+} mt_slice;
+
+/* ---- library --------------------------------------------------------- */
+
+int mt_hip_abi_version(void);
+
+/* Message of the last failed call made by this thread ("" if none). */
+const char *mt_hip_last_error(void);
+
+/* Fills out->output_type and the sizes.  No device work.
+   Replaces the sizing half of Output_Multislice::set_input_data
+   (output_multislice.hpp:416-632). */
+mt_status mt_hip_output_dims(const mt_system_conf *sys, const mt_input_multislice *in,
+                             mt_output_multislice *out);
+
+/* Full simulation.  Replaces run_multislice<T, e_device>
+   (ilc_multem.cu:483-518) = Multislice::set_input_data + operator()
+   (tem_simulation.cuh:51-97) + Output_Multislice::gather.
+   sys->precision selects the _f32 / _f64 variant; the _f32 / _f64 entries
+   force it. */
+mt_status mt_hip_run_multislice(const mt_system_conf *sys, const mt_input_multislice *in,
+                                mt_output_multislice *out);
+mt_status mt_hip_run_multislice_f32(const mt_system_conf *sys, const mt_input_multislice *in,
+                                    mt_output_multislice *out);
+
+/* ---- per-piece entries (same input struct) --------------------------- */
+
+/* Slices of the (undisplaced) specimen.  Replaces ilc_spec_slicing
+   (mex_files_multem/ilc_spec_slicing.cpp:98-135).  *n_slice receives the
+   count; slices (capacity max_slice, may be NULL) receives the slices. */
+mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multislice *in,
+                              int iconf, mt_slice *slices, int max_slice, int *n_slice);
+
+/* Projected potential of slice islice (0-based) for phonon configuration
+   iconf (1-based, ignored for still atoms), ny x nx real, natural layout.
+   Replaces ilc_projected_potential (ilc_projected_potential.cu:136-158). */
+mt_status mt_hip_projected_potential(const mt_system_conf *sys, const mt_input_multislice *in,
+                                     int iconf, int islice, double *V);
+
+/* Transmission function exp(i sigma V) of slice islice, ny x nx complex,
+   natural layout.  Replaces ilc_transmission_function
+   (ilc_transmission_function.cu:142-168). */
+mt_status mt_hip_transmission_function(const mt_system_conf *sys, const mt_input_multislice *in,
+                                       int iconf, int islice, double *trans);
+
+/* Incident wave at (iw_x, iw_y) in real space, ny x nx complex, natural
+   layout.  Replaces ilc_incident_wave (ilc_incident_wave.cu:144-169). */
+mt_status mt_hip_incident_wave(const mt_system_conf *sys, const mt_input_multislice *in,
+                               double *psi0);
+
+/* Fresnel propagation of psi_in (ny x nx complex, natural layout) by z
+   (Angstrom); space 1 real, 2 reciprocal output.  Replaces ilc_propagate
+   (ilc_propagate.cu:115-140). */
+mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice *in,
+                           const double *psi_in, double z, int space, double *psi_out);
+
+/* ---- run control (Multislice::ext_stop_sim / ext_niter / ext_iter,
+        tem_simulation.cuh:47-49) ---------------------------------------- */
+void mt_hip_request_stop(void);
+void mt_hip_progress(int *iter, int *niter);
+
+/* Caps the device memory the engine plans for (bytes; 0 = no cap).  The
+   counterpart of Memory_Slice's free-memory probe
+   (transmission_function.cuh:161-182): a smaller budget makes the engine
+   keep fewer transmission slices resident and rebuild the rest. */
+void mt_hip_set_memory_budget(size_t bytes);
+
+/* Engine choices made for the last run (batch width, FFT schedule, tiles),
+   as a NUL-terminated text line written into buf (capacity n). */
+mt_status mt_hip_last_plan(char *buf, size_t n);
+
+/* Device time (ms) of the slice-step kernels of the last run, measured with
+   HIP events on the engine's stream, and the number of probe.slices they
+   covered. */
+mt_status mt_hip_last_timing(double *ms_slice_kernels, double *ms_total, double *probe_slices);
+
+/* ---- engine tuning hooks (no counterpart in the reference; for the
+        benchmark and tests) ------------------------------------------------ */
+
+/* Batch width (waves per batch; 0 = planner's choice) and whether the slice
+   loop of a batch is replayed from a captured hipGraph. */
+void mt_hip_set_batch(int batch, int use_graph);
+
+/* Per-kernel device time of the last run, from HIP events around the row and
+   column passes of the first batch of each configuration: summed ms and
+   launch counts; ms_tf is the transmission-function build time. */
+void mt_hip_kernel_timing(double *ms_row, int *n_row, double *ms_col, int *n_col, double *ms_tf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MULTEM_HIP_H */

@@ -1,0 +1,171 @@
+// extern "C" entry points declared in include/multem_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/multem_hip.h"
+#include "engine.hpp"
+#include "host_model.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+std::mutex g_engine_mutex;     // one engine call at a time per process
+
+mt_status fail(mt_status c, const std::string &m)
+{
+    g_last_error = m;
+    return c;
+}
+
+template <class F>
+mt_status guarded(const mt_system_conf *sys, const mt_input_multislice *in, F &&f)
+{
+    if (!sys || !in) return fail(MT_ERR_INVALID_ARGUMENT, "null argument");
+    if (sys->device == 1) return fail(MT_ERR_UNSUPPORTED, "system_conf.device = 1 (CPU): this library runs on the GPU");
+    std::lock_guard<std::mutex> lock(g_engine_mutex);
+    try {
+        mtg::Model m(*in);
+        f(m);
+        g_last_error.clear();
+        return MT_OK;
+    } catch (const mtg::Error &e) {
+        return fail(e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(MT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(MT_ERR_DEVICE, e.what());
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int mt_hip_abi_version(void) { return MT_ABI_VERSION; }
+
+const char *mt_hip_last_error(void) { return g_last_error.c_str(); }
+
+mt_status mt_hip_output_dims(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
+{
+    if (!out) return fail(MT_ERR_INVALID_ARGUMENT, "null output");
+    return guarded(sys, in, [&](const mtg::Model &m) { m.output_dims(out); });
+}
+
+mt_status mt_hip_run_multislice_f32(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
+{
+    if (!out) return fail(MT_ERR_INVALID_ARGUMENT, "null output");
+    mtg::run_control().stop = false;
+    return guarded(sys, in, [&](const mtg::Model &m) { mtg::run_multislice(m, sys->gpu_device, out); });
+}
+
+mt_status mt_hip_run_multislice(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
+{
+    if (sys && sys->precision == 2)
+        return fail(MT_ERR_UNSUPPORTED, "precision 2 (double): the engine runs the FP32 wave path");
+    return mt_hip_run_multislice_f32(sys, in, out);
+}
+
+mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, mt_slice *slices,
+                              int max_slice, int *n_slice)
+{
+    if (!n_slice) return fail(MT_ERR_INVALID_ARGUMENT, "null n_slice");
+    (void)iconf;   // plane slicing without z displacement: identical for every configuration
+    if (!sys || !in) return fail(MT_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lock(g_engine_mutex);
+    try {
+        mtg::Model m(*in);
+        *n_slice = m.n_slice();
+        for (int i = 0; slices && i < m.n_slice() && i < max_slice; ++i) {
+            const auto &s = m.slices[i];
+            slices[i].z_0 = s.z_0; slices[i].z_e = s.z_e;
+            slices[i].z_int_0 = s.z_0; slices[i].z_int_e = s.z_e;
+            slices[i].iatom_0 = s.iatom_0; slices[i].iatom_e = s.iatom_e;
+            slices[i].ithk = s.ithk;
+        }
+        g_last_error.clear();
+        return MT_OK;
+    } catch (const mtg::Error &e) {
+        return fail(e.code, e.msg);
+    }
+}
+
+mt_status mt_hip_projected_potential(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, int islice,
+                                     double *V)
+{
+    if (!V) return fail(MT_ERR_INVALID_ARGUMENT, "null V");
+    return guarded(sys, in, [&](const mtg::Model &m) { mtg::projected_potential(m, sys->gpu_device, iconf, islice, V); });
+}
+
+mt_status mt_hip_transmission_function(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, int islice,
+                                       double *trans)
+{
+    if (!trans) return fail(MT_ERR_INVALID_ARGUMENT, "null trans");
+    return guarded(sys, in,
+                   [&](const mtg::Model &m) { mtg::transmission_function(m, sys->gpu_device, iconf, islice, trans); });
+}
+
+mt_status mt_hip_incident_wave(const mt_system_conf *sys, const mt_input_multislice *in, double *psi0)
+{
+    if (!psi0) return fail(MT_ERR_INVALID_ARGUMENT, "null psi0");
+    return guarded(sys, in, [&](const mtg::Model &m) { mtg::incident_wave(m, sys->gpu_device, psi0); });
+}
+
+mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice *in, const double *psi_in, double z,
+                           int space, double *psi_out)
+{
+    if (!psi_in || !psi_out) return fail(MT_ERR_INVALID_ARGUMENT, "null wave");
+    return guarded(sys, in,
+                   [&](const mtg::Model &m) { mtg::propagate(m, sys->gpu_device, psi_in, z, space, psi_out); });
+}
+
+void mt_hip_request_stop(void) { mtg::run_control().stop = true; }
+
+void mt_hip_progress(int *iter, int *niter)
+{
+    if (iter) *iter = mtg::run_control().iter.load();
+    if (niter) *niter = mtg::run_control().niter.load();
+}
+
+void mt_hip_set_memory_budget(size_t bytes) { mtg::run_control().memory_budget = bytes; }
+
+mt_status mt_hip_last_plan(char *buf, size_t n)
+{
+    if (!buf || n == 0) return fail(MT_ERR_INVALID_ARGUMENT, "null buffer");
+    std::snprintf(buf, n, "%s", mtg::last_timing().plan.c_str());
+    return MT_OK;
+}
+
+mt_status mt_hip_last_timing(double *ms_slice_kernels, double *ms_total, double *probe_slices)
+{
+    const mtg::Timing &t = mtg::last_timing();
+    if (ms_slice_kernels) *ms_slice_kernels = t.ms_slice;
+    if (ms_total) *ms_total = t.ms_total;
+    if (probe_slices) *probe_slices = t.probe_slices;
+    return MT_OK;
+}
+
+// ---- engine tuning hooks (no counterpart in the reference) ----
+
+// Batch width override (0: planner's choice); disables the hipGraph replay if use_graph == 0.
+void mt_hip_set_batch(int batch, int use_graph)
+{
+    mtg::run_control().batch_override = batch;
+    mtg::run_control().use_graph = use_graph != 0;
+}
+
+// Per-kernel timing of the last run: summed row / column pass durations and launch counts.
+void mt_hip_kernel_timing(double *ms_row, int *n_row, double *ms_col, int *n_col, double *ms_tf)
+{
+    const mtg::Timing &t = mtg::last_timing();
+    if (ms_row) *ms_row = t.ms_row;
+    if (n_row) *n_row = t.n_row;
+    if (ms_col) *ms_col = t.ms_col;
+    if (n_col) *n_col = t.n_col;
+    if (ms_tf) *ms_tf = t.ms_tf;
+}
+
+} // extern "C"

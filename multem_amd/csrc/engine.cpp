@@ -1,0 +1,742 @@
+// Multislice engine: plans device memory, builds transmission stacks, and
+// runs batches of waves through the two-pass slice step on one GPU.
+//
+// Replaces Multislice::STEM_ISTEM / EWFS_EWRS / CBED_CBEI
+// (src/tem_simulation.cuh:100-327) and Wave_Function::psi
+// (src/wave_function.cuh:229-252): instead of one probe at a time with a host
+// synchronisation per detector, a batch of B waves (probe positions or
+// phonon configurations) advances slice by slice, each slice one row pass and
+// one column pass over the whole batch, and the slice loop of a batch is
+// captured once in a hipGraph and replayed.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <sstream>
+
+namespace mtg {
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238463;
+
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        throw Error{e == hipErrorOutOfMemory ? MT_ERR_OUT_OF_MEMORY : MT_ERR_DEVICE, m};
+    }
+}
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    void alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count) check(hipMalloc((void **)&p, count * sizeof(T)), "hipMalloc");
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    // Host buffers are pageable and short-lived: the copy is complete on return.
+    void upload(const T *h, size_t count, hipStream_t s)
+    {
+        check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+        check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+    ~Stream() { if (s) (void)hipStreamDestroy(s); }
+};
+
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { check(hipEventCreate(&e), "hipEventCreate"); }
+    ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+
+float elapsed(const Event &a, const Event &b)
+{
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, a.e, b.e), "hipEventElapsedTime");
+    return ms;
+}
+
+std::vector<float2> twiddles(int n)
+{
+    std::vector<float2> w(n);
+    for (int m = 0; m < n; ++m) {
+        double a = -2.0 * kPi * (double)m / (double)n;
+        w[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    return w;
+}
+
+// Device-resident data shared by all entries of one model.
+struct Setup {
+    const Model &m;
+    hipStream_t s;
+    int nx, ny;
+    size_t nxy;
+    DBuf<float2> tw_x, tw_y;
+    DBuf<float> tables;
+    DBuf<float2> rmax;
+    DBuf<int2> ranges;
+    std::vector<int2> h_ranges;
+
+    Setup(const Model &mm, hipStream_t st) : m(mm), s(st), nx(mm.nx), ny(mm.ny), nxy((size_t)mm.nx * mm.ny)
+    {
+        auto wx = twiddles(nx), wy = twiddles(ny);
+        tw_x.alloc(nx); tw_x.upload(wx.data(), nx, s);
+        tw_y.alloc(ny); tw_y.upload(wy.data(), ny, s);
+        const int nt = (int)m.tables.size();
+        if (nt > kMaxTypes) throw Error{MT_ERR_UNSUPPORTED, "more than 16 distinct elements"};
+        std::vector<float> tb((size_t)nt * 5 * 128);
+        std::vector<float2> rm(nt);
+        for (int i = 0; i < nt; ++i) {
+            const AtomTable &t = m.tables[i];
+            for (int k = 0; k < 128; ++k) {
+                tb[(size_t)i * 640 + k] = t.R2[k];
+                tb[(size_t)i * 640 + 128 + k] = t.c0[k];
+                tb[(size_t)i * 640 + 256 + k] = t.c1[k];
+                tb[(size_t)i * 640 + 384 + k] = t.c2[k];
+                tb[(size_t)i * 640 + 512 + k] = t.c3[k];
+            }
+            rm[i] = make_float2(t.R_max_f, t.R2_max_f);
+        }
+        tables.alloc(tb.size()); tables.upload(tb.data(), tb.size(), s);
+        rmax.alloc(nt); rmax.upload(rm.data(), nt, s);
+        for (const SliceInfo &sl : m.slices) h_ranges.push_back(make_int2(sl.iatom_0, sl.iatom_e));
+        ranges.alloc(h_ranges.size()); ranges.upload(h_ranges.data(), h_ranges.size(), s);
+    }
+
+    // float4 atoms (x, y, occ, type) of configuration iconf
+    std::vector<float4> atoms_of(int iconf) const
+    {
+        std::vector<float> x, y;
+        m.displaced(iconf, x, y);
+        std::vector<float4> a(x.size());
+        for (size_t i = 0; i < x.size(); ++i) {
+            int ty = m.type_of_atom[i];
+            float tyf;
+            std::memcpy(&tyf, &ty, sizeof tyf);
+            a[i] = make_float4(x[i], y[i], m.atoms_u.occ[i], tyf);
+        }
+        return a;
+    }
+
+    // Transmission stack of one configuration into trans (n_slice * nxy),
+    // optionally the potential V (natural layout) of the same slices.
+    void build_tf(const DBuf<float4> &atoms, int slice0, int n_slices, float2 *trans, float *V,
+                  DBuf<float2> *bwl_work)
+    {
+        TfArgs a{};
+        a.nx = nx; a.ny = ny; a.nxh = m.nxh; a.nyh = m.nyh;
+        a.dRx = m.dRx; a.dRy = m.dRy;
+        a.atoms = atoms.p;
+        a.slice_range = ranges.p;
+        a.slice0 = slice0; a.n_slices = n_slices;
+        a.tables = tables.p; a.rmax = rmax.p; a.n_types = (int)m.tables.size();
+        a.w = m.Vr_factor;
+        a.trans = trans; a.V = V;
+        check(launch_tf_build(a, s), "tf_build");
+        if (m.bwl && trans) bandwidth_limit(trans, n_slices);
+        (void)bwl_work;
+    }
+
+    // FFT -> x bwl/(nx ny) -> IFFT on n waves in place (Transmission_Function::trans,
+    // src/transmission_function.cuh:72-82)
+    void bandwidth_limit(float2 *w, int n)
+    {
+        std::vector<Member> hm(n);
+        for (int i = 0; i < n; ++i) hm[i] = Member{0, -1, 0, 0};
+        RowArgs r{};
+        r.src = w; r.dst = w; r.member_stride = (long long)nxy; r.nrows = nx; r.n_members = n;
+        r.tw = tw_y.p;
+        r.flags = ROW_LOAD | ROW_FFT | ROW_STORE;
+        check(launch_row_pass(ny, r, s), "row pass");
+        ColArgs c = col_base(w, w, n);
+        c.flags = COL_LOAD | COL_FFT | COL_BWL | COL_IFFT | COL_STORE;
+        check(launch_col_pass(nx, c, false, false, s), "col pass");
+        r.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+        check(launch_row_pass(ny, r, s), "row pass");
+    }
+
+    ColArgs col_base(const float2 *src, float2 *dst, int n_members) const
+    {
+        ColArgs c{};
+        c.src = src; c.dst = dst; c.member_stride = (long long)nxy;
+        c.ncols = ny; c.n_members = n_members;
+        c.tw = tw_x.p;
+        c.dgx_f = m.dgx; c.dgy_f = m.dgy;
+        c.nx = nx; c.ny = ny;
+        c.bwl_alpha = m.bwl_alpha; c.gl2_max = m.gl2_max;
+        c.inv_nxy = 1.0f / (float)nxy;
+        c.p_scale = 1.0f;
+        return c;
+    }
+
+    ProbeParams probe_params(unsigned int n_aperture) const
+    {
+        ProbeParams P{};
+        const double lam = m.lambda_d;
+        // defocus referred to the first slice (Incident_Wave::operator(), src/incident_wave.cuh:72-91)
+        const float f_s = m.cond.c_10 - (m.cond.zero_defocus_plane - m.z_init());
+        P.c10 = (std::fabs(f_s) <= 1e-5f) ? 0.0 : -kPi * (double)f_s * lam;
+        P.c30 = (std::fabs(m.cond.c_30) <= 1e-5f) ? 0.0 : -kPi * (double)m.cond.c_30 * std::pow(lam, 3) / 2.0;
+        P.c50 = (std::fabs(m.cond.c_50) <= 1e-5f) ? 0.0 : -kPi * (double)m.cond.c_50 * std::pow(lam, 5) / 3.0;
+        P.dgx = 1.0 / m.lx_d; P.dgy = 1.0 / m.ly_d;
+        P.dgx_f = m.dgx; P.dgy_f = m.dgy;
+        P.g2_min = m.cond.g2_min; P.g2_max = m.cond.g2_max;
+        P.norm = n_aperture ? (float)std::sqrt(1.0 / (double)n_aperture) : 0.f;
+        return P;
+    }
+
+    Beam beam(float xb, float yb) const
+    {
+        // x = -2 pi (x_b - lx/2)  (Grid_2d::exp_factor_Rx, src/types.cuh:2599-2611)
+        return Beam{-2.0 * kPi * ((double)xb - 0.5 * m.lx_d), -2.0 * kPi * ((double)yb - 0.5 * m.ly_d)};
+    }
+
+    unsigned int count_aperture()
+    {
+        DBuf<unsigned int> cnt(1);
+        unsigned int zero = 0;
+        cnt.upload(&zero, 1, s);
+        check(launch_count_aperture(nx, ny, m.dgx, m.dgy, m.cond.g2_min, m.cond.g2_max, cnt.p, s), "count aperture");
+        unsigned int h = 0;
+        check(hipMemcpyAsync(&h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s), "D2H");
+        check(hipStreamSynchronize(s), "sync");
+        return h;
+    }
+
+    // propagator tables for the given distances: px [nz][nx], py [nz][ny]
+    void prop_tables(const std::vector<double> &z, DBuf<float2> &px, DBuf<float2> &py)
+    {
+        DBuf<double> dz(z.size());
+        dz.upload(z.data(), z.size(), s);
+        px.alloc(z.size() * nx);
+        py.alloc(z.size() * ny);
+        check(launch_prop_tables(nx, ny, dz.p, (int)z.size(), m.lambda_d, m.mul_sign, m.lx_d, m.ly_d, px.p, py.p, s),
+              "prop tables");
+        check(hipStreamSynchronize(s), "sync");
+    }
+};
+
+size_t free_device_memory()
+{
+    size_t fr = 0, tot = 0;
+    check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    return fr;
+}
+
+size_t budget_bytes()
+{
+    size_t fr = free_device_memory();
+    size_t b = (size_t)(0.85 * (double)fr);
+    if (run_control().memory_budget) b = std::min(b, run_control().memory_budget);
+    return b;
+}
+
+void select_device(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw Error{MT_ERR_NO_DEVICE, "no HIP device visible"};
+    if (device < 0 || device >= n) throw Error{MT_ERR_INVALID_ARGUMENT, "gpu_device out of range"};
+    check(hipSetDevice(device), "hipSetDevice");
+}
+
+void progress_add(int k) { run_control().iter.fetch_add(k); }
+void stop_check()
+{
+    if (run_control().stop.load()) throw Error{MT_ERR_STOPPED, "stopped by mt_hip_request_stop"};
+}
+
+// ---------------------------------------------------------------------------
+// STEM: batches of probe positions of one configuration.
+void run_stem(const Model &m, mt_output_multislice *out)
+{
+    Stream stream;
+    hipStream_t s = stream.s;
+    Setup S(m, s);
+    Timing &tm = last_timing();
+    tm = Timing{};
+    const int ns_run = [&] {
+        int last = 0;
+        for (const ThickInfo &t : m.thk) last = std::max(last, t.islice);
+        return last + 1;                     // slices after the deepest thickness add nothing
+    }();
+    const int nscan = (int)m.sc_x.size();
+    const int n_thk = (int)m.thk.size(), n_det = (int)m.det_g2_min.size();
+    if (n_det > kMaxDet) throw Error{MT_ERR_UNSUPPORTED, "more than 8 detectors"};
+    const size_t nxy = S.nxy;
+
+    // memory plan: one transmission stack + B waves
+    const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(float2);
+    const size_t budget = budget_bytes();
+    if (stack_bytes + 2 * nxy * sizeof(float2) > budget) throw Error{MT_ERR_OUT_OF_MEMORY, "transmission stack exceeds the memory budget"};
+    int B = run_control().batch_override;
+    if (B <= 0) {
+        // enough waves to fill 256 CUs in the column pass, and a working set
+        // that stays near the 256 MB last-level cache
+        const int ntile = m.ny / col_tile_width(m.nx);
+        B = std::max(1, (int)std::min<size_t>(256, (size_t)(160u << 20) / (nxy * sizeof(float2))));
+        B = std::max(B, (2048 + ntile - 1) / ntile);
+    }
+    B = std::min(B, std::max(1, nscan));
+    B = (int)std::min<size_t>(B, (budget - stack_bytes) / (nxy * sizeof(float2)));
+    if (B < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for a wave batch"};
+    const int ntile = m.ny / col_tile_width(m.nx);
+
+    DBuf<float2> trans(m.n_slice() * nxy), psi((size_t)B * nxy);
+    DBuf<Member> members(B);
+    DBuf<Beam> beams(B);
+    DBuf<double> part((size_t)B * n_thk * n_det * ntile);
+    DBuf<double> image((size_t)n_thk * n_det * nscan);
+    {
+        std::vector<double> zero(image.n, 0.0);
+        image.upload(zero.data(), zero.size(), s);
+    }
+    std::vector<double> zl;
+    for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
+    DBuf<float2> px, py;
+    S.prop_tables(zl, px, py);
+    const unsigned int nap = S.count_aperture();
+    const ProbeParams P = S.probe_params(nap);
+
+    std::ostringstream plan;
+    plan << "stem nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
+         << " col_C=" << col_tile_width(m.nx) << " row_RW=" << row_group(m.ny)
+         << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
+    tm.plan = plan.str();
+
+    const float w = m.phonon_weight();
+    const int nbatch = (nscan + B - 1) / B;
+    run_control().niter = m.n_conf_run() * nscan;
+    run_control().iter = 0;
+
+    // the batch pipeline (probe -> ns_run slices -> detector partials)
+    auto pipeline = [&](bool timed, std::vector<Event> *ev) {
+        ColArgs c0 = S.col_base(nullptr, psi.p, B);
+        c0.flags = COL_IFFT | COL_STORE;
+        c0.members = members.p; c0.beams = beams.p; c0.probe = P;
+        check(launch_col_pass(m.nx, c0, true, false, s), "probe pass");
+        RowArgs r{};
+        r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = B;
+        r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = 0; r.members = members.p;
+        r.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
+        ColArgs c = S.col_base(psi.p, psi.p, B);
+        c.n_det = n_det;
+        for (int d = 0; d < n_det; ++d) { c.det_g2_min[d] = m.det_g2_min[d]; c.det_g2_max[d] = m.det_g2_max[d]; }
+        c.det_part = part.p; c.n_thk = n_thk;
+        int k = 0;
+        for (int is = 0; is < ns_run; ++is) {
+            r.slice = is;
+            if (timed) check(hipEventRecord((*ev)[k++].e, s), "event");
+            check(launch_row_pass(m.ny, r, s), "row pass");
+            if (timed) check(hipEventRecord((*ev)[k++].e, s), "event");
+            const int ithk = m.slices[is].ithk;
+            const bool last = is == ns_run - 1;
+            c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (last ? 0 : (COL_IFFT | COL_STORE));
+            c.px = px.p + (size_t)is * m.nx;
+            c.py = py.p + (size_t)is * m.ny;
+            c.ithk = ithk;
+            check(launch_col_pass(m.nx, c, false, ithk >= 0, s), "col pass");
+            if (timed) check(hipEventRecord((*ev)[k++].e, s), "event");
+        }
+        check(launch_det_reduce(part.p, B, n_thk, n_det, ntile, members.p, (double)w, image.p, nscan, s), "det reduce");
+    };
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    if (run_control().use_graph) {
+        check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        pipeline(false, nullptr);
+        check(hipStreamEndCapture(s, &graph), "end capture");
+        check(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate");
+    }
+    struct GraphGuard {
+        hipGraph_t g; hipGraphExec_t e;
+        ~GraphGuard() { if (e) (void)hipGraphExecDestroy(e); if (g) (void)hipGraphDestroy(g); }
+    } guard{graph, gexec};
+
+    std::vector<Event> kev(run_control().time_kernels ? 3 * ns_run : 0);
+    Event t0, t1, tf0, tf1, sl0, sl1;
+    double ms_slice = 0, ms_tf = 0, ps = 0;
+    check(hipEventRecord(t0.e, s), "event");
+    std::vector<Member> hm(B);
+    std::vector<Beam> hb(B);
+    DBuf<float4> atoms(m.atoms_u.x.size());
+    for (int iconf = m.fp_iconf_0; iconf <= m.pn_nconf; ++iconf) {
+        stop_check();
+        auto a = S.atoms_of(iconf);
+        atoms.upload(a.data(), a.size(), s);
+        check(hipEventRecord(tf0.e, s), "event");
+        S.build_tf(atoms, 0, m.n_slice(), trans.p, nullptr, nullptr);
+        check(hipEventRecord(tf1.e, s), "event");
+        check(hipEventSynchronize(tf1.e), "sync");
+        ms_tf += elapsed(tf0, tf1);
+        for (int ib = 0; ib < nbatch; ++ib) {
+            stop_check();
+            for (int j = 0; j < B; ++j) {
+                int is = ib * B + j;
+                bool valid = is < nscan;
+                int iq = valid ? is : nscan - 1;
+                hm[j] = Member{0, valid ? is : -1, j, 1};
+                hb[j] = S.beam(m.sc_x[iq], m.sc_y[iq]);
+            }
+            members.upload(hm.data(), B, s);
+            beams.upload(hb.data(), B, s);
+            const bool timed = run_control().time_kernels && ib == 0;
+            check(hipEventRecord(sl0.e, s), "event");
+            if (timed || !gexec) pipeline(timed, &kev);
+            else check(hipGraphLaunch(gexec, s), "graph launch");
+            check(hipEventRecord(sl1.e, s), "event");
+            check(hipEventSynchronize(sl1.e), "sync");   // host arrays are reused next batch
+            ms_slice += elapsed(sl0, sl1);
+            ps += (double)std::min(B, nscan - ib * B) * ns_run;
+            if (timed) {
+                for (int is = 0; is < ns_run; ++is) {
+                    tm.ms_row += elapsed(kev[3 * is], kev[3 * is + 1]);
+                    tm.ms_col += elapsed(kev[3 * is + 1], kev[3 * is + 2]);
+                }
+                tm.n_row += ns_run;
+                tm.n_col += ns_run;
+            }
+            progress_add(std::min(B, nscan - ib * B));
+        }
+    }
+    check(hipEventRecord(t1.e, s), "event");
+    check(hipEventSynchronize(t1.e), "sync");
+    tm.ms_total = elapsed(t0, t1);
+    tm.ms_slice = ms_slice;
+    tm.ms_tf = ms_tf;
+    tm.probe_slices = ps;
+
+    if (out->image_tot) {
+        check(hipMemcpyAsync(out->image_tot, image.p, image.n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        check(hipStreamSynchronize(s), "sync");
+    }
+    if (out->image_coh) throw Error{MT_ERR_UNSUPPORTED, "STEM coherent image (pn_coh_contrib)"};
+}
+
+// ---------------------------------------------------------------------------
+// Exit wave / CBED: the members of a batch are phonon configurations.
+void run_ew(const Model &m, mt_output_multislice *out)
+{
+    Stream stream;
+    hipStream_t s = stream.s;
+    Setup S(m, s);
+    Timing &tm = last_timing();
+    tm = Timing{};
+    const size_t nxy = S.nxy;
+    const int n_thk = (int)m.thk.size();
+    mt_output_multislice dims{};
+    m.output_dims(&dims);
+    const int ot = dims.output_type;
+    const int nxo = m.oa_ix_e - m.oa_ix_0, nyo = m.oa_iy_e - m.oa_iy_0;
+    const size_t no = (size_t)nxo * nyo;
+    const bool fs = !m.is_ewrs();     // Fourier-space output (EWFS, CBED)
+    const int ns_run = [&] {
+        int last = 0;
+        for (const ThickInfo &t : m.thk) last = std::max(last, t.islice);
+        return last + 1;
+    }();
+
+    const int nconf = m.n_conf_run();
+    const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(float2);
+    const size_t per_member = stack_bytes + 2 * nxy * sizeof(float2);
+    const size_t fixed = (size_t)n_thk * (no * 3 + 2 * nxy) * sizeof(double);
+    const size_t budget = budget_bytes();
+    if (fixed + per_member > budget) throw Error{MT_ERR_OUT_OF_MEMORY, "one configuration exceeds the memory budget"};
+    int K = (int)std::min<size_t>((size_t)nconf, (budget - fixed) / per_member);
+    if (run_control().batch_override > 0) K = std::min(K, run_control().batch_override);
+    K = std::max(K, 1);
+
+    DBuf<float2> trans((size_t)K * m.n_slice() * nxy), psi((size_t)K * nxy), scratch((size_t)K * nxy);
+    DBuf<Member> members(K);
+    DBuf<Beam> beams(std::max<size_t>(1, m.iw_x.size()));
+    DBuf<double> m2(std::max<size_t>(1, (size_t)n_thk * no)), pc(std::max<size_t>(1, (size_t)n_thk * no * 2));
+    const bool need_full = (ot == MT_OT_M2PSI_TOT_COH);
+    DBuf<double> pfull(need_full ? (size_t)n_thk * nxy * 2 : 1);
+    {
+        std::vector<double> zero(std::max(m2.n, std::max(pc.n, pfull.n)), 0.0);
+        m2.upload(zero.data(), m2.n, s);
+        pc.upload(zero.data(), pc.n, s);
+        pfull.upload(zero.data(), pfull.n, s);
+    }
+    std::vector<double> zl;
+    for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
+    for (const ThickInfo &t : m.thk) zl.push_back((double)t.z_back_prop);
+    DBuf<float2> px, py;
+    S.prop_tables(zl, px, py);
+    ProbeParams P{};
+    if (m.iw_type == 2) {
+        P = S.probe_params(S.count_aperture());
+        std::vector<Beam> hb;
+        for (size_t i = 0; i < m.iw_x.size(); ++i) hb.push_back(S.beam(m.iw_x[i], m.iw_y[i]));
+        beams.upload(hb.data(), hb.size(), s);
+    }
+    std::ostringstream plan;
+    plan << (m.is_cbed() ? "cbed" : (m.is_ewrs() ? "ewrs" : "ewfs")) << " nx=" << m.nx << " ny=" << m.ny
+         << " configs_per_batch=" << K << " slices=" << ns_run;
+    tm.plan = plan.str();
+
+    const double w = (double)m.phonon_weight();
+    run_control().niter = nconf;
+    run_control().iter = 0;
+    Event t0, t1;
+    check(hipEventRecord(t0.e, s), "event");
+    DBuf<float4> atoms(m.atoms_u.x.size());
+    for (int c0 = 0; c0 < nconf; c0 += K) {
+        stop_check();
+        const int nb = std::min(K, nconf - c0);
+        std::vector<Member> hm(nb);
+        for (int j = 0; j < nb; ++j) {
+            hm[j] = Member{j, -1, 0, m.iw_type == 2 ? (int)m.iw_x.size() : 0};
+            auto a = S.atoms_of(m.fp_iconf_0 + c0 + j);
+            atoms.upload(a.data(), a.size(), s);
+            S.build_tf(atoms, 0, m.n_slice(), trans.p + (size_t)j * m.n_slice() * nxy, nullptr, nullptr);
+            check(hipStreamSynchronize(s), "sync");   // atoms buffer is reused by the next configuration
+        }
+        members.upload(hm.data(), nb, s);
+
+        RowArgs r{};
+        r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nb;
+        r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = (long long)m.n_slice() * nxy; r.members = members.p;
+        if (m.iw_type == 2) {
+            ColArgs cp = S.col_base(nullptr, psi.p, nb);
+            cp.flags = COL_IFFT | COL_STORE;
+            cp.members = members.p; cp.beams = beams.p; cp.probe = P;
+            check(launch_col_pass(m.nx, cp, true, false, s), "probe pass");
+        }
+        for (int is = 0; is < ns_run; ++is) {
+            r.slice = is;
+            r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
+            check(launch_row_pass(m.ny, r, s), "row pass");
+            ColArgs c = S.col_base(psi.p, psi.p, nb);
+            c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+            c.px = px.p + (size_t)is * m.nx;
+            c.py = py.p + (size_t)is * m.ny;
+            check(launch_col_pass(m.nx, c, false, false, s), "col pass");
+            const int ithk = m.slices[is].ithk;
+            if (ithk < 0) continue;
+            // O = F_x(L)/nx * bwl * exp(i phi_back)  (get_psi, src/wave_function.cuh:96-105)
+            const float zb = m.thk[ithk].z_back_prop;
+            const float2 *osrc = psi.p;
+            if (fs || zb != 0.f || m.bwl) {
+                ColArgs co = S.col_base(psi.p, scratch.p, nb);
+                co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) | COL_STORE;
+                co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
+                co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
+                co.p_scale = (float)m.ny;
+                check(launch_col_pass(m.nx, co, false, false, s), "output col pass");
+                osrc = scratch.p;
+            }
+            if (!fs) {
+                RowArgs ro = r;
+                ro.src = osrc; ro.dst = scratch.p;
+                ro.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+                check(launch_row_pass(m.ny, ro, s), "output row pass");
+                osrc = scratch.p;
+            }
+            OutArgs oa{};
+            oa.src = osrc; oa.member_stride = (long long)nxy; oa.n_members = nb;
+            oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
+            oa.w = w;
+            oa.m2 = m2.p + (size_t)ithk * no;
+            oa.psi = pc.p + (size_t)ithk * no * 2;
+            oa.psi_full = pfull.p + (need_full ? (size_t)ithk * nxy * 2 : 0);
+            if (ot == MT_OT_PSI_COH) oa.mode = OUT_SET_PSI;
+            else if (ot == MT_OT_M2PSI_TOT_PSI_COH) oa.mode = OUT_ADD_M2 | OUT_ADD_PSI;
+            else oa.mode = OUT_ADD_M2 | (need_full ? OUT_ADD_PSI_FULL : 0);
+            check(launch_out_accum(oa, s), "output accumulate");
+        }
+        progress_add(nb);
+    }
+    check(hipEventRecord(t1.e, s), "event");
+    check(hipEventSynchronize(t1.e), "sync");
+    tm.ms_total = elapsed(t0, t1);
+    tm.probe_slices = (double)nconf * ns_run;
+
+    if (out->m2psi_tot && (ot == MT_OT_M2PSI_TOT || ot == MT_OT_M2PSI_TOT_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
+        check(hipMemcpyAsync(out->m2psi_tot, m2.p, (size_t)n_thk * no * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    if (out->psi_coh && (ot == MT_OT_PSI_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
+        check(hipMemcpyAsync(out->psi_coh, pc.p, (size_t)n_thk * no * 2 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    if (need_full && out->m2psi_coh) {
+        DBuf<double> tmp(no);
+        for (int k = 0; k < n_thk; ++k) {
+            check(launch_m2_from_psi(pfull.p + (size_t)k * nxy * 2, m.nx, m.ny, m.oa_ix_0, m.oa_iy_0, nxo, nyo, tmp.p, s),
+                  "m2 from psi");
+            check(hipMemcpyAsync(out->m2psi_coh + (size_t)k * no, tmp.p, no * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "sync");
+        }
+    }
+    check(hipStreamSynchronize(s), "sync");
+}
+
+} // namespace
+
+RunControl &run_control()
+{
+    static RunControl rc;
+    return rc;
+}
+
+Timing &last_timing()
+{
+    static Timing t;
+    return t;
+}
+
+void run_multislice(const Model &m, int device, mt_output_multislice *out)
+{
+    select_device(device);
+    if (m.cond.phi_required) throw Error{MT_ERR_UNSUPPORTED, "azimuthal condenser aberrations"};
+    m.output_dims(out);
+    m.fill_axes(out);
+    if (m.is_stem()) run_stem(m, out);
+    else run_ew(m, out);
+}
+
+// ---------------------------------------------------------------------------
+// per-piece entries
+
+void projected_potential(const Model &m, int device, int iconf, int islice, double *V)
+{
+    select_device(device);
+    if (islice < 0 || islice >= m.n_slice()) throw Error{MT_ERR_INVALID_ARGUMENT, "islice out of range"};
+    Stream st;
+    Setup S(m, st.s);
+    auto a = S.atoms_of(iconf);
+    DBuf<float4> atoms(a.size());
+    atoms.upload(a.data(), a.size(), st.s);
+    DBuf<float> dv(S.nxy);
+    S.build_tf(atoms, islice, 1, nullptr, dv.p, nullptr);
+    std::vector<float> h(S.nxy);
+    check(hipMemcpyAsync(h.data(), dv.p, S.nxy * sizeof(float), hipMemcpyDeviceToHost, st.s), "D2H");
+    check(hipStreamSynchronize(st.s), "sync");
+    for (size_t i = 0; i < S.nxy; ++i) V[i] = h[i];
+}
+
+static void download_unshifted(Setup &S, const float2 *d, double *out)
+{
+    DBuf<float2> tmp(S.nxy);
+    check(launch_shift_complex(d, tmp.p, S.nx, S.ny, S.s), "shift");
+    std::vector<float2> h(S.nxy);
+    check(hipMemcpyAsync(h.data(), tmp.p, S.nxy * sizeof(float2), hipMemcpyDeviceToHost, S.s), "D2H");
+    check(hipStreamSynchronize(S.s), "sync");
+    for (size_t i = 0; i < S.nxy; ++i) { out[2 * i] = h[i].x; out[2 * i + 1] = h[i].y; }
+}
+
+void transmission_function(const Model &m, int device, int iconf, int islice, double *trans)
+{
+    select_device(device);
+    if (islice < 0 || islice >= m.n_slice()) throw Error{MT_ERR_INVALID_ARGUMENT, "islice out of range"};
+    Stream st;
+    Setup S(m, st.s);
+    auto a = S.atoms_of(iconf);
+    DBuf<float4> atoms(a.size());
+    atoms.upload(a.data(), a.size(), st.s);
+    DBuf<float2> t(S.nxy);
+    S.build_tf(atoms, islice, 1, t.p, nullptr, nullptr);
+    download_unshifted(S, t.p, trans);
+}
+
+void incident_wave(const Model &m, int device, double *psi0)
+{
+    select_device(device);
+    if (m.cond.phi_required) throw Error{MT_ERR_UNSUPPORTED, "azimuthal condenser aberrations"};
+    Stream st;
+    Setup S(m, st.s);
+    DBuf<float2> w(S.nxy);
+    if (m.iw_type == 1) {
+        std::vector<float2> one(S.nxy, make_float2(1.f, 0.f));
+        w.upload(one.data(), S.nxy, st.s);
+    } else {
+        ProbeParams P = S.probe_params(S.count_aperture());
+        std::vector<Beam> hb;
+        for (size_t i = 0; i < m.iw_x.size(); ++i) hb.push_back(S.beam(m.iw_x[i], m.iw_y[i]));
+        if (hb.empty()) throw Error{MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y"};
+        DBuf<Beam> beams(hb.size());
+        beams.upload(hb.data(), hb.size(), st.s);
+        Member hm{0, -1, 0, (int)hb.size()};
+        DBuf<Member> mem(1);
+        mem.upload(&hm, 1, st.s);
+        ColArgs c = S.col_base(nullptr, w.p, 1);
+        c.flags = COL_IFFT | COL_STORE;
+        c.members = mem.p; c.beams = beams.p; c.probe = P;
+        check(launch_col_pass(m.nx, c, true, false, st.s), "probe pass");
+        RowArgs r{};
+        r.src = w.p; r.dst = w.p; r.member_stride = (long long)S.nxy; r.nrows = m.nx; r.n_members = 1;
+        r.tw = S.tw_y.p; r.members = mem.p;
+        r.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+        check(launch_row_pass(m.ny, r, st.s), "row pass");
+        check(hipStreamSynchronize(st.s), "sync");
+    }
+    download_unshifted(S, w.p, psi0);
+}
+
+// Propagator::operator()(space, ..., output) (src/propagator.cuh:95-104)
+void propagate(const Model &m, int device, const double *psi_in, double z, int space, double *psi_out)
+{
+    select_device(device);
+    if (space != 1 && space != 2) throw Error{MT_ERR_INVALID_ARGUMENT, "space must be 1 (real) or 2 (reciprocal)"};
+    Stream st;
+    Setup S(m, st.s);
+    std::vector<float2> h(S.nxy);
+    for (size_t i = 0; i < S.nxy; ++i) h[i] = make_float2((float)psi_in[2 * i], (float)psi_in[2 * i + 1]);
+    DBuf<float2> a(S.nxy), w(S.nxy);
+    a.upload(h.data(), S.nxy, st.s);
+    check(launch_shift_complex(a.p, w.p, S.nx, S.ny, st.s), "shift");
+    DBuf<float2> px, py;
+    std::vector<double> zl{z};
+    S.prop_tables(zl, px, py);
+    const bool zero = std::fabs((float)z) <= 1e-5f;
+    Member hm{0, -1, 0, 0};
+    DBuf<Member> mem(1);
+    mem.upload(&hm, 1, st.s);
+    RowArgs r{};
+    r.src = w.p; r.dst = w.p; r.member_stride = (long long)S.nxy; r.nrows = m.nx; r.n_members = 1;
+    r.tw = S.tw_y.p; r.members = mem.p;
+    bool transformed = false;
+    if (!zero || m.bwl || space == 2) {
+        r.flags = ROW_LOAD | ROW_FFT | ROW_STORE;
+        check(launch_row_pass(m.ny, r, st.s), "row pass");
+        ColArgs c = S.col_base(w.p, w.p, 1);
+        if (!zero) c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE;
+        else c.flags = COL_LOAD | COL_FFT | (m.bwl ? COL_BWL : 0) | COL_STORE;
+        if (zero && !m.bwl) {
+            c.flags = COL_LOAD | COL_FFT | COL_MUL_P | COL_STORE;   // scale 1/(nx ny): z = 0 tables
+        }
+        c.px = px.p; c.py = py.p;
+        if (space == 1) c.flags |= COL_IFFT;
+        check(launch_col_pass(m.nx, c, false, false, st.s), "col pass");
+        transformed = true;
+    }
+    if (transformed && space == 1) {
+        r.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+        check(launch_row_pass(m.ny, r, st.s), "row pass");
+    }
+    check(hipStreamSynchronize(st.s), "sync");
+    download_unshifted(S, w.p, psi_out);
+}
+
+} // namespace mtg

@@ -1,0 +1,45 @@
+// Engine: drives the HIP kernels for one validated Model on one GPU.
+#pragma once
+
+#include <atomic>
+#include <string>
+#include <vector>
+
+#include "host_model.hpp"
+#include "kernels.hpp"
+
+namespace mtg {
+
+struct RunControl {
+    std::atomic<bool> stop{false};
+    std::atomic<int> iter{0};
+    std::atomic<int> niter{0};
+    size_t memory_budget = 0;       // 0: derived from free device memory
+    int batch_override = 0;         // 0: planner's choice
+    bool use_graph = true;
+    bool time_kernels = true;       // per-kernel HIP-event timing of one batch per configuration
+};
+
+struct Timing {
+    double ms_total = 0;            // device time of the whole run (events on the engine stream)
+    double ms_slice = 0;            // device time of the batched slice loops
+    double probe_slices = 0;        // probe.slices covered by ms_slice
+    double ms_row = 0, ms_col = 0;  // summed durations of timed row / column passes
+    int n_row = 0, n_col = 0;       // launches timed
+    double ms_tf = 0;               // transmission build (all configurations)
+    std::string plan;
+};
+
+RunControl &run_control();
+Timing &last_timing();
+
+// Full simulation into caller-owned output arrays.
+void run_multislice(const Model &m, int device, mt_output_multislice *out);
+
+// Per-piece entries (natural, unshifted layouts).
+void projected_potential(const Model &m, int device, int iconf, int islice, double *V);
+void transmission_function(const Model &m, int device, int iconf, int islice, double *trans);
+void incident_wave(const Model &m, int device, double *psi0);
+void propagate(const Model &m, int device, const double *psi_in, double z, int space, double *psi_out);
+
+} // namespace mtg

@@ -1,0 +1,206 @@
+// Register-resident Stockham FFT building blocks for CDNA4.
+//
+// A length-N sequence is held by T = N/E threads, E complex values each:
+// thread t holds x[t + T*e], e = 0..E-1.  Each stage of radix R performs E/R
+// butterflies per thread on registers v[q + r*(E/R)], then scatters its
+// results through LDS so that, after the exchange, thread t again holds
+// x'[t + T*e].  After the last stage the layout is the natural-order
+// transform in the same t + T*e pattern, so a forward transform can be
+// followed by a point-wise multiply and an inverse transform with no extra
+// data movement, and loads/stores from HBM stay coalesced.
+//
+// Transforms are unnormalised in both directions (forward: exp(-2 pi i nk/N)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mtg {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a)
+{
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+constexpr float kSqrtHalf = 0.70710678118654752440f;
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2 &a0, float2 &a1)
+{
+    float2 t = csub(a0, a1);
+    a0 = cadd(a0, a1);
+    a1 = t;
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3)
+{
+    float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+    float2 s13 = cadd(a1, a3), d13 = mul_mi<INV>(csub(a1, a3));
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = cadd(d02, d13);
+    a3 = csub(d02, d13);
+}
+
+// W8^k for k = 1, 3 (forward) or their conjugates (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 w8_1(float2 a)
+{
+    // (1 -/+ i)/sqrt2 * a
+    return INV ? make_float2((a.x - a.y) * kSqrtHalf, (a.x + a.y) * kSqrtHalf)
+               : make_float2((a.x + a.y) * kSqrtHalf, (a.y - a.x) * kSqrtHalf);
+}
+template <bool INV>
+__device__ __forceinline__ float2 w8_3(float2 a)
+{
+    // (-1 -/+ i)/sqrt2 * a
+    return INV ? make_float2((-a.x - a.y) * kSqrtHalf, (a.x - a.y) * kSqrtHalf)
+               : make_float2((a.y - a.x) * kSqrtHalf, (-a.x - a.y) * kSqrtHalf);
+}
+
+// 8-point DFT of v[0], v[S], ..., v[7S], result in natural order
+template <bool INV, int S, int E>
+__device__ __forceinline__ void dft8(float2 (&v)[E], int o)
+{
+    float2 e0 = v[o], e1 = v[o + 2 * S], e2 = v[o + 4 * S], e3 = v[o + 6 * S];
+    float2 d0 = v[o + S], d1 = v[o + 3 * S], d2 = v[o + 5 * S], d3 = v[o + 7 * S];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(d0, d1, d2, d3);
+    d1 = w8_1<INV>(d1);
+    d2 = mul_mi<INV>(d2);
+    d3 = w8_3<INV>(d3);
+    v[o] = cadd(e0, d0);         v[o + 4 * S] = csub(e0, d0);
+    v[o + S] = cadd(e1, d1);     v[o + 5 * S] = csub(e1, d1);
+    v[o + 2 * S] = cadd(e2, d2); v[o + 6 * S] = csub(e2, d2);
+    v[o + 3 * S] = cadd(e3, d3); v[o + 7 * S] = csub(e3, d3);
+}
+
+// 16-point DFT as 4 x 4 with internal twiddles W16^(q*kp)
+template <bool INV, int S, int E>
+__device__ __forceinline__ void dft16(float2 (&v)[E], int o)
+{
+    float2 y[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        y[q][0] = v[o + q * S];
+        y[q][1] = v[o + (q + 4) * S];
+        y[q][2] = v[o + (q + 8) * S];
+        y[q][3] = v[o + (q + 12) * S];
+        dft4<INV>(y[q][0], y[q][1], y[q][2], y[q][3]);
+    }
+    // twiddles W16^(q*kp), q, kp in 1..3
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float sg = INV ? 1.f : -1.f;
+    const float2 w1 = make_float2(c1, sg * s1);                // W16^1
+    const float2 w2 = make_float2(kSqrtHalf, sg * kSqrtHalf);  // W16^2
+    const float2 w3 = make_float2(s1, sg * c1);                // W16^3
+    const float2 w6 = make_float2(-kSqrtHalf, sg * kSqrtHalf); // W16^6
+    const float2 w9 = make_float2(-c1, -sg * s1);              // W16^9
+    y[1][1] = cmul(y[1][1], w1); y[1][2] = cmul(y[1][2], w2); y[1][3] = cmul(y[1][3], w3);
+    y[2][1] = cmul(y[2][1], w2); y[2][2] = mul_mi<INV>(y[2][2]); y[2][3] = cmul(y[2][3], w6);
+    y[3][1] = cmul(y[3][1], w3); y[3][2] = cmul(y[3][2], w6); y[3][3] = cmul(y[3][3], w9);
+#pragma unroll
+    for (int kp = 0; kp < 4; ++kp) {
+        float2 a0 = y[0][kp], a1 = y[1][kp], a2 = y[2][kp], a3 = y[3][kp];
+        dft4<INV>(a0, a1, a2, a3);
+        v[o + kp * S] = a0;          // X[4*kq + kp], kq = 0..3
+        v[o + (kp + 4) * S] = a1;
+        v[o + (kp + 8) * S] = a2;
+        v[o + (kp + 12) * S] = a3;
+    }
+}
+
+template <int R, bool INV, int S, int E>
+__device__ __forceinline__ void dftR(float2 (&v)[E], int o)
+{
+    if constexpr (R == 2) dft2<INV>(v[o], v[o + S]);
+    else if constexpr (R == 4) dft4<INV>(v[o], v[o + S], v[o + 2 * S], v[o + 3 * S]);
+    else if constexpr (R == 8) dft8<INV, S, E>(v, o);
+    else if constexpr (R == 16) dft16<INV, S, E>(v, o);
+}
+
+// One Stockham stage: radix R, NS = product of the radices of earlier stages.
+// tw: W_N^m = exp(-2 pi i m / N), m in [0, N).
+template <int N, int E, int R, int NS, bool INV>
+__device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, const float2 *__restrict__ tw)
+{
+    constexpr int T = N / E;
+    constexpr int Q = E / R;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if constexpr (NS > 1) {
+            const int j = t + T * q;
+            const int k = j & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[r * k * (N / (NS * R))];
+                if (INV) w.y = -w.y;
+                v[q + r * Q] = cmul(v[q + r * Q], w);
+            }
+        }
+        dftR<R, INV, Q, E>(v, q);
+    }
+}
+
+// LDS index written by element (q, r) of a stage; see fft_stage.
+template <int N, int E, int R, int NS>
+__device__ __forceinline__ int stage_out_index(int t, int q, int r)
+{
+    constexpr int T = N / E;
+    const int j = t + T * q;
+    return (j / NS) * NS * R + (j & (NS - 1)) + r * NS;
+}
+
+// Radix schedules.  Each radix divides E.
+template <int N> struct Sched;
+template <> struct Sched<64>   { static constexpr int E = 8;  static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
+template <> struct Sched<128>  { static constexpr int E = 8;  static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 2, 0}; };
+template <> struct Sched<256>  { static constexpr int E = 8;  static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 4, 0}; };
+template <> struct Sched<512>  { static constexpr int E = 8;  static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
+template <> struct Sched<1024> { static constexpr int E = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 8, 8, 0}; };
+template <> struct Sched<2048> { static constexpr int E = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 8, 0}; };
+
+template <int N, int ST>
+struct StageNS {
+    static constexpr int value = (ST == 0) ? 1 : StageNS<N, ST - 1>::value * Sched<N>::R[ST - 1];
+};
+template <int N>
+struct StageNS<N, 0> { static constexpr int value = 1; };
+
+// Full transform.  lds(i) maps a sequence index to this sequence's LDS slot;
+// sync() is the barrier that orders LDS writes and reads of all threads
+// sharing the buffer.  The data starts and ends in the t + T*e layout.
+template <int N, bool INV, int ST = 0, class LdsIdx, class Sync>
+__device__ __forceinline__ void fft_run(float2 (&v)[Sched<N>::E], int t, const float2 *__restrict__ tw,
+                                        float2 *lds, LdsIdx lds_idx, Sync sync)
+{
+    constexpr int E = Sched<N>::E;
+    constexpr int R = Sched<N>::R[ST];
+    constexpr int NS = StageNS<N, ST>::value;
+    constexpr int T = N / E;
+    fft_stage<N, E, R, NS, INV>(v, t, tw);
+    if constexpr (ST + 1 < Sched<N>::NST) {
+        constexpr int Q = E / R;
+        sync();   // previous readers of the buffer are done
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                lds[lds_idx(stage_out_index<N, E, R, NS>(t, q, r))] = v[q + r * Q];
+        sync();
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = lds[lds_idx(t + T * e)];
+        fft_run<N, INV, ST + 1>(v, t, tw, lds, lds_idx, sync);
+    }
+}
+
+} // namespace mtg

@@ -1,0 +1,691 @@
+// Host model: input reading and validation, spline tables, slicing, phonons.
+//
+// Follows the reference's FP32 path (Input_Multislice<float>): quantities the
+// reference keeps in T are float here, tables are built in double and cast.
+#include "host_model.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <numeric>
+#include <random>
+
+#include "lobato_table.inc"
+
+namespace mtg {
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238463;
+constexpr double kTwoPi = 6.283185307179586476925;
+constexpr double kPi2 = 9.869604401089358618834;      // c_Pi2 (src/types.cuh:88)
+constexpr double kPotf = 47.877645145863056;      // c_Potf (src/types.cuh:66)
+constexpr double kDegToRad = 0.01745329251994329576924;
+constexpr double kMradToRad = 1.0e-3;
+constexpr double kMmToAngs = 1.0e7;
+
+[[noreturn]] void fail(mt_status c, const std::string &m) { throw Error{c, m}; }
+
+// isEqual<float> / isEqual<double> (src/types.cuh:562-592)
+bool near_zero(float a) { return std::fabs(a) <= 1e-5f; }
+bool near_zero(double a) { return std::fabs(a) <= 1e-13; }
+bool same_value(float a, float b)
+{
+    float d = std::fabs(a - b);
+    return d <= 1e-5f || d <= std::max(std::fabs(a), std::fabs(b)) * 1e-4f;
+}
+
+// Polynomial approximations of I0, I1, K0, K1: Abramowitz & Stegun 9.8.1-9.8.8.
+double horner(const double *c, int n, double y)
+{
+    double r = c[n - 1];
+    for (int i = n - 2; i >= 0; --i) r = c[i] + y * r;
+    return r;
+}
+double i0_as(double x)
+{
+    static const double a[] = {1.0, 3.5156229, 3.0899424, 1.2067492, 0.2659732, 0.360768e-1, 0.45813e-2};
+    static const double b[] = {0.39894228, 0.1328592e-1, 0.225319e-2, -0.157565e-2, 0.916281e-2,
+                               -0.2057706e-1, 0.2635537e-1, -0.1647633e-1, 0.392377e-2};
+    double ax = std::fabs(x);
+    if (ax < 3.75) { double t = x / 3.75; return horner(a, 7, t * t); }
+    return (std::exp(ax) / std::sqrt(ax)) * horner(b, 9, 3.75 / ax);
+}
+double i1_as(double x)
+{
+    static const double a[] = {0.5, 0.87890594, 0.51498869, 0.15084934, 0.2658733e-1, 0.301532e-2, 0.32411e-3};
+    double ax = std::fabs(x), r;
+    if (ax < 3.75) {
+        double t = x / 3.75;
+        r = ax * horner(a, 7, t * t);
+    } else {
+        double y = 3.75 / ax;
+        double tail = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+        r = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y * (-0.1031555e-1 + y * tail))));
+        r *= std::exp(ax) / std::sqrt(ax);
+    }
+    return x < 0.0 ? -r : r;
+}
+double k0_as(double x)
+{
+    static const double a[] = {-0.57721566, 0.42278420, 0.23069756, 0.3488590e-1, 0.262698e-2, 0.10750e-3, 0.74e-5};
+    static const double b[] = {1.25331414, -0.7832358e-1, 0.2189568e-1, -0.1062446e-1, 0.587872e-2, -0.251540e-2, 0.53208e-3};
+    if (x <= 2.0) return (-std::log(x / 2.0) * i0_as(x)) + horner(a, 7, x * x / 4.0);
+    return (std::exp(-x) / std::sqrt(x)) * horner(b, 7, 2.0 / x);
+}
+double k1_as(double x)
+{
+    static const double a[] = {1.0, 0.15443144, -0.67278579, -0.18156897, -0.1919402e-1, -0.110404e-2, -0.4686e-4};
+    static const double b[] = {1.25331414, 0.23498619, -0.3655620e-1, 0.1504268e-1, -0.780353e-2, 0.325614e-2, -0.68245e-3};
+    if (x <= 2.0) return (std::log(x / 2.0) * i1_as(x)) + (1.0 / x) * horner(a, 7, x * x / 4.0);
+    return (std::exp(-x) / std::sqrt(x)) * horner(b, 7, 2.0 / x);
+}
+
+// Lobato hydrogen-like terms for the 3D potential Vr and the projected VR
+// (set_Lobato_neutral_coef, src/atomic_data.hpp:1440-1485).
+struct Terms { double a[5], k[5]; };
+void lobato_terms(int Z, Terms &vr, Terms &VR)
+{
+    const double *t = mt_lobato_coef[Z - 1];
+    for (int j = 0; j < 5; ++j) {
+        double cl = t[2 * j], cnl = near_zero(cl) ? 0.0 : t[2 * j + 1];
+        if (near_zero(cl) && near_zero(cnl)) {
+            vr.a[j] = VR.a[j] = 0.0;
+            vr.k[j] = VR.k[j] = 1.0;
+            continue;
+        }
+        // same operation order as the reference, so the tables round identically
+        vr.a[j] = kPotf * kPi2 * cl / std::pow(cnl, 1.5);
+        VR.a[j] = 2.0 * kPotf * kPi2 * cl / std::pow(cnl, 1.5);
+        vr.k[j] = VR.k[j] = kTwoPi / std::sqrt(cnl);
+    }
+}
+double vr3(const Terms &t, double r)
+{
+    double ir = 1 / r, s = 0;
+    for (int i = 0; i < 5; ++i) s += t.a[i] * std::exp(-t.k[i] * r) * ir * (2 / t.k[i] + r);
+    return s;
+}
+
+} // namespace
+
+AtomTable build_atom_table(int Z, double dR_min, double Vrl, int nR)
+{
+    AtomTable tb;
+    tb.Z = Z;
+    Terms vr, VR;
+    lobato_terms(Z, vr, VR);
+    const double rn_c = 1.2e-05 * std::pow((double)mt_element_table[Z - 1].A, 1.0 / 3.0);
+    const double R_min = std::max(rn_c, dR_min);
+
+    // cut-off radius where Vr = Vrl: bisection on [rn_c, 25]
+    // (AtomicRadius_Cutoff + Root_Finder, src/atomic_fcns_mt.hpp:269-296, src/cgpu_fcns.cuh:262-286)
+    double R_max = 0.0;
+    if (!(near_zero(Vrl) || near_zero(vr.a[0]))) {
+        double lo = rn_c, hi = 25.0, f_hi = vr3(vr, hi) - Vrl, x, fx;
+        int it = 0;
+        do {
+            x = 0.5 * (lo + hi);
+            fx = vr3(vr, x) - Vrl;
+            if (fx * f_hi < 0) lo = x;
+            else { hi = x; f_hi = fx; }
+            ++it;
+        } while (std::fabs(fx) > 1e-8 && it < 200);
+        R_max = x;
+    }
+    R_max = std::max(R_max, 2.0 * R_min);
+    if (near_zero(R_max)) R_max = 1.75 * mt_element_table[Z - 1].ra;
+
+    tb.R_min = R_min;
+    tb.R_max = R_max;
+    tb.R_tap = R_min + 0.85 * (R_max - R_min);
+    const double R2_tap = tb.R_tap * tb.R_tap;
+    tb.tap_cf = (kPi / 2) / (R_max * R_max - R2_tap);
+
+    std::vector<double> R(nR), R2(nR), y(nR), dy(nR);
+    const double dlnr = std::log(R_max / R_min) / double(nR - 1);
+    for (int i = 0; i < nR; ++i) {
+        R[i] = R_min * std::exp(i * dlnr);
+        R2[i] = std::pow(R[i], 2);
+        double v = 0, dv = 0;
+        for (int j = 0; j < 5; ++j) {
+            double kx = VR.k[j] * R[i], k0 = k0_as(kx), k1 = k1_as(kx);
+            v += VR.a[j] * (2 * k0 / VR.k[j] + R[i] * k1);
+            dv += -VR.a[j] * (VR.k[j] * R[i] * k0 + 2 * k1);
+        }
+        double d = 0.5 * dv / R[i];       // derivative with respect to R^2
+        if (R2_tap < R2[i]) {             // cosine taper (apply_tapering, src/cgpu_fcns.cuh:1528-1538)
+            double arg = tb.tap_cf * (R2[i] - R2_tap);
+            double c = std::cos(arg), s = std::sin(arg);
+            d = d * c - tb.tap_cf * v * s;
+            v *= c;
+        }
+        y[i] = v;
+        dy[i] = d;
+    }
+    tb.R2.resize(nR); tb.c0.resize(nR); tb.c1.resize(nR); tb.c2.assign(nR, 0.f); tb.c3.assign(nR, 0.f);
+    for (int i = 0; i < nR; ++i) {
+        tb.R2[i] = (float)R2[i];
+        tb.c0[i] = (float)y[i];
+        tb.c1[i] = (float)dy[i];
+    }
+    for (int i = 0; i + 1 < nR; ++i) {   // Hermite cubic on [R2_i, R2_i+1]
+        double h = 1.0 / (R2[i + 1] - R2[i]);
+        double m = (y[i + 1] - y[i]) * h, n = dy[i] + dy[i + 1];
+        tb.c2[i] = (float)((3.0 * m - n - dy[i]) * h);
+        tb.c3[i] = (float)((n - 2.0 * m) * h * h);
+    }
+    tb.R_max_f = (float)R_max;
+    tb.R2_max_f = tb.R_max_f * tb.R_max_f;
+    return tb;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+float lambda_f(float E_0)                 // get_lambda<float> (src/cgpu_fcns.cuh:51-58)
+{
+    float emass = 510.99906f, hc = 12.3984244f;
+    return hc / std::sqrt(E_0 * (2 * emass + E_0));
+}
+
+void read_lens(const mt_lens &m, LensModel &L)
+{
+    L.m = (int)m.m;
+    L.c_10 = (float)m.c_10; L.c_12 = (float)m.c_12; L.phi_12 = (float)(m.phi_12 * kDegToRad);
+    L.c_21 = (float)m.c_21; L.phi_21 = (float)(m.phi_21 * kDegToRad);
+    L.c_23 = (float)m.c_23; L.phi_23 = (float)(m.phi_23 * kDegToRad);
+    L.c_30 = (float)(m.c_30 * kMmToAngs); L.c_32 = (float)m.c_32; L.phi_32 = (float)(m.phi_32 * kDegToRad);
+    L.c_34 = (float)m.c_34; L.phi_34 = (float)(m.phi_34 * kDegToRad);
+    L.c_41 = (float)m.c_41; L.phi_41 = (float)(m.phi_41 * kDegToRad);
+    L.c_43 = (float)m.c_43; L.phi_43 = (float)(m.phi_43 * kDegToRad);
+    L.c_45 = (float)m.c_45; L.phi_45 = (float)(m.phi_45 * kDegToRad);
+    L.c_50 = (float)(m.c_50 * kMmToAngs); L.c_52 = (float)m.c_52; L.phi_52 = (float)(m.phi_52 * kDegToRad);
+    L.c_54 = (float)m.c_54; L.phi_54 = (float)(m.phi_54 * kDegToRad);
+    L.c_56 = (float)m.c_56; L.phi_56 = (float)(m.phi_56 * kDegToRad);
+    L.inner_aper_ang = (float)(m.inner_aper_ang * kMradToRad);
+    L.outer_aper_ang = (float)(m.outer_aper_ang * kMradToRad);
+    L.zero_defocus_type = m.zero_defocus_type;
+    L.zero_defocus_plane = (float)m.zero_defocus_plane;
+}
+
+// aperture bounds of Lens::set_input_data (src/types.cuh:2924-2976)
+void set_lens(LensModel &L, float E_0, float g_max)
+{
+    L.lambda = lambda_f(E_0);
+    L.g2_min = (near_zero(L.inner_aper_ang) || L.inner_aper_ang < 0)
+                   ? 0.f : (float)std::pow(std::sin(L.inner_aper_ang) / L.lambda, 2);
+    L.g2_max = (near_zero(L.outer_aper_ang) || L.outer_aper_ang < 0)
+                   ? g_max * g_max : (float)std::pow(std::sin(L.outer_aper_ang) / L.lambda, 2);
+    const float c[] = {(float)L.m, L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
+    L.phi_required = false;
+    for (float v : c) L.phi_required = L.phi_required || !near_zero(v);
+}
+
+float zero_defocus_plane(const LensModel &L, float z_min, float z_max)
+{
+    switch (L.zero_defocus_type) {
+    case 1: return z_min;
+    case 2: return 0.5f * (z_min + z_max);
+    case 3: return z_max;
+    default: return L.zero_defocus_plane;
+    }
+}
+
+// Plane finder of Identify_Planes (src/types.cuh:1166-1372): a 0.1 A histogram
+// whose one-bin gaps inside a layer are closed, then Kahan means per layer.
+std::vector<float> find_planes(std::vector<float> v)
+{
+    std::vector<float> planes;
+    if (v.empty()) return planes;
+    std::sort(v.begin(), v.end());
+    const double dv = 0.1;
+    const double v_min = v.front(), v_max = v.back();
+    int nb = (int)std::ceil(std::max(v_max - v_min, dv) / dv);
+    std::vector<int> h(nb, 0);
+    for (float f : v) {
+        double x = f;
+        int ih = (int)std::floor((x - v_min) / dv);
+        // move to the bin [v_min + ih dv, v_min + (ih+1) dv) that holds x
+        if (x < v_min + (ih - 1) * dv) {
+            for (int k = ih; k >= 0; --k) {
+                double lo = v_min + (k - 1) * dv;
+                if (lo <= x && x < lo + dv) { ih = k - 1; break; }
+            }
+        } else if (x > v_min + ih * dv) {
+            for (int k = ih; k < nb; ++k) {
+                double lo = v_min + k * dv;
+                if (lo <= x && x < lo + dv) { ih = k; break; }
+            }
+        }
+        ih = std::max(0, std::min(nb - 1, ih));
+        h[ih]++;
+    }
+    while (!h.empty() && h.back() == 0) h.pop_back();
+    nb = (int)h.size();
+    for (int i = 1; i < nb - 1; ++i) {
+        bool ahead = (i < nb - 2) ? (h[i + 2] > 0) : false;
+        if (h[i] == 0 && h[i - 1] > 0 && (h[i + 1] > 0 || ahead)) h[i] = 1;
+    }
+    if (nb == 1) {
+        float s = 0;
+        for (float f : v) s += f;
+        planes.push_back(s / float(v.size()));
+        return planes;
+    }
+    std::vector<float> lim;
+    for (int i = 0; i + 1 < nb; ++i)
+        if (h[i] > 0 && h[i + 1] == 0) lim.push_back((float)(v.front() + (i + 1) * dv));
+    lim.push_back((float)(v.back() + dv));
+    float mean = v.front(), carry = 0;
+    int cnt = 1;
+    size_t il = 0;
+    for (float f : v) {
+        if (f < lim[il]) {
+            float y = f - carry, t = mean + y;   // Kahan summation
+            carry = (t - mean) - y;
+            mean = t;
+            ++cnt;
+        } else {
+            planes.push_back(mean / cnt);
+            mean = f; carry = 0; cnt = 1; ++il;
+        }
+    }
+    planes.push_back(mean / cnt);
+    return planes;
+}
+
+// evenly spaced planes with matched borders (Identify_Planes(v_min, v_max, dv), eMB_MinMax)
+std::vector<float> even_planes(float v_min, float v_max, float dv)
+{
+    const float eps = 1e-4f;
+    if (std::fabs(v_max - v_min) < eps) return {0.5f * (v_min + v_max)};
+    if (v_max < v_min) return {};
+    float s = v_max - v_min;
+    int nv = std::max(1, (int)std::floor(s / dv + eps)) + 1;
+    float db = dv + 0.5f * (s - (nv - 1) * dv);
+    std::vector<float> p(nv);
+    p[0] = v_min;
+    for (int i = 1; i < nv; ++i) p[i] = p[i - 1] + ((i == 1 || i == nv - 1) ? db : dv);
+    return p;
+}
+
+void unique_sorted(std::vector<float> &v)
+{
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end(), same_value), v.end());
+}
+
+float spacing(const std::vector<float> &p, int i, float dz)
+{
+    if (p.size() == 1) return dz;
+    i = std::max(1, std::min(i, (int)p.size() - 1));
+    return p[i] - p[i - 1];
+}
+
+std::vector<float> slice_bounds(const std::vector<float> &p, float dz, float z_int_min, float z_int_max)
+{
+    if (p.size() == 1) return {z_int_min, z_int_max};
+    std::vector<float> zs(p.size() + 1);
+    zs[0] = p[0] - 0.5f * spacing(p, 0, dz);
+    for (size_t i = 1; i + 1 < zs.size(); ++i) zs[i] = 0.5f * (p[i] + p[i - 1]);
+    zs.back() = zs[zs.size() - 2] + spacing(p, (int)p.size(), dz);
+    return zs;
+}
+
+} // namespace
+
+Model::Model(const mt_input_multislice &in)
+{
+    sim_type = in.simulation_type;
+    if (!(is_stem() || is_cbed() || is_ewfs() || is_ewrs()))
+        fail(MT_ERR_UNSUPPORTED, "simulation_type must be 11 (STEM), 21 (CBED), 51 (EWFS) or 52 (EWRS)");
+    if (in.interaction_model != 1) fail(MT_ERR_UNSUPPORTED, "only interaction_model 1 (multislice)");
+    if (in.potential_type != 6) fail(MT_ERR_UNSUPPORTED, "only potential_type 6 (Lobato)");
+    if (in.pn_model == 2) fail(MT_ERR_UNSUPPORTED, "absorptive potential model");
+    if (in.pn_model < 1 || in.pn_model > 3) fail(MT_ERR_INVALID_ARGUMENT, "pn_model must be 1..3");
+    if (!(in.potential_slicing == 1 || in.potential_slicing == 2))
+        fail(MT_ERR_UNSUPPORTED, "potential_slicing 3/4 (sub-slicing)");
+    if (std::fabs(in.spec_rot_theta) > 0) fail(MT_ERR_UNSUPPORTED, "specimen rotation");
+    if (std::fabs(in.theta) > 0) fail(MT_ERR_UNSUPPORTED, "tilted illumination");
+    if (in.illumination_model == 4) fail(MT_ERR_UNSUPPORTED, "illumination_model 4 (full integration)");
+
+    pn_model = in.pn_model;
+    pn_coh_contrib = in.pn_coh_contrib;
+    pn_single_conf = in.pn_single_conf;
+    pn_nconf = in.pn_nconf;
+    pn_seed = in.pn_seed;
+    switch (in.pn_dim) {                // FP_Dim::set (src/types.cuh:1687-1740)
+    case 111: pn_x = pn_y = pn_z = true; break;
+    case 110: pn_x = pn_y = true; pn_z = false; break;
+    case 101: pn_x = true; pn_y = false; pn_z = true; break;
+    case 11: pn_x = false; pn_y = pn_z = true; break;
+    case 100: pn_x = true; pn_y = pn_z = false; break;
+    case 10: pn_x = false; pn_y = true; pn_z = false; break;
+    case 1: pn_x = pn_y = false; pn_z = true; break;
+    default: break;
+    }
+
+    // grid (Grid_2d::set_input_data, src/types.cuh:2154-2178), periodic
+    nx = in.nx; ny = in.ny;
+    auto pow2 = [](int n) { return n >= 64 && n <= 2048 && (n & (n - 1)) == 0; };
+    if (nx <= 0 || ny <= 0) fail(MT_ERR_INVALID_ARGUMENT, "nx, ny must be positive");
+    if (!pow2(nx) || !pow2(ny)) fail(MT_ERR_UNSUPPORTED, "nx, ny must be powers of two in [64, 2048]");
+    nxh = nx / 2; nyh = ny / 2;
+    lx = (float)in.spec_lx; ly = (float)in.spec_ly;
+    lx_d = in.spec_lx; ly_d = in.spec_ly;
+    if (!(lx > 0 && ly > 0)) fail(MT_ERR_INVALID_ARGUMENT, "spec_lx, spec_ly must be positive");
+    dRx = lx / nx; dRy = ly / ny;
+    dgx = 1.0f / lx; dgy = 1.0f / ly;
+    bwl = in.bwl != 0;
+    const float g_max = std::fmin((float)nxh * dgx, (float)nyh * dgy);
+    const float gl_max = 2.0f * g_max / 3.0f;
+    gl2_max = gl_max * gl_max;
+    bwl_alpha = (float)(std::log(1.0 / 1e-2 - 1.0) / ((gl_max + 0.25f) * (gl_max + 0.25f) - gl2_max));
+
+    iw_type = in.iw_type;
+    if (iw_type == 4) iw_type = (is_stem() || is_cbed()) ? 2 : 1;
+    if (iw_type == 3) fail(MT_ERR_UNSUPPORTED, "user-defined incident wave");
+    if (iw_type != 1 && iw_type != 2) fail(MT_ERR_INVALID_ARGUMENT, "iw_type must be 1..4");
+    for (int i = 0; i < in.n_iw_xy; ++i) { iw_x.push_back((float)in.iw_x[i]); iw_y.push_back((float)in.iw_y[i]); }
+    if (!is_stem() && iw_type == 2 && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y");
+
+    E_0 = (float)in.E_0;
+    if (!(E_0 > 0)) fail(MT_ERR_INVALID_ARGUMENT, "E_0 must be positive");
+    theta = 0.f; phi = (float)(in.phi * kDegToRad);
+    lambda = lambda_f(E_0);
+    {
+        double e = in.E_0, emass = 510.99906, hc = 12.3984244;
+        lambda_d = hc / std::sqrt(e * (2 * emass + e));
+    }
+    mul_sign = in.reverse_multislice ? -1 : 1;
+    {
+        float emass = 510.99906f;
+        float gamma = 1 + E_0 / emass;
+        float cpotf = (float)kPotf;
+        Vr_factor = mul_sign * (gamma * lambda / (cpotf * std::cos(theta)));
+    }
+
+    read_lens(in.cond_lens, cond);
+    set_lens(cond, E_0, g_max);
+    read_lens(in.obj_lens, obj);
+    set_lens(obj, E_0, g_max);
+
+    // scanning (Scanning::set_grid, src/types.cuh:4294-4389)
+    if (is_stem()) {
+        int ns = in.scanning_ns;
+        bool pbc = in.scanning_periodic != 0, spxs = in.scanning_square_pxs != 0;
+        if (ns <= 0) fail(MT_ERR_INVALID_ARGUMENT, "scanning_ns must be positive");
+        float x0 = (float)in.scanning_x0, y0 = (float)in.scanning_y0;
+        float xe = (float)in.scanning_xe, ye = (float)in.scanning_ye;
+        float xu = xe - x0, yu = ye - y0;
+        if (in.scanning_type == 1) {
+            float ds = std::sqrt(yu * yu + xu * xu), th = std::atan2(yu, xu);
+            float ct = std::cos(th);
+            ct = near_zero(ct) ? 0.f : ct;
+            float st = std::sin(th);
+            sc_dRx = ds * ct / (pbc ? ns : (ns - 1));
+            sc_dRy = ds * st / (pbc ? ns : (ns - 1));
+            sc_nx = ns; sc_ny = 1;
+            for (int i = 0; i < ns; ++i) { sc_x.push_back(x0 + i * sc_dRx); sc_y.push_back(y0 + i * sc_dRy); }
+        } else {
+            sc_nx = sc_ny = ns;
+            if (std::fabs(xu) > std::fabs(yu)) {
+                sc_dRx = xu / (pbc ? ns : (ns - 1));
+                sc_dRy = std::copysign(sc_dRx, yu);
+                sc_ny = (int)std::floor(yu / sc_dRy + 1e-4f + 0.5f) + (pbc ? 0 : 1);
+                if (!spxs) sc_dRy = yu / (pbc ? sc_ny : (sc_ny - 1));
+            } else {
+                sc_dRy = yu / (pbc ? ns : (ns - 1));
+                sc_dRx = std::copysign(sc_dRy, xu);
+                sc_nx = (int)std::floor(xu / sc_dRx + 1e-4f + 0.5f) + (pbc ? 0 : 1);
+                if (!spxs) sc_dRx = xu / (pbc ? sc_nx : (sc_nx - 1));
+            }
+            if (sc_nx <= 0 || sc_ny <= 0) fail(MT_ERR_INVALID_ARGUMENT, "empty scan grid");
+            for (int ix = 0; ix < sc_nx; ++ix)
+                for (int iy = 0; iy < sc_ny; ++iy) {
+                    sc_x.push_back(x0 + ix * sc_dRx);
+                    sc_y.push_back(y0 + iy * sc_dRy);
+                }
+        }
+        sc_x0 = x0; sc_y0 = y0;
+        if (in.detector_type != 1) fail(MT_ERR_UNSUPPORTED, "only circular detectors");
+        if (in.n_detector <= 0 || !in.detector_cir) fail(MT_ERR_INVALID_ARGUMENT, "STEM needs at least one detector");
+        for (int i = 0; i < in.n_detector; ++i) {
+            float gi = std::sin((float)(in.detector_cir[i].inner_ang * kMradToRad)) / lambda;
+            float go = std::sin((float)(in.detector_cir[i].outer_ang * kMradToRad)) / lambda;
+            det_g2_min.push_back((float)std::pow(gi, 2));
+            det_g2_max.push_back((float)std::pow(go, 2));
+        }
+    }
+
+    // atoms (Atom_Data::set_atoms, src/atomic_data_mt.hpp:320-350)
+    const int na = in.spec_atoms_rows, nc = in.spec_atoms_cols;
+    if (na <= 0 || !in.spec_atoms || nc < 4) fail(MT_ERR_INVALID_ARGUMENT, "spec_atoms must be n x (4..8), n > 0");
+    Specimen in_atoms;
+    for (int i = 0; i < na; ++i) {
+        const double *A = in.spec_atoms;
+        int Zi = (int)A[i];
+        int charge = (int)(nc > 7 ? A[7 * na + i] : 0);
+        if (Zi % 1000 < 1 || Zi % 1000 > 103) fail(MT_ERR_INVALID_ARGUMENT, "atomic number out of range 1..103");
+        if (charge != 0) fail(MT_ERR_UNSUPPORTED, "ionic charges");
+        in_atoms.Z.push_back(Zi);
+        in_atoms.x.push_back((float)A[na + i]);
+        in_atoms.y.push_back((float)A[2 * na + i]);
+        in_atoms.z.push_back((float)A[3 * na + i]);
+        in_atoms.sigma.push_back((float)(nc > 4 ? A[4 * na + i] : 0.085));
+        in_atoms.occ.push_back((float)(nc > 5 ? A[5 * na + i] : 1.0));
+        in_atoms.region.push_back(std::abs((int)(nc > 6 ? A[6 * na + i] : 0)));
+    }
+    const float spec_dz = (float)in.spec_dz;
+
+    // validate_parameters (src/input_multislice.cuh:622-796)
+    pn_seed = std::max(1, pn_seed);
+    pn_nconf = frozen() ? std::max(1, pn_nconf) : 1;
+    fp_iconf_0 = frozen() ? (pn_single_conf ? pn_nconf : 1) : 1;
+    if (is_ewfs() || is_ewrs()) pn_coh_contrib = 1;
+    pn_z = false;
+
+    auto zminmax = [](const Specimen &s, float &lo, float &hi) {
+        auto mm = std::minmax_element(s.z.begin(), s.z.end());
+        lo = *mm.first; hi = *mm.second;
+    };
+    float zmin, zmax;
+    zminmax(in_atoms, zmin, zmax);
+
+    // thicknesses matched to planes (Slicing::match_thickness, src/slicing.hpp:53-71)
+    const int thick_type = in.thick_type;
+    if (thick_type == 1) {
+        thick = {zmax};
+    } else {
+        if (thick_type != 2 && thick_type != 3) fail(MT_ERR_INVALID_ARGUMENT, "thick_type must be 1..3");
+        std::vector<float> zc;
+        for (int i = 0; i < na; ++i) if (in_atoms.region[i] == 0) zc.push_back(in_atoms.z[i]);
+        std::sort(zc.begin(), zc.end());
+        std::vector<float> zp = (in.potential_slicing == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
+        unique_sorted(zp);
+        if (thick_type == 3) zp = slice_bounds(zp, spec_dz, zmin - 2.5f, zmax + 2.5f);
+        std::vector<float> req;
+        for (int i = 0; i < in.n_thick; ++i) req.push_back((float)in.thick[i]);
+        // match_vectors (src/cpu_fcns.hpp:3613-3654): each request snaps to the
+        // nearest plane (ties to the lower), each plane used once
+        float dv = std::fmax(0.25f * spec_dz, 0.1f);
+        float lo = *std::min_element(zp.begin(), zp.end()) - dv, hi = *std::max_element(zp.begin(), zp.end()) + dv;
+        std::vector<bool> used(zp.size(), false);
+        for (float r : req) {
+            if (r < lo || r > hi) continue;
+            size_t best = 0;
+            for (size_t k = 1; k < zp.size(); ++k)
+                if (std::fabs(zp[k] - r - 1e-4f) < std::fabs(zp[best] - r)) best = k;
+            if (!used[best]) { used[best] = true; thick.push_back(zp[best]); }
+        }
+        if (thick.empty()) fail(MT_ERR_INVALID_ARGUMENT, "no requested thickness lies inside the specimen");
+    }
+    // drop atoms beyond the last thickness
+    {
+        float z_e = thick.back() + 0.1f;
+        if (zmax > z_e) {
+            float z_0 = zmin - 0.1f;
+            Specimen t;
+            for (int i = 0; i < (int)in_atoms.Z.size(); ++i) {
+                float z = in_atoms.z[i];
+                if (z_0 < z && z < z_e) {
+                    t.Z.push_back(in_atoms.Z[i]); t.x.push_back(in_atoms.x[i]); t.y.push_back(in_atoms.y[i]);
+                    t.z.push_back(z); t.sigma.push_back(in_atoms.sigma[i]); t.occ.push_back(in_atoms.occ[i]);
+                    t.region.push_back(in_atoms.region[i]);
+                }
+            }
+            in_atoms = std::move(t);
+            zminmax(in_atoms, zmin, zmax);
+        }
+    }
+    cond.zero_defocus_plane = zero_defocus_plane(cond, zmin, zmax);
+    cond.zero_defocus_type = 4;
+
+    if (is_stem()) { oa_ix_0 = 0; oa_ix_e = nx; oa_iy_0 = 0; oa_iy_e = ny; }
+    else {
+        oa_ix_0 = in.output_area_ix_0 - 1; oa_ix_e = in.output_area_ix_e - 1;
+        oa_iy_0 = in.output_area_iy_0 - 1; oa_iy_e = in.output_area_iy_e - 1;
+        if (oa_ix_0 == oa_ix_e) { oa_ix_0 = 0; oa_ix_e = nx; }
+        if (oa_iy_0 == oa_iy_e) { oa_iy_0 = 0; oa_iy_e = ny; }
+        if (oa_ix_0 > oa_ix_e) std::swap(oa_ix_0, oa_ix_e);
+        if (oa_iy_0 > oa_iy_e) std::swap(oa_iy_0, oa_iy_e);
+        oa_ix_0 = std::clamp(oa_ix_0, 0, nx); oa_ix_e = std::clamp(oa_ix_e, 0, nx);
+        oa_iy_0 = std::clamp(oa_iy_0, 0, ny); oa_iy_e = std::clamp(oa_iy_e, 0, ny);
+    }
+
+    // Spec::set_input_data (src/spec.hpp:44-82): periodic de-duplication,
+    // stable z sort, tables of the elements present
+    const float lx_b = lx - 1e-4f, ly_b = ly - 1e-4f;
+    std::vector<int> keep;
+    for (int i = 0; i < (int)in_atoms.Z.size(); ++i)
+        if (in_atoms.x[i] < lx_b && in_atoms.y[i] < ly_b) keep.push_back(i);
+    if (keep.empty()) fail(MT_ERR_INVALID_ARGUMENT, "no atoms inside the simulation box");
+    std::stable_sort(keep.begin(), keep.end(), [&](int a, int b) { return in_atoms.z[a] < in_atoms.z[b]; });
+    for (int i : keep) {
+        atoms_u.Z.push_back(in_atoms.Z[i] % 1000); atoms_u.x.push_back(in_atoms.x[i]); atoms_u.y.push_back(in_atoms.y[i]);
+        atoms_u.z.push_back(in_atoms.z[i]); atoms_u.sigma.push_back(in_atoms.sigma[i]); atoms_u.occ.push_back(in_atoms.occ[i]);
+        atoms_u.region.push_back(in_atoms.region[i]);
+    }
+    z_types = atoms_u.Z;
+    std::sort(z_types.begin(), z_types.end());
+    z_types.erase(std::unique(z_types.begin(), z_types.end()), z_types.end());
+    const double dR_min = std::fmin(dRx, dRy);
+    for (int Zt : z_types) tables.push_back(build_atom_table(Zt, dR_min));
+    for (int Zi : atoms_u.Z)
+        type_of_atom.push_back((int)(std::lower_bound(z_types.begin(), z_types.end(), Zi) - z_types.begin()));
+    R_int_max = 0;
+    for (auto &t : tables) R_int_max = std::max(R_int_max, t.R_max_f);
+
+    // planes of the undisplaced crystal
+    {
+        std::vector<float> zc;
+        for (size_t i = 0; i < atoms_u.z.size(); ++i) if (atoms_u.region[i] == 0) zc.push_back(atoms_u.z[i]);
+        z_plane = (in.potential_slicing == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
+        unique_sorted(z_plane);
+    }
+    float uzmin, uzmax;
+    zminmax(atoms_u, uzmin, uzmax);
+    const float s_z_int = (uzmax + R_int_max) - (uzmin - R_int_max);
+    if (s_z_int < 2.0f * spec_dz || (z_plane.size() == 1 && in.potential_slicing == 1))
+        fail(MT_ERR_UNSUPPORTED, "specimen reduces to a single phase object");
+
+    // slices and thicknesses (Slicing::calculate, src/slicing.hpp:93-100 / 253-403)
+    std::vector<float> zs = slice_bounds(z_plane, spec_dz, uzmin - R_int_max, uzmax + R_int_max);
+    const std::vector<float> &az = atoms_u.z;
+    auto lower = [&](float v) { return (int)(std::lower_bound(az.begin(), az.end(), v) - az.begin()); };
+    for (size_t is = 0; is + 1 < zs.size(); ++is) {
+        SliceInfo s{zs[is], zs[is + 1], 1, 0, -1};
+        float z0 = s.z_0, ze = s.z_e;
+        if (!(z0 > ze || az.back() < z0 || ze < az.front())) {
+            int i0 = (z0 <= az.front()) ? 0 : lower(z0);
+            int ie = (az.back() <= ze) ? (int)az.size() - 1 : lower(ze) - 1;
+            if (!(i0 > ie || az[ie] < z0 || ze < az[i0])) { s.iatom_0 = i0; s.iatom_e = ie; }
+        }
+        slices.push_back(s);
+    }
+    for (float t : thick) {
+        ThickInfo ti{t, 0, 0.f};
+        if (thick_type == 3) {
+            for (size_t i = 0; i + 1 < zs.size(); ++i) if (std::fabs(zs[i + 1] - t) < 1e-4f) { ti.islice = (int)i; break; }
+        } else {
+            for (size_t i = 0; i + 1 < zs.size(); ++i) if (zs[i] < t && t <= zs[i + 1]) { ti.islice = (int)i; break; }
+        }
+        float ze = zs[ti.islice + 1];
+        int iatom_e = (az.back() <= ze) ? (int)az.size() - 1 : lower(ze) - 1;
+        if (iatom_e < 0) iatom_e = 0;
+        float zdp = zero_defocus_plane(obj, az[0], az[iatom_e]);
+        if (thick_type == 3) ti.z_back_prop = 0.f;
+        else {
+            ti.z_back_prop = zdp - 0.5f * (zs[ti.islice] + zs[ti.islice + 1]);
+            if (std::fabs(ti.z_back_prop) < 1e-4f) ti.z_back_prop = 0.f;
+        }
+        thk.push_back(ti);
+    }
+    for (size_t k = 0; k < thk.size(); ++k) slices[thk[k].islice].ithk = (int)k;
+}
+
+float Model::phonon_weight() const
+{
+    int nconf = (!frozen() || pn_single_conf) ? 1 : pn_nconf;
+    return 1.0f / float(nconf);
+}
+
+// Randn_3d<T, e_host> (src/cgpu_rand.cuh:303-402): a 64-bit Mersenne Twister
+// seeded with pn_seed draws three seeds per configuration index; each axis
+// then has its own generator and a float normal distribution.
+void Model::displaced(int iconf, std::vector<float> &x, std::vector<float> &y) const
+{
+    x = atoms_u.x;
+    y = atoms_u.y;
+    if (!frozen()) return;
+    std::mt19937_64 gen_u((uint64_t)(int64_t)pn_seed);
+    std::uniform_int_distribution<int> rand_u(1, INT_MAX);
+    unsigned int sx = 0, sy = 0, sz = 0;
+    for (int ic = 0; ic < std::max(1, iconf); ++ic) {
+        sx = (unsigned int)rand_u(gen_u);
+        sy = (unsigned int)rand_u(gen_u);
+        sz = (unsigned int)rand_u(gen_u);
+    }
+    (void)sz;
+    std::mt19937_64 gx(sx), gy(sy);
+    std::normal_distribution<float> nx_(0.f, 1.f), ny_(0.f, 1.f);
+    for (size_t i = 0; i < x.size(); ++i) {
+        float s = atoms_u.sigma[i];
+        float dx = pn_x ? s * nx_(gx) : 0.f;
+        float dy = pn_y ? s * ny_(gy) : 0.f;
+        x[i] = atoms_u.x[i] + dx;
+        y[i] = atoms_u.y[i] + dy;
+    }
+}
+
+void Model::output_dims(mt_output_multislice *out) const
+{
+    out->n_thk = (int)thk.size();
+    if (is_stem()) {
+        out->output_type = pn_coh_contrib ? MT_OT_IMAGE_TOT_COH : MT_OT_IMAGE_TOT;
+        out->nx = sc_nx; out->ny = sc_ny; out->n_det = (int)det_g2_min.size();
+        out->dx = sc_dRx; out->dy = sc_dRy;
+        return;
+    }
+    out->nx = oa_ix_e - oa_ix_0; out->ny = oa_iy_e - oa_iy_0; out->n_det = 0;
+    out->dx = is_ewrs() ? dRx : dgx;
+    out->dy = is_ewrs() ? dRy : dgy;
+    if (is_cbed()) out->output_type = pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
+    else out->output_type = (!frozen() || pn_single_conf) ? MT_OT_PSI_COH : MT_OT_M2PSI_TOT_PSI_COH;
+}
+
+void Model::fill_axes(mt_output_multislice *out) const
+{
+    if (out->thick) for (size_t i = 0; i < thk.size(); ++i) out->thick[i] = thk[i].z;
+    if (is_stem()) {
+        if (out->x) for (int i = 0; i < sc_nx; ++i) out->x[i] = sc_x0 + i * sc_dRx;
+        if (out->y) for (int i = 0; i < sc_ny; ++i) out->y[i] = sc_y0 + i * sc_dRy;
+        return;
+    }
+    bool rs = is_ewrs();
+    if (out->x) for (int ix = oa_ix_0; ix < oa_ix_e; ++ix) out->x[ix - oa_ix_0] = rs ? ix * dRx : (ix - nxh) * dgx;
+    if (out->y) for (int iy = oa_iy_0; iy < oa_iy_e; ++iy) out->y[iy - oa_iy_0] = rs ? iy * dRy : (iy - nyh) * dgy;
+}
+
+} // namespace mtg

@@ -1,0 +1,553 @@
+// HIP kernels of the multislice engine (gfx950).
+//
+// Layout of a wave in HBM: column-major, element (ix, iy) at ix*ny + iy, real
+// space fft-shifted (box centre at index 0), reciprocal space in natural FFT
+// order -- the conventions of the reference (src/types.cuh:2528-2618).
+//
+// One slice step is two passes over the wave (see row/col pass below):
+//   row pass  (along y, contiguous):  IFFT_y -> x T_s(r) -> FFT_y
+//   col pass  (along x, strided):     FFT_x -> x P_s(g) [-> detector sums] -> IFFT_x
+// so each probe.slice reads and writes the wave twice and reads T once.
+#include "kernels.hpp"
+#include "fft_core.hpp"
+
+namespace mtg {
+
+namespace {
+
+constexpr double kTwoPiD = 6.283185307179586476925;
+
+__device__ __forceinline__ int shift_idx(int i, int nh) { return i < nh ? i + nh : i - nh; }
+__device__ __forceinline__ int signed_freq(int i, int n) { return i < (n >> 1) ? i : i - n; }
+
+// g^2 on the float grid exactly as the FP32 path rounds it (no contraction):
+// Grid_2d::g2_shift (src/types.cuh:2549-2552)
+__device__ __forceinline__ float g2_float(int igx, int igy, float dgx, float dgy)
+{
+    float gx = __fmul_rn((float)igx, dgx), gy = __fmul_rn((float)igy, dgy);
+    return __fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy));
+}
+
+// bandwidth-limit factor 1/(1+exp(alpha (g^2 - gl2_max))) (Grid_2d::bwl_factor_shift)
+__device__ __forceinline__ float bwl_factor(float g2, float alpha, float gl2_max)
+{
+    return 1.0f / (1.0f + expf(alpha * (g2 - gl2_max)));
+}
+
+// blockIdx -> (row group, member) such that one row group of every member
+// runs on the same XCD (blockIdx % 8): the transmission rows it multiplies
+// by stay in that XCD's L2 across the batch.
+__device__ __forceinline__ void map_rows(int wg, int G, int M, int &rg, int &member)
+{
+    if ((G & 7) == 0) {
+        const int xcd = wg & 7, k = wg >> 3;
+        rg = (k / M) * 8 + xcd;
+        member = k % M;
+    } else {
+        rg = wg % G;
+        member = wg / G;
+    }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Row pass: RW rows of length N per workgroup, T = N/E threads per row.
+template <int N, int RW>
+__global__ __launch_bounds__(RW *(N / Sched<N>::E)) void k_row_pass(RowArgs a)
+{
+    constexpr int E = Sched<N>::E;
+    constexpr int T = N / E;
+    constexpr int PADN = N + N / 8;
+    __shared__ float2 lds[RW * PADN];
+
+    const int tid = threadIdx.x;
+    const int t = tid % T, r = tid / T;
+    int rg, member;
+    map_rows(blockIdx.x, a.nrows / RW, a.n_members, rg, member);
+    const int row = rg * RW + r;
+    float2 *lp = lds + r * PADN;
+    auto idx = [](int i) { return i + (i >> 3); };
+    auto sync = [] { __syncthreads(); };
+
+    const long long off = (long long)member * a.member_stride + (long long)row * N;
+    float2 v[E];
+    if (a.flags & ROW_LOAD) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = a.src[off + t + T * e];
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = make_float2(1.f, 0.f);
+    }
+    if (a.flags & ROW_IFFT) fft_run<N, true>(v, t, a.tw, lp, idx, sync);
+    if (a.flags & ROW_MUL_T) {
+        const int slot = a.members[member].tslot;
+        const float2 *T_s = a.trans + (long long)slot * a.tslot_stride + (long long)a.slice * a.nrows * N
+                            + (long long)row * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cmul(T_s[t + T * e], v[e]);
+    }
+    if (a.flags & ROW_FFT) fft_run<N, false>(v, t, a.tw, lp, idx, sync);
+    if (a.flags & ROW_STORE) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) a.dst[off + t + T * e] = v[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass: C adjacent columns of length N per workgroup.  Thread (c, t)
+// holds elements ix = t + T*e of column tile*C + c.
+template <int N, int C, bool PROBE, bool DET>
+__global__ __launch_bounds__(C *(N / Sched<N>::E)) void k_col_pass(ColArgs a)
+{
+    constexpr int E = Sched<N>::E;
+    constexpr int T = N / E;
+    constexpr int NWAVE = (C * T + 63) / 64;
+    __shared__ float2 lds[N * C];
+    __shared__ double red[DET ? NWAVE * kMaxDet : 1];
+
+    const int tid = threadIdx.x;
+    const int c = tid % C, t = tid / C;
+    const int ntile = a.ncols / C;
+    const int tile = blockIdx.x % ntile, member = blockIdx.x / ntile;
+    const int col = tile * C + c;
+    auto idx = [c](int i) { return i * C + c; };
+    auto sync = [] { __syncthreads(); };
+    const long long base = (long long)member * a.member_stride + col;
+    const int igy = signed_freq(col, a.ny);
+
+    float2 v[E];
+    if constexpr (PROBE) {
+        // Convergent probe in Fourier space (exp_i_chi, src/cgpu_fcns.cuh:1949-1979):
+        // float aperture mask, FP64 phase reduced mod 2 pi, FP32 sincos.
+        const Member m = a.members[member];
+        const ProbeParams &P = a.probe;
+        const double gy = igy * P.dgy;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ix = t + T * e;
+            const int igx = signed_freq(ix, a.nx);
+            const float g2f = g2_float(igx, igy, P.dgx_f, P.dgy_f);
+            float2 acc = make_float2(0.f, 0.f);
+            if (P.g2_min <= g2f && g2f < P.g2_max) {
+                const double gx = igx * P.dgx;
+                const double g2 = gx * gx + gy * gy;
+                const double aber = (P.c10 + (P.c30 + P.c50 * g2) * g2) * g2;
+                for (int b = 0; b < m.nbeam; ++b) {
+                    const Beam bm = a.beams[m.beam0 + b];
+                    double chi = bm.xph * gx + bm.yph * gy + aber;
+                    chi -= kTwoPiD * rint(chi * (1.0 / kTwoPiD));
+                    float s, co;
+                    sincosf((float)chi, &s, &co);
+                    acc.x += co * P.norm;
+                    acc.y += s * P.norm;
+                }
+            }
+            v[e] = acc;
+        }
+    } else {
+        if (a.flags & COL_LOAD) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = a.src[base + (long long)(t + T * e) * a.ncols];
+        }
+    }
+    if (a.flags & COL_FFT) fft_run<N, false>(v, t, a.tw, lds, idx, sync);
+    if (a.flags & (COL_MUL_P | COL_BWL)) {
+        const float2 pyv = (a.flags & COL_MUL_P) ? a.py[col] : make_float2(a.inv_nxy, 0.f);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ix = t + T * e;
+            float2 p = (a.flags & COL_MUL_P) ? cscale(cmul(a.px[ix], pyv), a.p_scale) : pyv;
+            if (a.flags & COL_BWL) {
+                const float f = bwl_factor(g2_float(signed_freq(ix, a.nx), igy, a.dgx_f, a.dgy_f),
+                                           a.bwl_alpha, a.gl2_max);
+                p = cscale(p, f);
+            }
+            v[e] = cmul(p, v[e]);
+        }
+    }
+    if constexpr (DET) {
+        // Annular detector sums (sum_square_over_Det, src/cgpu_fcns.cuh:1794-1805):
+        // FP64 per-thread sums, wave and workgroup reduction in fixed order.
+        double acc[kMaxDet];
+#pragma unroll
+        for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ix = t + T * e;
+            const float g2f = g2_float(signed_freq(ix, a.nx), igy, a.dgx_f, a.dgy_f);
+            float m2 = v[e].x * v[e].x + v[e].y * v[e].y;
+            if (a.flags & COL_BWL) {
+                const float f = bwl_factor(g2f, a.bwl_alpha, a.gl2_max);
+                m2 *= f * f;
+            }
+            for (int d = 0; d < a.n_det; ++d)
+                if (a.det_g2_min[d] <= g2f && g2f < a.det_g2_max[d]) acc[d] += (double)m2;
+        }
+        const int lane = tid & 63, wave = tid >> 6;
+        for (int d = 0; d < a.n_det; ++d) {
+            double s = acc[d];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) red[wave * kMaxDet + d] = s;
+        }
+        __syncthreads();
+        if (tid < a.n_det) {
+            double s = 0.0;
+            for (int w = 0; w < NWAVE; ++w) s += red[w * kMaxDet + tid];
+            a.det_part[(((long long)member * a.n_thk + a.ithk) * a.n_det + tid) * ntile + tile] = s;
+        }
+    }
+    if (a.flags & COL_IFFT) fft_run<N, true>(v, t, a.tw, lds, idx, sync);
+    if (a.flags & COL_STORE) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * a.ncols] = v[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Projected potential -> transmission function, gathered per pixel tile.
+// V(r) = sum over atoms of the slice of occ * spline(|r - r_a|^2)
+// (Projected_Potential, src/projected_potential.cuh:89-182; eval_cubic_poly,
+// src/cgpu_fcns.cuh:1496-1566), restricted to each atom's window
+// [floor((x-R_max)/dR), ceil((x+R_max)/dR)] (get_bn, src/cpu_fcns.hpp:1669-1684)
+// with periodic images; T = exp(i w V) written fft-shifted.
+constexpr int kTfTy = 64, kTfTx = 8, kTfThreads = 256;
+
+__global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
+{
+    __shared__ float tab[kMaxTypes * 5 * 128];
+    __shared__ float2 rmx[kMaxTypes];
+    __shared__ int list[kTfThreads];
+    __shared__ int wave_cnt[kTfThreads / 64];
+
+    const int tid = threadIdx.x;
+    const int ntx = a.nx / kTfTx;
+    const int tile = blockIdx.x;
+    const int px0 = (tile % ntx) * kTfTx;
+    const int py0 = (tile / ntx) * kTfTy;
+    const int is = a.slice0 + blockIdx.y;
+
+    for (int i = tid; i < a.n_types * 5 * 128; i += kTfThreads) tab[i] = a.tables[i];
+    if (tid < a.n_types) rmx[tid] = a.rmax[tid];
+
+    const int ty = tid % kTfTy, tx = tid / kTfTy;   // pixels (px0+tx, py) and (px0+tx+4, py)
+    const int py = py0 + ty;
+    const bool py_ok = py < a.ny;
+    float vacc[2] = {0.f, 0.f};
+
+    const int2 rng = a.slice_range[is];
+    __syncthreads();
+    for (int c0 = rng.x; c0 <= rng.y; c0 += kTfThreads) {
+        // cull this chunk of atoms against the tile (all periodic images)
+        const int ia = c0 + tid;
+        bool hit = false;
+        if (ia <= rng.y) {
+            const float4 at = a.atoms[ia];
+            const float R_max = rmx[__float_as_int(at.w)].x;
+            int ix0 = (int)floorf((at.x - R_max) / a.dRx), ixe = (int)ceilf((at.x + R_max) / a.dRx);
+            int iy0 = (int)floorf((at.y - R_max) / a.dRy), iye = (int)ceilf((at.y + R_max) / a.dRy);
+            if (ix0 != ixe && iy0 != iye) {
+                // smallest image of the tile's last column/row not left of the window start
+                auto fdiv = [](int p, int q) { return p >= 0 ? p / q : -((-p + q - 1) / q); };
+                int kx = -fdiv(-(ix0 - (px0 + kTfTx - 1)), a.nx);   // ceil((ix0 - px_last)/nx)
+                int ky = -fdiv(-(iy0 - (py0 + kTfTy - 1)), a.ny);
+                hit = (px0 + kx * a.nx <= ixe) && (py0 + ky * a.ny <= iye);
+            }
+        }
+        const unsigned long long mask = __ballot(hit);
+        const int lane = tid & 63, wave = tid >> 6;
+        if (lane == 0) wave_cnt[wave] = __popcll(mask);
+        __syncthreads();
+        int base = 0, total = 0;
+        for (int w = 0; w < kTfThreads / 64; ++w) {
+            if (w < wave) base += wave_cnt[w];
+            total += wave_cnt[w];
+        }
+        if (hit) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = ia;
+        __syncthreads();
+        for (int li = 0; li < total; ++li) {
+            const float4 at = a.atoms[list[li]];
+            const int ty_ = __float_as_int(at.w);
+            const float *tb = tab + ty_ * 5 * 128;
+            const float R_max = rmx[ty_].x, R2_max = rmx[ty_].y;
+            const int ix0 = (int)floorf((at.x - R_max) / a.dRx), ixe = (int)ceilf((at.x + R_max) / a.dRx);
+            const int iy0 = (int)floorf((at.y - R_max) / a.dRy), iye = (int)ceilf((at.y + R_max) / a.dRy);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int px = px0 + tx + 4 * h;
+                // images ix = px + k*nx inside [ix0, ixe]
+                int kx = (ix0 - px >= 0) ? (ix0 - px + a.nx - 1) / a.nx : -((px - ix0) / a.nx);
+                for (int ix = px + kx * a.nx; ix <= ixe; ix += a.nx) {
+                    const float rx = __fsub_rn(__fmul_rn((float)ix, a.dRx), at.x);
+                    int ky = (iy0 - py >= 0) ? (iy0 - py + a.ny - 1) / a.ny : -((py - iy0) / a.ny);
+                    for (int iy = py + ky * a.ny; iy <= iye; iy += a.ny) {
+                        const float ry = __fsub_rn(__fmul_rn((float)iy, a.dRy), at.y);
+                        const float R2 = __fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry));
+                        if (R2 < R2_max) {
+                            // 7-step binary search over the 128 knots, then the cubic
+                            int i0 = 0, ie = 127;
+#pragma unroll
+                            for (int k = 0; k < 7; ++k) {
+                                const int im = (i0 + ie) >> 1;
+                                if (R2 < tb[im]) ie = im; else i0 = im;
+                            }
+                            const float dx = __fsub_rn(R2, tb[i0]);
+                            float p = __fadd_rn(__fmul_rn(tb[4 * 128 + i0], dx), tb[3 * 128 + i0]);
+                            p = __fadd_rn(__fmul_rn(p, dx), tb[2 * 128 + i0]);
+                            p = __fadd_rn(__fmul_rn(p, dx), tb[1 * 128 + i0]);
+                            vacc[h] = __fadd_rn(vacc[h], __fmul_rn(at.z, p));
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!py_ok) return;
+    const long long nxy = (long long)a.nx * a.ny;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int px = px0 + tx + 4 * h;
+        const long long sl = is - a.slice0;
+        if (a.V) a.V[sl * nxy + (long long)px * a.ny + py] = vacc[h];
+        if (a.trans) {
+            float s, c;
+            sincosf(__fmul_rn(a.w, vacc[h]), &s, &c);
+            a.trans[sl * nxy + (long long)shift_idx(px, a.nxh) * a.ny + shift_idx(py, a.nyh)] = make_float2(c, s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Detector partials -> image (FP64), in fixed order.
+__global__ void k_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
+                             const Member *members, double w, double *image, int nscan)
+{
+    const int m = blockIdx.x;
+    const int iscan = members[m].iscan;
+    if (iscan < 0) return;
+    for (int k = threadIdx.x; k < n_thk * n_det; k += blockDim.x) {
+        const double *p = part + ((long long)m * n_thk * n_det + k) * ntile;
+        double s = 0.0;
+        for (int i = 0; i < ntile; ++i) s += p[i];
+        image[(long long)k * nscan + iscan] += w * s;
+    }
+}
+
+// Fresnel propagator factors, separable in gx and gy, with the FP64 phase
+// reduced mod 2 pi before the FP32 sincos:
+// polar(1/(nx ny), w g^2), w = -mul_sign pi lambda z  (propagate, src/cgpu_fcns.cuh:1835-1845).
+__global__ void k_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
+                              double lx, double ly, float2 *px, float2 *py)
+{
+    const int n = nx + ny;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * nz) return;
+    const int zi = (int)(i / n), j = (int)(i % n);
+    const double w = -mul_sign * 3.141592653589793238463 * lambda * z[zi];
+    double g;
+    if (j < nx) g = signed_freq(j, nx) / lx;
+    else g = signed_freq(j - nx, ny) / ly;
+    double arg = w * g * g;
+    arg -= kTwoPiD * rint(arg * (1.0 / kTwoPiD));
+    float s, c;
+    sincosf((float)arg, &s, &c);
+    if (j < nx) {
+        const float inv = 1.0f / (float)(nx * ny);
+        px[(long long)zi * nx + j] = make_float2(c * inv, s * inv);
+    } else {
+        py[(long long)zi * ny + (j - nx)] = make_float2(c, s);
+    }
+}
+
+__global__ void k_count_aperture(int nx, int ny, float dgx, float dgy, float g2_min, float g2_max, unsigned int *count)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    if (i < (long long)nx * ny) {
+        const int ix = (int)(i / ny), iy = (int)(i % ny);
+        const float g2 = g2_float(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy);
+        in = (g2_min <= g2 && g2 < g2_max);
+    }
+    const unsigned long long m = __ballot(in);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (unsigned int)__popcll(m));
+}
+
+// Output accumulation with fftshift + crop (add_scale_*_crop_shift_2d,
+// src/cgpu_fcns.cuh:1664-1778); members summed in order.
+__global__ void k_out_accum(OutArgs a)
+{
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long long)a.nx_o * a.ny_o) return;
+    const int ixo = (int)(o / a.ny_o), iyo = (int)(o % a.ny_o);
+    const int sx = shift_idx(ixo + a.ix_0, a.nx >> 1), sy = shift_idx(iyo + a.iy_0, a.ny >> 1);
+    const long long si = (long long)sx * a.ny + sy;
+    for (int m = 0; m < a.n_members; ++m) {
+        const float2 v = a.src[(long long)m * a.member_stride + si];
+        if (a.mode & OUT_SET_PSI) {
+            a.psi[2 * o] = v.x;
+            a.psi[2 * o + 1] = v.y;
+        }
+        if (a.mode & OUT_ADD_M2) a.m2[o] += a.w * (double)(v.x * v.x + v.y * v.y);
+        if (a.mode & OUT_ADD_PSI) {
+            a.psi[2 * o] += a.w * (double)v.x;
+            a.psi[2 * o + 1] += a.w * (double)v.y;
+        }
+    }
+    if (a.mode & OUT_ADD_PSI_FULL) {
+        // natural index o maps to the same pixel of the uncropped wave
+        for (long long k = o; k < (long long)a.nx * a.ny; k += (long long)a.nx_o * a.ny_o) {
+            for (int m = 0; m < a.n_members; ++m) {
+                const float2 v = a.src[(long long)m * a.member_stride + k];
+                a.psi_full[2 * k] += a.w * (double)v.x;
+                a.psi_full[2 * k + 1] += a.w * (double)v.y;
+            }
+        }
+    }
+}
+
+__global__ void k_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o, double *m2)
+{
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long long)nx_o * ny_o) return;
+    const int ixo = (int)(o / ny_o), iyo = (int)(o % ny_o);
+    const long long si = (long long)shift_idx(ixo + ix_0, nx >> 1) * ny + shift_idx(iyo + iy_0, ny >> 1);
+    const double re = psi_full[2 * si], im = psi_full[2 * si + 1];
+    m2[o] = re * re + im * im;
+}
+
+__global__ void k_shift_complex(const float2 *src, float2 *dst, int nx, int ny)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nx * ny) return;
+    const int ix = (int)(i / ny), iy = (int)(i % ny);
+    dst[i] = src[(long long)shift_idx(ix, nx >> 1) * ny + shift_idx(iy, ny >> 1)];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+int col_tile_width(int nx) { return nx >= 2048 ? 8 : 16; }
+
+int row_group(int ny)
+{
+    switch (ny) {
+    case 64: return 32;
+    case 128: return 16;
+    case 256: return 8;
+    case 512: return 4;
+    case 1024: return 4;
+    default: return 2;
+    }
+}
+
+template <int N>
+static hipError_t row_launch(const RowArgs &a, hipStream_t s)
+{
+    constexpr int RW = (N == 64) ? 32 : (N == 128) ? 16 : (N == 256) ? 8 : (N <= 1024) ? 4 : 2;
+    constexpr int threads = RW * (N / Sched<N>::E);
+    if (a.nrows % RW) return hipErrorInvalidValue;
+    const int grid = (a.nrows / RW) * a.n_members;
+    hipLaunchKernelGGL((k_row_pass<N, RW>), dim3(grid), dim3(threads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_pass(int n, const RowArgs &a, hipStream_t s)
+{
+    switch (n) {
+    case 64: return row_launch<64>(a, s);
+    case 128: return row_launch<128>(a, s);
+    case 256: return row_launch<256>(a, s);
+    case 512: return row_launch<512>(a, s);
+    case 1024: return row_launch<1024>(a, s);
+    case 2048: return row_launch<2048>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int N>
+static hipError_t col_launch(const ColArgs &a, bool probe, bool det, hipStream_t s)
+{
+    constexpr int C = (N >= 2048) ? 8 : 16;
+    constexpr int threads = C * (N / Sched<N>::E);
+    if (a.ncols % C) return hipErrorInvalidValue;
+    const int grid = (a.ncols / C) * a.n_members;
+    if (probe) hipLaunchKernelGGL((k_col_pass<N, C, true, false>), dim3(grid), dim3(threads), 0, s, a);
+    else if (det) hipLaunchKernelGGL((k_col_pass<N, C, false, true>), dim3(grid), dim3(threads), 0, s, a);
+    else hipLaunchKernelGGL((k_col_pass<N, C, false, false>), dim3(grid), dim3(threads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_col_pass(int n, const ColArgs &a, bool probe, bool det, hipStream_t s)
+{
+    if (probe && det) return hipErrorInvalidValue;
+    switch (n) {
+    case 64: return col_launch<64>(a, probe, det, s);
+    case 128: return col_launch<128>(a, probe, det, s);
+    case 256: return col_launch<256>(a, probe, det, s);
+    case 512: return col_launch<512>(a, probe, det, s);
+    case 1024: return col_launch<1024>(a, probe, det, s);
+    case 2048: return col_launch<2048>(a, probe, det, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_tf_build(const TfArgs &a, hipStream_t s)
+{
+    if (a.nx % kTfTx || a.n_types > kMaxTypes || a.n_slices <= 0) return hipErrorInvalidValue;
+    const int tiles = (a.nx / kTfTx) * ((a.ny + kTfTy - 1) / kTfTy);
+    hipLaunchKernelGGL(k_tf_build, dim3(tiles, a.n_slices), dim3(kTfThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
+                             const Member *members, double w, double *image, int nscan, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_det_reduce, dim3(n_members), dim3(64), 0, s, part, n_members, n_thk, n_det, ntile,
+                       members, w, image, nscan);
+    return hipGetLastError();
+}
+
+hipError_t launch_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
+                              double lx, double ly, float2 *px, float2 *py, hipStream_t s)
+{
+    const long long n = (long long)(nx + ny) * nz;
+    hipLaunchKernelGGL(k_prop_tables, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, z, nz, lambda,
+                       mul_sign, lx, ly, px, py);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float g2_min, float g2_max,
+                                 unsigned int *count, hipStream_t s)
+{
+    const long long n = (long long)nx * ny;
+    hipLaunchKernelGGL(k_count_aperture, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, dgx, dgy,
+                       g2_min, g2_max, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_out_accum(const OutArgs &a, hipStream_t s)
+{
+    const long long n = (long long)a.nx_o * a.ny_o;
+    hipLaunchKernelGGL(k_out_accum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o,
+                              double *m2, hipStream_t s)
+{
+    const long long n = (long long)nx_o * ny_o;
+    hipLaunchKernelGGL(k_m2_from_psi, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, psi_full, nx, ny, ix_0,
+                       iy_0, nx_o, ny_o, m2);
+    return hipGetLastError();
+}
+
+hipError_t launch_shift_complex(const float2 *src, float2 *dst, int nx, int ny, hipStream_t s)
+{
+    const long long n = (long long)nx * ny;
+    hipLaunchKernelGGL(k_shift_complex, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, nx, ny);
+    return hipGetLastError();
+}
+
+} // namespace mtg

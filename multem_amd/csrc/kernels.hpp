@@ -1,0 +1,143 @@
+// Host-visible interface of the HIP kernels: argument blocks and launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mtg {
+
+constexpr int kMaxDet = 8;
+constexpr int kMaxTypes = 16;
+
+// One wave in a batch: its probe position(s), its configuration's
+// transmission stack and its scan index.
+struct Member {
+    int tslot;      // which transmission stack (configuration slot)
+    int iscan;      // STEM scan index, -1 for padding members
+    int beam0;      // first beam in the beam table
+    int nbeam;      // number of beams (0: plane wave)
+};
+
+// Probe phase x*gx + y*gy with x = -2 pi (x_b - lx/2): per beam, FP64.
+struct Beam {
+    double xph, yph;
+};
+
+struct ProbeParams {
+    double c10, c30, c50;         // -pi*lambda*f, -pi*Cs3*lambda^3/2, -pi*Cs5*lambda^5/3 (FP64)
+    double dgx, dgy;              // FP64 reciprocal sampling (1/lx, 1/ly)
+    float dgx_f, dgy_f;           // float sampling, for the aperture mask as the FP32 path evaluates it
+    float g2_min, g2_max;         // aperture
+    float norm;                   // 1/sqrt(number of aperture pixels)
+};
+
+enum RowFlags : int {
+    ROW_LOAD = 1,       // load the wave (else start from psi = 1, a plane wave)
+    ROW_IFFT = 2,       // inverse transform along y first
+    ROW_MUL_T = 4,      // multiply by the transmission function of slice `slice`
+    ROW_FFT = 8,        // forward transform along y
+    ROW_STORE = 16,
+};
+
+struct RowArgs {
+    const float2 *src;
+    float2 *dst;
+    long long member_stride;      // complex elements between members
+    int nrows;                    // rows per member (= nx)
+    int n_members;
+    int flags;
+    const float2 *tw;             // twiddles W_ny
+    const float2 *trans;          // transmission stacks: [tslot][n_slice][nx*ny]
+    long long tslot_stride;       // complex elements between stacks
+    int slice;
+    const Member *members;
+};
+
+enum ColFlags : int {
+    COL_LOAD = 1,
+    COL_FFT = 2,
+    COL_MUL_P = 4,      // multiply by px[ix]*py[iy] (Fresnel propagator incl. 1/(nx ny))
+    COL_BWL = 8,        // multiply by the bandwidth-limit factor (and 1/(nx ny) when !COL_MUL_P)
+    COL_IFFT = 16,
+    COL_STORE = 32,
+};
+
+struct ColArgs {
+    const float2 *src;
+    float2 *dst;
+    long long member_stride;
+    int ncols;                    // = ny (row length, element stride of x)
+    int n_members;
+    int flags;
+    const float2 *tw;             // twiddles W_nx
+    const float2 *px;             // [nx]
+    const float2 *py;             // [ny]
+    float dgx_f, dgy_f;           // float sampling for masks
+    int nx, ny;
+    float bwl_alpha, gl2_max;
+    float inv_nxy;
+    float p_scale;                // extra factor on px*py (1 for a slice step)
+    // detector epilogue
+    int n_det;
+    float det_g2_min[kMaxDet], det_g2_max[kMaxDet];
+    double *det_part;             // [member][n_thk][n_det][ntile]
+    int n_thk, ithk;
+    // probe generation
+    const Member *members;
+    const Beam *beams;
+    ProbeParams probe;
+};
+
+struct TfArgs {
+    int nx, ny, nxh, nyh;
+    float dRx, dRy;
+    const float4 *atoms;          // x, y, occ, type (bit pattern of an int)
+    const int2 *slice_range;      // per slice: first, last atom (inclusive); empty if first > last
+    int slice0, n_slices;         // slices built by this launch
+    const float *tables;          // [type][5][128]: R2, c0, c1, c2, c3
+    const float2 *rmax;           // [type]: R_max, R2_max
+    int n_types;
+    float w;                      // sigma factor (Vr_factor)
+    float2 *trans;                // out, shifted layout: [slice][nx*ny] (relative to slice0), may be null
+    float *V;                     // out, natural layout: [slice][nx*ny], may be null
+};
+
+enum OutMode : int {
+    OUT_SET_PSI = 1,       // psi_coh = O                     (single configuration)
+    OUT_ADD_M2 = 2,        // m2psi_tot += w |O|^2
+    OUT_ADD_PSI = 4,       // psi_coh += w O (cropped, shifted)
+    OUT_ADD_PSI_FULL = 8,  // psi_full += w O (uncropped, unshifted; for m2psi_coh)
+};
+
+struct OutArgs {
+    const float2 *src;
+    long long member_stride;
+    int n_members;
+    int nx, ny;
+    int ix_0, iy_0, nx_o, ny_o;
+    double w;
+    int mode;
+    double *m2;            // [nx_o*ny_o]
+    double *psi;           // complex [nx_o*ny_o]
+    double *psi_full;      // complex [nx*ny]
+};
+
+// launchers (return hipError_t of the launch)
+hipError_t launch_row_pass(int n, const RowArgs &a, hipStream_t s);
+hipError_t launch_col_pass(int n, const ColArgs &a, bool probe, bool det, hipStream_t s);
+hipError_t launch_tf_build(const TfArgs &a, hipStream_t s);
+hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
+                             const Member *members, double w, double *image, int nscan, hipStream_t s);
+hipError_t launch_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
+                              double lx, double ly, float2 *px, float2 *py, hipStream_t s);
+hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float g2_min, float g2_max,
+                                 unsigned int *count, hipStream_t s);
+hipError_t launch_out_accum(const OutArgs &a, hipStream_t s);
+hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o,
+                              double *m2, hipStream_t s);
+hipError_t launch_shift_complex(const float2 *src, float2 *dst, int nx, int ny, hipStream_t s);
+
+int col_tile_width(int nx);   // columns per column-pass workgroup
+int row_group(int ny);        // rows per row-pass workgroup
+
+} // namespace mtg

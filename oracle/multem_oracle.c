@@ -1,0 +1,2062 @@
+/*
+ * multem_oracle.c -- CPU restatement of MULTEM's multislice path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP engine is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it.  The engine never links or calls it.
+ *
+ * It restates the reference's algorithm function by function (each function
+ * cites the reference file:line it follows) in plain C, built twice:
+ *   liboracle_f64.so  (real = double): the reference's double-precision path,
+ *                     used as the FP64 oracle;
+ *   liboracle_f32.so  (real = float, -DORC_F32): the reference's FP32 path,
+ *                     with the float-evaluated constants of Input_Multislice<float>.
+ * <tgmath.h> makes sqrt/sin/exp/... resolve on the argument type the way the
+ * C++ overloads do in the reference, so float expressions stay float.
+ *
+ * Deliberate differences from the reference, all tolerance-level:
+ *  - the projected potential is accumulated atom by atom in slice order
+ *    (the reference uses atomicAdd / mutex completion order);
+ *  - the FFT is this file's own radix-2 transform (FFTW / cuFFT in the
+ *    reference), unnormalised in both directions like those;
+ *  - detector sums are summed pixel by pixel in column-major order.
+ *
+ * Parity pins: see tests/test_oracle_*.py (Bessel vs scipy, FFT vs numpy,
+ * RNG vs libstdc++, table invariants, slicing counts, closed forms).
+ */
+#define _GNU_SOURCE
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include <float.h>
+#include <pthread.h>
+#include <tgmath.h>
+#include <limits.h>
+
+#include "../include/multem_hip.h"
+#include "../multem_amd/csrc/lobato_table.inc"
+
+#ifdef ORC_F32
+typedef float real;
+#define REAL_IS_FLOAT 1
+#else
+typedef double real;
+#define REAL_IS_FLOAT 0
+#endif
+
+typedef struct { real re, im; } cplx;
+
+static __thread char g_err[512];
+static void set_err(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
+const char *orc_last_error(void) { return g_err; }
+int orc_is_f32(void) { return REAL_IS_FLOAT; }
+
+/* ------------------------------------------------------------------ */
+/* constants (src/types.cuh:64-117)                                   */
+/* ------------------------------------------------------------------ */
+static const double c_Potf = 47.877645145863056;
+static const double c_Pi = 3.141592653589793238463;
+static const double c_2Pi = 6.283185307179586476925;
+static const double c_Pi2 = 9.869604401089358618834;
+static const double c_i2Pi = 1.570796326794896619231;
+static const double c_mrad_2_rad = 1.0e-03;
+static const double c_deg_2_rad = 0.01745329251994329576924;
+static const double c_mm_2_Angs = 1.0e+07;
+enum { c_nR = 128, c_nAtomsTypes = 103 };
+static const double c_Vrl = 0.015;
+
+/* isEqual / isZero (src/types.cuh:562-605): absolute then relative test,
+   thresholds depend on the value type. */
+static int is_equal_f(float a, float b)
+{
+    float diff = fabsf(a - b);
+    if (diff <= 1e-5f) return 1;
+    return diff <= ((fabsf(a) < fabsf(b) ? fabsf(b) : fabsf(a)) * 1e-4f);
+}
+static int is_equal_d(double a, double b)
+{
+    double diff = fabs(a - b);
+    if (diff <= 1e-13) return 1;
+    return diff <= ((fabs(a) < fabs(b) ? fabs(b) : fabs(a)) * 1e-8);
+}
+#if REAL_IS_FLOAT
+#define is_zero_r(x) is_equal_f((x), 0.0f)
+#define is_equal_r(a, b) is_equal_f((a), (b))
+static const real eps_rel_r = 1e-4f;   /* Epsilon<float>::rel */
+#else
+#define is_zero_r(x) is_equal_d((x), 0.0)
+#define is_equal_r(a, b) is_equal_d((a), (b))
+static const real eps_rel_r = 1e-8;    /* Epsilon<double>::rel */
+#endif
+#define is_zero_d(x) is_equal_d((x), 0.0)
+
+static inline real rsquare(real x) { return x * x; }
+
+/* ------------------------------------------------------------------ */
+/* electron constants (src/cgpu_fcns.cuh:50-144), evaluated in real    */
+/* ------------------------------------------------------------------ */
+static real get_lambda(real E_0)
+{
+    real emass = 510.99906;
+    real hc = 12.3984244;
+    real lambda = hc / sqrt(E_0 * (2 * emass + E_0));
+    return lambda;
+}
+static real get_gamma(real E_0)
+{
+    real emass = 510.99906;
+    return 1 + E_0 / emass;
+}
+static real get_Vr_factor(real E_0, real theta)
+{
+    real cpotf = 47.877645145863056;
+    real fPot = get_gamma(E_0) * get_lambda(E_0) / (cpotf * cos(theta));
+    return fPot;
+}
+double orc_lambda(double E_0) { return (double)get_lambda((real)E_0); }
+double orc_vr_factor(double E_0, double theta) { return (double)get_Vr_factor((real)E_0, (real)theta); }
+
+/* ------------------------------------------------------------------ */
+/* Modified Bessel functions: the polynomial approximations of          */
+/* Abramowitz & Stegun 9.8.1-9.8.8, as used by src/math.cuh:462-605.    */
+/* Always evaluated in double (the tables are built in double).        */
+/* ------------------------------------------------------------------ */
+static double bes_i0(double x)
+{
+    double ax = fabs(x), y;
+    if (ax < 3.75) {
+        y = x / 3.75; y = y * y;
+        return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492
+               + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+    }
+    y = 3.75 / ax;
+    return (exp(ax) / sqrt(ax)) * (0.39894228 + y * (0.1328592e-1
+           + y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2
+           + y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1
+           + y * 0.392377e-2))))))));
+}
+static double bes_i1(double x)
+{
+    double ax = fabs(x), y, ans;
+    if (ax < 3.75) {
+        y = x / 3.75; y = y * y;
+        ans = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934
+              + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+    } else {
+        y = 3.75 / ax;
+        ans = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+        ans = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2
+              + y * (0.163801e-2 + y * (-0.1031555e-1 + y * ans))));
+        ans *= (exp(ax) / sqrt(ax));
+    }
+    return (x < 0.0) ? -ans : ans;
+}
+static double bes_k0(double x)
+{
+    double y;
+    if (x <= 2.0) {
+        y = x * x / 4.0;
+        return (-log(x / 2.0) * bes_i0(x)) + (-0.57721566 + y * (0.42278420
+               + y * (0.23069756 + y * (0.3488590e-1 + y * (0.262698e-2
+               + y * (0.10750e-3 + y * 0.74e-5))))));
+    }
+    y = 2.0 / x;
+    return (exp(-x) / sqrt(x)) * (1.25331414 + y * (-0.7832358e-1
+           + y * (0.2189568e-1 + y * (-0.1062446e-1 + y * (0.587872e-2
+           + y * (-0.251540e-2 + y * 0.53208e-3))))));
+}
+static double bes_k1(double x)
+{
+    double y;
+    if (x <= 2.0) {
+        y = x * x / 4.0;
+        return (log(x / 2.0) * bes_i1(x)) + (1.0 / x) * (1.0 + y * (0.15443144
+               + y * (-0.67278579 + y * (-0.18156897 + y * (-0.1919402e-1
+               + y * (-0.110404e-2 + y * (-0.4686e-4)))))));
+    }
+    y = 2.0 / x;
+    return (exp(-x) / sqrt(x)) * (1.25331414 + y * (0.23498619
+           + y * (-0.3655620e-1 + y * (0.1504268e-1 + y * (-0.780353e-2
+           + y * (0.325614e-2 + y * (-0.68245e-3)))))));
+}
+double orc_bessel(int which, double x)
+{
+    switch (which) {
+    case 0: return bes_i0(x);
+    case 1: return bes_i1(x);
+    case 2: return bes_k0(x);
+    case 3: return bes_k1(x);
+    }
+    return 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Lobato coefficients for Vr (3D) and VR (projected)                   */
+/* (src/atomic_data.hpp:1440-1485, set_Lobato_neutral_coef)            */
+/* ------------------------------------------------------------------ */
+typedef struct { double cl[5], cnl[5]; } pp_coef;
+
+static void lobato_coef(int Z, pp_coef *Vr, pp_coef *VR)
+{
+    const double *t = mt_lobato_coef[Z - 1];
+    for (int j = 0; j < 5; j++) {
+        double cl = t[2 * j];
+        double cnl = is_zero_d(cl) ? 0.0 : t[2 * j + 1];
+        if (!(is_zero_d(cl) && is_zero_d(cnl))) {
+            Vr->cl[j] = c_Potf * c_Pi2 * cl / pow(cnl, 1.5);
+            Vr->cnl[j] = c_2Pi / sqrt(cnl);
+            VR->cl[j] = 2.0 * c_Potf * c_Pi2 * cl / pow(cnl, 1.5);
+            VR->cnl[j] = c_2Pi / sqrt(cnl);
+        } else {
+            Vr->cl[j] = 0.0; Vr->cnl[j] = 1.0;
+            VR->cl[j] = 0.0; VR->cnl[j] = 1.0;
+        }
+    }
+}
+
+/* Vr_Lobato_neutral_0_12 (src/cgpu_fcns.cuh:1111-1121) */
+static double lobato_Vr(double x, const pp_coef *c)
+{
+    double ix = 1 / x, y = 0;
+    for (int i = 0; i < 5; i++) y += c->cl[i] * exp(-c->cnl[i] * x) * ix * (2 / c->cnl[i] + x);
+    return y;
+}
+/* VR_dVR_Lobato_neutral_0_12 (src/cgpu_fcns.cuh:1256-1268) */
+static void lobato_VR_dVR(double x, const pp_coef *c, double *y, double *dy)
+{
+    *y = *dy = 0;
+    for (int i = 0; i < 5; i++) {
+        double k0 = bes_k0(c->cnl[i] * x);
+        double k1 = bes_k1(c->cnl[i] * x);
+        *y += c->cl[i] * (2 * k0 / c->cnl[i] + x * k1);
+        *dy += -c->cl[i] * (c->cnl[i] * x * k0 + 2 * k1);
+    }
+}
+int orc_lobato_VR(int Z, int n, const double *R, double *VR, double *dVR, double *Vr3)
+{
+    pp_coef Vr, VRc;
+    if (Z < 1 || Z > 103) { set_err("Z out of range"); return MT_ERR_INVALID_ARGUMENT; }
+    lobato_coef(Z, &Vr, &VRc);
+    for (int i = 0; i < n; i++) {
+        double y, dy;
+        lobato_VR_dVR(R[i], &VRc, &y, &dy);
+        if (VR) VR[i] = y;
+        if (dVR) dVR[i] = dy;
+        if (Vr3) Vr3[i] = lobato_Vr(R[i], &Vr);
+    }
+    return MT_OK;
+}
+
+/* Root_Finder: bisection (src/cgpu_fcns.cuh:262-286) on Vr(r) - Vrl */
+static double root_vr(const pp_coef *Vr, double Vrl, double x0, double xe)
+{
+    const double Tol = 1e-8;
+    const int itMax = 200;
+    int it = 0;
+    double x, fx, fxe = lobato_Vr(xe, Vr) - Vrl;
+    do {
+        x = 0.5 * (x0 + xe);
+        fx = lobato_Vr(x, Vr) - Vrl;
+        if (fx * fxe < 0) {
+            x0 = x;
+        } else {
+            xe = x;
+            fxe = fx;
+        }
+        it++;
+    } while ((fabs(fx) > Tol) && (it < itMax));
+    return x;
+}
+
+/* ------------------------------------------------------------------ */
+/* Spline table of the projected potential of one element              */
+/* Atomic_Data::To_atom_type_CPU + get_atom_coef                       */
+/* (src/atomic_data.hpp:148-175, 1546-1668)                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int Z;
+    double R_min_d, R_max_d;  /* as built (double) */
+    real R2_max, R2_tap, tap_cf, R_max;
+    real R2[c_nR], c0[c_nR], c1[c_nR], c2[c_nR], c3[c_nR];
+    real R[c_nR];
+    int valid;
+} atom_tab;
+
+static void build_atom_tab(int Z, double Vrl, int nR, double R_min_i, atom_tab *t)
+{
+    pp_coef Vr, VRc;
+    double dR[c_nR], VR[c_nR], dVR[c_nR], R[c_nR], R2[c_nR];
+    double c0[c_nR], c1[c_nR], c2[c_nR], c3[c_nR];
+    int iZ = Z - 1;
+    double rn_c = (1.2e-05) * pow((double)mt_element_table[iZ].A, 1.0 / 3.0);
+    double R_min = fmax(rn_c, R_min_i);
+    (void)dR;
+    memset(t, 0, sizeof *t);
+    t->Z = Z;
+    lobato_coef(Z, &Vr, &VRc);
+
+    /* R_max: AtomicRadius_Cutoff(3, Vrl) (src/atomic_fcns_mt.hpp:269-296),
+       bisection on [rn_c, 25] */
+    double R_max = 0.0;
+    if (!(is_zero_d(Vrl) || is_zero_d(Vr.cl[0]))) R_max = root_vr(&Vr, Vrl, rn_c, 25.0);
+    R_max = fmax(R_max, 2.0 * R_min);
+    if (is_zero_d(R_max)) R_max = 1.75 * mt_element_table[iZ].ra;
+
+    double R_tap = R_min + 0.85 * (R_max - R_min);
+    double R2_max = R_max * R_max, R2_tap = R_tap * R_tap;
+    double tap_cf = c_i2Pi / (R2_max - R2_tap);
+
+    double dlnr = log(R_max / R_min) / (double)(nR - 1);
+    for (int i = 0; i < nR; i++) {
+        R[i] = R_min * exp(i * dlnr);
+        R2[i] = pow(R[i], 2);
+    }
+    for (int i = 0; i < nR; i++) lobato_VR_dVR(R[i], &VRc, &VR[i], &dVR[i]);
+    for (int i = 0; i < nR; i++) {
+        double x = R2[i], y = VR[i], dy = 0.5 * dVR[i] / R[i];
+        /* apply_tapering (src/cgpu_fcns.cuh:1528-1538) */
+        if (R2_tap < x) {
+            double tap = cos(tap_cf * (x - R2_tap)), dtap = sin(tap_cf * (x - R2_tap));
+            dy = dy * tap - tap_cf * y * dtap;
+            y *= tap;
+        }
+        c0[i] = y;
+        c1[i] = dy;
+        c2[i] = c3[i] = 0.0;
+    }
+    for (int i = 0; i < nR - 1; i++) {
+        double idR = 1.0 / (R2[i + 1] - R2[i]);
+        double V = c0[i], Vn = c0[i + 1], dV = c1[i], dVn = c1[i + 1];
+        double m = (Vn - V) * idR, n = dV + dVn;
+        c2[i] = (3.0 * m - n - dV) * idR;
+        c3[i] = (n - 2.0 * m) * idR * idR;
+    }
+    t->R_min_d = R_min;
+    t->R_max_d = R_max;
+    t->R_max = (real)R_max;
+    /* R2_max() / R2_tap() are evaluated on the cast members (types.cuh Atom_Coef) */
+    t->R2_max = rsquare((real)R_max);
+    t->R2_tap = rsquare((real)R_tap);
+    t->tap_cf = (real)tap_cf;
+    for (int i = 0; i < nR; i++) {
+        t->R[i] = (real)R[i];
+        t->R2[i] = (real)R2[i];
+        t->c0[i] = (real)c0[i];
+        t->c1[i] = (real)c1[i];
+        t->c2[i] = (real)c2[i];
+        t->c3[i] = (real)c3[i];
+    }
+    t->valid = 1;
+}
+
+/* Exported for table tests: R, R2, c0..c3 (nR each) and {R_min, R_max, R2_tap, tap_cf} */
+int orc_atom_table(int Z, double dR_min, double *R, double *R2, double *c0, double *c1,
+                   double *c2, double *c3, double *scal)
+{
+    atom_tab t;
+    if (Z < 1 || Z > 103) { set_err("Z out of range"); return MT_ERR_INVALID_ARGUMENT; }
+    build_atom_tab(Z, c_Vrl, c_nR, dR_min, &t);
+    for (int i = 0; i < c_nR; i++) {
+        R[i] = t.R[i]; R2[i] = t.R2[i];
+        c0[i] = t.c0[i]; c1[i] = t.c1[i]; c2[i] = t.c2[i]; c3[i] = t.c3[i];
+    }
+    scal[0] = t.R_min_d; scal[1] = t.R_max_d; scal[2] = t.R2_tap; scal[3] = t.tap_cf;
+    return MT_OK;
+}
+
+/* unrolledBinarySearch_c_nR + eval_cubic_poly (src/cgpu_fcns.cuh:1496-1566) */
+static inline real eval_spline(real R2v, const atom_tab *t)
+{
+    int i0 = 0, ie = c_nR - 1, im;
+    for (int k = 0; k < 7; k++) {
+        im = (i0 + ie) >> 1;
+        if (R2v < t->R2[im]) ie = im; else i0 = im;
+    }
+    const real dx = R2v - t->R2[i0];
+    return (((t->c3[i0] * dx + t->c2[i0]) * dx + t->c1[i0]) * dx + t->c0[i0]);
+}
+
+/* ------------------------------------------------------------------ */
+/* mt19937_64 (Matsumoto & Nishimura 2000), uniform_int_distribution,   */
+/* generate_canonical and normal_distribution as libstdc++ (GCC 11)     */
+/* implements them -- the generators behind Randn_3d                    */
+/* (src/cgpu_rand.cuh:303-402).                                         */
+/* ------------------------------------------------------------------ */
+typedef struct { uint64_t mt[312]; int mti; } mt64;
+
+static void mt64_seed(mt64 *g, uint64_t s)
+{
+    g->mt[0] = s;
+    for (int i = 1; i < 312; i++)
+        g->mt[i] = 6364136223846793005ULL * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t)i;
+    g->mti = 312;
+}
+static uint64_t mt64_next(mt64 *g)
+{
+    static const uint64_t UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL;
+    static const uint64_t MATRIX_A = 0xB5026F5AA96619E9ULL;
+    if (g->mti >= 312) {
+        int i;
+        for (i = 0; i < 312 - 156; i++) {
+            uint64_t x = (g->mt[i] & UM) | (g->mt[i + 1] & LM);
+            g->mt[i] = g->mt[i + 156] ^ (x >> 1) ^ ((x & 1ULL) ? MATRIX_A : 0ULL);
+        }
+        for (; i < 311; i++) {
+            uint64_t x = (g->mt[i] & UM) | (g->mt[i + 1] & LM);
+            g->mt[i] = g->mt[i + (156 - 312)] ^ (x >> 1) ^ ((x & 1ULL) ? MATRIX_A : 0ULL);
+        }
+        uint64_t x = (g->mt[311] & UM) | (g->mt[0] & LM);
+        g->mt[311] = g->mt[155] ^ (x >> 1) ^ ((x & 1ULL) ? MATRIX_A : 0ULL);
+        g->mti = 0;
+    }
+    uint64_t x = g->mt[g->mti++];
+    x ^= (x >> 29) & 0x5555555555555555ULL;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+    x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+    x ^= (x >> 43);
+    return x;
+}
+
+/* uniform_int_distribution<int>(1, INT_MAX) on a 64-bit engine: Lemire's
+   nearly-divisionless reduction with 128-bit products. */
+static int uniform_int_1_max(mt64 *g)
+{
+    const uint64_t range = (uint64_t)INT_MAX - 1ULL + 1ULL;   /* b - a + 1 */
+    unsigned __int128 product = (unsigned __int128)mt64_next(g) * range;
+    uint64_t low = (uint64_t)product;
+    if (low < range) {
+        uint64_t threshold = (0ULL - range) % range;
+        while (low < threshold) {
+            product = (unsigned __int128)mt64_next(g) * range;
+            low = (uint64_t)product;
+        }
+    }
+    return (int)((uint64_t)(product >> 64) + 1ULL);
+}
+
+typedef struct { mt64 g; int saved_available; real saved; } normal_gen;
+
+static real canonical(mt64 *g)
+{
+    /* generate_canonical<real, digits>: one 64-bit draw / 2^64, clamped below 1 */
+    real sum = (real)mt64_next(g);
+    real tmp = (real)18446744073709551616.0L;
+    real ret = sum / tmp;
+    if (ret >= (real)1) ret = nextafter((real)1, (real)0);
+    return ret;
+}
+static real normal_draw(normal_gen *n)
+{
+    real ret;
+    if (n->saved_available) {
+        n->saved_available = 0;
+        ret = n->saved;
+    } else {
+        real x, y, r2;
+        do {
+            x = (real)((real)2.0 * canonical(&n->g) - 1.0);
+            y = (real)((real)2.0 * canonical(&n->g) - 1.0);
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const real mult = sqrt(-2 * log(r2) / r2);
+        n->saved = x * mult;
+        n->saved_available = 1;
+        ret = y * mult;
+    }
+    return ret * (real)1 + (real)0;
+}
+
+/* Randn_3d<T, e_host>::seed (src/cgpu_rand.cuh:322-345) */
+typedef struct { normal_gen gx, gy, gz; int bx, by, bz; } randn3;
+static void randn3_seed(randn3 *r, int seed, int n_iter)
+{
+    mt64 gu;
+    unsigned int sx = 0, sy = 0, sz = 0;
+    if (n_iter < 1) n_iter = 1;
+    mt64_seed(&gu, (uint64_t)(int64_t)seed);
+    for (int ic = 0; ic < n_iter; ic++) {
+        sx = (unsigned int)uniform_int_1_max(&gu);
+        sy = (unsigned int)uniform_int_1_max(&gu);
+        sz = (unsigned int)uniform_int_1_max(&gu);
+    }
+    mt64_seed(&r->gx.g, sx); r->gx.saved_available = 0;
+    mt64_seed(&r->gy.g, sy); r->gy.saved_available = 0;
+    mt64_seed(&r->gz.g, sz); r->gz.saved_available = 0;
+}
+
+void orc_mt19937_64(uint64_t seed, int n, uint64_t *out)
+{
+    mt64 g;
+    mt64_seed(&g, seed);
+    for (int i = 0; i < n; i++) out[i] = mt64_next(&g);
+}
+void orc_uniform_int(uint64_t seed, int n, int *out)
+{
+    mt64 g;
+    mt64_seed(&g, seed);
+    for (int i = 0; i < n; i++) out[i] = uniform_int_1_max(&g);
+}
+void orc_normal(uint64_t seed, int n, double *out)
+{
+    normal_gen ng;
+    mt64_seed(&ng.g, seed);
+    ng.saved_available = 0;
+    for (int i = 0; i < n; i++) out[i] = (double)normal_draw(&ng);
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT: iterative radix-2, unnormalised both ways, on real precision.   */
+/* 2D layout column-major: element (ix, iy) at ix*ny + iy.              */
+/* ------------------------------------------------------------------ */
+typedef struct { int n, logn; cplx *w; int *rev; } fft1;
+
+static int ilog2i(int n) { int l = 0; while ((1 << l) < n) l++; return ((1 << l) == n) ? l : -1; }
+
+static int fft1_init(fft1 *p, int n)
+{
+    p->n = n;
+    p->logn = ilog2i(n);
+    if (p->logn < 0) return -1;
+    p->w = malloc(sizeof(cplx) * (size_t)(n / 2 + 1));
+    p->rev = malloc(sizeof(int) * (size_t)n);
+    for (int k = 0; k < n / 2 + 1; k++) {
+        double a = -c_2Pi * (double)k / (double)n;
+        p->w[k].re = (real)cos(a);
+        p->w[k].im = (real)sin(a);
+    }
+    for (int i = 0; i < n; i++) {
+        int r = 0;
+        for (int b = 0; b < p->logn; b++) if (i & (1 << b)) r |= 1 << (p->logn - 1 - b);
+        p->rev[i] = r;
+    }
+    return 0;
+}
+static void fft1_free(fft1 *p) { free(p->w); free(p->rev); }
+
+/* transform nb interleaved sequences: element k of sequence b at x[k*nb + b] */
+static void fft1_exec_multi(const fft1 *p, cplx *x, int nb, int inverse)
+{
+    const int n = p->n;
+    for (int i = 0; i < n; i++) {
+        int r = p->rev[i];
+        if (r > i) {
+            cplx *a = x + (size_t)i * nb, *b = x + (size_t)r * nb;
+            for (int q = 0; q < nb; q++) { cplx t = a[q]; a[q] = b[q]; b[q] = t; }
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1, step = n / len;
+        for (int i = 0; i < n; i += len) {
+            for (int j = 0; j < half; j++) {
+                cplx w = p->w[j * step];
+                if (inverse) w.im = -w.im;
+                cplx *u = x + (size_t)(i + j) * nb, *v = x + (size_t)(i + j + half) * nb;
+                for (int q = 0; q < nb; q++) {
+                    real tr = v[q].re * w.re - v[q].im * w.im;
+                    real ti = v[q].re * w.im + v[q].im * w.re;
+                    v[q].re = u[q].re - tr; v[q].im = u[q].im - ti;
+                    u[q].re = u[q].re + tr; u[q].im = u[q].im + ti;
+                }
+            }
+        }
+    }
+}
+
+typedef struct { int nx, ny; fft1 fx, fy; } fft2;
+
+static int fft2_init(fft2 *p, int nx, int ny)
+{
+    p->nx = nx; p->ny = ny;
+    if (fft1_init(&p->fx, nx) || fft1_init(&p->fy, ny)) return -1;
+    return 0;
+}
+static void fft2_free(fft2 *p) { fft1_free(&p->fx); fft1_free(&p->fy); }
+
+/* 2D: along y (contiguous) for each ix, then along x for blocks of 16 columns */
+static void fft2_exec(const fft2 *p, cplx *a, int inverse, cplx *work)
+{
+    const int nx = p->nx, ny = p->ny;
+    for (int ix = 0; ix < nx; ix++) fft1_exec_multi(&p->fy, a + (size_t)ix * ny, 1, inverse);
+    const int B = 16;
+    for (int iy0 = 0; iy0 < ny; iy0 += B) {
+        int nb = (ny - iy0 < B) ? ny - iy0 : B;
+        for (int ix = 0; ix < nx; ix++)
+            memcpy(work + (size_t)ix * nb, a + (size_t)ix * ny + iy0, sizeof(cplx) * (size_t)nb);
+        fft1_exec_multi(&p->fx, work, nb, inverse);
+        for (int ix = 0; ix < nx; ix++)
+            memcpy(a + (size_t)ix * ny + iy0, work + (size_t)ix * nb, sizeof(cplx) * (size_t)nb);
+    }
+}
+
+int orc_fft2(int nx, int ny, double *data, int inverse)
+{
+    fft2 p;
+    if (fft2_init(&p, nx, ny)) { set_err("fft sizes must be powers of two"); return MT_ERR_UNSUPPORTED; }
+    size_t n = (size_t)nx * ny;
+    cplx *a = malloc(sizeof(cplx) * n), *w = malloc(sizeof(cplx) * (size_t)nx * 16);
+    for (size_t i = 0; i < n; i++) { a[i].re = (real)data[2 * i]; a[i].im = (real)data[2 * i + 1]; }
+    fft2_exec(&p, a, inverse, w);
+    for (size_t i = 0; i < n; i++) { data[2 * i] = a[i].re; data[2 * i + 1] = a[i].im; }
+    free(a); free(w); fft2_free(&p);
+    return MT_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Simulation state: Input_Multislice<real> + Spec + Slicing           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    int *Z, *region, *charge;
+    real *x, *y, *z;
+    float *sigma, *occ;
+    real l_x, l_y, l_z, dz;
+    real z_min, z_max, R_int_max, z_int_min, z_int_max, s_z_int;
+} atoms_t;
+
+typedef struct {
+    int m;
+    real c_10, c_12, phi_12, c_21, phi_21, c_23, phi_23, c_30, c_32, phi_32, c_34, phi_34;
+    real c_41, phi_41, c_43, phi_43, c_45, phi_45, c_50, c_52, phi_52, c_54, phi_54, c_56, phi_56;
+    real inner_aper_ang, outer_aper_ang;
+    int zero_defocus_type;
+    real zero_defocus_plane;
+    real lambda;
+    real c_c_10, c_c_12, c_c_21, c_c_23, c_c_30, c_c_32, c_c_34, c_c_41, c_c_43, c_c_45;
+    real c_c_50, c_c_52, c_c_54, c_c_56;
+    real g2_min, g2_max;
+} lens_t;
+
+typedef struct {
+    int nx, ny, nxh, nyh;
+    real lx, ly, dz;
+    int bwl, pbc_xy;
+    real dRx, dRy, dgx, dgy, gl2_max, alpha;
+} grid_t;
+
+typedef struct { real z_0, z_e, z_int_0, z_int_e; int iatom_0, iatom_e, ithk; } slice_t;
+typedef struct { real z, z_zero_def_plane, z_back_prop; int islice, iatom_e; } thick_t;
+
+typedef struct {
+    int simulation_type, interaction_model, potential_type;
+    int pn_model, pn_coh_contrib, pn_single_conf, pn_nconf, fp_iconf_0, pn_seed;
+    int pn_dim_x, pn_dim_y, pn_dim_z;
+    int thick_type, potential_slicing, illumination_model;
+    int n_thick; real *thick;
+    grid_t g;
+    int iw_type; int n_iw; real *iw_x, *iw_y;
+    real E_0, lambda, theta, phi;
+    int mul_sign;
+    lens_t cond, obj;
+    /* scanning */
+    int sc_type, sc_pbc, sc_spxs, sc_ns, sc_nx, sc_ny;
+    real sc_x0, sc_y0, sc_xe, sc_ye, sc_dRx, sc_dRy;
+    real *sc_x, *sc_y; int sc_n;
+    /* detector */
+    int n_det; real *det_g_inner, *det_g_outer;
+    /* output area (0-based half-open) */
+    int oa_ix_0, oa_ix_e, oa_iy_0, oa_iy_e;
+    real Vrl; int nR;
+    /* specimen */
+    atoms_t in_atoms;      /* input_multislice.atoms */
+    atoms_t atoms_u;       /* Spec::atoms_u (pbc-deduped, z-sorted) */
+    atoms_t atoms;         /* displaced */
+    atom_tab *tab;         /* 103 entries, built lazily */
+    int n_plane; real *z_plane;
+    int n_zslice; real *z_slice;
+    int n_slice; slice_t *slice;
+    int n_thk; thick_t *thk;
+} sim_t;
+
+static void atoms_alloc(atoms_t *a, int n)
+{
+    memset(a, 0, sizeof *a);
+    a->n = n;
+    size_t m = (size_t)(n > 0 ? n : 1);
+    a->Z = calloc(m, sizeof(int)); a->region = calloc(m, sizeof(int)); a->charge = calloc(m, sizeof(int));
+    a->x = calloc(m, sizeof(real)); a->y = calloc(m, sizeof(real)); a->z = calloc(m, sizeof(real));
+    a->sigma = calloc(m, sizeof(float)); a->occ = calloc(m, sizeof(float));
+}
+static void atoms_free(atoms_t *a)
+{
+    free(a->Z); free(a->region); free(a->charge); free(a->x); free(a->y); free(a->z);
+    free(a->sigma); free(a->occ);
+    memset(a, 0, sizeof *a);
+}
+static void atoms_copy_row(atoms_t *d, int i, const atoms_t *s, int j)
+{
+    d->Z[i] = s->Z[j]; d->x[i] = s->x[j]; d->y[i] = s->y[j]; d->z[i] = s->z[j];
+    d->sigma[i] = s->sigma[j]; d->occ[i] = s->occ[j]; d->region[i] = s->region[j]; d->charge[i] = s->charge[j];
+}
+
+static const atom_tab *get_tab(sim_t *s, int Z)
+{
+    atom_tab *t = &s->tab[Z - 1];
+    if (!t->valid) {
+        real dRmin = fmin(s->g.dRx, s->g.dRy);
+        build_atom_tab(Z, (double)s->Vrl, s->nR, (double)dRmin, t);
+    }
+    return t;
+}
+
+/* Atom_Data::get_statistic (src/atomic_data_mt.hpp:425-512), the fields used */
+static void atoms_statistic(sim_t *s, atoms_t *a, int with_types)
+{
+    if (a->n == 0) return;
+    real zmin = a->z[0], zmax = a->z[0];
+    for (int i = 1; i < a->n; i++) { if (a->z[i] < zmin) zmin = a->z[i]; if (a->z[i] > zmax) zmax = a->z[i]; }
+    a->z_min = zmin; a->z_max = zmax;
+    a->R_int_max = 2.5;
+    if (with_types) {
+        a->R_int_max = get_tab(s, a->Z[0] % 1000)->R_max;
+        for (int i = 0; i < a->n; i++) a->R_int_max = fmax(get_tab(s, a->Z[i] % 1000)->R_max, a->R_int_max);
+    }
+    a->z_int_min = a->z_min - a->R_int_max;
+    a->z_int_max = a->z_max + a->R_int_max;
+    a->s_z_int = a->z_int_max - a->z_int_min;
+    a->l_z = fmax(a->z_max - a->z_min, a->l_z);
+}
+
+static int cmp_z_stable_idx(const void *pa, const void *pb, void *arg)
+{
+    const real *z = (const real *)arg;
+    int a = *(const int *)pa, b = *(const int *)pb;
+    if (z[a] < z[b]) return -1;
+    if (z[b] < z[a]) return 1;
+    return (a < b) ? -1 : (a > b);
+}
+/* sort_by_z: thrust::sort on the host backend is a stable merge sort
+   (src/atomic_data_mt.hpp:515-522) */
+static void atoms_sort_by_z(atoms_t *a)
+{
+    int *idx = malloc(sizeof(int) * (size_t)(a->n > 0 ? a->n : 1));
+    for (int i = 0; i < a->n; i++) idx[i] = i;
+    qsort_r(idx, (size_t)a->n, sizeof(int), cmp_z_stable_idx, a->z);
+    atoms_t t;
+    atoms_alloc(&t, a->n);
+    for (int i = 0; i < a->n; i++) atoms_copy_row(&t, i, a, idx[i]);
+    t.l_x = a->l_x; t.l_y = a->l_y; t.l_z = a->l_z; t.dz = a->dz;
+    atoms_free(a);
+    *a = t;
+    free(idx);
+}
+
+/* ------------------------------------------------------------------ */
+/* Identify_Planes (src/types.cuh:1166-1372)                            */
+/* ------------------------------------------------------------------ */
+static int *hist_planes(const real *v, int n, double dv, double v_min, double v_max, int *nbins_out)
+{
+    const double v_l = fmax(v_max - v_min, dv);
+    int nbins = (int)ceil(v_l / dv);
+    int *h = calloc((size_t)nbins, sizeof(int));
+    for (int iv = 0; iv < n; iv++) {
+        double v_id = (double)v[iv];
+        int ih = (int)floor((v_id - v_min) / dv);
+        double v_imin = v_min + (ih - 1) * dv;
+        double v_imax = v_imin + dv;
+        if (v_id < v_imin) {
+            for (int ik = ih; ik >= 0; ik--) {
+                v_imin = v_min + (ik - 1) * dv; v_imax = v_imin + dv;
+                if ((v_imin <= v_id) && (v_id < v_imax)) { ih = ik - 1; break; }
+            }
+        } else if (v_id > v_imax) {
+            for (int ik = ih; ik < nbins; ik++) {
+                v_imin = v_min + ik * dv; v_imax = v_imin + dv;
+                if ((v_imin <= v_id) && (v_id < v_imax)) { ih = ik; break; }
+            }
+        }
+        ih = (ih < 0) ? 0 : ((ih > nbins - 1) ? nbins - 1 : ih);
+        h[ih]++;
+    }
+    while (nbins > 0 && h[nbins - 1] == 0) nbins--;
+    for (int ih = 1; ih < nbins - 1; ih++) {
+        int bn = (ih < nbins - 2) ? (0 < h[ih + 2]) : 0;
+        bn = (0 < h[ih - 1]) && ((0 < h[ih + 1]) || bn);
+        if ((h[ih] == 0) && bn) h[ih] = 1;
+    }
+    *nbins_out = nbins;
+    return h;
+}
+
+/* planes of a sorted vector v */
+static real *identify_planes_v(const real *v, int n, int *np)
+{
+    const double dv = 0.1;
+    real *out;
+    if (n == 0) { *np = 0; return NULL; }
+    real v_min = v[0], v_max = v[n - 1];
+    int nbins;
+    int *h = hist_planes(v, n, dv, v_min, v_max, &nbins);
+    if (nbins == 1) {
+        real s = 0;   /* thrust::reduce: plain sum in order */
+        for (int i = 0; i < n; i++) s += v[i];
+        out = malloc(sizeof(real));
+        out[0] = s / (real)n;
+        *np = 1;
+        free(h);
+        return out;
+    }
+    real *v_lim = malloc(sizeof(real) * (size_t)(nbins + 1));
+    int nlim = 0;
+    for (int iz = 0; iz < nbins - 1; iz++)
+        if ((h[iz] > 0) && (h[iz + 1] == 0)) v_lim[nlim++] = (real)(v[0] + (iz + 1) * dv);
+    v_lim[nlim++] = (real)(v[n - 1] + dv);
+    out = malloc(sizeof(real) * (size_t)nlim);
+    int nout = 0;
+    real v_m = v[0], v_m_ee = 0;
+    int v_c = 1, izl = 0;
+    for (int iz = 0; iz < n; iz++) {
+        real v_v = v[iz];
+        if (v_v < v_lim[izl]) {
+            /* kh_sum (src/cgpu_fcns.cuh:253-260) */
+            real vv = v_v - v_m_ee;
+            real t = v_m + vv;
+            v_m_ee = (t - v_m) - vv;
+            v_m = t;
+            v_c++;
+        } else {
+            out[nout++] = v_m / v_c;
+            v_m = v_v;
+            v_m_ee = 0;
+            v_c = 1;
+            izl++;
+        }
+    }
+    out[nout++] = v_m / v_c;
+    free(v_lim); free(h);
+    *np = nout;
+    return out;
+}
+
+/* planes between v_min and v_max with spacing dv, eMB_MinMax (src/types.cuh:1240-1305) */
+static real *identify_planes_range(real v_min, real v_max, real dv, int *np)
+{
+    const real v_eps = 1e-4;
+    real *out;
+    if (fabs(v_max - v_min) < v_eps) {
+        out = malloc(sizeof(real)); out[0] = 0.5 * (v_min + v_max); *np = 1; return out;
+    }
+    if (v_max < v_min) { *np = 0; return NULL; }
+    real s_v = v_max - v_min;
+    int q = (int)floor(s_v / dv + v_eps);
+    const int nv = ((q > 1) ? q : 1) + 1;
+    const real dv_b = dv + 0.5 * (s_v - (nv - 1) * dv);
+    out = malloc(sizeof(real) * (size_t)nv);
+    out[0] = v_min;
+    for (int iv = 1; iv < nv; iv++) {
+        const real dv_t = ((iv == 1) || (iv == nv - 1)) ? dv_b : dv;
+        out[iv] = out[iv - 1] + dv_t;
+    }
+    *np = nv;
+    return out;
+}
+
+static int cmp_real(const void *a, const void *b)
+{
+    real x = *(const real *)a, y = *(const real *)b;
+    return (x < y) ? -1 : (x > y);
+}
+
+/* unique_vector (src/cpu_fcns.hpp:3603-3611): sort + unique with isEqual */
+static int unique_real(real *v, int n)
+{
+    qsort(v, (size_t)n, sizeof(real), cmp_real);
+    if (n == 0) return 0;
+    int m = 1;
+    for (int i = 1; i < n; i++) if (!is_equal_r(v[m - 1], v[i])) v[m++] = v[i];
+    return m;
+}
+
+/* Slicing::get_z_plane (src/slicing.hpp:130-193), no amorphous layers */
+static real *get_z_plane(int pot_sli, const atoms_t *a, int *np)
+{
+    real *zp;
+    if (a->n == 0) { zp = malloc(sizeof(real)); zp[0] = 0; *np = 1; return zp; }
+    real *zct = malloc(sizeof(real) * (size_t)a->n);
+    int nct = 0;
+    for (int i = 0; i < a->n; i++) if (a->region[i] == 0) zct[nct++] = a->z[i];
+    qsort(zct, (size_t)nct, sizeof(real), cmp_real);
+    if (pot_sli == 1) zp = identify_planes_v(zct, nct, np);
+    else zp = identify_planes_range(zct[0], zct[nct - 1], a->dz, np);
+    free(zct);
+    *np = unique_real(zp, *np);
+    return zp;
+}
+
+/* Slicing::get_spacing (src/slicing.hpp:118-127) */
+static real get_spacing(int ix, const real *x, int n, real dz)
+{
+    if (n == 1) return dz;
+    ix = (ix <= 0) ? 1 : ((ix < n - 1) ? ix : n - 1);
+    return (n > 1) ? x[ix] - x[ix - 1] : 0.0;
+}
+
+/* Slicing::get_z_slice (src/slicing.hpp:196-250), planes and dz_Proj */
+static real *get_z_slice(int pot_sli, const real *zp, int np, const atoms_t *a, int *nz)
+{
+    real *zs;
+    if ((pot_sli != 3) && (np == 1)) {
+        zs = malloc(2 * sizeof(real));
+        zs[0] = a->z_int_min; zs[1] = a->z_int_max;
+        *nz = 2;
+        return zs;
+    }
+    int n = np + 1;
+    zs = malloc(sizeof(real) * (size_t)n);
+    zs[0] = zp[0] - 0.5 * get_spacing(0, zp, np, a->dz);
+    for (int iz = 1; iz < n - 1; iz++) zs[iz] = 0.5 * (zp[iz] + zp[iz - 1]);
+    zs[n - 1] = zs[n - 2] + get_spacing(np, zp, np, a->dz);
+    *nz = n;
+    return zs;
+}
+
+/* Slicing::fd_by_z (src/slicing.hpp:406-452) */
+static int lower_bound_r(const real *z, int n, real v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (z[mid] < v) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+static int fd_by_z_e(const real *z, int n, real z_e, int inc)
+{
+    if (n == 0) return -1;
+    int iz_e = -1;
+    z_e = inc ? z_e + eps_rel_r : z_e;
+    if (z_e < z[0]) return iz_e;
+    iz_e = (z[n - 1] <= z_e) ? (n - 1) : (lower_bound_r(z, n, z_e) - 1);
+    if (z[iz_e] < z[0]) iz_e = -1;
+    return iz_e;
+}
+static void fd_by_z_range(const real *z, int n, real z_0, real z_e, int *iz_0, int *iz_e, int inc)
+{
+    z_0 = inc ? (z_0 - eps_rel_r) : z_0;
+    z_e = inc ? (z_e + eps_rel_r) : z_e;
+    if ((z_0 > z_e) || (z[n - 1] < z_0) || (z_e < z[0])) { *iz_0 = 1; *iz_e = 0; return; }
+    *iz_0 = (z_0 <= z[0]) ? 0 : lower_bound_r(z, n, z_0);
+    *iz_e = (z[n - 1] <= z_e) ? (n - 1) : (lower_bound_r(z, n, z_e) - 1);
+    if ((*iz_0 > *iz_e) || (z[*iz_e] < z_0) || (z_e < z[*iz_0])) { *iz_0 = 1; *iz_e = 0; }
+}
+
+/* Lens::get_zero_defocus_plane (src/types.cuh:3013-3041) */
+static real lens_zero_defocus_plane(const lens_t *l, real z_min, real z_max)
+{
+    switch (l->zero_defocus_type) {
+    case 1: return z_min;
+    case 2: return 0.5 * (z_min + z_max);
+    case 3: return z_max;
+    default: return l->zero_defocus_plane;
+    }
+}
+
+/* Slicing::get_thick (src/slicing.hpp:253-336) */
+static void get_thick(sim_t *s, const atoms_t *ar)
+{
+    free(s->thk);
+    s->n_thk = s->n_thick;
+    s->thk = calloc((size_t)s->n_thk, sizeof(thick_t));
+    const real *zs = s->z_slice;
+    int nzs = s->n_zslice;
+    if (ar->n == 0) {
+        for (int ik = 0; ik < s->n_thk; ik++) s->thk[ik].z = s->thick[ik];
+        return;
+    }
+    for (int ik = 0; ik < s->n_thk; ik++) {
+        thick_t *t = &s->thk[ik];
+        t->z = s->thick[ik];
+        int islice = 0;
+        if (s->thick_type == 3) {
+            for (int i = 0; i < nzs - 1; i++) if (fabs(zs[i + 1] - t->z) < 1e-4f) { islice = i; break; }
+        } else {
+            for (int i = 0; i < nzs - 1; i++) if ((zs[i] < t->z) && (t->z <= zs[i + 1])) { islice = i; break; }
+        }
+        t->islice = islice;
+        int iatom_e = fd_by_z_e(ar->z, ar->n, zs[islice + 1], 0);
+        t->iatom_e = iatom_e;
+        t->z_zero_def_plane = lens_zero_defocus_plane(&s->obj, ar->z[0], ar->z[iatom_e]);
+        if (s->thick_type == 3) {
+            t->z_back_prop = 0;
+        } else {
+            t->z_back_prop = t->z_zero_def_plane - 0.5 * (zs[islice] + zs[islice + 1]);
+            if (fabs(t->z_back_prop) < 1e-4f) t->z_back_prop = 0;
+        }
+    }
+}
+
+/* Slicing::get_slicing (src/slicing.hpp:339-403), multislice with planes / dz_Proj */
+static void get_slicing(sim_t *s, const atoms_t *a)
+{
+    free(s->slice);
+    s->n_slice = s->n_zslice - 1;
+    s->slice = calloc((size_t)s->n_slice, sizeof(slice_t));
+    for (int is = 0; is < s->n_slice; is++) {
+        slice_t *sl = &s->slice[is];
+        sl->z_0 = s->z_slice[is];
+        sl->z_e = s->z_slice[is + 1];
+        sl->z_int_0 = sl->z_0;
+        sl->z_int_e = sl->z_e;
+        fd_by_z_range(a->z, a->n, sl->z_int_0, sl->z_int_e, &sl->iatom_0, &sl->iatom_e, 0);
+        sl->ithk = -1;
+    }
+    for (int ik = 0; ik < s->n_thk; ik++) s->slice[s->thk[ik].islice].ithk = ik;
+}
+
+/* Slicing::calculate (src/slicing.hpp:93-100) */
+static void slicing_calculate(sim_t *s)
+{
+    free(s->z_slice);
+    s->z_slice = get_z_slice(s->potential_slicing, s->z_plane, s->n_plane, &s->atoms, &s->n_zslice);
+    get_thick(s, &s->atoms_u);
+    get_slicing(s, &s->atoms);
+}
+
+/* ------------------------------------------------------------------ */
+/* Reader: read_input_multislice + validate_parameters                 */
+/* (mex_files_multem/ilc_multem.cu:37-379; input_multislice.cuh:622-796)*/
+/* ------------------------------------------------------------------ */
+static int is_stem(const sim_t *s) { return s->simulation_type == 11; }
+static int is_cbed(const sim_t *s) { return s->simulation_type == 21; }
+static int is_ewfs(const sim_t *s) { return s->simulation_type == 51; }
+static int is_ewrs(const sim_t *s) { return s->simulation_type == 52; }
+static int is_frozen(const sim_t *s) { return s->pn_model == 3; }
+static int is_scanning_t(int st) { return st == 11 || st == 12 || st == 61; }
+
+static void lens_read(lens_t *l, const mt_lens *m)
+{
+    memset(l, 0, sizeof *l);
+    l->m = (int)m->m;
+    l->c_10 = (real)m->c_10; l->c_12 = (real)m->c_12; l->phi_12 = (real)(m->phi_12 * c_deg_2_rad);
+    l->c_21 = (real)m->c_21; l->phi_21 = (real)(m->phi_21 * c_deg_2_rad);
+    l->c_23 = (real)m->c_23; l->phi_23 = (real)(m->phi_23 * c_deg_2_rad);
+    l->c_30 = (real)(m->c_30 * c_mm_2_Angs); l->c_32 = (real)m->c_32; l->phi_32 = (real)(m->phi_32 * c_deg_2_rad);
+    l->c_34 = (real)m->c_34; l->phi_34 = (real)(m->phi_34 * c_deg_2_rad);
+    l->c_41 = (real)m->c_41; l->phi_41 = (real)(m->phi_41 * c_deg_2_rad);
+    l->c_43 = (real)m->c_43; l->phi_43 = (real)(m->phi_43 * c_deg_2_rad);
+    l->c_45 = (real)m->c_45; l->phi_45 = (real)(m->phi_45 * c_deg_2_rad);
+    l->c_50 = (real)(m->c_50 * c_mm_2_Angs); l->c_52 = (real)m->c_52; l->phi_52 = (real)(m->phi_52 * c_deg_2_rad);
+    l->c_54 = (real)m->c_54; l->phi_54 = (real)(m->phi_54 * c_deg_2_rad);
+    l->c_56 = (real)m->c_56; l->phi_56 = (real)(m->phi_56 * c_deg_2_rad);
+    l->inner_aper_ang = (real)(m->inner_aper_ang * c_mrad_2_rad);
+    l->outer_aper_ang = (real)(m->outer_aper_ang * c_mrad_2_rad);
+    l->zero_defocus_type = m->zero_defocus_type;
+    l->zero_defocus_plane = (real)m->zero_defocus_plane;
+}
+
+/* Lens::set_input_data (src/types.cuh:2924-2976), coefficient part */
+static void lens_set(lens_t *l, real E_0, const grid_t *g)
+{
+    real lambda = get_lambda(E_0);
+    l->lambda = lambda;
+    l->c_c_10 = is_zero_r(l->c_10) ? 0 : -c_Pi * l->c_10 * lambda;
+    l->c_c_12 = is_zero_r(l->c_12) ? 0 : -c_Pi * l->c_12 * lambda;
+    l->c_c_21 = is_zero_r(l->c_21) ? 0 : -2.0 * c_Pi * l->c_21 * pow(lambda, 2) / 3.0;
+    l->c_c_23 = is_zero_r(l->c_23) ? 0 : -2.0 * c_Pi * l->c_23 * pow(lambda, 2) / 3.0;
+    l->c_c_30 = is_zero_r(l->c_30) ? 0 : -c_Pi * l->c_30 * pow(lambda, 3) / 2.0;
+    l->c_c_32 = is_zero_r(l->c_32) ? 0 : -c_Pi * l->c_32 * pow(lambda, 3) / 2.0;
+    l->c_c_34 = is_zero_r(l->c_34) ? 0 : -c_Pi * l->c_34 * pow(lambda, 3) / 2.0;
+    l->c_c_41 = is_zero_r(l->c_41) ? 0 : -2.0 * c_Pi * l->c_41 * pow(lambda, 4) / 5.0;
+    l->c_c_43 = is_zero_r(l->c_43) ? 0 : -2.0 * c_Pi * l->c_43 * pow(lambda, 4) / 5.0;
+    l->c_c_45 = is_zero_r(l->c_45) ? 0 : -2.0 * c_Pi * l->c_45 * pow(lambda, 4) / 5.0;
+    l->c_c_50 = is_zero_r(l->c_50) ? 0 : -c_Pi * l->c_50 * pow(lambda, 5) / 3.0;
+    l->c_c_52 = is_zero_r(l->c_52) ? 0 : -c_Pi * l->c_52 * pow(lambda, 5) / 3.0;
+    l->c_c_54 = is_zero_r(l->c_54) ? 0 : -c_Pi * l->c_54 * pow(lambda, 5) / 3.0;
+    l->c_c_56 = is_zero_r(l->c_56) ? 0 : -c_Pi * l->c_56 * pow(lambda, 5) / 3.0;
+    real gmax = fmin((real)g->nxh * g->dgx, (real)g->nyh * g->dgy);
+    l->g2_min = (is_zero_r(l->inner_aper_ang) || (l->inner_aper_ang < 0)) ? 0 : pow(sin(l->inner_aper_ang) / lambda, 2);
+    l->g2_max = (is_zero_r(l->outer_aper_ang) || (l->outer_aper_ang < 0)) ? rsquare(gmax)
+                : pow(sin(l->outer_aper_ang) / lambda, 2);
+}
+static void lens_set_defocus(lens_t *l, real f)
+{
+    l->c_10 = f;
+    l->c_c_10 = is_zero_r(l->c_10) ? 0 : -c_Pi * l->c_10 * l->lambda;
+}
+static int lens_phi_required(const lens_t *l)
+{
+    return !(is_zero_r((real)l->m) && is_zero_r(l->c_12) && is_zero_r(l->c_21) && is_zero_r(l->c_23)
+             && is_zero_r(l->c_32) && is_zero_r(l->c_34) && is_zero_r(l->c_41) && is_zero_r(l->c_43)
+             && is_zero_r(l->c_45) && is_zero_r(l->c_52) && is_zero_r(l->c_54) && is_zero_r(l->c_56));
+}
+
+/* Scanning::set_grid (src/types.cuh:4294-4389), regular grid */
+static void scanning_set_grid(sim_t *s)
+{
+    free(s->sc_x); free(s->sc_y);
+    s->sc_x = s->sc_y = NULL; s->sc_n = 0;
+    if (s->sc_ns <= 0) { s->sc_ns = s->sc_nx = s->sc_ny = 0; return; }
+    s->sc_nx = s->sc_ny = s->sc_ns;
+    int ns = s->sc_ns, pbc = s->sc_pbc;
+    if (s->sc_type == 1) {
+        real xu = s->sc_xe - s->sc_x0, yu = s->sc_ye - s->sc_y0;
+        real ds = sqrt(yu * yu + xu * xu);
+        real theta = atan2(yu, xu);
+        real cos_theta = cos(theta);
+        cos_theta = is_zero_r(cos_theta) ? 0 : cos_theta;
+        real sin_theta = sin(theta);
+        s->sc_dRx = ds * cos_theta / (pbc ? ns : (ns - 1));
+        s->sc_dRy = ds * sin_theta / (pbc ? ns : (ns - 1));
+        s->sc_n = ns;
+        s->sc_x = malloc(sizeof(real) * (size_t)ns); s->sc_y = malloc(sizeof(real) * (size_t)ns);
+        for (int i = 0; i < ns; i++) { s->sc_x[i] = s->sc_x0 + i * s->sc_dRx; s->sc_y[i] = s->sc_y0 + i * s->sc_dRy; }
+        return;
+    }
+    real xu = s->sc_xe - s->sc_x0, yu = s->sc_ye - s->sc_y0;
+    if (fabs(xu) > fabs(yu)) {
+        s->sc_dRx = xu / (pbc ? ns : (ns - 1));
+        s->sc_dRy = copysign(s->sc_dRx, yu);
+        s->sc_ny = (int)floor(yu / s->sc_dRy + eps_rel_r + 0.5);
+        s->sc_ny += pbc ? 0 : 1;
+        if (!s->sc_spxs) s->sc_dRy = yu / (pbc ? s->sc_ny : (s->sc_ny - 1));
+    } else {
+        s->sc_dRy = yu / (pbc ? ns : (ns - 1));
+        s->sc_dRx = copysign(s->sc_dRy, xu);
+        s->sc_nx = (int)floor(xu / s->sc_dRx + eps_rel_r + 0.5);
+        s->sc_nx += pbc ? 0 : 1;
+        if (!s->sc_spxs) s->sc_dRx = xu / (pbc ? s->sc_nx : (s->sc_nx - 1));
+    }
+    s->sc_n = s->sc_nx * s->sc_ny;
+    s->sc_x = malloc(sizeof(real) * (size_t)(s->sc_n > 0 ? s->sc_n : 1));
+    s->sc_y = malloc(sizeof(real) * (size_t)(s->sc_n > 0 ? s->sc_n : 1));
+    for (int ix = 0; ix < s->sc_nx; ix++)
+        for (int iy = 0; iy < s->sc_ny; iy++) {
+            s->sc_x[ix * s->sc_ny + iy] = s->sc_x0 + ix * s->sc_dRx;
+            s->sc_y[ix * s->sc_ny + iy] = s->sc_y0 + iy * s->sc_dRy;
+        }
+}
+
+/* match_vectors (src/cpu_fcns.hpp:3613-3654) */
+static void match_vectors(const real *vr, int nvr, real **vs_io, int *nvs_io, real dv)
+{
+    real *vs = *vs_io;
+    int nvs = *nvs_io;
+    dv = fmax(dv, (real)0.1);
+    real vr_min = vr[0], vr_max = vr[0];
+    for (int i = 1; i < nvr; i++) { if (vr[i] < vr_min) vr_min = vr[i]; if (vr[i] > vr_max) vr_max = vr[i]; }
+    vr_min -= dv; vr_max += dv;
+    int m = 0;
+    for (int i = 0; i < nvs; i++) if (!((vs[i] < vr_min) || (vs[i] > vr_max))) vs[m++] = vs[i];
+    nvs = m;
+    char *used = calloc((size_t)nvr, 1);
+    real *A = malloc(sizeof(real) * (size_t)(nvs > 0 ? nvs : 1));
+    int nA = 0;
+    for (int i = 0; i < nvs; i++) {
+        real val = vs[i];
+        /* std::min_element with closest_element (src/cgpu_fcns.cuh:3089-3100) */
+        int imin = 0;
+        for (int k = 1; k < nvr; k++)
+            if (fabs(vr[k] - val - (real)1e-4) < fabs(vr[imin] - val)) imin = k;
+        if (!used[imin]) { used[imin] = 1; A[nA++] = vr[imin]; }
+    }
+    free(used);
+    free(vs);
+    *vs_io = A;
+    *nvs_io = nA;
+}
+
+static void sim_free(sim_t *s)
+{
+    if (!s) return;
+    free(s->thick); free(s->iw_x); free(s->iw_y); free(s->sc_x); free(s->sc_y);
+    free(s->det_g_inner); free(s->det_g_outer);
+    atoms_free(&s->in_atoms); atoms_free(&s->atoms_u); atoms_free(&s->atoms);
+    free(s->tab); free(s->z_plane); free(s->z_slice); free(s->slice); free(s->thk);
+    free(s);
+}
+
+static sim_t *sim_create(const mt_input_multislice *in, int *status)
+{
+    sim_t *s = calloc(1, sizeof(sim_t));
+    s->tab = calloc(c_nAtomsTypes, sizeof(atom_tab));
+    *status = MT_OK;
+#define FAIL(code, msg) do { set_err(msg); *status = code; sim_free(s); return NULL; } while (0)
+    s->simulation_type = in->simulation_type;
+    if (!(is_stem(s) || is_cbed(s) || is_ewfs(s) || is_ewrs(s)))
+        FAIL(MT_ERR_UNSUPPORTED, "simulation_type not provided (STEM 11, CBED 21, EWFS 51, EWRS 52)");
+    s->interaction_model = in->interaction_model;
+    if (s->interaction_model != 1) FAIL(MT_ERR_UNSUPPORTED, "only interaction_model 1 (multislice)");
+    s->potential_type = in->potential_type;
+    if (s->potential_type != 6) FAIL(MT_ERR_UNSUPPORTED, "only potential_type 6 (Lobato)");
+    s->pn_model = in->pn_model;
+    if (s->pn_model == 2) FAIL(MT_ERR_UNSUPPORTED, "absorptive potential model");
+    s->pn_coh_contrib = in->pn_coh_contrib;
+    s->pn_single_conf = in->pn_single_conf;
+    s->pn_nconf = in->pn_nconf;
+    /* FP_Dim::set (src/types.cuh:1687-1740) */
+    {
+        int d = in->pn_dim;
+        s->pn_dim_x = (d / 100) % 10 != 0;
+        s->pn_dim_y = (d / 10) % 10 != 0;
+        s->pn_dim_z = d % 10 != 0;
+        if (!(d == 111 || d == 110 || d == 101 || d == 11 || d == 100 || d == 10 || d == 1))
+            { s->pn_dim_x = 1; s->pn_dim_y = 1; s->pn_dim_z = 0; }
+    }
+    s->pn_seed = in->pn_seed;
+
+    real lx = (real)in->spec_lx, ly = (real)in->spec_ly, lz = (real)in->spec_lz, dz = (real)in->spec_dz;
+    if (in->spec_atoms_rows < 0 || (in->spec_atoms_rows > 0 && (!in->spec_atoms || in->spec_atoms_cols < 4)))
+        FAIL(MT_ERR_INVALID_ARGUMENT, "spec_atoms must be n x (>=4)");
+    /* Atom_Data::set_atoms(rows, cols, double*) (src/atomic_data_mt.hpp:320-350) */
+    {
+        int nr = in->spec_atoms_rows, nc = in->spec_atoms_cols;
+        atoms_alloc(&s->in_atoms, nr);
+        for (int i = 0; i < nr; i++) {
+            const double *A = in->spec_atoms;
+            s->in_atoms.Z[i] = (int)A[0 * nr + i];
+            s->in_atoms.x[i] = (real)A[1 * nr + i];
+            s->in_atoms.y[i] = (real)A[2 * nr + i];
+            s->in_atoms.z[i] = (real)A[3 * nr + i];
+            s->in_atoms.sigma[i] = (float)((nc > 4) ? A[4 * nr + i] : 0.085);
+            s->in_atoms.occ[i] = (float)((nc > 5) ? A[5 * nr + i] : 1.0);
+            s->in_atoms.region[i] = abs((int)((nc > 6) ? A[6 * nr + i] : 0));
+            s->in_atoms.charge[i] = (int)((nc > 7) ? A[7 * nr + i] : 0);
+            int Z = s->in_atoms.Z[i] % 1000;
+            if (Z < 1 || Z > 103) FAIL(MT_ERR_INVALID_ARGUMENT, "atomic number out of range");
+            if (s->in_atoms.charge[i] != 0) FAIL(MT_ERR_UNSUPPORTED, "ionic charges (Peng ion tables)");
+        }
+        s->in_atoms.l_x = lx; s->in_atoms.l_y = ly; s->in_atoms.l_z = lz; s->in_atoms.dz = dz;
+    }
+    if (fabs(in->spec_rot_theta) > 0) FAIL(MT_ERR_UNSUPPORTED, "specimen rotation");
+    s->thick_type = in->thick_type;
+    if (s->thick_type != 1) {
+        s->n_thick = in->n_thick;
+        s->thick = malloc(sizeof(real) * (size_t)(in->n_thick > 0 ? in->n_thick : 1));
+        for (int i = 0; i < in->n_thick; i++) s->thick[i] = (real)in->thick[i];
+    }
+    s->potential_slicing = in->potential_slicing;
+    if (!(s->potential_slicing == 1 || s->potential_slicing == 2))
+        FAIL(MT_ERR_UNSUPPORTED, "potential_slicing 3/4 (sub-slicing)");
+
+    /* Grid_2d::set_input_data (src/types.cuh:2154-2178), pbc_xy = true */
+    if (in->nx <= 0 || in->ny <= 0) FAIL(MT_ERR_INVALID_ARGUMENT, "nx, ny must be positive");
+    if (ilog2i(in->nx) < 0 || ilog2i(in->ny) < 0) FAIL(MT_ERR_UNSUPPORTED, "nx, ny must be powers of two");
+    {
+        grid_t *g = &s->g;
+        g->nx = in->nx; g->ny = in->ny; g->nxh = g->nx / 2; g->nyh = g->ny / 2;
+        g->lx = lx; g->ly = ly; g->dz = dz; g->bwl = in->bwl != 0; g->pbc_xy = 1;
+        g->dRx = g->lx / g->nx; g->dRy = g->ly / g->ny;
+        g->dgx = (real)1.0 / g->lx; g->dgy = (real)1.0 / g->ly;
+        real gmax = fmin((real)g->nxh * g->dgx, (real)g->nyh * g->dgy);
+        real glmax = 2.0 * gmax / 3.0;
+        g->gl2_max = rsquare(glmax);
+        real dg0 = 0.25, fg0 = 1e-02;
+        g->alpha = log(1.0 / fg0 - 1.0) / (rsquare(glmax + dg0) - g->gl2_max);
+    }
+
+    /* incident wave type (validate_incident_wave_type, input_multislice.cuh:439-449) */
+    s->iw_type = in->iw_type;
+    if (s->iw_type == 4) s->iw_type = (is_scanning_t(s->simulation_type) || is_cbed(s)) ? 2 : 1;
+    if (s->iw_type == 3) FAIL(MT_ERR_UNSUPPORTED, "user-defined incident wave");
+    s->n_iw = in->n_iw_xy;
+    s->iw_x = malloc(sizeof(real) * (size_t)(s->n_iw > 0 ? s->n_iw : 1));
+    s->iw_y = malloc(sizeof(real) * (size_t)(s->n_iw > 0 ? s->n_iw : 1));
+    for (int i = 0; i < s->n_iw; i++) { s->iw_x[i] = (real)in->iw_x[i]; s->iw_y[i] = (real)in->iw_y[i]; }
+
+    s->E_0 = (real)in->E_0;
+    s->theta = (real)(in->theta * c_deg_2_rad);
+    s->phi = (real)(in->phi * c_deg_2_rad);
+    if (fabs(in->theta) > 0) FAIL(MT_ERR_UNSUPPORTED, "tilted illumination");
+    s->illumination_model = in->illumination_model;
+    if (s->illumination_model == 4) FAIL(MT_ERR_UNSUPPORTED, "illumination_model 4 (full integration)");
+
+    lens_read(&s->cond, &in->cond_lens);
+    lens_set(&s->cond, s->E_0, &s->g);
+    lens_read(&s->obj, &in->obj_lens);
+    lens_set(&s->obj, s->E_0, &s->g);
+
+    if (is_scanning_t(s->simulation_type)) {
+        s->sc_type = in->scanning_type; s->sc_pbc = in->scanning_periodic != 0;
+        s->sc_spxs = in->scanning_square_pxs != 0; s->sc_ns = in->scanning_ns;
+        s->sc_x0 = (real)in->scanning_x0; s->sc_y0 = (real)in->scanning_y0;
+        s->sc_xe = (real)in->scanning_xe; s->sc_ye = (real)in->scanning_ye;
+        scanning_set_grid(s);
+    }
+    if (is_stem(s)) {
+        real lambda = get_lambda(s->E_0);
+        if (in->detector_type != 1) FAIL(MT_ERR_UNSUPPORTED, "only circular detectors");
+        s->n_det = in->n_detector;
+        s->det_g_inner = malloc(sizeof(real) * (size_t)(s->n_det > 0 ? s->n_det : 1));
+        s->det_g_outer = malloc(sizeof(real) * (size_t)(s->n_det > 0 ? s->n_det : 1));
+        for (int i = 0; i < s->n_det; i++) {
+            real ia = (real)(in->detector_cir[i].inner_ang * c_mrad_2_rad);
+            real oa = (real)(in->detector_cir[i].outer_ang * c_mrad_2_rad);
+            s->det_g_inner[i] = sin(ia) / lambda;
+            s->det_g_outer[i] = sin(oa) / lambda;
+        }
+    }
+    s->oa_ix_0 = in->output_area_ix_0 - 1; s->oa_iy_0 = in->output_area_iy_0 - 1;
+    s->oa_ix_e = in->output_area_ix_e - 1; s->oa_iy_e = in->output_area_iy_e - 1;
+
+    /* ---- validate_parameters (src/input_multislice.cuh:622-796) ---- */
+    s->pn_seed = (s->pn_seed > 1) ? s->pn_seed : 1;
+    s->pn_nconf = !is_frozen(s) ? 1 : ((s->pn_nconf > 1) ? s->pn_nconf : 1);
+    s->fp_iconf_0 = !is_frozen(s) ? 1 : (s->pn_single_conf ? s->pn_nconf : 1);
+    s->Vrl = c_Vrl; s->nR = c_nR;
+    if (!is_scanning_t(s->simulation_type)) {
+        s->sc_type = 1; s->sc_pbc = 0; s->sc_spxs = 1; s->sc_ns = 1;
+        s->sc_x0 = s->sc_y0 = s->sc_xe = s->sc_ye = 0;
+    }
+    scanning_set_grid(s);
+    s->lambda = get_lambda(s->E_0);
+    s->mul_sign = in->reverse_multislice ? -1 : 1;
+    {
+        /* set_incident_angle (input_multislice.cuh:891-895) */
+        real dgmin = fmin(s->g.dgx, s->g.dgy);
+        real n = round(sin(s->theta) / (s->lambda * dgmin));
+        s->theta = is_zero_r(s->theta) ? 0 : asin(n * s->lambda * dgmin);
+    }
+    if (is_ewfs(s) || is_ewrs(s)) { s->pn_coh_contrib = 1; s->illumination_model = 1; }
+    if (is_cbed(s)) s->illumination_model = 1;
+    s->pn_dim_z = 0;  /* not sub-slicing */
+
+    /* match_thickness (src/slicing.hpp:53-71) */
+    atoms_statistic(s, &s->in_atoms, 0);
+    if (s->thick_type == 1) {
+        free(s->thick);
+        s->n_thick = 1;
+        s->thick = malloc(sizeof(real));
+        s->thick[0] = s->in_atoms.z_max;
+    } else {
+        int np;
+        atoms_t tmp;
+        atoms_alloc(&tmp, s->in_atoms.n);
+        for (int i = 0; i < tmp.n; i++) atoms_copy_row(&tmp, i, &s->in_atoms, i);
+        tmp.dz = s->in_atoms.dz;
+        atoms_statistic(s, &tmp, 0);
+        real *zp = get_z_plane(s->potential_slicing, &tmp, &np);
+        if (s->thick_type == 3) {
+            int nz;
+            real *zs = get_z_slice(s->potential_slicing, zp, np, &tmp, &nz);
+            free(zp); zp = zs; np = nz;
+        }
+        match_vectors(zp, np, &s->thick, &s->n_thick, (real)(0.25 * tmp.dz));
+        free(zp);
+        atoms_free(&tmp);
+        if (s->n_thick == 0) FAIL(MT_ERR_INVALID_ARGUMENT, "no thickness matches the specimen");
+    }
+    /* remove atoms outside the thickness range */
+    {
+        real ee_z = 0.1;
+        real z_e = s->thick[s->n_thick - 1] + ee_z;
+        if (s->in_atoms.z_max > z_e) {
+            real z_0 = s->in_atoms.z_min - ee_z;
+            int m = 0;
+            for (int i = 0; i < s->in_atoms.n; i++)
+                if ((z_0 < s->in_atoms.z[i]) && (s->in_atoms.z[i] < z_e)) atoms_copy_row(&s->in_atoms, m++, &s->in_atoms, i);
+            s->in_atoms.n = m;
+            atoms_statistic(s, &s->in_atoms, 0);
+        }
+    }
+    s->cond.zero_defocus_plane = (s->in_atoms.n == 0) ? 0
+        : lens_zero_defocus_plane(&s->cond, s->in_atoms.z_min, s->in_atoms.z_max);
+    s->cond.zero_defocus_type = 4;
+    if (is_stem(s)) { s->oa_ix_0 = 0; s->oa_ix_e = s->g.nx; s->oa_iy_0 = 0; s->oa_iy_e = s->g.ny; }
+    else {
+        if (s->oa_ix_0 == s->oa_ix_e) { s->oa_ix_0 = 0; s->oa_ix_e = s->g.nx; }
+        if (s->oa_iy_0 == s->oa_iy_e) { s->oa_iy_0 = 0; s->oa_iy_e = s->g.ny; }
+        if (s->oa_ix_0 > s->oa_ix_e) { int t = s->oa_ix_0; s->oa_ix_0 = s->oa_ix_e; s->oa_ix_e = t; }
+        if (s->oa_iy_0 > s->oa_iy_e) { int t = s->oa_iy_0; s->oa_iy_0 = s->oa_iy_e; s->oa_iy_e = t; }
+        s->oa_ix_0 = (s->oa_ix_0 < 0) ? 0 : ((s->oa_ix_0 > s->g.nx) ? s->g.nx : s->oa_ix_0);
+        s->oa_ix_e = (s->oa_ix_e < 0) ? 0 : ((s->oa_ix_e > s->g.nx) ? s->g.nx : s->oa_ix_e);
+        s->oa_iy_0 = (s->oa_iy_0 < 0) ? 0 : ((s->oa_iy_0 > s->g.ny) ? s->g.ny : s->oa_iy_0);
+        s->oa_iy_e = (s->oa_iy_e < 0) ? 0 : ((s->oa_iy_e > s->g.ny) ? s->g.ny : s->oa_iy_e);
+    }
+
+    /* ---- Spec::set_input_data (src/spec.hpp:44-82) ---- */
+    {
+        /* atoms_u.set_atoms(input atoms, pbc_xy, atom_type) + sort_by_z */
+        atoms_t *a = &s->in_atoms;
+        real dl = 1e-04;
+        real lx_b = a->l_x - dl, ly_b = a->l_y - dl;
+        atoms_alloc(&s->atoms_u, a->n);
+        int m = 0;
+        for (int i = 0; i < a->n; i++)
+            if ((a->x[i] < lx_b) && (a->y[i] < ly_b)) atoms_copy_row(&s->atoms_u, m++, a, i);
+        s->atoms_u.n = m;
+        s->atoms_u.l_x = a->l_x; s->atoms_u.l_y = a->l_y; s->atoms_u.l_z = a->l_z; s->atoms_u.dz = a->dz;
+        if (m == 0) FAIL(MT_ERR_INVALID_ARGUMENT, "no atoms inside the simulation box");
+        atoms_statistic(s, &s->atoms_u, 1);
+        atoms_sort_by_z(&s->atoms_u);
+        atoms_statistic(s, &s->atoms_u, 1);
+        s->z_plane = get_z_plane(s->potential_slicing, &s->atoms_u, &s->n_plane);
+        if ((s->atoms_u.s_z_int < 2.0 * s->g.dz) || (s->n_plane == 1 && s->potential_slicing == 1))
+            FAIL(MT_ERR_UNSUPPORTED, "specimen reduces to a single phase object");
+        atoms_alloc(&s->atoms, s->atoms_u.n);
+        for (int i = 0; i < s->atoms_u.n; i++) atoms_copy_row(&s->atoms, i, &s->atoms_u, i);
+        s->atoms.l_x = s->atoms_u.l_x; s->atoms.l_y = s->atoms_u.l_y; s->atoms.l_z = s->atoms_u.l_z;
+        s->atoms.dz = s->atoms_u.dz;
+        atoms_statistic(s, &s->atoms, 1);
+        slicing_calculate(s);
+    }
+#undef FAIL
+    return s;
+}
+
+/* Spec::move_atoms (src/spec.hpp:85-127) */
+static void move_atoms(sim_t *s, int iconf)
+{
+    randn3 rnd;
+    int fp = is_frozen(s);
+    if (fp) {
+        randn3_seed(&rnd, s->pn_seed, iconf);
+        rnd.bx = s->pn_dim_x; rnd.by = s->pn_dim_y; rnd.bz = s->pn_dim_z;
+    }
+    for (int i = 0; i < s->atoms_u.n; i++) {
+        atoms_copy_row(&s->atoms, i, &s->atoms_u, i);
+        if (fp) {
+            real sg = s->atoms_u.sigma[i];
+            real dx = rnd.bx ? sg * normal_draw(&rnd.gx) : 0;
+            real dy = rnd.by ? sg * normal_draw(&rnd.gy) : 0;
+            real dzv = rnd.bz ? sg * normal_draw(&rnd.gz) : 0;
+            s->atoms.x[i] = s->atoms_u.x[i] + dx;
+            s->atoms.y[i] = s->atoms_u.y[i] + dy;
+            s->atoms.z[i] = s->atoms_u.z[i] + dzv;
+        }
+    }
+    atoms_statistic(s, &s->atoms, 1);
+    slicing_calculate(s);
+}
+
+/* ------------------------------------------------------------------ */
+/* Per-slice pieces                                                    */
+/* ------------------------------------------------------------------ */
+static inline int iRx_shift(const grid_t *g, int ix) { return (ix < g->nxh) ? ix + g->nxh : ix - g->nxh; }
+static inline int iRy_shift(const grid_t *g, int iy) { return (iy < g->nyh) ? iy + g->nyh : iy - g->nyh; }
+
+/* get_bn (src/cpu_fcns.hpp:1669-1684), pbc */
+static void get_bn(real R, real dR, real R_max, int *i0, int *n)
+{
+    int iR_0 = (int)floor((R - R_max) / dR);
+    int iR_e = (int)ceil((R + R_max) / dR);
+    *i0 = iR_0;
+    *n = (iR_0 == iR_e) ? 0 : iR_e - iR_0 + 1;
+}
+
+/* Projected potential of a slice, fft-shifted column-major layout:
+   Projected_Potential::operator() (src/projected_potential.cuh:89-124)
+   with host_detail::eval_cubic_poly (src/cpu_fcns.hpp:234-264).  V (nx*ny) */
+static int slice_potential(sim_t *s, int islice, real *V)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    for (size_t i = 0; i < nxy; i++) V[i] = 0;
+    if (islice < 0 || islice >= s->n_slice) return MT_OK;
+    const slice_t *sl = &s->slice[islice];
+    for (int ia = sl->iatom_0; ia <= sl->iatom_e; ia++) {
+        const atom_tab *t = get_tab(s, s->atoms.Z[ia] % 1000);
+        real x = s->atoms.x[ia], y = s->atoms.y[ia], occ = s->atoms.occ[ia];
+        int ix0, nxw, iy0, nyw;
+        get_bn(x, g->dRx, t->R_max, &ix0, &nxw);
+        get_bn(y, g->dRy, t->R_max, &iy0, &nyw);
+        for (int ixw = 0; ixw < nxw; ixw++) {
+            for (int iyw = 0; iyw < nyw; iyw++) {
+                int ix = ixw + ix0, iy = iyw + iy0;
+                real Rx = ix * g->dRx + 0 - x, Ry = iy * g->dRy + 0 - y;
+                const real R2 = Rx * Rx + Ry * Ry;
+                if (R2 < t->R2_max) {
+                    const real v = occ * eval_spline(R2, t);
+                    /* ind_col_pbc_shift (src/types.cuh:2228-2234) */
+                    int px = ix - (int)floor((ix * g->dRx + 0) / g->lx) * g->nx;
+                    int py = iy - (int)floor((iy * g->dRy + 0) / g->ly) * g->ny;
+                    px = iRx_shift(g, px); py = iRy_shift(g, py);
+                    if (px < 0 || px >= g->nx || py < 0 || py >= g->ny) {
+                        set_err("potential index out of range");
+                        return MT_ERR_INVALID_ARGUMENT;
+                    }
+                    V[(size_t)px * g->ny + py] += v;
+                }
+            }
+        }
+    }
+    return MT_OK;
+}
+
+/* g2 in natural FFT order: Grid_2d::g2_shift (src/types.cuh:2549-2552) */
+static inline real g2_shift(const grid_t *g, int ix, int iy, real gx0, real gy0)
+{
+    int igx = (ix < g->nxh) ? ix : ix - g->nx;
+    int igy = (iy < g->nyh) ? iy : iy - g->ny;
+    real gx = igx * g->dgx - gx0, gy = igy * g->dgy - gy0;
+    return gx * gx + gy * gy;
+}
+static inline real bwl_factor_shift(const grid_t *g, int ix, int iy)
+{
+    return g->bwl ? (real)(1.0 / (1.0 + exp(g->alpha * (g2_shift(g, ix, iy, 0, 0) - g->gl2_max)))) : (real)1.0;
+}
+
+/* bandwidth_limit (src/cgpu_fcns.cuh:1816-1822) */
+static void bandwidth_limit(const grid_t *g, cplx *M)
+{
+    for (int ix = 0; ix < g->nx; ix++)
+        for (int iy = 0; iy < g->ny; iy++) {
+            real f = bwl_factor_shift(g, ix, iy) / (real)(g->nx * g->ny);
+            cplx *m = &M[(size_t)ix * g->ny + iy];
+            m->re *= f; m->im *= f;
+        }
+}
+
+/* Transmission_Function::trans (src/transmission_function.cuh:72-82) with
+   the transmission functor (src/cgpu_fcns.cuh:3071-3086) */
+static void slice_transmission(sim_t *s, const real *V, cplx *T, const fft2 *F, cplx *work)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    real w = s->mul_sign * get_Vr_factor(s->E_0, s->theta);
+    for (size_t i = 0; i < nxy; i++) {
+        real theta = w * V[i];
+        T[i].re = cos(theta);
+        T[i].im = sin(theta);
+    }
+    if (g->bwl) {
+        fft2_exec(F, T, 0, work);
+        bandwidth_limit(g, T);
+        fft2_exec(F, T, 1, work);
+    }
+}
+
+/* propagate (src/cgpu_fcns.cuh:1835-1845) applied in Fourier space */
+static void propagate_k(const sim_t *s, real z, cplx *M)
+{
+    const grid_t *g = &s->g;
+    real w = (real)(-s->mul_sign * c_Pi * s->lambda * z);
+    real gx0 = sin(s->theta) * cos(s->phi) / s->lambda, gy0 = sin(s->theta) * sin(s->phi) / s->lambda;
+    for (int ix = 0; ix < g->nx; ix++)
+        for (int iy = 0; iy < g->ny; iy++) {
+            const real m = bwl_factor_shift(g, ix, iy) / (real)(g->nx * g->ny);
+            const real th = w * g2_shift(g, ix, iy, gx0, gy0);
+            cplx p = { m * cos(th), m * sin(th) };
+            cplx *v = &M[(size_t)ix * g->ny + iy];
+            cplx r = { p.re * v->re - p.im * v->im, p.re * v->im + p.im * v->re };
+            *v = r;
+        }
+}
+
+/* Propagator::operator() (src/propagator.cuh:52-87); space 1 real, 2 reciprocal */
+static void propagator(const sim_t *s, int space, real z, cplx *psi, const fft2 *F, cplx *work)
+{
+    const grid_t *g = &s->g;
+    if (is_zero_r(z)) {
+        if (g->bwl) {
+            fft2_exec(F, psi, 0, work);
+            bandwidth_limit(g, psi);
+            if (space == 1) fft2_exec(F, psi, 1, work);
+        } else if (space == 2) {
+            fft2_exec(F, psi, 0, work);
+            real f = (real)(1.0 / (real)(g->nx * g->ny));
+            size_t nxy = (size_t)g->nx * g->ny;
+            for (size_t i = 0; i < nxy; i++) { psi[i].re *= f; psi[i].im *= f; }
+        }
+    } else {
+        fft2_exec(F, psi, 0, work);
+        propagate_k(s, z, psi);
+        if (space == 1) fft2_exec(F, psi, 1, work);
+    }
+}
+
+/* Convergent probe: Incident_Wave::operator() (src/incident_wave.cuh:72-91),
+   exp_i_chi (src/cgpu_fcns.cuh:1949-1979); psi in shifted real space */
+static void incident_wave(sim_t *s, const real *xb, const real *yb, int nb, cplx *psi,
+                          const fft2 *F, cplx *work, cplx *fpsi)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    if (s->iw_type == 1) {
+        for (size_t i = 0; i < nxy; i++) { psi[i].re = 1; psi[i].im = 0; }
+        return;
+    }
+    real z_init = (s->n_slice > 0) ? (real)(0.5 * (s->slice[0].z_0 + s->slice[0].z_e)) : 0;
+    lens_t lens = s->cond;
+    real f_0 = lens.c_10;
+    real f_s = f_0 - (lens.zero_defocus_plane - z_init);
+    lens_set_defocus(&lens, f_s);
+    for (size_t i = 0; i < nxy; i++) { psi[i].re = 0; psi[i].im = 0; }
+    const real c2pi = 6.283185307179586476925;
+    int phireq = lens_phi_required(&lens);
+    for (int ib = 0; ib < nb; ib++) {
+        real x = -c2pi * (xb[ib] - 0.5 * g->lx);
+        real y = -c2pi * (yb[ib] - 0.5 * g->ly);
+        double total = 0;
+        for (int ix = 0; ix < g->nx; ix++)
+            for (int iy = 0; iy < g->ny; iy++) {
+                int igx = (ix < g->nxh) ? ix : ix - g->nx;
+                int igy = (iy < g->nyh) ? iy : iy - g->ny;
+                real gx = igx * g->dgx + 0, gy = igy * g->dgy + 0;
+                real g2 = gx * gx + gy * gy;
+                cplx v = { 0, 0 };
+                if ((lens.g2_min <= g2) && (g2 < lens.g2_max)) {
+                    real g4 = g2 * g2, g6 = g4 * g2;
+                    real chi = x * gx + y * gy + (is_zero_r(lens.c_10) ? 0 : lens.c_c_10 * g2)
+                               + (is_zero_r(lens.c_30) ? 0 : lens.c_c_30 * g4)
+                               + (is_zero_r(lens.c_50) ? 0 : lens.c_c_50 * g6);
+                    if (phireq) {
+                        real gg = sqrt(g2), g3 = g2 * gg, g5 = g4 * gg;
+                        real ph = atan2(gy, gx);
+                        chi += (is_zero_r(ph) ? 0 : lens.m * ph);
+                        chi += is_zero_r(lens.c_12) ? 0 : lens.c_c_12 * g2 * sin(2 * (ph - lens.phi_12));
+                        chi += (is_zero_r(lens.c_21) ? 0 : lens.c_c_21 * g3 * sin(ph - lens.phi_21))
+                               + (is_zero_r(lens.c_23) ? 0 : lens.c_c_23 * g3 * sin(3 * (ph - lens.phi_23)));
+                        chi += (is_zero_r(lens.c_32) ? 0 : lens.c_c_32 * g4 * sin(2 * (ph - lens.phi_32)))
+                               + (is_zero_r(lens.c_34) ? 0 : lens.c_c_34 * g4 * sin(4 * (ph - lens.phi_34)));
+                        chi += (is_zero_r(lens.c_41) ? 0 : lens.c_c_41 * g5 * sin(ph - lens.phi_41))
+                               + (is_zero_r(lens.c_43) ? 0 : lens.c_c_43 * g5 * sin(3 * (ph - lens.phi_43)))
+                               + (is_zero_r(lens.c_45) ? 0 : lens.c_c_45 * g5 * sin(5 * (ph - lens.phi_45)));
+                        chi += (is_zero_r(lens.c_52) ? 0 : lens.c_c_52 * g6 * sin(2 * (ph - lens.phi_52)))
+                               + (is_zero_r(lens.c_54) ? 0 : lens.c_c_54 * g6 * sin(4 * (ph - lens.phi_54)))
+                               + (is_zero_r(lens.c_56) ? 0 : lens.c_c_56 * g6 * sin(6 * (ph - lens.phi_56)));
+                    }
+                    v.re = cos(chi); v.im = sin(chi);
+                }
+                fpsi[(size_t)ix * g->ny + iy] = v;
+                total += (double)(v.re * v.re + v.im * v.im);
+            }
+        /* probe(): scale by sqrt(1/sum|fpsi|^2) (src/gpu_fcns.cuh:1612-1625) */
+        real sc = (real)sqrt(1.0 / (real)total);
+        for (size_t i = 0; i < nxy; i++) { psi[i].re += fpsi[i].re * sc; psi[i].im += fpsi[i].im * sc; }
+    }
+    fft2_exec(F, psi, 1, work);
+}
+
+/* ------------------------------------------------------------------ */
+/* Engine run                                                          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    sim_t *s;
+    cplx *trans;            /* n_slice x nxy, one configuration */
+    int n_work;             /* number of (probe) work items */
+    const int *items;       /* scan indices */
+    double *result;         /* [item][n_thk][n_det] */
+    int next;
+    pthread_mutex_t lock;
+    int status;
+} stem_job;
+
+/* Wave_Function::psi for one probe with detector sums (wave_function.cuh:107-156,240-252) */
+static void *stem_worker(void *arg)
+{
+    stem_job *J = arg;
+    sim_t *s = J->s;
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    fft2 F;
+    fft2_init(&F, g->nx, g->ny);
+    cplx *psi = malloc(sizeof(cplx) * nxy), *fpsi = malloc(sizeof(cplx) * nxy);
+    cplx *po = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    for (;;) {
+        pthread_mutex_lock(&J->lock);
+        int it = J->next++;
+        pthread_mutex_unlock(&J->lock);
+        if (it >= J->n_work) break;
+        int iscan = J->items[it];
+        real xb = s->sc_x[iscan], yb = s->sc_y[iscan];
+        incident_wave(s, &xb, &yb, 1, psi, &F, work, fpsi);
+        for (int is = 0; is < s->n_slice; is++) {
+            const cplx *T = J->trans + (size_t)is * nxy;
+            for (size_t i = 0; i < nxy; i++) {
+                cplx a = psi[i], b = T[i];
+                psi[i].re = b.re * a.re - b.im * a.im;
+                psi[i].im = b.re * a.im + b.im * a.re;
+            }
+            real dzs = (s->slice[is].z_e - s->slice[is].z_0) / cos(s->theta);
+            propagator(s, 1, dzs, psi, &F, work);
+            int ithk = s->slice[is].ithk;
+            if (ithk >= 0) {
+                memcpy(po, psi, sizeof(cplx) * nxy);
+                propagator(s, 2, s->thk[ithk].z_back_prop, po, &F, work);
+                for (int id = 0; id < s->n_det; id++) {
+                    real g2_min = (real)pow(s->det_g_inner[id], 2), g2_max = (real)pow(s->det_g_outer[id], 2);
+                    real sum = 0;
+                    for (int ix = 0; ix < g->nx; ix++)
+                        for (int iy = 0; iy < g->ny; iy++) {
+                            real g2 = g2_shift(g, ix, iy, 0, 0);
+                            if ((g2_min <= g2) && (g2 < g2_max)) {
+                                cplx v = po[(size_t)ix * g->ny + iy];
+                                sum += v.re * v.re + v.im * v.im;
+                            }
+                        }
+                    J->result[((size_t)it * s->n_thk + ithk) * s->n_det + id] = (double)sum;
+                }
+            }
+        }
+    }
+    free(psi); free(fpsi); free(po); free(work);
+    fft2_free(&F);
+    return NULL;
+}
+
+static int build_trans_stack(sim_t *s, cplx *trans)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    fft2 F;
+    fft2_init(&F, g->nx, g->ny);
+    real *V = malloc(sizeof(real) * nxy);
+    cplx *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    int st = MT_OK;
+    for (int is = 0; is < s->n_slice && st == MT_OK; is++) {
+        st = slice_potential(s, is, V);
+        if (st == MT_OK) slice_transmission(s, V, trans + (size_t)is * nxy, &F, work);
+    }
+    free(V); free(work);
+    fft2_free(&F);
+    return st;
+}
+
+static void output_dims_from_sim(const sim_t *s, mt_output_multislice *out)
+{
+    out->n_thk = s->n_thk;
+    if (is_stem(s)) {
+        out->output_type = s->pn_coh_contrib ? MT_OT_IMAGE_TOT_COH : MT_OT_IMAGE_TOT;
+        out->nx = s->sc_nx; out->ny = s->sc_ny; out->n_det = s->n_det;
+        out->dx = s->sc_dRx; out->dy = s->sc_dRy;
+    } else {
+        out->nx = s->oa_ix_e - s->oa_ix_0; out->ny = s->oa_iy_e - s->oa_iy_0; out->n_det = 0;
+        int rs = is_ewrs(s);
+        out->dx = rs ? s->g.dRx : s->g.dgx;
+        out->dy = rs ? s->g.dRy : s->g.dgy;
+        if (is_cbed(s)) out->output_type = s->pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
+        else {
+            int sc = !is_frozen(s) || s->pn_single_conf;
+            out->output_type = sc ? MT_OT_PSI_COH : MT_OT_M2PSI_TOT_PSI_COH;
+        }
+    }
+}
+
+int orc_output_dims(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
+{
+    int st;
+    (void)sys;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    output_dims_from_sim(s, out);
+    sim_free(s);
+    return MT_OK;
+}
+
+static void fill_axes(const sim_t *s, mt_output_multislice *out)
+{
+    if (out->thick) for (int i = 0; i < s->n_thk; i++) out->thick[i] = s->thk[i].z;
+    if (is_stem(s)) {
+        if (out->x) for (int i = 0; i < s->sc_nx; i++) out->x[i] = s->sc_x0 + i * s->sc_dRx;
+        if (out->y) for (int i = 0; i < s->sc_ny; i++) out->y[i] = s->sc_y0 + i * s->sc_dRy;
+    } else {
+        int rs = is_ewrs(s);
+        if (out->x) for (int ix = s->oa_ix_0; ix < s->oa_ix_e; ix++)
+            out->x[ix - s->oa_ix_0] = rs ? ix * s->g.dRx : (ix - s->g.nxh) * s->g.dgx;
+        if (out->y) for (int iy = s->oa_iy_0; iy < s->oa_iy_e; iy++)
+            out->y[iy - s->oa_iy_0] = rs ? iy * s->g.dRy : (iy - s->g.nyh) * s->g.dgy;
+    }
+}
+
+static int run_stem(sim_t *s, mt_output_multislice *out, int nthreads)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    int nscan = s->sc_n;
+    size_t nimg = (size_t)s->n_thk * s->n_det * nscan;
+    real *img = calloc(nimg > 0 ? nimg : 1, sizeof(real));
+    cplx *trans = malloc(sizeof(cplx) * nxy * (size_t)s->n_slice);
+    double *res = malloc(sizeof(double) * (size_t)nscan * s->n_thk * (s->n_det > 0 ? s->n_det : 1));
+    int *items = malloc(sizeof(int) * (size_t)(nscan > 0 ? nscan : 1));
+    if (!img || !trans || !res || !items) { free(img); free(trans); free(res); free(items); set_err("out of memory"); return MT_ERR_OUT_OF_MEMORY; }
+    for (int i = 0; i < nscan; i++) items[i] = i;
+    int nconf_w = (!is_frozen(s) || s->pn_single_conf) ? 1 : s->pn_nconf;
+    real w = (real)(1.0 / (real)(nconf_w * 1));
+    int st = MT_OK;
+    for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
+        move_atoms(s, iconf);
+        st = build_trans_stack(s, trans);
+        if (st != MT_OK) break;
+        stem_job J = { s, trans, nscan, items, res, 0, PTHREAD_MUTEX_INITIALIZER, 0 };
+        int nt = nthreads > 0 ? nthreads : 1;
+        pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nt);
+        for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, stem_worker, &J);
+        for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+        free(th);
+        /* image_tot[ithk].image[iDet][iscan] += w*I, in probe order (wave_function.cuh:145-151) */
+        for (int iscan = 0; iscan < nscan; iscan++)
+            for (int ithk = 0; ithk < s->n_thk; ithk++)
+                for (int id = 0; id < s->n_det; id++) {
+                    real val = (real)res[((size_t)iscan * s->n_thk + ithk) * s->n_det + id];
+                    img[((size_t)ithk * s->n_det + id) * nscan + iscan] += w * val;
+                }
+    }
+    if (st == MT_OK && out->image_tot)
+        for (size_t i = 0; i < nimg; i++) out->image_tot[i] = img[i];
+    free(img); free(trans); free(res); free(items);
+    return st;
+}
+
+/* crop-shift output index: out(ix_o, iy_o) <- in(fftshift) (cgpu_fcns.cuh:1664-1778) */
+static void accumulate_output(const sim_t *s, const cplx *po, double w, int square, int set,
+                              double *dst, int complex_out)
+{
+    const grid_t *g = &s->g;
+    int nyo = s->oa_iy_e - s->oa_iy_0;
+    for (int ix = s->oa_ix_0; ix < s->oa_ix_e; ix++)
+        for (int iy = s->oa_iy_0; iy < s->oa_iy_e; iy++) {
+            int sx = (ix < g->nxh) ? ix + g->nxh : ix - g->nxh;
+            int sy = (iy < g->nyh) ? iy + g->nyh : iy - g->nyh;
+            cplx v = po[(size_t)sx * g->ny + sy];
+            size_t o = (size_t)(ix - s->oa_ix_0) * nyo + (iy - s->oa_iy_0);
+            if (complex_out) {
+                if (set) { dst[2 * o] = v.re; dst[2 * o + 1] = v.im; }
+                else { dst[2 * o] += w * v.re; dst[2 * o + 1] += w * v.im; }
+            } else {
+                double m2 = square ? (double)(v.re * v.re + v.im * v.im) : (double)v.re;
+                if (set) dst[o] = m2; else dst[o] += w * m2;
+            }
+        }
+}
+
+typedef struct {
+    sim_t *s;
+    int iconf;
+    cplx **po;           /* [n_thk] k-space or real-space wave at each thickness */
+    int status;
+} ew_conf;
+
+/* EWFS_EWRS (tem_simulation.cuh:303-327) for one configuration */
+static int run_ew_conf(sim_t *s, cplx **po_thk)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    fft2 F;
+    fft2_init(&F, g->nx, g->ny);
+    cplx *trans = malloc(sizeof(cplx) * nxy * (size_t)s->n_slice);
+    cplx *psi = malloc(sizeof(cplx) * nxy), *fpsi = malloc(sizeof(cplx) * nxy);
+    cplx *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    int st = build_trans_stack(s, trans);
+    if (st == MT_OK) {
+        incident_wave(s, s->iw_x, s->iw_y, s->n_iw, psi, &F, work, fpsi);
+        int space = is_ewrs(s) ? 1 : 2;
+        for (int is = 0; is < s->n_slice; is++) {
+            const cplx *T = trans + (size_t)is * nxy;
+            for (size_t i = 0; i < nxy; i++) {
+                cplx a = psi[i], b = T[i];
+                psi[i].re = b.re * a.re - b.im * a.im;
+                psi[i].im = b.re * a.im + b.im * a.re;
+            }
+            real dzs = (s->slice[is].z_e - s->slice[is].z_0) / cos(s->theta);
+            propagator(s, 1, dzs, psi, &F, work);
+            int ithk = s->slice[is].ithk;
+            if (ithk >= 0) {
+                memcpy(po_thk[ithk], psi, sizeof(cplx) * nxy);
+                propagator(s, space, s->thk[ithk].z_back_prop, po_thk[ithk], &F, work);
+            }
+        }
+    }
+    free(trans); free(psi); free(fpsi); free(work);
+    fft2_free(&F);
+    return st;
+}
+
+static int run_ew(sim_t *s, mt_output_multislice *out)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    int nxo = s->oa_ix_e - s->oa_ix_0, nyo = s->oa_iy_e - s->oa_iy_0;
+    size_t no = (size_t)nxo * nyo;
+    int nconf_w = (!is_frozen(s) || s->pn_single_conf) ? 1 : s->pn_nconf;
+    double w = (double)(real)(1.0 / (real)nconf_w);
+    mt_output_multislice dims;
+    memset(&dims, 0, sizeof dims);
+    output_dims_from_sim(s, &dims);
+    int ot = dims.output_type;
+    double *m2 = calloc(no * (size_t)s->n_thk, sizeof(double));
+    double *pc = calloc(2 * nxy * (size_t)s->n_thk, sizeof(double));   /* psi_coh, uncropped for CBED */
+    double *pcs = calloc(2 * no * (size_t)s->n_thk, sizeof(double));
+    cplx **po = malloc(sizeof(cplx *) * (size_t)s->n_thk);
+    for (int k = 0; k < s->n_thk; k++) po[k] = malloc(sizeof(cplx) * nxy);
+    int st = MT_OK;
+    for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
+        move_atoms(s, iconf);
+        st = run_ew_conf(s, po);
+        if (st != MT_OK) break;
+        for (int k = 0; k < s->n_thk; k++) {
+            if (ot == MT_OT_PSI_COH) {
+                accumulate_output(s, po[k], 1.0, 0, 1, pcs + 2 * no * k, 1);
+            } else if (ot == MT_OT_M2PSI_TOT_PSI_COH) {
+                accumulate_output(s, po[k], w, 1, 0, m2 + no * k, 0);
+                accumulate_output(s, po[k], w, 0, 0, pcs + 2 * no * k, 1);
+            } else {
+                accumulate_output(s, po[k], w, 1, 0, m2 + no * k, 0);
+                if (s->pn_coh_contrib)
+                    for (size_t i = 0; i < nxy; i++) {
+                        pc[2 * (nxy * k + i)] += w * po[k][i].re;
+                        pc[2 * (nxy * k + i) + 1] += w * po[k][i].im;
+                    }
+            }
+        }
+    }
+    if (st == MT_OK) {
+        if (ot == MT_OT_M2PSI_TOT_COH && out->m2psi_coh) {
+            /* set_m2psi_coh (wave_function.cuh:188-227): |psi_coh|^2 */
+            cplx *tmp = malloc(sizeof(cplx) * nxy);
+            for (int k = 0; k < s->n_thk; k++) {
+                for (size_t i = 0; i < nxy; i++) { tmp[i].re = (real)pc[2 * (nxy * k + i)]; tmp[i].im = (real)pc[2 * (nxy * k + i) + 1]; }
+                accumulate_output(s, tmp, 1.0, 1, 1, out->m2psi_coh + no * k, 0);
+            }
+            free(tmp);
+        }
+        if (out->m2psi_tot && (ot == MT_OT_M2PSI_TOT || ot == MT_OT_M2PSI_TOT_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
+            memcpy(out->m2psi_tot, m2, sizeof(double) * no * (size_t)s->n_thk);
+        if (out->psi_coh && (ot == MT_OT_PSI_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
+            memcpy(out->psi_coh, pcs, sizeof(double) * 2 * no * (size_t)s->n_thk);
+    }
+    for (int k = 0; k < s->n_thk; k++) free(po[k]);
+    free(po); free(m2); free(pc); free(pcs);
+    return st;
+}
+
+int orc_run(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out, int nthreads)
+{
+    int st;
+    (void)sys;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    mt_output_multislice d;
+    memset(&d, 0, sizeof d);
+    output_dims_from_sim(s, &d);
+    out->output_type = d.output_type; out->nx = d.nx; out->ny = d.ny; out->n_thk = d.n_thk;
+    out->n_det = d.n_det; out->dx = d.dx; out->dy = d.dy;
+    fill_axes(s, out);
+    st = is_stem(s) ? run_stem(s, out, nthreads) : run_ew(s, out);
+    sim_free(s);
+    return st;
+}
+
+/* ---- per-piece entries ---- */
+int orc_spec_slicing(const mt_system_conf *sys, const mt_input_multislice *in, int iconf,
+                     mt_slice *slices, int max_slice, int *n_slice)
+{
+    int st;
+    (void)sys;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    if (iconf >= 1 && is_frozen(s)) move_atoms(s, iconf);
+    *n_slice = s->n_slice;
+    for (int i = 0; i < s->n_slice && i < max_slice && slices; i++) {
+        slices[i].z_0 = s->slice[i].z_0; slices[i].z_e = s->slice[i].z_e;
+        slices[i].z_int_0 = s->slice[i].z_int_0; slices[i].z_int_e = s->slice[i].z_int_e;
+        slices[i].iatom_0 = s->slice[i].iatom_0; slices[i].iatom_e = s->slice[i].iatom_e;
+        slices[i].ithk = s->slice[i].ithk;
+    }
+    sim_free(s);
+    return MT_OK;
+}
+
+/* displaced atoms (x, y, z, Z) in z-sorted order for configuration iconf */
+int orc_displaced_atoms(const mt_input_multislice *in, int iconf, double *xyzZ, int max_atoms, int *n_atoms)
+{
+    int st;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    if (iconf >= 1) move_atoms(s, iconf);
+    *n_atoms = s->atoms.n;
+    for (int i = 0; i < s->atoms.n && i < max_atoms && xyzZ; i++) {
+        xyzZ[4 * i] = s->atoms.x[i]; xyzZ[4 * i + 1] = s->atoms.y[i];
+        xyzZ[4 * i + 2] = s->atoms.z[i]; xyzZ[4 * i + 3] = s->atoms.Z[i];
+    }
+    sim_free(s);
+    return MT_OK;
+}
+
+static void unshift_real(const grid_t *g, const real *src, double *dst)
+{
+    for (int ix = 0; ix < g->nx; ix++)
+        for (int iy = 0; iy < g->ny; iy++) {
+            int sx = iRx_shift(g, ix), sy = iRy_shift(g, iy);
+            dst[(size_t)ix * g->ny + iy] = src[(size_t)sx * g->ny + sy];
+        }
+}
+static void unshift_cplx(const grid_t *g, const cplx *src, double *dst)
+{
+    for (int ix = 0; ix < g->nx; ix++)
+        for (int iy = 0; iy < g->ny; iy++) {
+            int sx = iRx_shift(g, ix), sy = iRy_shift(g, iy);
+            cplx v = src[(size_t)sx * g->ny + sy];
+            dst[2 * ((size_t)ix * g->ny + iy)] = v.re;
+            dst[2 * ((size_t)ix * g->ny + iy) + 1] = v.im;
+        }
+}
+
+int orc_projected_potential(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, int islice, double *V)
+{
+    int st;
+    (void)sys;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    if (iconf >= 1 && is_frozen(s)) move_atoms(s, iconf);
+    size_t nxy = (size_t)s->g.nx * s->g.ny;
+    real *v = malloc(sizeof(real) * nxy);
+    st = slice_potential(s, islice, v);
+    if (st == MT_OK) unshift_real(&s->g, v, V);
+    free(v);
+    sim_free(s);
+    return st;
+}
+
+int orc_transmission_function(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, int islice, double *trans)
+{
+    int st;
+    (void)sys;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    if (iconf >= 1 && is_frozen(s)) move_atoms(s, iconf);
+    size_t nxy = (size_t)s->g.nx * s->g.ny;
+    real *v = malloc(sizeof(real) * nxy);
+    cplx *t = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * (size_t)s->g.nx * 16);
+    fft2 F;
+    fft2_init(&F, s->g.nx, s->g.ny);
+    st = slice_potential(s, islice, v);
+    if (st == MT_OK) { slice_transmission(s, v, t, &F, work); unshift_cplx(&s->g, t, trans); }
+    fft2_free(&F);
+    free(v); free(t); free(work);
+    sim_free(s);
+    return st;
+}
+
+int orc_incident_wave(const mt_system_conf *sys, const mt_input_multislice *in, double *psi0)
+{
+    int st;
+    (void)sys;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    size_t nxy = (size_t)s->g.nx * s->g.ny;
+    cplx *p = malloc(sizeof(cplx) * nxy), *f = malloc(sizeof(cplx) * nxy);
+    cplx *work = malloc(sizeof(cplx) * (size_t)s->g.nx * 16);
+    fft2 F;
+    fft2_init(&F, s->g.nx, s->g.ny);
+    incident_wave(s, s->iw_x, s->iw_y, s->n_iw, p, &F, work, f);
+    unshift_cplx(&s->g, p, psi0);
+    fft2_free(&F);
+    free(p); free(f); free(work);
+    sim_free(s);
+    return MT_OK;
+}
+
+/* Propagator::operator()(space, ..., output) (src/propagator.cuh:95-104):
+   fft2_shift in, propagate, fft2_shift out */
+int orc_propagate(const mt_system_conf *sys, const mt_input_multislice *in, const double *psi_in,
+                  double z, int space, double *psi_out)
+{
+    int st;
+    (void)sys;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return st;
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    cplx *p = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    for (int ix = 0; ix < g->nx; ix++)
+        for (int iy = 0; iy < g->ny; iy++) {
+            size_t src = (size_t)ix * g->ny + iy;
+            size_t dst = (size_t)iRx_shift(g, ix) * g->ny + iRy_shift(g, iy);
+            p[dst].re = (real)psi_in[2 * src]; p[dst].im = (real)psi_in[2 * src + 1];
+        }
+    fft2 F;
+    fft2_init(&F, g->nx, g->ny);
+    propagator(s, space, (real)z, p, &F, work);
+    unshift_cplx(g, p, psi_out);
+    fft2_free(&F);
+    free(p); free(work);
+    sim_free(s);
+    return MT_OK;
+}

@@ -1,0 +1,107 @@
+"""The C-ABI library on the CPU: it loads, exports every symbol the header
+declares, and its host-side model (input reading, slicing, output sizes,
+error codes) agrees with the oracle.  No device work."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from multem_amd import capi, configs, engine
+from oracle import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_exports_every_declared_symbol(engine_lib):
+    hdr = open(os.path.join(ROOT, "include", "multem_hip.h")).read()
+    declared = set(re.findall(r"\b(mt_hip_\w+)\s*\(", hdr))
+    assert len(declared) >= 15
+    nm = subprocess.run(["nm", "-D", "--defined-only", engine.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (mt_hip_\w+)", nm))
+    assert declared <= exported, declared - exported
+    for name in declared:
+        getattr(engine_lib, name)
+
+
+def test_abi_struct_sizes():
+    import ctypes as C
+    # 31 doubles of lens parameters, then 4 ints and 4 more doubles
+    assert C.sizeof(capi.Lens) == 8 * 31 + 8 + 8 * 3 + 8 + 8 + 8
+    assert C.sizeof(capi.DetectorCir) == 16
+
+
+@pytest.mark.parametrize("maker", [
+    configs.si_ewrs_256,
+    lambda: configs.au_stem_512(ns=4, nconf=2),
+    lambda: configs.pt_cbed_1024(nconf=2, nc=30),
+    lambda: configs.sto_stem(n=512, ncells=12, nc=25),
+])
+def test_slicing_matches_oracle(maker):
+    p = maker()
+    a = engine.spec_slicing(p)
+    b = oracle.spec_slicing(p, precision="f32")
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert x[4:] == y[4:]
+        assert abs(x[0] - y[0]) < 1e-6 and abs(x[1] - y[1]) < 1e-6
+
+
+@pytest.mark.parametrize("maker", [
+    configs.si_ewrs_256,
+    lambda: configs.au_stem_512(ns=8, nconf=2),
+    lambda: configs.pt_cbed_1024(nconf=2, nc=30),
+])
+def test_output_dims_match_oracle(maker):
+    import ctypes as C
+    p = maker()
+    d = engine.output_dims(p)
+    s, keep = p.to_struct()
+    sysc = p.system_conf.as_struct()
+    o = capi.OutputMultislice()
+    assert oracle.lib("f32").orc_output_dims(C.byref(sysc), C.byref(s), C.byref(o)) == 0
+    for f in ("output_type", "nx", "ny", "n_thk", "n_det"):
+        assert getattr(d, f) == getattr(o, f), f
+    assert abs(d.dx - o.dx) < 1e-6 and abs(d.dy - o.dy) < 1e-6
+
+
+def test_scan_grid_square_region():
+    p = configs.au_stem_512(ns=64, nconf=1)
+    d = engine.output_dims(p)
+    assert (d.nx, d.ny) == (64, 64)
+    assert abs(d.dx - 4.078 / 64) < 1e-6
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("simulation_type", 12, 2),       # ISTEM: not provided
+    ("potential_type", 4, 2),         # Kirkland tables: not provided
+    ("nx", 300, 2),                   # not a power of two
+    ("nx", 0, 1),
+    ("E_0", 0.0, 1),
+    ("iw_type", 3, 2),                # user-defined wave
+    ("theta", 0.5, 2),                # tilt
+])
+def test_errors_are_reported(field, value, status):
+    p = configs.si_ewrs_256()
+    setattr(p, field, value)
+    with pytest.raises(engine.EngineError) as e:
+        engine.output_dims(p)
+    assert e.value.status == status
+    assert str(e.value)
+
+
+def test_cpu_device_is_refused():
+    p = configs.si_ewrs_256()
+    p.system_conf.device = 1
+    with pytest.raises(engine.EngineError) as e:
+        engine.output_dims(p)
+    assert e.value.status == 2
+
+
+def test_empty_specimen_is_refused():
+    p = configs.si_ewrs_256()
+    p.spec_atoms = np.zeros((0, 8))
+    with pytest.raises(engine.EngineError) as e:
+        engine.output_dims(p)
+    assert e.value.status == 1

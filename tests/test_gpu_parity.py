@@ -1,0 +1,128 @@
+"""Parity of the HIP engine against the CPU oracle, through the C ABI.
+
+Tolerances (relative L2):
+  projected potential, transmission: 2e-6 vs the FP32 restatement (same
+      summation order; sincosf differs by ulps), 1e-6 vs FP64 for V;
+  single pieces (probe, propagate): 1e-5 vs FP64;
+  exit wave (Si [001], 256^2, 41 slices): 1e-5 vs the FP32 restatement of
+      the reference path, 1e-4 vs FP64;
+  STEM / CBED images: 1e-4 vs FP64.
+"""
+import numpy as np
+import pytest
+
+from multem_amd import configs, engine
+from oracle import oracle
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stem(n=256, ns=4, nconf=2, nc=5, ncells=4):
+    return configs.au_stem_512(ns=ns, nconf=nconf, n=n, ncells=ncells, nc=nc)
+
+
+def test_projected_potential_all_slices_si():
+    p = configs.si_ewrs_256()
+    ns = len(oracle.spec_slicing(p))
+    for islice in range(ns):
+        V = engine.projected_potential(p, islice)
+        V32 = oracle.projected_potential(p, islice, precision="f32")
+        V64 = oracle.projected_potential(p, islice, precision="f64")
+        print("slice", islice, rel_l2(V, V32), rel_l2(V, V64))
+        assert rel_l2(V, V32) < 2e-6
+        assert rel_l2(V, V64) < 1e-5
+
+
+def test_projected_potential_phonon_au():
+    p = small_stem()
+    for iconf in (1, 2):
+        for islice in (0, 3, 10):
+            V = engine.projected_potential(p, islice, iconf)
+            V32 = oracle.projected_potential(p, islice, iconf, precision="f32")
+            assert rel_l2(V, V32) < 2e-6, (iconf, islice)
+
+
+def test_transmission_function():
+    p = configs.si_ewrs_256()
+    for islice in (0, 1, 20, 40):
+        T = engine.transmission_function(p, islice)
+        T32 = oracle.transmission_function(p, islice, precision="f32")
+        T64 = oracle.transmission_function(p, islice, precision="f64")
+        assert rel_l2(T, T32) < 2e-6
+        assert rel_l2(T, T64) < 1e-5
+
+
+def test_transmission_function_bwl():
+    p = configs.si_ewrs_256()
+    p.bwl = 1
+    T = engine.transmission_function(p, 3)
+    T64 = oracle.transmission_function(p, 3, precision="f64")
+    assert rel_l2(T, T64) < 1e-5
+
+
+def test_incident_wave_probe():
+    p = small_stem()
+    p.simulation_type = 21
+    p.iw_type = 2
+    p.cond_lens_c_10 = 35.0
+    p.cond_lens_c_30 = 0.002
+    p.iw_x = np.array([0.3 * p.spec_lx])
+    p.iw_y = np.array([0.6 * p.spec_ly])
+    psi = engine.incident_wave(p)
+    ref = oracle.incident_wave(p, precision="f64")
+    assert rel_l2(psi, ref) < 1e-5
+    assert abs(np.sum(np.abs(psi) ** 2) / psi.size - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("space", [1, 2])
+def test_propagate(space):
+    p = configs.si_ewrs_256()
+    rng = np.random.default_rng(1)
+    psi = rng.standard_normal((p.nx, p.ny)) + 1j * rng.standard_normal((p.nx, p.ny))
+    out = engine.propagate(p, psi, 7.5, space)
+    ref = oracle.propagate(p, psi, 7.5, space, precision="f64")
+    assert rel_l2(out, ref) < 1e-5
+
+
+def test_exit_wave_si():
+    p = configs.si_ewrs_256()
+    out = engine.run_multislice(p)
+    r32 = oracle.run(p, "f32")
+    r64 = oracle.run(p, "f64")
+    e32, e64 = rel_l2(out.psi_coh, r32.psi_coh), rel_l2(out.psi_coh, r64.psi_coh)
+    print("exit wave rel L2 vs FP32 restatement %.3e, vs FP64 %.3e" % (e32, e64))
+    assert e32 < 1e-5
+    assert e64 < 1e-4
+
+
+def test_stem_small():
+    p = small_stem()
+    out = engine.run_multislice(p)
+    r64 = oracle.run(p, "f64")
+    e = rel_l2(out.image_tot, r64.image_tot)
+    print("STEM image rel L2 vs FP64 %.3e" % e)
+    assert e < 1e-4
+
+
+def test_stem_batch_and_graph_invariance():
+    p = small_stem(ns=5, nconf=1)
+    engine.set_batch(0, True)
+    a = engine.run_multislice(p).image_tot.copy()
+    engine.set_batch(3, True)
+    b = engine.run_multislice(p).image_tot.copy()
+    engine.set_batch(7, False)
+    c = engine.run_multislice(p).image_tot.copy()
+    engine.set_batch(0, True)
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, c)
+
+
+def test_cbed_small():
+    p = configs.pt_cbed_1024(nconf=2, n=256, na=4, nb=6, nc=12, thick=[15.0, 30.0])
+    out = engine.run_multislice(p)
+    r64 = oracle.run(p, "f64")
+    for k in range(out.n_thk):
+        e = rel_l2(out.m2psi_tot[k], r64.m2psi_tot[k])
+        print("CBED thickness", k, "rel L2 vs FP64 %.3e" % e)
+        assert e < 1e-4

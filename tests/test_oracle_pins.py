@@ -1,0 +1,245 @@
+"""Pins of the CPU oracle against what fixes it independently of its own code:
+scipy's exact Bessel functions, numpy's FFT, libstdc++'s random engines, the
+reference's own parameter table, closed forms and physical invariants.
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from multem_amd import configs
+from oracle import oracle
+from tests.conftest import rel_l2
+
+REF = "/root/reference/src/atomic_data.hpp"
+
+
+def test_bessel_against_scipy():
+    L = oracle.lib("f64")
+    x = np.concatenate([np.linspace(0.01, 2.0, 60), np.linspace(2.0, 40.0, 80)])
+    for which, f, tol in [(0, special.i0, 2e-7), (1, special.i1, 2e-7), (2, special.k0, 2e-7), (3, special.k1, 3e-7)]:
+        got = np.array([L.orc_bessel(which, float(v)) for v in x])
+        ref = f(x)
+        err = np.max(np.abs(got - ref) / np.abs(ref))
+        assert err < tol, (which, err)
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (128, 32), (256, 512)])
+def test_fft_against_numpy(shape):
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    f = oracle.fft2(a, inverse=False)
+    b = oracle.fft2(a, inverse=True)
+    assert rel_l2(f, np.fft.fft2(a)) < 1e-13
+    assert rel_l2(b, np.fft.ifft2(a) * a.size) < 1e-13
+    assert rel_l2(oracle.fft2(a, precision="f32"), np.fft.fft2(a)) < 2e-6
+
+
+def test_fft_known_answers():
+    n = 64
+    d = np.zeros((n, n), complex)
+    d[0, 0] = 1
+    assert np.allclose(oracle.fft2(d), np.ones((n, n)))
+    ix, iy = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    plane = np.exp(2j * np.pi * (3 * ix + 5 * iy) / n)
+    F = oracle.fft2(plane)
+    expect = np.zeros((n, n), complex)
+    expect[3, 5] = n * n
+    assert np.allclose(F, expect, atol=1e-9)
+
+
+@pytest.fixture(scope="module")
+def rng_ref(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    d = tmp_path_factory.mktemp("rng")
+    exe = str(d / "rng_ref")
+    src = os.path.join(os.path.dirname(__file__), "golden", "rng_ref.cpp")
+    subprocess.run(["g++", "-O2", "-std=c++17", src, "-o", exe], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("seed", [300183, 1, 2147483647])
+def test_rng_against_libstdcxx(rng_ref, seed):
+    n = 2000
+    out = subprocess.run([rng_ref, str(seed), str(n)], capture_output=True, text=True, check=True).stdout.split()
+    mt = np.array([int(v) for v in out[:n]], dtype=np.uint64)
+    ui = np.array([int(v) for v in out[n:2 * n]])
+    nf = np.array([float(v) for v in out[2 * n:3 * n]])
+    nd = np.array([float(v) for v in out[3 * n:]])
+    import ctypes as C
+    L32, L64 = oracle.lib("f32"), oracle.lib("f64")
+    got_mt = (C.c_uint64 * n)()
+    L64.orc_mt19937_64(seed, n, got_mt)
+    assert np.array_equal(np.array(got_mt[:], dtype=np.uint64), mt)
+    got_ui = (C.c_int * n)()
+    L64.orc_uniform_int(seed, n, got_ui)
+    assert np.array_equal(np.array(got_ui[:]), ui)
+    g32 = np.zeros(n)
+    L32.orc_normal(seed, n, g32.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.array_equal(g32.astype(np.float32), nf.astype(np.float32))
+    g64 = np.zeros(n)
+    L64.orc_normal(seed, n, g64.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.array_equal(g64, nd)
+
+
+def test_wavelength_closed_form():
+    L = oracle.lib("f64")
+    # lambda = hc / sqrt(E (2 m c^2 + E)), E in keV
+    for E, lam in [(300.0, 0.0196875), (200.0, 0.0250793)]:
+        assert abs(L.orc_lambda(E) - lam) < 1e-7
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference sources not present")
+def test_lobato_table_matches_reference_file():
+    src = open(REF).read()
+    body = src[src.index("void Load_feg_Lobato_neutral_0_12"):]
+    body = body[:body.index("\n\t\t\t}")]
+    ref = {}
+    for m in re.finditer(r"data_table\[(\d+)\]\.feg\[0\]\.(cl|cnl)\[(\d)\] = ([0-9.eE+-]+);", body):
+        ref[(int(m.group(1)), m.group(2), int(m.group(3)))] = float(m.group(4))
+    inc = open(os.path.join(os.path.dirname(__file__), "..", "multem_amd", "csrc", "lobato_table.inc")).read()
+    rows = re.findall(r"^\s*\{([-0-9.eE+, ]+)\},\s*$", inc[inc.index("mt_lobato_coef"):], re.M)
+    assert len(rows) == 103
+    for z, row in enumerate(rows):
+        vals = [float(v) for v in row.split(",")]
+        for j in range(5):
+            assert vals[2 * j] == ref[(z, "cl", j)]
+            assert vals[2 * j + 1] == ref[(z, "cnl", j)]
+    A = {int(m.group(1)): int(m.group(2)) for m in re.finditer(r"data_table\[(\d+)\]\.Z = \d+;\s*data_table\[\d+\]\.A = (\d+);", src)}
+    elem = re.findall(r"\{\s*(\d+),\s*(\d+),", inc[inc.index("mt_element_table"):inc.index("mt_lobato_coef")])
+    assert all(int(a) == A[int(z) - 1] for z, a in elem)
+
+
+@pytest.mark.parametrize("Z", [6, 8, 14, 22, 38, 78, 79])
+def test_projected_potential_is_z_integral_of_3d_potential(Z):
+    """VR(R) = integral over z of Vr(sqrt(R^2 + z^2)): pins the K0/K1 closed form
+    and its coefficients against the 3D Lobato potential."""
+    L = oracle.lib("f64")
+    import ctypes as C
+    R = np.array([0.05, 0.2, 0.5, 1.0, 2.0])
+    VR = np.zeros_like(R)
+    dp = C.POINTER(C.c_double)
+    L.orc_lobato_VR(Z, len(R), R.ctypes.data_as(dp), VR.ctypes.data_as(dp), None, None)
+
+    def vr3(r):
+        rr = np.array([r])
+        out = np.zeros(1)
+        L.orc_lobato_VR(Z, 1, rr.ctypes.data_as(dp), None, None, out.ctypes.data_as(dp))
+        return out[0]
+
+    for Ri, VRi in zip(R, VR):
+        num = 2 * integrate.quad(lambda z: vr3(np.hypot(Ri, z)), 0, np.inf, limit=400, epsabs=0, epsrel=1e-10)[0]
+        assert abs(num - VRi) / abs(VRi) < 2e-6, (Z, Ri, num, VRi)
+
+
+@pytest.mark.parametrize("Z", [8, 14, 79])
+def test_spline_table_invariants(Z):
+    t = oracle.atom_table(Z, 0.05)
+    R2, c0, c1, c2, c3 = t["R2"], t["c0"], t["c1"], t["c2"], t["c3"]
+    # Hermite: each cubic hits the next knot's value and slope (in R^2)
+    h = R2[1:] - R2[:-1]
+    v_end = ((c3[:-1] * h + c2[:-1]) * h + c1[:-1]) * h + c0[:-1]
+    d_end = (3 * c3[:-1] * h + 2 * c2[:-1]) * h + c1[:-1]
+    assert np.max(np.abs(v_end - c0[1:]) / np.max(np.abs(c0))) < 1e-12
+    assert np.max(np.abs(d_end - c1[1:]) / np.max(np.abs(c1))) < 1e-9
+    # the taper brings the last knot to zero; knots below R_tap are the exact VR
+    assert abs(c0[-1]) < 1e-12 * abs(c0[0])
+    R = np.sqrt(R2)
+    below = R < np.sqrt(t["R2_tap"])
+    VR = np.zeros_like(R)
+    import ctypes as C
+    dp = C.POINTER(C.c_double)
+    Rc = np.ascontiguousarray(R)
+    oracle.lib("f64").orc_lobato_VR(Z, len(R), Rc.ctypes.data_as(dp), VR.ctypes.data_as(dp), None, None)
+    assert np.allclose(c0[below], VR[below], rtol=1e-13)
+    # knots are log-spaced between R_min and R_max
+    assert np.allclose(np.diff(np.log(R)), np.log(t["R_max"] / t["R_min"]) / 127, rtol=1e-9)
+
+
+def test_single_atom_potential_is_projected_potential():
+    """One atom at the box centre: V(r) follows VR(|r - r_a|) up to the spline
+    interpolation error inside R_tap and vanishes beyond R_max."""
+    p = configs.si_ewrs_256()
+    a = 10.0
+    p.spec_atoms = np.array([[79, a / 2, a / 2, 0.0, 0.085, 1, 0, 0], [79, a / 2, a / 2, 3.0, 0.085, 1, 0, 0]])
+    p.spec_lx = p.spec_ly = a
+    p.spec_lz = 3.0
+    p.nx = p.ny = 128
+    V = oracle.projected_potential(p, 0, precision="f64")
+    d = a / 128
+    ix = np.arange(128)
+    X, Y = np.meshgrid(ix * d - a / 2, ix * d - a / 2, indexing="ij")
+    R = np.hypot(X, Y)
+    t = oracle.atom_table(79, d)
+    inside = (R > t["R_min"]) & (R < np.sqrt(t["R2_tap"]))
+    import ctypes as C
+    dp = C.POINTER(C.c_double)
+    Rc = np.ascontiguousarray(R[inside])
+    VR = np.zeros_like(Rc)
+    oracle.lib("f64").orc_lobato_VR(79, len(Rc), Rc.ctypes.data_as(dp), VR.ctypes.data_as(dp), None, None)
+    assert np.max(np.abs(V[inside] - VR) / VR) < 2e-3
+    assert np.all(V[R > t["R_max"] + 1e-9] == 0)
+
+
+@pytest.mark.parametrize("maker,count", [
+    (lambda: configs.si_ewrs_256(nc=10), 41),
+    (lambda: configs.au_stem_512(ns=2, nconf=1, nc=20), 41),
+    (lambda: configs.sto_stem(nc=10), 21),
+])
+def test_slice_counts(maker, count):
+    p = maker()
+    sl = oracle.spec_slicing(p)
+    assert len(sl) == count
+    # slices tile the specimen without gaps, each plane's atoms in one slice
+    for a, b in zip(sl[:-1], sl[1:]):
+        assert abs(a[1] - b[0]) < 1e-12
+        assert a[5] + 1 == b[4]
+
+
+def test_flux_conservation_and_fp32_path():
+    p = configs.si_ewrs_256()
+    r64 = oracle.run(p, "f64")
+    r32 = oracle.run(p, "f32")
+    psi = r64.psi_coh[0]
+    assert abs(np.mean(np.abs(psi) ** 2) - 1.0) < 1e-12
+    # the reference's FP32 path stays within a few 1e-6 of its FP64 path
+    assert rel_l2(r32.psi_coh, r64.psi_coh) < 1e-5
+
+
+def test_propagator_composition():
+    p = configs.si_ewrs_256()
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((256, 256)) + 1j * rng.standard_normal((256, 256))
+    a = oracle.propagate(p, oracle.propagate(p, psi, 3.0), 4.5)
+    b = oracle.propagate(p, psi, 7.5)
+    assert rel_l2(a, b) < 1e-12
+    # unitary without a bandwidth limit
+    assert abs(np.linalg.norm(b) / np.linalg.norm(psi) - 1) < 1e-12
+
+
+def test_probe_normalisation_and_position():
+    p = configs.au_stem_512(ns=2, nconf=1, n=256, ncells=4, nc=2)
+    p.simulation_type = 21
+    p.iw_x = np.array([0.25 * p.spec_lx])
+    p.iw_y = np.array([0.75 * p.spec_ly])
+    psi = oracle.incident_wave(p)
+    assert abs(np.sum(np.abs(psi) ** 2) / psi.size - 1) < 1e-12
+    ix, iy = np.unravel_index(np.argmax(np.abs(psi)), psi.shape)
+    assert abs(ix - 64) <= 1 and abs(iy - 192) <= 1
+
+
+def test_phonon_displacements():
+    p = configs.au_stem_512(ns=2, nconf=3, n=256, ncells=4, nc=10)
+    u = oracle.displaced_atoms(p, 0)
+    for iconf in (1, 2, 3):
+        d = oracle.displaced_atoms(p, iconf)
+        dx, dy, dz = (d[:, k] - u[:, k] for k in range(3))
+        assert np.all(dz == 0)                   # pn_dim = 110
+        assert abs(np.std(dx) - 0.085) < 0.01 and abs(np.std(dy) - 0.085) < 0.01
+        assert abs(np.mean(dx)) < 0.01
+    assert not np.array_equal(oracle.displaced_atoms(p, 1), oracle.displaced_atoms(p, 2))
