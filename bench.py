@@ -1,0 +1,167 @@
+"""Benchmark: batched FP32 STEM multislice on one MI355X.
+
+Workload (default): Au [001], a = 4.078 A, 8x8x20 cells, 512^2 grid, 41
+slices, 300 kV probe of 21.4 mrad, 64x64 scan over one unit cell, ADF ring
+80-200 mrad, 16 frozen-phonon configurations.  A step is one full run of that
+workload: 64*64*16*41 = 2 686 976 probe.slices.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  value = whole-job probe.slices/s (wall clock of the
+timed steps).  roofline: the dominant slice kernel's algorithmic bytes per
+launch over its mean duration (HIP events on the engine stream).
+cpu_baseline: the CPU oracle (a C port of the reference path, FP32, one
+thread per core) on 8x8 probes x 1 configuration of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ns", type=int, default=64, help="scan points per side")
+    ap.add_argument("--nconf", type=int, default=16)
+    ap.add_argument("--n", type=int, default=512, help="grid size")
+    ap.add_argument("--batch", type=int, default=0, help="waves per batch (0: planner)")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-ns", type=int, default=8)
+    return ap.parse_args()
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def cpu_baseline(args):
+    from multem_amd import configs
+    from oracle import oracle
+    p = configs.au_stem_512(ns=args.cpu_ns, nconf=1, n=args.n)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = min(cores, 16)
+    nsl = len(oracle.spec_slicing(p))
+    t0 = time.perf_counter()
+    oracle.run(p, "f32", nthreads=cores)
+    dt = time.perf_counter() - t0
+    ps = args.cpu_ns * args.cpu_ns * nsl
+    return {"value": ps / dt, "unit": "probe·slices/s", "cores": cores, "kind": "port",
+            "sample": "Au[001] 512^2, %dx%d probes x 1 configuration x %d slices (%d probe.slices), "
+                      "FP32 C oracle, %.1f s" % (args.cpu_ns, args.cpu_ns, nsl, ps, dt)}
+
+
+def traffic_from_profiles(kernel, n):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), if a summary has been recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_au_stem_%d.txt" % (kernel, n))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not provided (see DESIGN.md for how the "
+              "path shards)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    from multem_amd import configs, engine
+    p = configs.au_stem_512(ns=args.ns, nconf=args.nconf, n=args.n)
+    engine.set_batch(args.batch, not args.no_graph)
+    nsl = len(engine.spec_slicing(p))
+
+    for _ in range(args.warmup):
+        engine.run_multislice(p)
+    sync()
+    t0 = time.perf_counter()
+    rows = cols = 0.0
+    nrow = ncol = 0
+    dev_ms = 0.0
+    ps_total = 0.0
+    for k in range(args.steps):
+        engine.run_multislice(p)
+        t = engine.last_timing()
+        print("bench: step %d done, device %.1f ms" % (k, t["ms_total"]), file=sys.stderr, flush=True)
+        rows += t["ms_row"]
+        cols += t["ms_col"]
+        nrow += t["n_row"]
+        ncol += t["n_col"]
+        dev_ms += t["ms_slice"]
+        ps_total += t["probe_slices"]
+    sync()
+    wall = time.perf_counter() - t0
+    ps_step = args.ns * args.ns * args.nconf * nsl
+    value = ps_step * args.steps / wall
+    plan = engine.last_plan()
+    B = int([kv.split("=")[1] for kv in plan.split() if kv.startswith("batch=")][0])
+    N2 = args.n * args.n
+    # algorithmic bytes per launch: row pass reads+writes the wave and reads T
+    # (24 N^2 B per wave), column pass reads+writes the wave (16 N^2 B)
+    row_ms, col_ms = rows / max(nrow, 1), cols / max(ncol, 1)
+    row_bytes, col_bytes = 24.0 * N2 * B, 16.0 * N2 * B
+    if row_ms * nrow >= col_ms * ncol:
+        kern, kb, kms = "row_pass", row_bytes, row_ms
+    else:
+        kern, kb, kms = "col_pass", col_bytes, col_ms
+    achieved = kb / (kms * 1e-3) / 1e9
+    slice_loop_gbs = 40.0 * N2 * ps_total / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else None
+    out = {
+        "metric": "STEM probe·slices/s (+ HBM roofline %)",
+        "value": value,
+        "unit": "probe·slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (crystal built in-process, seeded frozen phonons)",
+        "config": {
+            "workload": "au_stem_512: Au[001] a=4.078 8x8x20 cells, %d^2 grid, %d slices, 300 kV 21.4 mrad probe, "
+                        "%dx%d scan, %d phonon configs, ADF 80-200 mrad" % (args.n, nsl, args.ns, args.ns, args.nconf),
+            "probe_slices_per_step": ps_step,
+            "batch": B,
+            "plan": plan,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kern,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(kern, args.n),
+            "bytes_alg_per_launch": kb,
+            "mean_launch_ms": kms,
+            "row_pass_ms": row_ms,
+            "col_pass_ms": col_ms,
+            "slice_loop_alg_GBs": slice_loop_gbs,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -224,6 +224,14 @@ mt_status mt_hip_incident_wave(const mt_system_conf *sys, const mt_input_multisl
 mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice *in,
                            const double *psi_in, double z, int space, double *psi_out);
 
+/* Projected-potential spline of element Z for pixel size dR_min (Angstrom),
+   as the engine builds it: 128 knots each of R^2 and the cubic coefficients
+   c0..c3, and scal = {R_min, R_max, R_tap, tap_cf}.  Host only.  Replaces
+   Atomic_Data::To_atom_type_CPU (src/atomic_data.hpp:148-175) as exposed by
+   the mex helpers. */
+mt_status mt_hip_atom_table(int Z, double dR_min, double *R2, double *c0, double *c1, double *c2,
+                            double *c3, double *scal);
+
 /* ---- run control (Multislice::ext_stop_sim / ext_niter / ext_iter,
         tem_simulation.cuh:47-49) ---------------------------------------- */
 void mt_hip_request_stop(void);

@@ -122,6 +122,24 @@ mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice 
                    [&](const mtg::Model &m) { mtg::propagate(m, sys->gpu_device, psi_in, z, space, psi_out); });
 }
 
+mt_status mt_hip_atom_table(int Z, double dR_min, double *R2, double *c0, double *c1, double *c2, double *c3,
+                            double *scal)
+{
+    if (Z < 1 || Z > 103) return fail(MT_ERR_INVALID_ARGUMENT, "Z out of range 1..103");
+    if (!(dR_min > 0)) return fail(MT_ERR_INVALID_ARGUMENT, "dR_min must be positive");
+    mtg::AtomTable t = mtg::build_atom_table(Z, (double)(float)dR_min, (double)0.015f);
+    for (int i = 0; i < 128; ++i) {
+        if (R2) R2[i] = t.R2[i];
+        if (c0) c0[i] = t.c0[i];
+        if (c1) c1[i] = t.c1[i];
+        if (c2) c2[i] = t.c2[i];
+        if (c3) c3[i] = t.c3[i];
+    }
+    if (scal) { scal[0] = t.R_min; scal[1] = t.R_max; scal[2] = t.R_tap; scal[3] = t.tap_cf; }
+    g_last_error.clear();
+    return MT_OK;
+}
+
 void mt_hip_request_stop(void) { mtg::run_control().stop = true; }
 
 void mt_hip_progress(int *iter, int *niter)

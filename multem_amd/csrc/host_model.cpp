@@ -570,8 +570,11 @@ Model::Model(const mt_input_multislice &in)
     z_types = atoms_u.Z;
     std::sort(z_types.begin(), z_types.end());
     z_types.erase(std::unique(z_types.begin(), z_types.end()), z_types.end());
+    // The FP32 path stores Vrl and dR_min as float and widens them for the
+    // double-precision table builder (Spec::set_input_data, src/spec.hpp:51-55).
     const double dR_min = std::fmin(dRx, dRy);
-    for (int Zt : z_types) tables.push_back(build_atom_table(Zt, dR_min));
+    const double Vrl = (double)0.015f;
+    for (int Zt : z_types) tables.push_back(build_atom_table(Zt, dR_min, Vrl));
     for (int Zi : atoms_u.Z)
         type_of_atom.push_back((int)(std::lower_bound(z_types.begin(), z_types.end(), Zi) - z_types.begin()));
     R_int_max = 0;

@@ -24,8 +24,9 @@ __device__ __forceinline__ int signed_freq(int i, int n) { return i < (n >> 1) ?
 // Grid_2d::g2_shift (src/types.cuh:2549-2552)
 __device__ __forceinline__ float g2_float(int igx, int igy, float dgx, float dgy)
 {
-    float gx = __fmul_rn((float)igx, dgx), gy = __fmul_rn((float)igy, dgy);
-    return __fadd_rn(__fmul_rn(gx, gx), __fmul_rn(gy, gy));
+#pragma clang fp contract(off)
+    const float gx = (float)igx * dgx, gy = (float)igy * dgy;
+    return gx * gx + gy * gy;
 }
 
 // bandwidth-limit factor 1/(1+exp(alpha (g^2 - gl2_max))) (Grid_2d::bwl_factor_shift)
@@ -216,6 +217,9 @@ constexpr int kTfTy = 64, kTfTx = 8, kTfThreads = 256;
 
 __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
 {
+    // Every product and sum rounds separately, as on the reference's FP32 host
+    // path: the pixel values then depend only on the summation order.
+#pragma clang fp contract(off)
     __shared__ float tab[kMaxTypes * 5 * 128];
     __shared__ float2 rmx[kMaxTypes];
     __shared__ int list[kTfThreads];
@@ -279,11 +283,11 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
                 // images ix = px + k*nx inside [ix0, ixe]
                 int kx = (ix0 - px >= 0) ? (ix0 - px + a.nx - 1) / a.nx : -((px - ix0) / a.nx);
                 for (int ix = px + kx * a.nx; ix <= ixe; ix += a.nx) {
-                    const float rx = __fsub_rn(__fmul_rn((float)ix, a.dRx), at.x);
+                    const float rx = (float)ix * a.dRx - at.x;
                     int ky = (iy0 - py >= 0) ? (iy0 - py + a.ny - 1) / a.ny : -((py - iy0) / a.ny);
                     for (int iy = py + ky * a.ny; iy <= iye; iy += a.ny) {
-                        const float ry = __fsub_rn(__fmul_rn((float)iy, a.dRy), at.y);
-                        const float R2 = __fadd_rn(__fmul_rn(rx, rx), __fmul_rn(ry, ry));
+                        const float ry = (float)iy * a.dRy - at.y;
+                        const float R2 = rx * rx + ry * ry;
                         if (R2 < R2_max) {
                             // 7-step binary search over the 128 knots, then the cubic
                             int i0 = 0, ie = 127;
@@ -292,11 +296,10 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
                                 const int im = (i0 + ie) >> 1;
                                 if (R2 < tb[im]) ie = im; else i0 = im;
                             }
-                            const float dx = __fsub_rn(R2, tb[i0]);
-                            float p = __fadd_rn(__fmul_rn(tb[4 * 128 + i0], dx), tb[3 * 128 + i0]);
-                            p = __fadd_rn(__fmul_rn(p, dx), tb[2 * 128 + i0]);
-                            p = __fadd_rn(__fmul_rn(p, dx), tb[1 * 128 + i0]);
-                            vacc[h] = __fadd_rn(vacc[h], __fmul_rn(at.z, p));
+                            const float dx = R2 - tb[i0];
+                            const float p = ((tb[4 * 128 + i0] * dx + tb[3 * 128 + i0]) * dx + tb[2 * 128 + i0]) * dx
+                                            + tb[1 * 128 + i0];
+                            vacc[h] = vacc[h] + at.z * p;
                         }
                     }
                 }
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
         if (a.V) a.V[sl * nxy + (long long)px * a.ny + py] = vacc[h];
         if (a.trans) {
             float s, c;
-            sincosf(__fmul_rn(a.w, vacc[h]), &s, &c);
+            sincosf(a.w * vacc[h], &s, &c);
             a.trans[sl * nxy + (long long)shift_idx(px, a.nxh) * a.ny + shift_idx(py, a.nyh)] = make_float2(c, s);
         }
     }
