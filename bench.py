@@ -37,16 +37,17 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ns", type=int, default=8)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB device copy used to calibrate HBM counters")
     return ap.parse_args()
 
 
 def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except ImportError:
-        pass
+    """Device-wide barrier on the engine's HIP runtime.  (torch is not imported:
+    it bundles a second HIP runtime whose synchronize would not order the
+    engine's work, and two runtimes in one process confuse tracers.)"""
+    from multem_amd import engine
+    engine.synchronize()
 
 
 def cpu_baseline(args):
@@ -65,6 +66,30 @@ def cpu_baseline(args):
                       "FP32 C oracle, %.1f s" % (args.cpu_ns, args.cpu_ns, nsl, ps, dt)}
 
 
+def copy_calibration():
+    """Device-to-device copy of 1 GiB on the engine's HIP runtime: the known-size
+    reference for rocprofv3's FETCH_SIZE / WRITE_SIZE, and the achievable copy
+    bandwidth on this GPU."""
+    import ctypes as C
+    from multem_amd import engine
+    engine.lib()
+    hip = C.CDLL("libamdhip64.so.7")
+    n = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    hip.hipDeviceSynchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0     # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t0) / reps
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes_per_launch": n, "read_plus_write_GBs": 2 * n / dt / 1e9, "ms": dt * 1e3}))
+
+
 def traffic_from_profiles(kernel, n):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration), if a summary has been recorded."""
@@ -79,6 +104,9 @@ def traffic_from_profiles(kernel, n):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not provided (see DESIGN.md for how the "
               "path shards)" % args.gpus, file=sys.stderr)
@@ -121,7 +149,7 @@ def main():
         kern, kb, kms = "row_pass", row_bytes, row_ms
     else:
         kern, kb, kms = "col_pass", col_bytes, col_ms
-    achieved = kb / (kms * 1e-3) / 1e9
+    achieved = kb / (kms * 1e-3) / 1e9 if kms > 0 else None   # None when engine events are off
     slice_loop_gbs = 40.0 * N2 * ps_total / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else None
     out = {
         "metric": "STEM probe·slices/s (+ HBM roofline %)",
@@ -149,7 +177,7 @@ def main():
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic_from_profiles(kern, args.n),
             "bytes_alg_per_launch": kb,
             "mean_launch_ms": kms,

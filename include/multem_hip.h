@@ -252,6 +252,11 @@ mt_status mt_hip_last_plan(char *buf, size_t n);
    covered. */
 mt_status mt_hip_last_timing(double *ms_slice_kernels, double *ms_total, double *probe_slices);
 
+/* Blocks until all work the library queued on GPU `device` has finished
+   (hipDeviceSynchronize on the library's HIP runtime).  No counterpart in
+   the reference, whose mex entry synchronises its stream before returning. */
+mt_status mt_hip_synchronize(int device);
+
 /* ---- engine tuning hooks (no counterpart in the reference; for the
         benchmark and tests) ------------------------------------------------ */
 

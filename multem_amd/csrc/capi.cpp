@@ -37,7 +37,7 @@ mt_status guarded(const mt_system_conf *sys, const mt_input_multislice *in, F &&
     } catch (const std::bad_alloc &) {
         return fail(MT_ERR_OUT_OF_MEMORY, "host allocation failed");
     } catch (const std::exception &e) {
-        return fail(MT_ERR_DEVICE, e.what());
+        return fail(MT_ERR_DEVICE, std::string(e.what()) + " (after " + mtg::last_successful_call() + ")");
     }
 }
 
@@ -137,6 +137,16 @@ mt_status mt_hip_atom_table(int Z, double dR_min, double *R2, double *c0, double
     }
     if (scal) { scal[0] = t.R_min; scal[1] = t.R_max; scal[2] = t.R_tap; scal[3] = t.tap_cf; }
     g_last_error.clear();
+    return MT_OK;
+}
+
+mt_status mt_hip_synchronize(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(MT_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(MT_ERR_INVALID_ARGUMENT, "device out of range");
+    if (hipSetDevice(device) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return fail(MT_ERR_DEVICE, hipGetErrorString(hipGetLastError()));
     return MT_OK;
 }
 

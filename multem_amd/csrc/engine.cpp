@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -23,8 +24,11 @@ namespace {
 
 constexpr double kPi = 3.141592653589793238463;
 
+thread_local const char *t_last_ok = "none";
+
 void check(hipError_t e, const char *what)
 {
+    if (e == hipSuccess) t_last_ok = what;
     if (e != hipSuccess) {
         std::string m = std::string(what) + ": " + hipGetErrorString(e);
         throw Error{e == hipErrorOutOfMemory ? MT_ERR_OUT_OF_MEMORY : MT_ERR_DEVICE, m};
@@ -72,8 +76,31 @@ struct Event {
     ~Event() { if (e) (void)hipEventDestroy(e); }
 };
 
+// HIP-event timing of the engine's own stream.  MULTEM_HIP_EVENTS=0 turns it
+// off (stream synchronisation only), for runs under an external tracer.
+bool events_enabled()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("MULTEM_HIP_EVENTS");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+void record(const Event &ev, hipStream_t s)
+{
+    if (events_enabled()) check(hipEventRecord(ev.e, s), "hipEventRecord");
+}
+
+void wait(const Event &ev, hipStream_t s)
+{
+    if (events_enabled()) check(hipEventSynchronize(ev.e), "hipEventSynchronize");
+    else check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
 float elapsed(const Event &a, const Event &b)
 {
+    if (!events_enabled()) return 0.f;
     float ms = 0;
     check(hipEventElapsedTime(&ms, a.e, b.e), "hipEventElapsedTime");
     return ms;
@@ -349,9 +376,9 @@ void run_stem(const Model &m, mt_output_multislice *out)
         int k = 0;
         for (int is = 0; is < ns_run; ++is) {
             r.slice = is;
-            if (timed) check(hipEventRecord((*ev)[k++].e, s), "event");
+            if (timed) record((*ev)[k++], s);
             check(launch_row_pass(m.ny, r, s), "row pass");
-            if (timed) check(hipEventRecord((*ev)[k++].e, s), "event");
+            if (timed) record((*ev)[k++], s);
             const int ithk = m.slices[is].ithk;
             const bool last = is == ns_run - 1;
             c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (last ? 0 : (COL_IFFT | COL_STORE));
@@ -359,7 +386,7 @@ void run_stem(const Model &m, mt_output_multislice *out)
             c.py = py.p + (size_t)is * m.ny;
             c.ithk = ithk;
             check(launch_col_pass(m.nx, c, false, ithk >= 0, s), "col pass");
-            if (timed) check(hipEventRecord((*ev)[k++].e, s), "event");
+            if (timed) record((*ev)[k++], s);
         }
         check(launch_det_reduce(part.p, B, n_thk, n_det, ntile, members.p, (double)w, image.p, nscan, s), "det reduce");
     };
@@ -380,7 +407,7 @@ void run_stem(const Model &m, mt_output_multislice *out)
     std::vector<Event> kev(run_control().time_kernels ? 3 * ns_run : 0);
     Event t0, t1, tf0, tf1, sl0, sl1;
     double ms_slice = 0, ms_tf = 0, ps = 0;
-    check(hipEventRecord(t0.e, s), "event");
+    record(t0, s);
     std::vector<Member> hm(B);
     std::vector<Beam> hb(B);
     DBuf<float4> atoms(m.atoms_u.x.size());
@@ -388,10 +415,10 @@ void run_stem(const Model &m, mt_output_multislice *out)
         stop_check();
         auto a = S.atoms_of(iconf);
         atoms.upload(a.data(), a.size(), s);
-        check(hipEventRecord(tf0.e, s), "event");
+        record(tf0, s);
         S.build_tf(atoms, 0, m.n_slice(), trans.p, nullptr, nullptr);
-        check(hipEventRecord(tf1.e, s), "event");
-        check(hipEventSynchronize(tf1.e), "sync");
+        record(tf1, s);
+        wait(tf1, s);
         ms_tf += elapsed(tf0, tf1);
         for (int ib = 0; ib < nbatch; ++ib) {
             stop_check();
@@ -405,11 +432,11 @@ void run_stem(const Model &m, mt_output_multislice *out)
             members.upload(hm.data(), B, s);
             beams.upload(hb.data(), B, s);
             const bool timed = run_control().time_kernels && ib == 0;
-            check(hipEventRecord(sl0.e, s), "event");
+            record(sl0, s);
             if (timed || !gexec) pipeline(timed, &kev);
             else check(hipGraphLaunch(gexec, s), "graph launch");
-            check(hipEventRecord(sl1.e, s), "event");
-            check(hipEventSynchronize(sl1.e), "sync");   // host arrays are reused next batch
+            record(sl1, s);
+            wait(sl1, s);   // host arrays are reused next batch
             ms_slice += elapsed(sl0, sl1);
             ps += (double)std::min(B, nscan - ib * B) * ns_run;
             if (timed) {
@@ -423,8 +450,8 @@ void run_stem(const Model &m, mt_output_multislice *out)
             progress_add(std::min(B, nscan - ib * B));
         }
     }
-    check(hipEventRecord(t1.e, s), "event");
-    check(hipEventSynchronize(t1.e), "sync");
+    record(t1, s);
+    wait(t1, s);
     tm.ms_total = elapsed(t0, t1);
     tm.ms_slice = ms_slice;
     tm.ms_tf = ms_tf;
@@ -503,7 +530,7 @@ void run_ew(const Model &m, mt_output_multislice *out)
     run_control().niter = nconf;
     run_control().iter = 0;
     Event t0, t1;
-    check(hipEventRecord(t0.e, s), "event");
+    record(t0, s);
     DBuf<float4> atoms(m.atoms_u.x.size());
     for (int c0 = 0; c0 < nconf; c0 += K) {
         stop_check();
@@ -571,8 +598,8 @@ void run_ew(const Model &m, mt_output_multislice *out)
         }
         progress_add(nb);
     }
-    check(hipEventRecord(t1.e, s), "event");
-    check(hipEventSynchronize(t1.e), "sync");
+    record(t1, s);
+    wait(t1, s);
     tm.ms_total = elapsed(t0, t1);
     tm.probe_slices = (double)nconf * ns_run;
 
@@ -593,6 +620,8 @@ void run_ew(const Model &m, mt_output_multislice *out)
 }
 
 } // namespace
+
+const char *last_successful_call() { return t_last_ok; }
 
 RunControl &run_control()
 {

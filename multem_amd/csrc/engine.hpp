@@ -31,6 +31,7 @@ struct Timing {
 };
 
 RunControl &run_control();
+const char *last_successful_call();   // name of the last HIP call that succeeded on this thread
 Timing &last_timing();
 
 // Full simulation into caller-owned output arrays.

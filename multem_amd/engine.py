@@ -48,6 +48,8 @@ def lib():
         L.mt_hip_last_timing.argtypes = [dp, dp, dp]
         L.mt_hip_set_batch.argtypes = [C.c_int, C.c_int]
         L.mt_hip_kernel_timing.argtypes = [dp, P(C.c_int), dp, P(C.c_int), dp]
+        L.mt_hip_synchronize.argtypes = [C.c_int]
+        L.mt_hip_atom_table.argtypes = [C.c_int, C.c_double] + [dp] * 6
         if L.mt_hip_abi_version() != capi.MT_ABI_VERSION:
             raise RuntimeError("libmultem_hip ABI mismatch")
         _lib = L
@@ -122,6 +124,18 @@ def propagate(params, psi, z, space=1):
     pout = np.zeros_like(pin)
     _check(lib().mt_hip_propagate(C.byref(sysc), C.byref(s), capi.dptr(pin), float(z), int(space), capi.dptr(pout)))
     return pout.view(np.complex128).reshape(params.nx, params.ny)
+
+
+def synchronize(device=0):
+    _check(lib().mt_hip_synchronize(int(device)))
+
+
+def atom_table(Z, dR_min):
+    arrs = [np.zeros(128) for _ in range(5)]
+    scal = np.zeros(4)
+    _check(lib().mt_hip_atom_table(int(Z), float(dR_min), *[capi.dptr(a) for a in arrs], capi.dptr(scal)))
+    R2, c0, c1, c2, c3 = arrs
+    return dict(R2=R2, c0=c0, c1=c1, c2=c2, c3=c3, R_min=scal[0], R_max=scal[1], R_tap=scal[2], tap_cf=scal[3])
 
 
 def set_batch(batch=0, use_graph=True):
