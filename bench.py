@@ -170,6 +170,7 @@ def main():
             "probe_slices_per_step": ps_step,
             "batch": B,
             "plan": plan,
+            "tune": engine.tune_log(),
         },
         "roofline": {
             "bound": "hbm",

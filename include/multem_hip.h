@@ -269,6 +269,27 @@ void mt_hip_set_batch(int batch, int use_graph);
    launch counts; ms_tf is the transmission-function build time. */
 void mt_hip_kernel_timing(double *ms_row, int *n_row, double *ms_col, int *n_col, double *ms_tf);
 
+/* Autotuner of the slice step.  Each pass has MT_PASS_VARIANTS launch
+   variants (radix schedule, rows/columns per workgroup, twiddles in registers
+   or read from the cached table).  Variants with the default schedule give
+   bit-identical waves; the alternative schedule rounds differently (FP32
+   FFT rounding only).  mode 0: fixed variants (0, or the forced ones);
+   1: each candidate is timed once per shape (grid, batch, detector
+   epilogue) and the choice kept for the process (default); 2: timed on
+   every run.  The choice is the lowest-numbered variant within 2 % of the
+   fastest. */
+#define MT_PASS_VARIANTS 7
+void mt_hip_set_autotune(int mode);
+
+/* Forces the variants of the row pass, the column pass and the last-slice
+   column pass with detector sums (-1: tuner's choice).  A run whose forced
+   variant is invalid for its grid fails with MT_ERR_INVALID_ARGUMENT. */
+void mt_hip_set_pass_variants(int row, int col, int col_det);
+
+/* Selection log of the last run: per pass the mean launch time of each
+   candidate in us ('-': not valid for the grid) and the variant chosen. */
+mt_status mt_hip_tune_log(char *buf, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

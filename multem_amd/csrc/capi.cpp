@@ -43,6 +43,8 @@ mt_status guarded(const mt_system_conf *sys, const mt_input_multislice *in, F &&
 
 } // namespace
 
+static_assert(MT_PASS_VARIANTS == mtg::kPassVariants, "variant count of the C ABI");
+
 extern "C" {
 
 int mt_hip_abi_version(void) { return MT_ABI_VERSION; }
@@ -183,6 +185,32 @@ void mt_hip_set_batch(int batch, int use_graph)
 {
     mtg::run_control().batch_override = batch;
     mtg::run_control().use_graph = use_graph != 0;
+}
+
+// Slice-step launch variants: mode 0 fixed (variant 0 or the forced ones),
+// 1 tuned once per shape and cached (default), 2 tuned on every run.
+void mt_hip_set_autotune(int mode)
+{
+    mtg::run_control().autotune = mode < 0 ? 0 : (mode > 2 ? 2 : mode);
+}
+
+// Force launch variants of the row pass, the column pass and the column pass
+// with detector sums (-1: leave to the tuner).  Out-of-range values are
+// refused when the run launches them.
+void mt_hip_set_pass_variants(int row, int col, int col_det)
+{
+    mtg::run_control().force_row = row;
+    mtg::run_control().force_col = col;
+    mtg::run_control().force_col_det = col_det;
+}
+
+// The tuner's selection log of the last run: per pass the candidates' mean
+// launch times (us, '-' = not valid for the shape) and the variant chosen.
+mt_status mt_hip_tune_log(char *buf, size_t n)
+{
+    if (!buf || n == 0) return fail(MT_ERR_INVALID_ARGUMENT, "null buffer");
+    std::snprintf(buf, n, "%s", mtg::last_timing().tune_log.c_str());
+    return MT_OK;
 }
 
 // Per-kernel timing of the last run: summed row / column pass durations and launch counts.

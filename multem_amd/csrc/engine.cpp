@@ -15,8 +15,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <sstream>
+#include <tuple>
 
 namespace mtg {
 
@@ -297,6 +299,130 @@ void stop_check()
 }
 
 // ---------------------------------------------------------------------------
+// Autotuner of the slice-step launch variants.  For each pass it launches
+// every candidate variant on the run's own buffers (one warm-up, then kReps
+// timed launches) and keeps the lowest-numbered variant within 2 % of the
+// fastest, so near-ties resolve the same way from run to run.  Choices are
+// cached per shape (nx, ny, batch, detector epilogue) for the process.
+//
+// Tuning launches overwrite the wave buffer and the detector partials; the
+// caller runs it before the first batch, which rewrites both.
+struct TuneKey {
+    int nx, ny, B, det_last;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(nx, ny, B, det_last) < std::tie(o.nx, o.ny, o.B, o.det_last);
+    }
+};
+
+std::map<TuneKey, PassChoice> &tune_cache()
+{
+    static std::map<TuneKey, PassChoice> c;
+    return c;
+}
+
+// mean duration (ms) of kReps launches of `launch`, or -1 if the variant is
+// not valid for this shape
+template <class F>
+double time_variant(F &&launch, hipStream_t s)
+{
+    constexpr int kReps = 4;
+    hipError_t e = launch();
+    if (e == hipErrorInvalidValue) {
+        (void)hipGetLastError();
+        return -1.0;
+    }
+    check(e, "tuning launch");
+    check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    double ms;
+    if (events_enabled()) {
+        Event a, b;
+        record(a, s);
+        for (int i = 0; i < kReps; ++i) check(launch(), "tuning launch");
+        record(b, s);
+        wait(b, s);
+        ms = elapsed(a, b) / kReps;
+    } else {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < kReps; ++i) check(launch(), "tuning launch");
+        check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / kReps;
+    }
+    return ms;
+}
+
+// forced >= 0: use that variant (checked to be valid for the shape);
+// tune == false: variant 0 unless forced
+template <class F>
+int pick_variant(const char *name, int forced, bool tune, F &&launch, hipStream_t s, std::ostringstream &log)
+{
+    if (forced >= 0 || !tune) {
+        const int v = std::max(forced, 0);
+        hipError_t e = v < kPassVariants ? launch(v) : hipErrorInvalidValue;
+        if (e == hipErrorInvalidValue) {
+            (void)hipGetLastError();
+            throw Error{MT_ERR_INVALID_ARGUMENT, std::string("launch variant ") + std::to_string(v) + " of the " +
+                                                     name + " pass is not valid for this grid"};
+        }
+        check(e, "variant launch");
+        log << name << "=v" << v << (forced >= 0 ? "(forced) " : "(fixed) ");
+        return v;
+    }
+    std::vector<double> t(kPassVariants, -1.0);
+    for (int v = 0; v < kPassVariants; ++v) t[v] = time_variant([&] { return launch(v); }, s);
+    double best = -1.0;
+    for (double x : t)
+        if (x >= 0 && (best < 0 || x < best)) best = x;
+    int pick = 0;
+    for (int v = 0; v < kPassVariants; ++v)
+        if (t[v] >= 0 && t[v] <= best * 1.02) { pick = v; break; }
+    log << name << "=v" << pick << "[";
+    for (int v = 0; v < kPassVariants; ++v) {
+        if (v) log << ",";
+        if (t[v] < 0) log << "-";
+        else log << (int)std::lround(t[v] * 1000.0);
+    }
+    log << "us] ";
+    return pick;
+}
+
+// r: a full row slice step over the batch; c: a full column slice step;
+// c_last: the last-slice column pass with detector sums (nullptr if the run
+// has none).
+PassChoice choose_variants(int nx, int ny, int B, RowArgs r, ColArgs c, const ColArgs *c_last, hipStream_t s)
+{
+    const RunControl &rc = run_control();
+    const bool forced = rc.force_row >= 0 || rc.force_col >= 0 || rc.force_col_det >= 0;
+    const bool tune = rc.autotune != 0;
+    const TuneKey key{nx, ny, B, c_last ? 1 : 0};
+    if (tune && rc.autotune == 1 && !forced) {
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end()) {
+            PassChoice cached = it->second;
+            cached.log = "cached: " + cached.log;
+            return cached;
+        }
+    }
+    PassChoice ch;
+    std::ostringstream log;
+    log << "nx=" << nx << " ny=" << ny << " batch=" << B << ": ";
+    ch.row = pick_variant("row", rc.force_row, tune, [&](int v) { r.variant = v; return launch_row_pass(ny, r, s); },
+                          s, log);
+    ch.col = pick_variant("col", rc.force_col, tune,
+                          [&](int v) { c.variant = v; return launch_col_pass(nx, c, false, false, s); }, s, log);
+    if (c_last) {
+        ColArgs cl = *c_last;
+        ch.col_det = pick_variant("col_det", rc.force_col_det, tune,
+                                  [&](int v) { cl.variant = v; return launch_col_pass(nx, cl, false, true, s); }, s,
+                                  log);
+    }
+    check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    ch.log = log.str();
+    if (tune && !forced) tune_cache()[key] = ch;
+    return ch;
+}
+
+// ---------------------------------------------------------------------------
 // STEM: batches of probe positions of one configuration.
 void run_stem(const Model &m, mt_output_multislice *out)
 {
@@ -348,16 +474,49 @@ void run_stem(const Model &m, mt_output_multislice *out)
     const unsigned int nap = S.count_aperture();
     const ProbeParams P = S.probe_params(nap);
 
-    std::ostringstream plan;
-    plan << "stem nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
-         << " col_C=" << col_tile_width(m.nx) << " row_RW=" << row_group(m.ny)
-         << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
-    tm.plan = plan.str();
 
     const float w = m.phonon_weight();
     const int nbatch = (nscan + B - 1) / B;
     run_control().niter = m.n_conf_run() * nscan;
     run_control().iter = 0;
+
+    // slice-step arguments shared by the tuner and the batch pipeline
+    RowArgs r{};
+    r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = B;
+    r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = 0; r.members = members.p;
+    r.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
+    ColArgs c = S.col_base(psi.p, psi.p, B);
+    c.n_det = n_det;
+    for (int d = 0; d < n_det; ++d) { c.det_g2_min[d] = m.det_g2_min[d]; c.det_g2_max[d] = m.det_g2_max[d]; }
+    c.det_part = part.p; c.n_thk = n_thk;
+    auto col_flags = [&](bool last) {
+        return COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (last ? 0 : (COL_IFFT | COL_STORE));
+    };
+    std::vector<Member> hm(B);
+    std::vector<Beam> hb(B);
+    PassChoice choice;
+    {   // pick the launch variants before the pipeline is captured
+        RowArgs rt = r;
+        rt.slice = 0;
+        ColArgs ct = c;
+        ct.flags = col_flags(false);
+        ct.px = px.p; ct.py = py.p; ct.ithk = -1;
+        ColArgs cl = c;
+        cl.flags = col_flags(true);
+        cl.px = px.p; cl.py = py.p; cl.ithk = m.slices[ns_run - 1].ithk;
+        // member slots must describe valid entries while tuning
+        for (int j = 0; j < B; ++j) hm[j] = Member{0, -1, j, 1};
+        members.upload(hm.data(), B, s);
+        check(hipMemsetAsync(psi.p, 0, psi.n * sizeof(float2), s), "hipMemsetAsync");
+        choice = choose_variants(m.nx, m.ny, B, rt, ct, cl.ithk >= 0 ? &cl : nullptr, s);
+        r.variant = choice.row;
+        tm.tune_log = choice.log;
+    }
+    std::ostringstream plan;
+    plan << "stem nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
+         << " row_variant=" << choice.row << " col_variant=" << choice.col << " col_det_variant=" << choice.col_det
+         << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
+    tm.plan = plan.str();
 
     // the batch pipeline (probe -> ns_run slices -> detector partials)
     auto pipeline = [&](bool timed, std::vector<Event> *ev) {
@@ -365,14 +524,6 @@ void run_stem(const Model &m, mt_output_multislice *out)
         c0.flags = COL_IFFT | COL_STORE;
         c0.members = members.p; c0.beams = beams.p; c0.probe = P;
         check(launch_col_pass(m.nx, c0, true, false, s), "probe pass");
-        RowArgs r{};
-        r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = B;
-        r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = 0; r.members = members.p;
-        r.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
-        ColArgs c = S.col_base(psi.p, psi.p, B);
-        c.n_det = n_det;
-        for (int d = 0; d < n_det; ++d) { c.det_g2_min[d] = m.det_g2_min[d]; c.det_g2_max[d] = m.det_g2_max[d]; }
-        c.det_part = part.p; c.n_thk = n_thk;
         int k = 0;
         for (int is = 0; is < ns_run; ++is) {
             r.slice = is;
@@ -381,10 +532,11 @@ void run_stem(const Model &m, mt_output_multislice *out)
             if (timed) record((*ev)[k++], s);
             const int ithk = m.slices[is].ithk;
             const bool last = is == ns_run - 1;
-            c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (last ? 0 : (COL_IFFT | COL_STORE));
+            c.flags = col_flags(last);
             c.px = px.p + (size_t)is * m.nx;
             c.py = py.p + (size_t)is * m.ny;
             c.ithk = ithk;
+            c.variant = ithk >= 0 ? choice.col_det : choice.col;
             check(launch_col_pass(m.nx, c, false, ithk >= 0, s), "col pass");
             if (timed) record((*ev)[k++], s);
         }
@@ -408,8 +560,6 @@ void run_stem(const Model &m, mt_output_multislice *out)
     Event t0, t1, tf0, tf1, sl0, sl1;
     double ms_slice = 0, ms_tf = 0, ps = 0;
     record(t0, s);
-    std::vector<Member> hm(B);
-    std::vector<Beam> hb(B);
     DBuf<float4> atoms(m.atoms_u.x.size());
     for (int iconf = m.fp_iconf_0; iconf <= m.pn_nconf; ++iconf) {
         stop_check();
@@ -532,6 +682,7 @@ void run_ew(const Model &m, mt_output_multislice *out)
     Event t0, t1;
     record(t0, s);
     DBuf<float4> atoms(m.atoms_u.x.size());
+    PassChoice choice;
     for (int c0 = 0; c0 < nconf; c0 += K) {
         stop_check();
         const int nb = std::min(K, nconf - c0);
@@ -548,6 +699,19 @@ void run_ew(const Model &m, mt_output_multislice *out)
         RowArgs r{};
         r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nb;
         r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = (long long)m.n_slice() * nxy; r.members = members.p;
+        if (c0 == 0) {   // pick the slice-step launch variants on this batch's buffers
+            RowArgs rt = r;
+            rt.slice = 0;
+            rt.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
+            ColArgs ct = S.col_base(psi.p, psi.p, nb);
+            ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+            ct.px = px.p; ct.py = py.p;
+            check(hipMemsetAsync(psi.p, 0, (size_t)nb * nxy * sizeof(float2), s), "hipMemsetAsync");
+            choice = choose_variants(m.nx, m.ny, nb, rt, ct, nullptr, s);
+            tm.tune_log = choice.log;
+            tm.plan += " row_variant=" + std::to_string(choice.row) + " col_variant=" + std::to_string(choice.col);
+        }
+        r.variant = choice.row;
         if (m.iw_type == 2) {
             ColArgs cp = S.col_base(nullptr, psi.p, nb);
             cp.flags = COL_IFFT | COL_STORE;
@@ -562,6 +726,7 @@ void run_ew(const Model &m, mt_output_multislice *out)
             c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
             c.px = px.p + (size_t)is * m.nx;
             c.py = py.p + (size_t)is * m.ny;
+            c.variant = choice.col;
             check(launch_col_pass(m.nx, c, false, false, s), "col pass");
             const int ithk = m.slices[is].ithk;
             if (ithk < 0) continue;

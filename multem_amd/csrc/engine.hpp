@@ -18,6 +18,15 @@ struct RunControl {
     int batch_override = 0;         // 0: planner's choice
     bool use_graph = true;
     bool time_kernels = true;       // per-kernel HIP-event timing of one batch per configuration
+    int autotune = 1;               // 0: default launch variants; 1: time the candidates once per
+                                    // shape and keep the choice; 2: time them on every run
+    int force_row = -1, force_col = -1, force_col_det = -1;   // >= 0: use this variant
+};
+
+// Launch variants chosen for one shape of the slice step.
+struct PassChoice {
+    int row = 0, col = 0, col_det = 0;
+    std::string log;                // candidates with their times and the choice
 };
 
 struct Timing {
@@ -28,6 +37,7 @@ struct Timing {
     int n_row = 0, n_col = 0;       // launches timed
     double ms_tf = 0;               // transmission build (all configurations)
     std::string plan;
+    std::string tune_log;           // the tuner's selection log for this run
 };
 
 RunControl &run_control();

@@ -18,9 +18,21 @@ namespace mtg {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Complex product with a fixed rounding pattern (the a.y terms rounded, the
+// a.x terms fused), so that every kernel instance rounds alike whatever the
+// compiler's contraction choices.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+#pragma clang fp contract(off)
+    const float yy = a.y * b.y, yx = a.y * b.x;
+    return make_float2(__builtin_fmaf(a.x, b.x, -yy), __builtin_fmaf(a.x, b.y, yx));
+}
+// |a|^2 with a fixed rounding pattern
+__device__ __forceinline__ float cnorm2(float2 a)
+{
+#pragma clang fp contract(off)
+    const float yy = a.y * a.y;
+    return __builtin_fmaf(a.x, a.x, yy);
 }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 // multiply by -i (forward) or +i (inverse)
@@ -128,78 +140,149 @@ __device__ __forceinline__ void dftR(float2 (&v)[E], int o)
     else if constexpr (R == 16) dft16<INV, S, E>(v, o);
 }
 
-// One Stockham stage: radix R, NS = product of the radices of earlier stages.
-// tw: W_N^m = exp(-2 pi i m / N), m in [0, N).
-template <int N, int E, int R, int NS, bool INV>
-__device__ __forceinline__ void fft_stage(float2 (&v)[E], int t, const float2 *__restrict__ tw)
+// Radix schedule of a length-N transform: E values per thread, NST stages
+// of radices R[0..NST-1] (each dividing E, product N).
+template <int N_, int E_, int R0, int R1, int R2 = 0, int R3 = 0>
+struct Radix {
+    static constexpr int N = N_, E = E_, T = N_ / E_;
+    static constexpr int NST = (R3 ? 4 : R2 ? 3 : 2);
+    static constexpr int R[4] = {R0, R1, R2, R3};
+    static_assert((R0 * R1 * (R2 ? R2 : 1) * (R3 ? R3 : 1)) == N_, "radices must multiply to N");
+    static_assert(E_ % R0 == 0 && E_ % R1 == 0 && (R2 == 0 || E_ % R2 == 0) && (R3 == 0 || E_ % R3 == 0),
+                  "each radix divides E");
+};
+
+// Schedules per length: 0 is the default, 1 the alternative the tuner may pick.
+template <int N, int SC> struct Sched;
+template <> struct Sched<64, 0>   : Radix<64, 8, 8, 8> {};
+template <> struct Sched<64, 1>   : Radix<64, 16, 16, 4> {};
+template <> struct Sched<128, 0>  : Radix<128, 8, 8, 8, 2> {};
+template <> struct Sched<128, 1>  : Radix<128, 16, 16, 8> {};
+template <> struct Sched<256, 0>  : Radix<256, 8, 8, 8, 4> {};
+template <> struct Sched<256, 1>  : Radix<256, 16, 16, 16> {};
+template <> struct Sched<512, 0>  : Radix<512, 8, 8, 8, 8> {};
+template <> struct Sched<512, 1>  : Radix<512, 16, 16, 8, 4> {};
+template <> struct Sched<1024, 0> : Radix<1024, 16, 16, 8, 8> {};
+template <> struct Sched<1024, 1> : Radix<1024, 8, 8, 8, 8, 2> {};
+template <> struct Sched<2048, 0> : Radix<2048, 16, 16, 16, 8> {};
+template <> struct Sched<2048, 1> : Radix<2048, 8, 8, 8, 8, 4> {};
+
+template <class S, int ST>
+struct StageNS {
+    static constexpr int value = StageNS<S, ST - 1>::value * S::R[ST - 1];
+};
+template <class S>
+struct StageNS<S, 0> { static constexpr int value = 1; };
+
+// One Stockham stage ST of schedule S.  tw.get<ST>(q, r) is this thread's
+// twiddle W_{NS R}^{r k} of butterfly q (forward sign; conjugated here for
+// the inverse).
+template <class S, int ST, bool INV, class TW>
+__device__ __forceinline__ void fft_stage(float2 (&v)[S::E], const TW &tw)
 {
-    constexpr int T = N / E;
-    constexpr int Q = E / R;
+    constexpr int R = S::R[ST], Q = S::E / R;
+    constexpr int NS = StageNS<S, ST>::value;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         if constexpr (NS > 1) {
-            const int j = t + T * q;
-            const int k = j & (NS - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * k * (N / (NS * R))];
+                float2 w = tw.template get<ST>(q, r);
                 if (INV) w.y = -w.y;
                 v[q + r * Q] = cmul(v[q + r * Q], w);
             }
         }
-        dftR<R, INV, Q, E>(v, q);
+        dftR<R, INV, Q, S::E>(v, q);
     }
 }
 
-// LDS index written by element (q, r) of a stage; see fft_stage.
-template <int N, int E, int R, int NS>
+// LDS index written by element (q, r) of stage ST: output j/NS*NS*R + j%NS + r*NS
+// of butterfly j = t + T*q (Stockham autosort).
+template <class S, int ST>
 __device__ __forceinline__ int stage_out_index(int t, int q, int r)
 {
-    constexpr int T = N / E;
-    const int j = t + T * q;
+    constexpr int R = S::R[ST], NS = StageNS<S, ST>::value;
+    const int j = t + S::T * q;
     return (j / NS) * NS * R + (j & (NS - 1)) + r * NS;
 }
 
-// Radix schedules.  Each radix divides E.
-template <int N> struct Sched;
-template <> struct Sched<64>   { static constexpr int E = 8;  static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
-template <> struct Sched<128>  { static constexpr int E = 8;  static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 2, 0}; };
-template <> struct Sched<256>  { static constexpr int E = 8;  static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 4, 0}; };
-template <> struct Sched<512>  { static constexpr int E = 8;  static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
-template <> struct Sched<1024> { static constexpr int E = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 8, 8, 0}; };
-template <> struct Sched<2048> { static constexpr int E = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 8, 0}; };
-
-template <int N, int ST>
-struct StageNS {
-    static constexpr int value = (ST == 0) ? 1 : StageNS<N, ST - 1>::value * Sched<N>::R[ST - 1];
+// Per-thread twiddles of all stages with NS > 1, in stage order.
+template <class S, int ST>
+struct TwOffset {
+    static constexpr int prev = (StageNS<S, ST - 1>::value > 1) ? (S::E / S::R[ST - 1]) * (S::R[ST - 1] - 1) : 0;
+    static constexpr int value = TwOffset<S, ST - 1>::value + prev;
 };
-template <int N>
-struct StageNS<N, 0> { static constexpr int value = 1; };
+template <class S>
+struct TwOffset<S, 0> { static constexpr int value = 0; };
 
-// Full transform.  lds(i) maps a sequence index to this sequence's LDS slot;
-// sync() is the barrier that orders LDS writes and reads of all threads
+// Twiddles held in registers: loaded once per thread, reused by the forward
+// and the inverse transform.
+template <class S>
+struct TwReg {
+    static constexpr int COUNT = TwOffset<S, S::NST>::value;
+    float2 w[COUNT > 0 ? COUNT : 1];
+
+    // tw: W_N^m = exp(-2 pi i m / N), m in [0, N); t: thread index in the sequence
+    template <int ST = 0>
+    __device__ __forceinline__ void load(int t, const float2 *__restrict__ tw)
+    {
+        constexpr int R = S::R[ST], NS = StageNS<S, ST>::value;
+        if constexpr (NS > 1) {
+            constexpr int Q = S::E / R, off = TwOffset<S, ST>::value;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int k = (t + S::T * q) & (NS - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[off + q * (R - 1) + r - 1] = tw[r * k * (S::N / (NS * R))];
+            }
+        }
+        if constexpr (ST + 1 < S::NST) load<ST + 1>(t, tw);
+    }
+    template <int ST>
+    __device__ __forceinline__ float2 get(int q, int r) const
+    {
+        return w[TwOffset<S, ST>::value + q * (S::R[ST] - 1) + r - 1];
+    }
+};
+
+// Twiddles read from the cache-resident table at each use: fewer registers.
+template <class S>
+struct TwTable {
+    const float2 *__restrict__ tab;
+    int t;
+    __device__ __forceinline__ void load(int t_, const float2 *__restrict__ tw)
+    {
+        t = t_;
+        tab = tw;
+    }
+    template <int ST>
+    __device__ __forceinline__ float2 get(int q, int r) const
+    {
+        constexpr int R = S::R[ST], NS = StageNS<S, ST>::value;
+        return tab[r * ((t + S::T * q) & (NS - 1)) * (S::N / (NS * R))];
+    }
+};
+
+// Full transform.  lds_idx(i) maps a sequence index to this sequence's LDS
+// slot; sync() is the barrier that orders LDS writes and reads of all threads
 // sharing the buffer.  The data starts and ends in the t + T*e layout.
-template <int N, bool INV, int ST = 0, class LdsIdx, class Sync>
-__device__ __forceinline__ void fft_run(float2 (&v)[Sched<N>::E], int t, const float2 *__restrict__ tw,
-                                        float2 *lds, LdsIdx lds_idx, Sync sync)
+template <class S, bool INV, int ST = 0, class TW, class LdsIdx, class Sync>
+__device__ __forceinline__ void fft_run(float2 (&v)[S::E], int t, const TW &tw, float2 *lds, LdsIdx lds_idx,
+                                        Sync sync)
 {
-    constexpr int E = Sched<N>::E;
-    constexpr int R = Sched<N>::R[ST];
-    constexpr int NS = StageNS<N, ST>::value;
-    constexpr int T = N / E;
-    fft_stage<N, E, R, NS, INV>(v, t, tw);
-    if constexpr (ST + 1 < Sched<N>::NST) {
-        constexpr int Q = E / R;
+    fft_stage<S, ST, INV>(v, tw);
+    if constexpr (ST + 1 < S::NST) {
+        constexpr int R = S::R[ST], Q = S::E / R;
         sync();   // previous readers of the buffer are done
 #pragma unroll
         for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                lds[lds_idx(stage_out_index<N, E, R, NS>(t, q, r))] = v[q + r * Q];
+                lds[lds_idx(stage_out_index<S, ST>(t, q, r))] = v[q + r * Q];
         sync();
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = lds[lds_idx(t + T * e)];
-        fft_run<N, INV, ST + 1>(v, t, tw, lds, lds_idx, sync);
+        for (int e = 0; e < S::E; ++e) v[e] = lds[lds_idx(t + S::T * e)];
+        fft_run<S, INV, ST + 1>(v, t, tw, lds, lds_idx, sync);
     }
 }
 

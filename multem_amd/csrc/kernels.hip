@@ -11,6 +11,9 @@
 #include "kernels.hpp"
 #include "fft_core.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace mtg {
 
 namespace {
@@ -54,13 +57,16 @@ __device__ __forceinline__ void map_rows(int wg, int G, int M, int &rg, int &mem
 
 // ---------------------------------------------------------------------------
 // Row pass: RW rows of length N per workgroup, T = N/E threads per row.
-template <int N, int RW>
-__global__ __launch_bounds__(RW *(N / Sched<N>::E)) void k_row_pass(RowArgs a)
+// S: radix schedule; F: compile-time RowFlags of the hot paths, or -1 for
+// a.flags at run time; TWT: twiddles from the table (1) or in registers (0).
+template <class S, int RW, int F, int TWT>
+__global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgs a)
 {
-    constexpr int E = Sched<N>::E;
-    constexpr int T = N / E;
+    constexpr int N = S::N, E = S::E, T = S::T;
+    using TW = std::conditional_t<TWT != 0, TwTable<S>, TwReg<S>>;
     constexpr int PADN = N + N / 8;
     __shared__ float2 lds[RW * PADN];
+    const int flags = (F < 0) ? a.flags : F;
 
     const int tid = threadIdx.x;
     const int t = tid % T, r = tid / T;
@@ -72,24 +78,31 @@ __global__ __launch_bounds__(RW *(N / Sched<N>::E)) void k_row_pass(RowArgs a)
     auto sync = [] { __syncthreads(); };
 
     const long long off = (long long)member * a.member_stride + (long long)row * N;
-    float2 v[E];
-    if (a.flags & ROW_LOAD) {
+    float2 v[E], tv[E];
+    // issue the wave and transmission loads together, then the twiddles
+    if (flags & ROW_LOAD) {
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = a.src[off + t + T * e];
     } else {
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = make_float2(1.f, 0.f);
     }
-    if (a.flags & ROW_IFFT) fft_run<N, true>(v, t, a.tw, lp, idx, sync);
-    if (a.flags & ROW_MUL_T) {
+    if (flags & ROW_MUL_T) {
         const int slot = a.members[member].tslot;
         const float2 *T_s = a.trans + (long long)slot * a.tslot_stride + (long long)a.slice * a.nrows * N
                             + (long long)row * N;
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = cmul(T_s[t + T * e], v[e]);
+        for (int e = 0; e < E; ++e) tv[e] = T_s[t + T * e];
     }
-    if (a.flags & ROW_FFT) fft_run<N, false>(v, t, a.tw, lp, idx, sync);
-    if (a.flags & ROW_STORE) {
+    TW tw;
+    tw.load(t, a.tw);
+    if (flags & ROW_IFFT) fft_run<S, true>(v, t, tw, lp, idx, sync);
+    if (flags & ROW_MUL_T) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cmul(tv[e], v[e]);
+    }
+    if (flags & ROW_FFT) fft_run<S, false>(v, t, tw, lp, idx, sync);
+    if (flags & ROW_STORE) {
 #pragma unroll
         for (int e = 0; e < E; ++e) a.dst[off + t + T * e] = v[e];
     }
@@ -98,14 +111,16 @@ __global__ __launch_bounds__(RW *(N / Sched<N>::E)) void k_row_pass(RowArgs a)
 // ---------------------------------------------------------------------------
 // Column pass: C adjacent columns of length N per workgroup.  Thread (c, t)
 // holds elements ix = t + T*e of column tile*C + c.
-template <int N, int C, bool PROBE, bool DET>
-__global__ __launch_bounds__(C *(N / Sched<N>::E)) void k_col_pass(ColArgs a)
+// S, F, TWT as for the row pass.
+template <class S, int C, int F, bool PROBE, bool DET, int TWT>
+__global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
 {
-    constexpr int E = Sched<N>::E;
-    constexpr int T = N / E;
+    constexpr int N = S::N, E = S::E, T = S::T;
+    using TW = std::conditional_t<TWT != 0, TwTable<S>, TwReg<S>>;
     constexpr int NWAVE = (C * T + 63) / 64;
     __shared__ float2 lds[N * C];
     __shared__ double red[DET ? NWAVE * kMaxDet : 1];
+    const int flags = (F < 0) ? a.flags : F;
 
     const int tid = threadIdx.x;
     const int c = tid % C, t = tid / C;
@@ -118,6 +133,14 @@ __global__ __launch_bounds__(C *(N / Sched<N>::E)) void k_col_pass(ColArgs a)
     const int igy = signed_freq(col, a.ny);
 
     float2 v[E];
+    if constexpr (!PROBE) {
+        if (flags & COL_LOAD) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = a.src[base + (long long)(t + T * e) * a.ncols];
+        }
+    }
+    TW tw;
+    tw.load(t, a.tw);
     if constexpr (PROBE) {
         // Convergent probe in Fourier space (exp_i_chi, src/cgpu_fcns.cuh:1949-1979):
         // float aperture mask, FP64 phase reduced mod 2 pi, FP32 sincos.
@@ -146,21 +169,16 @@ __global__ __launch_bounds__(C *(N / Sched<N>::E)) void k_col_pass(ColArgs a)
             }
             v[e] = acc;
         }
-    } else {
-        if (a.flags & COL_LOAD) {
-#pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = a.src[base + (long long)(t + T * e) * a.ncols];
-        }
     }
-    if (a.flags & COL_FFT) fft_run<N, false>(v, t, a.tw, lds, idx, sync);
-    if (a.flags & (COL_MUL_P | COL_BWL)) {
-        const float2 pyv = (a.flags & COL_MUL_P) ? a.py[col] : make_float2(a.inv_nxy, 0.f);
+    if (flags & COL_FFT) fft_run<S, false>(v, t, tw, lds, idx, sync);
+    if (flags & (COL_MUL_P | COL_BWL)) {
+        // the propagator factors are tiny tables, cache-resident across the batch
+        const float2 pyv = (flags & COL_MUL_P) ? cscale(a.py[col], a.p_scale) : make_float2(a.inv_nxy, 0.f);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const int ix = t + T * e;
-            float2 p = (a.flags & COL_MUL_P) ? cscale(cmul(a.px[ix], pyv), a.p_scale) : pyv;
-            if (a.flags & COL_BWL) {
-                const float f = bwl_factor(g2_float(signed_freq(ix, a.nx), igy, a.dgx_f, a.dgy_f),
+            float2 p = (flags & COL_MUL_P) ? cmul(a.px[t + T * e], pyv) : pyv;
+            if (flags & COL_BWL) {
+                const float f = bwl_factor(g2_float(signed_freq(t + T * e, a.nx), igy, a.dgx_f, a.dgy_f),
                                            a.bwl_alpha, a.gl2_max);
                 p = cscale(p, f);
             }
@@ -177,8 +195,8 @@ __global__ __launch_bounds__(C *(N / Sched<N>::E)) void k_col_pass(ColArgs a)
         for (int e = 0; e < E; ++e) {
             const int ix = t + T * e;
             const float g2f = g2_float(signed_freq(ix, a.nx), igy, a.dgx_f, a.dgy_f);
-            float m2 = v[e].x * v[e].x + v[e].y * v[e].y;
-            if (a.flags & COL_BWL) {
+            float m2 = cnorm2(v[e]);
+            if (flags & COL_BWL) {
                 const float f = bwl_factor(g2f, a.bwl_alpha, a.gl2_max);
                 m2 *= f * f;
             }
@@ -199,8 +217,8 @@ __global__ __launch_bounds__(C *(N / Sched<N>::E)) void k_col_pass(ColArgs a)
             a.det_part[(((long long)member * a.n_thk + a.ithk) * a.n_det + tid) * ntile + tile] = s;
         }
     }
-    if (a.flags & COL_IFFT) fft_run<N, true>(v, t, a.tw, lds, idx, sync);
-    if (a.flags & COL_STORE) {
+    if (flags & COL_IFFT) fft_run<S, true>(v, t, tw, lds, idx, sync);
+    if (flags & COL_STORE) {
 #pragma unroll
         for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * a.ncols] = v[e];
     }
@@ -431,30 +449,90 @@ __global__ void k_shift_complex(const float2 *src, float2 *dst, int nx, int ny)
 // ---------------------------------------------------------------------------
 // launchers
 
-int col_tile_width(int nx) { return nx >= 2048 ? 8 : 16; }
+int col_tile_width(int nx) { return col_variant(nx, 0).tile; }
+int row_group(int ny) { return row_variant(ny, 0).tile; }
 
-int row_group(int ny)
+namespace {
+
+constexpr int kRowSlice = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
+constexpr int kRowStart = ROW_MUL_T | ROW_FFT | ROW_STORE;
+constexpr int kColSlice = COL_LOAD | COL_FFT | COL_MUL_P | COL_IFFT | COL_STORE;
+constexpr int kColLast = COL_LOAD | COL_FFT | COL_MUL_P;
+constexpr int kColProbe = COL_IFFT | COL_STORE;
+
+// variant V of the row pass at length N with compile-time flags F
+template <int N, int V, int F>
+hipError_t row_go(const RowArgs &a, hipStream_t s)
 {
-    switch (ny) {
-    case 64: return 32;
-    case 128: return 16;
-    case 256: return 8;
-    case 512: return 4;
-    case 1024: return 4;
-    default: return 2;
-    }
+    constexpr PassVariant pv = row_variant(N, V);
+    using S = Sched<N, pv.sched>;
+    static_assert(S::E == sched_elems(N, pv.sched) && pv.tile * S::T <= 1024, "row variant");
+    if (a.nrows % pv.tile) return hipErrorInvalidValue;
+    const int grid = (a.nrows / pv.tile) * a.n_members;
+    hipLaunchKernelGGL((k_row_pass<S, pv.tile, F, pv.tw_table>), dim3(grid), dim3(pv.tile * S::T), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int N, int F, int... V>
+hipError_t row_pick(const RowArgs &a, hipStream_t s, std::integer_sequence<int, V...>)
+{
+    hipError_t e = hipErrorInvalidValue;
+    ((a.variant == V ? (e = row_go<N, V, F>(a, s), 0) : 0), ...);
+    return e;
 }
 
 template <int N>
-static hipError_t row_launch(const RowArgs &a, hipStream_t s)
+hipError_t row_launch(const RowArgs &a, hipStream_t s)
 {
-    constexpr int RW = (N == 64) ? 32 : (N == 128) ? 16 : (N == 256) ? 8 : (N <= 1024) ? 4 : 2;
-    constexpr int threads = RW * (N / Sched<N>::E);
-    if (a.nrows % RW) return hipErrorInvalidValue;
-    const int grid = (a.nrows / RW) * a.n_members;
-    hipLaunchKernelGGL((k_row_pass<N, RW>), dim3(grid), dim3(threads), 0, s, a);
-    return hipGetLastError();
+    if (a.flags == kRowSlice)
+        return row_pick<N, kRowSlice>(a, s, std::make_integer_sequence<int, kPassVariants>{});
+    if (a.flags == kRowStart) return row_go<N, 0, kRowStart>(a, s);
+    return row_go<N, 0, -1>(a, s);
 }
+
+template <int N, int V, int F, bool PROBE, bool DET>
+hipError_t col_go(const ColArgs &a, hipStream_t s)
+{
+    constexpr PassVariant pv = col_variant(N, V);
+    using S = Sched<N, pv.sched>;
+    static_assert(S::E == sched_elems(N, pv.sched) && pv.tile * S::T <= 1024, "column variant");
+    if constexpr (DET && pv.tile != col_tile0(N)) {
+        return hipErrorInvalidValue;   // the detector partials assume the default width
+    } else {
+        if (a.ncols % pv.tile) return hipErrorInvalidValue;
+        const int grid = (a.ncols / pv.tile) * a.n_members;
+        hipLaunchKernelGGL((k_col_pass<S, pv.tile, F, PROBE, DET, pv.tw_table>), dim3(grid), dim3(pv.tile * S::T), 0,
+                           s, a);
+        return hipGetLastError();
+    }
+}
+
+template <int N, int F, bool DET, int... V>
+hipError_t col_pick(const ColArgs &a, hipStream_t s, std::integer_sequence<int, V...>)
+{
+    hipError_t e = hipErrorInvalidValue;
+    ((a.variant == V ? (e = col_go<N, V, F, false, DET>(a, s), 0) : 0), ...);
+    return e;
+}
+
+template <int N>
+hipError_t col_launch(const ColArgs &a, bool probe, bool det, hipStream_t s)
+{
+    constexpr auto all = std::make_integer_sequence<int, kPassVariants>{};
+    if (probe) {
+        if (a.flags == kColProbe) return col_go<N, 0, kColProbe, true, false>(a, s);
+        return col_go<N, 0, -1, true, false>(a, s);
+    }
+    if (det) {
+        if (a.flags == kColLast) return col_pick<N, kColLast, true>(a, s, all);
+        if (a.flags == kColSlice) return col_go<N, 0, kColSlice, false, true>(a, s);
+        return col_go<N, 0, -1, false, true>(a, s);
+    }
+    if (a.flags == kColSlice) return col_pick<N, kColSlice, false>(a, s, all);
+    return col_go<N, 0, -1, false, false>(a, s);
+}
+
+} // namespace
 
 hipError_t launch_row_pass(int n, const RowArgs &a, hipStream_t s)
 {
@@ -467,19 +545,6 @@ hipError_t launch_row_pass(int n, const RowArgs &a, hipStream_t s)
     case 2048: return row_launch<2048>(a, s);
     }
     return hipErrorInvalidValue;
-}
-
-template <int N>
-static hipError_t col_launch(const ColArgs &a, bool probe, bool det, hipStream_t s)
-{
-    constexpr int C = (N >= 2048) ? 8 : 16;
-    constexpr int threads = C * (N / Sched<N>::E);
-    if (a.ncols % C) return hipErrorInvalidValue;
-    const int grid = (a.ncols / C) * a.n_members;
-    if (probe) hipLaunchKernelGGL((k_col_pass<N, C, true, false>), dim3(grid), dim3(threads), 0, s, a);
-    else if (det) hipLaunchKernelGGL((k_col_pass<N, C, false, true>), dim3(grid), dim3(threads), 0, s, a);
-    else hipLaunchKernelGGL((k_col_pass<N, C, false, false>), dim3(grid), dim3(threads), 0, s, a);
-    return hipGetLastError();
 }
 
 hipError_t launch_col_pass(int n, const ColArgs &a, bool probe, bool det, hipStream_t s)

@@ -51,6 +51,7 @@ struct RowArgs {
     long long tslot_stride;       // complex elements between stacks
     int slice;
     const Member *members;
+    int variant;                  // launch variant of the full slice step (kRowSlice flags)
 };
 
 enum ColFlags : int {
@@ -86,6 +87,7 @@ struct ColArgs {
     const Member *members;
     const Beam *beams;
     ProbeParams probe;
+    int variant;                  // launch variant of the slice step / last slice
 };
 
 struct TfArgs {
@@ -137,7 +139,63 @@ hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, 
                               double *m2, hipStream_t s);
 hipError_t launch_shift_complex(const float2 *src, float2 *dst, int nx, int ny, hipStream_t s);
 
-int col_tile_width(int nx);   // columns per column-pass workgroup
-int row_group(int ny);        // rows per row-pass workgroup
+// Launch variants of the slice passes.  Variant 0 is the default; the tuner
+// may pick another.  Variants differ in the radix schedule (different
+// rounding), the workgroup tile (rows or columns per workgroup: identical
+// results) and where twiddles live (registers or the cached table:
+// identical results).
+struct PassVariant {
+    int sched;      // Sched<N, sched>
+    int tile;       // rows (row pass) or columns (column pass) per workgroup
+    int tw_table;   // 1: twiddles read from the table at each use
+};
+constexpr int kPassVariants = 7;
+
+// elements per thread of schedule sc at length n (Sched<n, sc>::E)
+constexpr int sched_elems(int n, int sc)
+{
+    return sc == 0 ? (n >= 1024 ? 16 : 8) : (n >= 1024 ? 8 : 16);
+}
+constexpr int row_tile0(int n) { return n == 64 ? 32 : n == 128 ? 16 : n == 256 ? 8 : n <= 1024 ? 4 : 2; }
+constexpr int col_tile0(int n) { return n >= 2048 ? 8 : 16; }
+
+// Row pass, v: 0 default; 1 table twiddles; 2 twice the rows; 3 alternative
+// schedule at the same thread count; 4 = 3 with table twiddles; 5 half the
+// rows; 6 = 2 with table twiddles.
+constexpr PassVariant row_variant(int n, int v)
+{
+    const int r = row_tile0(n), alt = r * sched_elems(n, 1) / sched_elems(n, 0);
+    switch (v) {
+    case 1: return PassVariant{0, r, 1};
+    case 2: return PassVariant{0, 2 * r, 0};
+    case 3: return PassVariant{1, alt, 0};
+    case 4: return PassVariant{1, alt, 1};
+    case 5: return PassVariant{0, r > 1 ? r / 2 : 1, 0};
+    case 6: return PassVariant{0, 2 * r, 1};
+    default: return PassVariant{0, r, 0};
+    }
+}
+// Column pass, v: 0 default; 1 table twiddles; 2 half the columns;
+// 3 alternative schedule (columns capped at 1024 threads per workgroup);
+// 4 = 3 with table twiddles; 5 a quarter of the columns; 6 = 2 with table
+// twiddles.  Detector epilogues use only variants with the default width,
+// which fixes the layout of the detector partials.
+constexpr PassVariant col_variant(int n, int v)
+{
+    const int c = col_tile0(n), t1 = n / sched_elems(n, 1);
+    const int c1 = c * t1 > 1024 ? 1024 / t1 : c;
+    switch (v) {
+    case 1: return PassVariant{0, c, 1};
+    case 2: return PassVariant{0, c / 2, 0};
+    case 3: return PassVariant{1, c1, 0};
+    case 4: return PassVariant{1, c1, 1};
+    case 5: return PassVariant{0, c / 4, 0};
+    case 6: return PassVariant{0, c / 2, 1};
+    default: return PassVariant{0, c, 0};
+    }
+}
+
+int col_tile_width(int nx);   // columns per column-pass workgroup (variant 0; the detector partial layout)
+int row_group(int ny);        // rows per row-pass workgroup (variant 0)
 
 } // namespace mtg

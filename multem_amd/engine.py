@@ -50,6 +50,9 @@ def lib():
         L.mt_hip_kernel_timing.argtypes = [dp, P(C.c_int), dp, P(C.c_int), dp]
         L.mt_hip_synchronize.argtypes = [C.c_int]
         L.mt_hip_atom_table.argtypes = [C.c_int, C.c_double] + [dp] * 6
+        L.mt_hip_set_autotune.argtypes = [C.c_int]
+        L.mt_hip_set_pass_variants.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.mt_hip_tune_log.argtypes = [C.c_char_p, C.c_size_t]
         if L.mt_hip_abi_version() != capi.MT_ABI_VERSION:
             raise RuntimeError("libmultem_hip ABI mismatch")
         _lib = L
@@ -140,6 +143,25 @@ def atom_table(Z, dR_min):
 
 def set_batch(batch=0, use_graph=True):
     lib().mt_hip_set_batch(int(batch), int(bool(use_graph)))
+
+
+PASS_VARIANTS = 7
+
+
+def set_autotune(mode=1):
+    """0: fixed launch variants; 1: tune once per shape (default); 2: tune every run."""
+    lib().mt_hip_set_autotune(int(mode))
+
+
+def set_pass_variants(row=-1, col=-1, col_det=-1):
+    """Force slice-step launch variants (-1: tuner's choice)."""
+    lib().mt_hip_set_pass_variants(int(row), int(col), int(col_det))
+
+
+def tune_log():
+    buf = C.create_string_buffer(2048)
+    _check(lib().mt_hip_tune_log(buf, 2048))
+    return buf.value.decode()
 
 
 def set_memory_budget(nbytes):

@@ -107,16 +107,101 @@ def test_stem_small():
 
 
 def test_stem_batch_and_graph_invariance():
+    # fixed launch variants: the batch width and the graph replay alone must not
+    # change a single bit
     p = small_stem(ns=5, nconf=1)
-    engine.set_batch(0, True)
-    a = engine.run_multislice(p).image_tot.copy()
-    engine.set_batch(3, True)
-    b = engine.run_multislice(p).image_tot.copy()
-    engine.set_batch(7, False)
-    c = engine.run_multislice(p).image_tot.copy()
-    engine.set_batch(0, True)
+    engine.set_pass_variants(0, 0, 0)
+    try:
+        engine.set_batch(0, True)
+        a = engine.run_multislice(p).image_tot.copy()
+        engine.set_batch(3, True)
+        b = engine.run_multislice(p).image_tot.copy()
+        engine.set_batch(7, False)
+        c = engine.run_multislice(p).image_tot.copy()
+    finally:
+        engine.set_batch(0, True)
+        engine.set_pass_variants(-1, -1, -1)
     assert np.array_equal(a, b)
     assert np.array_equal(a, c)
+
+
+# launch variants with the default radix schedule (Sched<N, 0>): same
+# arithmetic in the same order as variant 0
+SAME_SCHEDULE = (0, 1, 2, 5, 6)
+
+
+def _run_forced(p, row, col, col_det):
+    engine.set_pass_variants(row, col, col_det)
+    try:
+        return engine.run_multislice(p)
+    finally:
+        engine.set_pass_variants(-1, -1, -1)
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_stem_pass_variant_sweep(n):
+    """Every launch variant of the row, column and detector column passes
+    against the FP64 oracle; variants that keep the radix schedule are
+    bit-identical to variant 0."""
+    p = small_stem(n=n, ns=3, nconf=1)
+    r64 = oracle.run(p, "f64")
+    base = _run_forced(p, 0, 0, 0).image_tot.copy()
+    assert rel_l2(base, r64.image_tot) < 1e-4
+    for v in range(1, engine.PASS_VARIANTS):
+        try:
+            img = _run_forced(p, v, v, v).image_tot.copy()
+            vdet = v
+        except engine.EngineError as e:
+            # detector epilogues keep the default column width
+            assert e.status == 1, e
+            img = _run_forced(p, v, v, 0).image_tot.copy()
+            vdet = 0
+        e64 = rel_l2(img, r64.image_tot)
+        print("n=%d variant %d (det %d): rel L2 vs FP64 %.3e" % (n, v, vdet, e64))
+        assert e64 < 1e-4, v
+        if v in SAME_SCHEDULE:
+            assert np.array_equal(img, base), v
+
+
+def test_exit_wave_pass_variants():
+    p = configs.si_ewrs_256()
+    r32 = oracle.run(p, "f32")
+    base = _run_forced(p, 0, 0, -1).psi_coh.copy()
+    for v in range(1, engine.PASS_VARIANTS):
+        psi = _run_forced(p, v, v, -1).psi_coh.copy()
+        e32 = rel_l2(psi, r32.psi_coh)
+        print("exit wave variant %d: rel L2 vs FP32 restatement %.3e" % (v, e32))
+        assert e32 < 1e-5, v
+        if v in SAME_SCHEDULE:
+            assert np.array_equal(psi, base), v
+
+
+def test_autotune_log_and_cache():
+    p = small_stem(ns=4, nconf=1)
+    engine.set_autotune(2)
+    try:
+        engine.run_multislice(p)
+        log = engine.tune_log()
+        plan = engine.last_plan()
+    finally:
+        engine.set_autotune(1)
+    print(log)
+    for name in ("row", "col", "col_det"):
+        assert (" %s=v" % name) in " " + log
+    chosen = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in plan.split() if kv.endswith(tuple("0123456789"))
+              and "_variant=" in kv}
+    assert set(chosen) == {"row_variant", "col_variant", "col_det_variant"}
+    assert all(0 <= v < engine.PASS_VARIANTS for v in chosen.values())
+    engine.run_multislice(p)           # mode 1: the same shape is served from the cache
+    engine.run_multislice(p)
+    assert engine.tune_log().startswith("cached: ")
+
+
+def test_forced_variant_out_of_range():
+    p = small_stem(ns=2, nconf=1)
+    with pytest.raises(engine.EngineError) as ei:
+        _run_forced(p, engine.PASS_VARIANTS, -1, -1)
+    assert ei.value.status == 1
 
 
 def test_cbed_small():
