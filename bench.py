@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=0, help="waves per batch (0: planner)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-ns", type=int, default=8)
+    ap.add_argument("--cpu-ns", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM counters")
     return ap.parse_args()
