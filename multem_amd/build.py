@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmultem_hip.so")
-SOURCES = ["kernels.hip", "engine.cpp", "host_model.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "surrogate.hip", "engine.cpp", "host_model.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code.
@@ -20,19 +20,25 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 
 
 def build(verbose=False):
+    """Compiles every source (in parallel) and links the library."""
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    objs, procs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(LIBDIR, src + ".o")
         cmd = [HIPCC] + FLAGS + ["-c", path, "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", path, "-o", obj]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError("compilation failed: " + src)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
+    failed = []
+    for src, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src)
+    if failed:
+        raise RuntimeError("compilation failed: " + ", ".join(failed))
     tmp = LIB + ".tmp"
     r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs,
                        capture_output=True, text=True)

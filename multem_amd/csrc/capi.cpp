@@ -43,6 +43,11 @@ mt_status guarded(const mt_system_conf *sys, const mt_input_multislice *in, F &&
 
 } // namespace
 
+namespace mtg {
+// error message of the calling thread, shared with the surrogate entries
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+} // namespace mtg
+
 static_assert(MT_PASS_VARIANTS == mtg::kPassVariants, "variant count of the C ABI");
 
 extern "C" {

@@ -15,9 +15,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_exports_every_declared_symbol(engine_lib):
-    hdr = open(os.path.join(ROOT, "include", "multem_hip.h")).read()
-    declared = set(re.findall(r"\b(mt_hip_\w+)\s*\(", hdr))
-    assert len(declared) >= 15
+    inc = os.path.join(ROOT, "include")
+    declared = set()
+    for h in sorted(os.listdir(inc)):
+        if h.endswith(".h"):
+            declared |= set(re.findall(r"\b(mt_hip_\w+)\s*\(", open(os.path.join(inc, h)).read()))
+    assert len(declared) >= 30
     nm = subprocess.run(["nm", "-D", "--defined-only", engine.LIB_PATH], capture_output=True, text=True, check=True).stdout
     exported = set(re.findall(r"\bT (mt_hip_\w+)", nm))
     assert declared <= exported, declared - exported

@@ -1,0 +1,132 @@
+"""Pins of the surrogate network's CPU oracle (oracle/surrogate_oracle.py).
+
+Each check is independent of the code it pins: bfloat16 rounding against an
+exact rational restatement of round-to-nearest-even, the convolution against
+explicit index loops, the two gradient kernels against the adjoint
+identities, the full backward pass against central finite differences of the
+FP64 network, and Adam's first step against its closed form.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import surrogate_oracle as so
+
+
+def _bf16_exact(x):
+    """Nearest bfloat16 of the float32 value x by exact rational comparison."""
+    f = Fraction(float(np.float32(x)))
+    if f == 0:
+        return 0.0
+    sign = -1 if f < 0 else 1
+    a = abs(f)
+    e = 0
+    while a >= 2:
+        a /= 2
+        e += 1
+    while a < 1:
+        a *= 2
+        e -= 1
+    # a in [1, 2): 7 fraction bits
+    scaled = a * 128
+    lo = int(scaled)
+    rem = scaled - lo
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and lo % 2 == 1):
+        lo += 1
+    return sign * float(Fraction(lo, 128) * (Fraction(2) ** e))
+
+
+def test_bf16_rounding_matches_exact_rne():
+    rng = np.random.default_rng(1)
+    bits = rng.integers(0, 2 ** 31, size=3000, dtype=np.int64).astype(np.uint32)
+    vals = bits.view(np.float32)
+    vals = vals[np.isfinite(vals) & (np.abs(vals) > 1e-30) & (np.abs(vals) < 1e30)]
+    # exact ties: the low 16 bits 0x8000, with even and odd kept parts
+    ties = np.array([0x3F808000, 0x3F818000, 0xBF808000, 0x40490000 | 0x8000], dtype=np.uint32).view(np.float32)
+    vals = np.concatenate([vals[:2000], ties, np.float32([1.0, -2.5, 3.1415927, 65504.0])])
+    got = so.bf16(vals)
+    want = np.array([_bf16_exact(v) for v in vals])
+    assert np.array_equal(got, want)
+
+
+def _conv_loops(x, W, b):
+    B, H, Wd, cin = x.shape
+    cout = W.shape[0]
+    out = np.zeros((B, H, Wd, cout))
+    for n in range(B):
+        for y in range(H):
+            for xx in range(Wd):
+                for co in range(cout):
+                    s = b[co]
+                    for ky in range(3):
+                        for kx in range(3):
+                            for ci in range(cin):
+                                s += x[n, (y + ky - 1) % H, (xx + kx - 1) % Wd, ci] * W[co, ky, kx, ci]
+                    out[n, y, xx, co] = s
+    return out
+
+
+def test_conv3x3_periodic_matches_loops():
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((2, 5, 4, 3))
+    W = rng.standard_normal((2, 3, 3, 3))
+    b = rng.standard_normal(2)
+    assert np.allclose(so.conv3x3(x, W, b), _conv_loops(x, W, b), rtol=1e-12, atol=1e-12)
+
+
+def test_conv_gradients_are_adjoints():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 6, 5, 4))
+    W = rng.standard_normal((3, 3, 3, 4))
+    dy = rng.standard_normal((2, 6, 5, 3))
+    lhs = np.sum(so.conv3x3(x, W) * dy)
+    assert np.isclose(lhs, np.sum(x * so.conv3x3_grad_input(dy, W)), rtol=1e-12)
+    dW, db = so.conv3x3_grad_weight(x, dy)
+    assert np.isclose(lhs, np.sum(W * dW), rtol=1e-12)
+    assert np.allclose(db, dy.reshape(-1, 3).sum(0))
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    nb, ch = 2, 4
+    theta = so.init_params(7, nb, ch).astype(np.float64)
+    theta += 0.05 * rng.standard_normal(theta.size)   # nonzero biases, nontrivial second convs
+    x = rng.standard_normal((2, 6, 6, 2))
+    t = rng.standard_normal((2, 6, 6))
+    loss, g = so.loss_and_grad(theta, x, t, nb, ch)
+    idx = rng.choice(theta.size, size=40, replace=False)
+    h = 1e-6
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (so.loss_and_grad(tp, x, t, nb, ch)[0] - so.loss_and_grad(tm, x, t, nb, ch)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_param_layout():
+    assert so.n_params(6, 32) == 32 * 9 * 2 + 32 + 12 * (32 * 9 * 32 + 32) + 9 * 32 + 1
+    theta = np.arange(so.n_params(1, 4), dtype=np.float64)
+    P = so.unpack(theta, 1, 4)
+    assert [w.shape for w, _ in P] == [(4, 3, 3, 2), (4, 3, 3, 4), (4, 3, 3, 4), (1, 3, 3, 4)]
+    assert P[1][0].ravel()[0] == 4 * 9 * 2 + 4
+
+
+def test_adam_first_step_closed_form():
+    g = np.array([0.5, -2.0, 1e-3])
+    th, m, v = so.adam_step(np.zeros(3), np.zeros(3), np.zeros(3), g, 1, 0.01)
+    # bias-corrected first step is lr * g / (|g| + eps)
+    assert np.allclose(th, -0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+
+
+@pytest.mark.parametrize("rounding", ["none", "bf16"])
+def test_identity_at_zero_head(rounding):
+    # with the head's weights and bias at zero the network returns channel 0
+    theta = so.init_params(0, 2, 4).astype(np.float64)
+    P = so.unpack(theta, 2, 4)
+    P[-1][0][...] = 0.0
+    P[-1][1][...] = 0.0
+    x = np.random.default_rng(5).standard_normal((1, 4, 4, 2))
+    y, _ = so.forward(theta, x, 2, 4, rounding)
+    assert np.array_equal(y, x[..., 0])
