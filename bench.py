@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ns", type=int, default=16)
+    ap.add_argument("--no-group", action="store_true", help="skip the uneven-size group measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM counters")
     return ap.parse_args()
@@ -64,6 +65,40 @@ def cpu_baseline(args):
     return {"value": ps / dt, "unit": "probe·slices/s", "cores": cores, "kind": "port",
             "sample": "Au[001] 512^2, %dx%d probes x 1 configuration x %d slices (%d probe.slices), "
                       "FP32 C oracle, %.1f s" % (args.cpu_ns, args.cpu_ns, nsl, ps, dt)}
+
+
+def group_measurement():
+    """SrTiO3 [001] at 256^2 / 512^2 / 1024^2 (21 / 51 / 101 slices), 16x16
+    probes and 4 phonon configurations each, 256 probes per batch: one group
+    run (one hipGraph launch per round) against the three runs one after the
+    other."""
+    from multem_amd import configs, engine
+    ps = configs.sto_group(ns=16, nconf=4)
+    nsl = [len(engine.spec_slicing(p)) for p in ps]
+    work = sum(16 * 16 * 4 * n for n in nsl)
+    engine.set_batch(256, True)
+    try:
+        engine.run_multislice_group(ps)          # warm-up (tuning, graphs)
+        sync()
+        t0 = time.perf_counter()
+        engine.run_multislice_group(ps)
+        sync()
+        t_group = time.perf_counter() - t0
+        plan = engine.last_plan()
+        for p in ps:
+            engine.run_multislice(p)
+        sync()
+        t0 = time.perf_counter()
+        for p in ps:
+            engine.run_multislice(p)
+        sync()
+        t_seq = time.perf_counter() - t0
+    finally:
+        engine.set_batch(0, True)
+    return {"workload": "sto_group: SrTiO3[001] 256^2/512^2/1024^2, %s slices, 16x16 probes x 4 configs each, "
+                        "256 probes per batch" % "/".join(str(n) for n in nsl),
+            "probe_slices": work, "value": work / t_group, "unit": "probe·slices/s",
+            "sequential_value": work / t_seq, "plan": plan}
 
 
 def copy_calibration():
@@ -187,6 +222,8 @@ def main():
             "slice_loop_alg_GBs": slice_loop_gbs,
         },
     }
+    if not args.no_group:
+        out["uneven_group"] = group_measurement()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))

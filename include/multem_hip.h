@@ -193,6 +193,17 @@ mt_status mt_hip_run_multislice(const mt_system_conf *sys, const mt_input_multis
 mt_status mt_hip_run_multislice_f32(const mt_system_conf *sys, const mt_input_multislice *in,
                                     mt_output_multislice *out);
 
+/* Several STEM inputs run together: grid sizes, slice counts, scans and
+   configuration counts may differ.  Each round advances every input with
+   work left by one batch of probes through all its slices, and the round's
+   device work (one branch per input, each with its own tuned kernels) is a
+   single hipGraph launch.  out[i] is filled as mt_hip_run_multislice would
+   fill it for in[i] alone (the caller provides the arrays that
+   mt_hip_output_dims reports).  No counterpart in the reference, which runs
+   one specimen per call (Multislice::operator(), src/tem_simulation.cuh:51-97). */
+mt_status mt_hip_run_multislice_group(const mt_system_conf *sys, const mt_input_multislice *const *in, int n,
+                                      mt_output_multislice *out);
+
 /* ---- per-piece entries (same input struct) --------------------------- */
 
 /* Slices of the (undisplaced) specimen.  Replaces ilc_spec_slicing

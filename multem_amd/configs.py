@@ -3,6 +3,7 @@
 si_ewrs_256     Si [001] 4x4x10 cells, 256^2, 300 kV plane wave, exit wave (EWRS)
 au_stem_512     Au [001] 8x8x20 cells, 512^2, STEM ADF 80-200 mrad, 64x64 scan, 16 phonon configs
 sto_stem        SrTiO3 [001] STEM at 256^2 / 512^2 / 1024^2 (6/12/24 cells, 10/25/50 thick)
+sto_group       the three sto_stem sizes as one uneven-size group
 pt_cbed_1024    Pt [110] CBED, 1024^2, 200 kV, 10 mrad, -100 A defocus, 20 phonon configs
 """
 import numpy as np
@@ -103,6 +104,14 @@ def sto_stem(n=256, ncells=6, nc=10, ns=16, nconf=1):
     p.scanning_xe = p.scanning_ye = (c0 + 1) * a
     p.detector_cir = [(80.0, 200.0)]
     return p
+
+
+def sto_group(ns=16, nconf=4):
+    """SrTiO3 [001] at 256^2 (6x6 cells, 10 thick), 512^2 (12x12, 25) and
+    1024^2 (24x24, 50): 21, 51 and 101 slices, ns x ns probes each."""
+    return [sto_stem(n=256, ncells=6, nc=10, ns=ns, nconf=nconf),
+            sto_stem(n=512, ncells=12, nc=25, ns=ns, nconf=nconf),
+            sto_stem(n=1024, ncells=24, nc=50, ns=ns, nconf=nconf)]
 
 
 def pt_cbed_1024(nconf=20, n=1024, na=10, nb=14, nc=None, thick=None):

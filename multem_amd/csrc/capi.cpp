@@ -3,7 +3,9 @@
 
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <vector>
 #include <string>
 
 #include "../../include/multem_hip.h"
@@ -74,6 +76,36 @@ mt_status mt_hip_run_multislice(const mt_system_conf *sys, const mt_input_multis
     if (sys && sys->precision == 2)
         return fail(MT_ERR_UNSUPPORTED, "precision 2 (double): the engine runs the FP32 wave path");
     return mt_hip_run_multislice_f32(sys, in, out);
+}
+
+mt_status mt_hip_run_multislice_group(const mt_system_conf *sys, const mt_input_multislice *const *in, int n,
+                                      mt_output_multislice *out)
+{
+    if (!sys || !in || !out || n < 1 || n > 16) return fail(MT_ERR_INVALID_ARGUMENT, "need 1..16 inputs and outputs");
+    if (sys->device == 1) return fail(MT_ERR_UNSUPPORTED, "system_conf.device = 1 (CPU): this library runs on the GPU");
+    for (int i = 0; i < n; ++i)
+        if (!in[i]) return fail(MT_ERR_INVALID_ARGUMENT, "null input");
+    mtg::run_control().stop = false;
+    std::lock_guard<std::mutex> lock(g_engine_mutex);
+    try {
+        std::vector<std::unique_ptr<mtg::Model>> models;
+        std::vector<const mtg::Model *> mp;
+        std::vector<mt_output_multislice *> op;
+        for (int i = 0; i < n; ++i) {
+            models.emplace_back(new mtg::Model(*in[i]));
+            mp.push_back(models.back().get());
+            op.push_back(&out[i]);
+        }
+        mtg::run_multislice_group(mp, sys->gpu_device, op);
+        g_last_error.clear();
+        return MT_OK;
+    } catch (const mtg::Error &e) {
+        return fail(e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(MT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(MT_ERR_DEVICE, std::string(e.what()) + " (after " + mtg::last_successful_call() + ")");
+    }
 }
 
 mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, mt_slice *slices,

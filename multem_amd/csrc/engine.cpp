@@ -244,16 +244,32 @@ struct Setup {
         return Beam{-2.0 * kPi * ((double)xb - 0.5 * m.lx_d), -2.0 * kPi * ((double)yb - 0.5 * m.ly_d)};
     }
 
-    unsigned int count_aperture()
+    // pixels inside the condenser aperture, |g + u| evaluated as the probe does
+    unsigned int count_aperture(float ux = 0.f, float uy = 0.f)
     {
         DBuf<unsigned int> cnt(1);
         unsigned int zero = 0;
         cnt.upload(&zero, 1, s);
-        check(launch_count_aperture(nx, ny, m.dgx, m.dgy, m.cond.g2_min, m.cond.g2_max, cnt.p, s), "count aperture");
+        check(launch_count_aperture(nx, ny, m.dgx, m.dgy, ux, uy, m.cond.g2_min, m.cond.g2_max, cnt.p, s),
+              "count aperture");
         unsigned int h = 0;
         check(hipMemcpyAsync(&h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s), "D2H");
         check(hipStreamSynchronize(s), "sync");
         return h;
+    }
+
+    // tilt ramp of the outputs, exp(-2 pi i g0.(r - r_c)), with 1/ny folded
+    // into rx (it follows an unnormalised inverse transform along y)
+    void ramp_tables(DBuf<float2> &rx, DBuf<float2> &ry)
+    {
+        rx.alloc(nx);
+        ry.alloc(ny);
+        // Grid_2d::Rx_c() is an int (src/types.cuh:2284-2287)
+        const double Rx_c = (double)(int)(0.5f * m.lx), Ry_c = (double)(int)(0.5f * m.ly);
+        check(launch_ramp_tables(nx, ny, (double)m.gx0, (double)m.gy0, m.lx_d / nx, m.ly_d / ny, Rx_c, Ry_c,
+                                 1.0f / (float)ny, rx.p, ry.p, s),
+              "ramp tables");
+        check(hipStreamSynchronize(s), "sync");
     }
 
     // propagator tables for the given distances: px [nz][nx], py [nz][ny]
@@ -263,7 +279,8 @@ struct Setup {
         dz.upload(z.data(), z.size(), s);
         px.alloc(z.size() * nx);
         py.alloc(z.size() * ny);
-        check(launch_prop_tables(nx, ny, dz.p, (int)z.size(), m.lambda_d, m.mul_sign, m.lx_d, m.ly_d, px.p, py.p, s),
+        check(launch_prop_tables(nx, ny, dz.p, (int)z.size(), m.lambda_d, m.mul_sign, m.lx_d, m.ly_d, (double)m.gx0,
+                                 (double)m.gy0, px.p, py.p, s),
               "prop tables");
         check(hipStreamSynchronize(s), "sync");
     }
@@ -424,78 +441,113 @@ PassChoice choose_variants(int nx, int ny, int B, RowArgs r, ColArgs c, const Co
 
 // ---------------------------------------------------------------------------
 // STEM: batches of probe positions of one configuration.
-void run_stem(const Model &m, mt_output_multislice *out)
-{
-    Stream stream;
-    hipStream_t s = stream.s;
-    Setup S(m, s);
-    Timing &tm = last_timing();
-    tm = Timing{};
-    const int ns_run = [&] {
-        int last = 0;
-        for (const ThickInfo &t : m.thk) last = std::max(last, t.islice);
-        return last + 1;                     // slices after the deepest thickness add nothing
-    }();
-    const int nscan = (int)m.sc_x.size();
-    const int n_thk = (int)m.thk.size(), n_det = (int)m.det_g2_min.size();
-    if (n_det > kMaxDet) throw Error{MT_ERR_UNSUPPORTED, "more than 8 detectors"};
-    const size_t nxy = S.nxy;
-
-    // memory plan: one transmission stack + B waves
-    const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(float2);
-    const size_t budget = budget_bytes();
-    if (stack_bytes + 2 * nxy * sizeof(float2) > budget) throw Error{MT_ERR_OUT_OF_MEMORY, "transmission stack exceeds the memory budget"};
-    int B = run_control().batch_override;
-    if (B <= 0) {
-        // enough waves to fill 256 CUs in the column pass, and a working set
-        // that stays near the 256 MB last-level cache
-        const int ntile = m.ny / col_tile_width(m.nx);
-        B = std::max(1, (int)std::min<size_t>(256, (size_t)(160u << 20) / (nxy * sizeof(float2))));
-        B = std::max(B, (2048 + ntile - 1) / ntile);
-    }
-    B = std::min(B, std::max(1, nscan));
-    B = (int)std::min<size_t>(B, (budget - stack_bytes) / (nxy * sizeof(float2)));
-    if (B < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for a wave batch"};
-    const int ntile = m.ny / col_tile_width(m.nx);
-
-    DBuf<float2> trans(m.n_slice() * nxy), psi((size_t)B * nxy);
-    DBuf<Member> members(B);
-    DBuf<Beam> beams(B);
-    DBuf<double> part((size_t)B * n_thk * n_det * ntile);
-    DBuf<double> image((size_t)n_thk * n_det * nscan);
-    {
-        std::vector<double> zero(image.n, 0.0);
-        image.upload(zero.data(), zero.size(), s);
-    }
-    std::vector<double> zl;
-    for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
-    DBuf<float2> px, py;
-    S.prop_tables(zl, px, py);
-    const unsigned int nap = S.count_aperture();
-    const ProbeParams P = S.probe_params(nap);
-
-
-    const float w = m.phonon_weight();
-    const int nbatch = (nscan + B - 1) / B;
-    run_control().niter = m.n_conf_run() * nscan;
-    run_control().iter = 0;
-
-    // slice-step arguments shared by the tuner and the batch pipeline
+//
+// A StemJob holds one specimen's device state.  Its work is a sequence of
+// items (configuration, probe batch); prepare(k) makes item k's inputs
+// resident (transmission stack of the configuration, the batch's member and
+// beam tables) and pipeline(st) enqueues the item's whole device work on
+// stream st: probe generation, the slice loop, detector partials -> image.
+class StemJob {
+public:
+    const Model &m;
+    Stream own;
+    hipStream_t s;            // uploads and transmission builds
+    Setup S;
+    int ns_run, nscan, n_thk, n_det, B, ntile, nbatch;
+    size_t nxy;
+    DBuf<float2> trans, psi;
+    DBuf<Member> members;
+    DBuf<Beam> beams;
+    DBuf<double> part, image;
+    DBuf<float2> px, py;           // [n_slice + n_thk]: slice propagators, then back-propagations
+    DBuf<float2> rx, ry, scratch;   // tilted illumination: output ramp and the output copy of the batch
+    DBuf<double2> coh;              // pn_coh_contrib: sum over configurations of w psi(g), [scan][n_thk][nxy]
+    DBuf<float4> atoms;
+    ProbeParams P{};
+    unsigned int nap = 0;
+    float w = 0;
     RowArgs r{};
-    r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = B;
-    r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = 0; r.members = members.p;
-    r.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
-    ColArgs c = S.col_base(psi.p, psi.p, B);
-    c.n_det = n_det;
-    for (int d = 0; d < n_det; ++d) { c.det_g2_min[d] = m.det_g2_min[d]; c.det_g2_max[d] = m.det_g2_max[d]; }
-    c.det_part = part.p; c.n_thk = n_thk;
-    auto col_flags = [&](bool last) {
-        return COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (last ? 0 : (COL_IFFT | COL_STORE));
-    };
-    std::vector<Member> hm(B);
-    std::vector<Beam> hb(B);
+    ColArgs c{};
     PassChoice choice;
-    {   // pick the launch variants before the pipeline is captured
+    std::vector<Member> hm;
+    std::vector<Beam> hb;
+    int cur_conf = -1;
+    double ms_tf = 0;
+
+    // budget: bytes this job may use for its transmission stack and waves
+    StemJob(const Model &mm, size_t budget, int batch_override)
+        : m(mm), s(own.s), S(mm, own.s), nxy((size_t)mm.nx * mm.ny)
+    {
+        ns_run = 0;
+        for (const ThickInfo &t : m.thk) ns_run = std::max(ns_run, t.islice + 1);   // deeper slices add nothing
+        nscan = (int)m.sc_x.size();
+        n_thk = (int)m.thk.size();
+        n_det = (int)m.det_g2_min.size();
+        if (n_det > kMaxDet) throw Error{MT_ERR_UNSUPPORTED, "more than 8 detectors"};
+
+        // memory plan: one transmission stack + B waves (+ the coherent
+        // accumulators of every scan point, all resident)
+        const size_t coh_bytes = m.pn_coh_contrib ? (size_t)nscan * n_thk * nxy * sizeof(double2) : 0;
+        const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(float2) + coh_bytes;
+        if (stack_bytes + 2 * nxy * sizeof(float2) > budget)
+            throw Error{MT_ERR_OUT_OF_MEMORY, m.pn_coh_contrib ? "transmission stack and coherent accumulators "
+                                                                  "exceed the memory budget"
+                                                                : "transmission stack exceeds the memory budget"};
+        B = batch_override;
+        if (B <= 0) {
+            // enough waves to fill 256 CUs in the column pass, and a working set
+            // that stays near the 256 MB last-level cache
+            const int nt = m.ny / col_tile_width(m.nx);
+            B = std::max(1, (int)std::min<size_t>(256, (size_t)(160u << 20) / (nxy * sizeof(float2))));
+            B = std::max(B, (2048 + nt - 1) / nt);
+        }
+        B = std::min(B, std::max(1, nscan));
+        B = (int)std::min<size_t>(B, (budget - stack_bytes) / ((m.tilted() ? 2 : 1) * nxy * sizeof(float2)));
+        if (B < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for a wave batch"};
+        ntile = m.ny / col_tile_width(m.nx);
+        nbatch = (nscan + B - 1) / B;
+
+        trans.alloc(m.n_slice() * nxy);
+        psi.alloc((size_t)B * nxy);
+        members.alloc(B);
+        beams.alloc(B);
+        part.alloc((size_t)B * n_thk * n_det * ntile);
+        image.alloc((size_t)n_thk * n_det * nscan);
+        {
+            std::vector<double> zero(image.n, 0.0);
+            image.upload(zero.data(), zero.size(), s);
+        }
+        std::vector<double> zl;
+        for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
+        for (const ThickInfo &t : m.thk) zl.push_back((double)t.z_back_prop);
+        S.prop_tables(zl, px, py);
+        if (m.tilted()) {
+            S.ramp_tables(rx, ry);
+            scratch.alloc((size_t)B * nxy);
+        }
+        if (m.pn_coh_contrib) {
+            coh.alloc((size_t)nscan * n_thk * nxy);
+            check(hipMemsetAsync(coh.p, 0, coh.n * sizeof(double2), s), "hipMemsetAsync");
+        }
+        nap = S.count_aperture();
+        P = S.probe_params(nap);
+        w = m.phonon_weight();
+        atoms.alloc(m.atoms_u.x.size());
+        hm.resize(B);
+        hb.resize(B);
+
+        // slice-step arguments shared by the tuner and the pipeline
+        r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = B;
+        r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = 0; r.members = members.p;
+        r.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
+        c = S.col_base(psi.p, psi.p, B);
+        c.n_det = n_det;
+        for (int d = 0; d < n_det; ++d) { c.det_g2_min[d] = m.det_g2_min[d]; c.det_g2_max[d] = m.det_g2_max[d]; }
+        c.det_part = part.p; c.n_thk = n_thk;
+        c.members = members.p;
+        c.coh = coh.p; c.coh_w = (double)w;
+
+        // launch variants, picked before any pipeline is captured
         RowArgs rt = r;
         rt.slice = 0;
         ColArgs ct = c;
@@ -504,114 +556,298 @@ void run_stem(const Model &m, mt_output_multislice *out)
         ColArgs cl = c;
         cl.flags = col_flags(true);
         cl.px = px.p; cl.py = py.p; cl.ithk = m.slices[ns_run - 1].ithk;
-        // member slots must describe valid entries while tuning
-        for (int j = 0; j < B; ++j) hm[j] = Member{0, -1, j, 1};
+        for (int j = 0; j < B; ++j) hm[j] = Member{0, -1, j, 1};   // valid entries while tuning
         members.upload(hm.data(), B, s);
         check(hipMemsetAsync(psi.p, 0, psi.n * sizeof(float2), s), "hipMemsetAsync");
         choice = choose_variants(m.nx, m.ny, B, rt, ct, cl.ithk >= 0 ? &cl : nullptr, s);
         r.variant = choice.row;
-        tm.tune_log = choice.log;
     }
-    std::ostringstream plan;
-    plan << "stem nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
-         << " row_variant=" << choice.row << " col_variant=" << choice.col << " col_det_variant=" << choice.col_det
-         << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
-    tm.plan = plan.str();
 
-    // the batch pipeline (probe -> ns_run slices -> detector partials)
-    auto pipeline = [&](bool timed, std::vector<Event> *ev) {
+    int col_flags(bool last) const
+    {
+        return COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL | COL_DET_BWL : 0) |
+               (last ? 0 : (COL_IFFT | COL_STORE));
+    }
+
+    int n_work() const { return m.n_conf_run() * nbatch; }
+    int batch_size(int k) const { return std::min(B, nscan - (k % nbatch) * B); }
+
+    std::string plan() const
+    {
+        std::ostringstream o;
+        o << "stem nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
+          << " row_variant=" << choice.row << " col_variant=" << choice.col << " col_det_variant=" << choice.col_det
+          << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
+        return o.str();
+    }
+
+    // inputs of work item k resident on the device (returns after the uploads)
+    void prepare(int k)
+    {
+        const int iconf = m.fp_iconf_0 + k / nbatch, ib = k % nbatch;
+        if (iconf != cur_conf) {
+            auto a = S.atoms_of(iconf);
+            atoms.upload(a.data(), a.size(), s);
+            Event t0, t1;
+            record(t0, s);
+            S.build_tf(atoms, 0, m.n_slice(), trans.p, nullptr, nullptr);
+            record(t1, s);
+            wait(t1, s);
+            ms_tf += elapsed(t0, t1);
+            cur_conf = iconf;
+        }
+        for (int j = 0; j < B; ++j) {
+            const int is = ib * B + j;
+            const bool valid = is < nscan;
+            const int iq = valid ? is : nscan - 1;
+            hm[j] = Member{0, valid ? is : -1, j, 1};
+            hb[j] = S.beam(m.sc_x[iq], m.sc_y[iq]);
+        }
+        members.upload(hm.data(), B, s);
+        beams.upload(hb.data(), B, s);
+    }
+
+    // the batch pipeline on stream st; ev (optional): 3 events per slice
+    // bracketing the row and column passes
+    void pipeline(hipStream_t st, std::vector<Event> *ev)
+    {
         ColArgs c0 = S.col_base(nullptr, psi.p, B);
         c0.flags = COL_IFFT | COL_STORE;
         c0.members = members.p; c0.beams = beams.p; c0.probe = P;
-        check(launch_col_pass(m.nx, c0, true, false, s), "probe pass");
+        check(launch_col_pass(m.nx, c0, true, false, st), "probe pass");
         int k = 0;
         for (int is = 0; is < ns_run; ++is) {
             r.slice = is;
-            if (timed) record((*ev)[k++], s);
-            check(launch_row_pass(m.ny, r, s), "row pass");
-            if (timed) record((*ev)[k++], s);
+            if (ev) record((*ev)[k++], st);
+            check(launch_row_pass(m.ny, r, st), "row pass");
+            if (ev) record((*ev)[k++], st);
             const int ithk = m.slices[is].ithk;
-            const bool last = is == ns_run - 1;
-            c.flags = col_flags(last);
             c.px = px.p + (size_t)is * m.nx;
             c.py = py.p + (size_t)is * m.ny;
-            c.ithk = ithk;
-            c.variant = ithk >= 0 ? choice.col_det : choice.col;
-            check(launch_col_pass(m.nx, c, false, ithk >= 0, s), "col pass");
-            if (timed) record((*ev)[k++], s);
+            if (m.tilted() && ithk >= 0) {
+                // complete the slice step, then detect a copy with the tilt ramp
+                // removed: O = FFT[psi exp(-2 pi i g0.r)] x P_back (get_psi,
+                // src/wave_function.cuh:96-105)
+                c.flags = col_flags(false);
+                c.ithk = -1;
+                c.variant = choice.col;
+                check(launch_col_pass(m.nx, c, false, false, st), "col pass");
+                RowArgs ro = r;
+                ro.dst = scratch.p;
+                ro.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_RAMP | ROW_FFT | ROW_STORE;
+                ro.ramp_x = rx.p; ro.ramp_y = ry.p;
+                check(launch_row_pass(m.ny, ro, st), "output ramp pass");
+                ColArgs cd = c;
+                cd.src = scratch.p;
+                cd.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (m.pn_coh_contrib ? COL_COH : 0);
+                cd.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
+                cd.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
+                cd.ithk = ithk;
+                cd.variant = 0;
+                check(launch_col_pass(m.nx, cd, false, true, st), "output detector pass");
+            } else {
+                c.flags = col_flags(is == ns_run - 1) | ((ithk >= 0 && m.pn_coh_contrib) ? COL_COH : 0);
+                c.ithk = ithk;
+                c.variant = ithk >= 0 ? choice.col_det : choice.col;
+                check(launch_col_pass(m.nx, c, false, ithk >= 0, st), "col pass");
+            }
+            if (ev) record((*ev)[k++], st);
         }
-        check(launch_det_reduce(part.p, B, n_thk, n_det, ntile, members.p, (double)w, image.p, nscan, s), "det reduce");
-    };
+        check(launch_det_reduce(part.p, B, n_thk, n_det, ntile, members.p, (double)w, image.p, nscan, st),
+              "det reduce");
+    }
 
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    void finish(mt_output_multislice *out)
+    {
+        if (out->image_tot) {
+            check(hipMemcpyAsync(out->image_tot, image.p, image.n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "sync");
+        }
+        if (out->image_coh && m.pn_coh_contrib) {
+            DBuf<double> ic((size_t)n_thk * n_det * nscan);
+            check(launch_coh_image(coh.p, nscan, n_thk, n_det, m.nx, m.ny, m.dgx, m.dgy, m.det_g2_min.data(),
+                                   m.det_g2_max.data(), (m.bwl && !m.tilted()) ? 1 : 0, m.bwl_alpha, m.gl2_max, ic.p,
+                                   s),
+                  "coherent image");
+            check(hipMemcpyAsync(out->image_coh, ic.p, ic.n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "sync");
+        }
+    }
+};
+
+struct GraphExec {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec &) = delete;
+    GraphExec &operator=(const GraphExec &) = delete;
+    ~GraphExec()
+    {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+void run_stem(const Model &m, mt_output_multislice *out)
+{
+    Timing &tm = last_timing();
+    tm = Timing{};
+    StemJob J(m, budget_bytes(), run_control().batch_override);
+    hipStream_t s = J.s;
+    tm.plan = J.plan();
+    tm.tune_log = J.choice.log;
+    run_control().niter = m.n_conf_run() * J.nscan;
+    run_control().iter = 0;
+
+    GraphExec G;
     if (run_control().use_graph) {
         check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-        pipeline(false, nullptr);
-        check(hipStreamEndCapture(s, &graph), "end capture");
-        check(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate");
+        J.pipeline(s, nullptr);
+        check(hipStreamEndCapture(s, &G.g), "end capture");
+        check(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0), "graph instantiate");
     }
-    struct GraphGuard {
-        hipGraph_t g; hipGraphExec_t e;
-        ~GraphGuard() { if (e) (void)hipGraphExecDestroy(e); if (g) (void)hipGraphDestroy(g); }
-    } guard{graph, gexec};
 
-    std::vector<Event> kev(run_control().time_kernels ? 3 * ns_run : 0);
-    Event t0, t1, tf0, tf1, sl0, sl1;
-    double ms_slice = 0, ms_tf = 0, ps = 0;
+    std::vector<Event> kev(run_control().time_kernels ? 3 * J.ns_run : 0);
+    Event t0, t1, sl0, sl1;
+    double ms_slice = 0, ps = 0;
     record(t0, s);
-    DBuf<float4> atoms(m.atoms_u.x.size());
-    for (int iconf = m.fp_iconf_0; iconf <= m.pn_nconf; ++iconf) {
+    int last_conf = -1;
+    for (int k = 0; k < J.n_work(); ++k) {
         stop_check();
-        auto a = S.atoms_of(iconf);
-        atoms.upload(a.data(), a.size(), s);
-        record(tf0, s);
-        S.build_tf(atoms, 0, m.n_slice(), trans.p, nullptr, nullptr);
-        record(tf1, s);
-        wait(tf1, s);
-        ms_tf += elapsed(tf0, tf1);
-        for (int ib = 0; ib < nbatch; ++ib) {
-            stop_check();
-            for (int j = 0; j < B; ++j) {
-                int is = ib * B + j;
-                bool valid = is < nscan;
-                int iq = valid ? is : nscan - 1;
-                hm[j] = Member{0, valid ? is : -1, j, 1};
-                hb[j] = S.beam(m.sc_x[iq], m.sc_y[iq]);
+        J.prepare(k);
+        const int iconf = k / J.nbatch;
+        // per-kernel events on the first batch of each configuration
+        const bool timed = run_control().time_kernels && iconf != last_conf;
+        last_conf = iconf;
+        record(sl0, s);
+        if (timed || !G.e) J.pipeline(s, timed ? &kev : nullptr);
+        else check(hipGraphLaunch(G.e, s), "graph launch");
+        record(sl1, s);
+        wait(sl1, s);   // host tables are reused by the next batch
+        ms_slice += elapsed(sl0, sl1);
+        ps += (double)J.batch_size(k) * J.ns_run;
+        if (timed) {
+            for (int is = 0; is < J.ns_run; ++is) {
+                tm.ms_row += elapsed(kev[3 * is], kev[3 * is + 1]);
+                tm.ms_col += elapsed(kev[3 * is + 1], kev[3 * is + 2]);
             }
-            members.upload(hm.data(), B, s);
-            beams.upload(hb.data(), B, s);
-            const bool timed = run_control().time_kernels && ib == 0;
-            record(sl0, s);
-            if (timed || !gexec) pipeline(timed, &kev);
-            else check(hipGraphLaunch(gexec, s), "graph launch");
-            record(sl1, s);
-            wait(sl1, s);   // host arrays are reused next batch
-            ms_slice += elapsed(sl0, sl1);
-            ps += (double)std::min(B, nscan - ib * B) * ns_run;
-            if (timed) {
-                for (int is = 0; is < ns_run; ++is) {
-                    tm.ms_row += elapsed(kev[3 * is], kev[3 * is + 1]);
-                    tm.ms_col += elapsed(kev[3 * is + 1], kev[3 * is + 2]);
-                }
-                tm.n_row += ns_run;
-                tm.n_col += ns_run;
-            }
-            progress_add(std::min(B, nscan - ib * B));
+            tm.n_row += J.ns_run;
+            tm.n_col += J.ns_run;
         }
+        progress_add(J.batch_size(k));
     }
     record(t1, s);
     wait(t1, s);
     tm.ms_total = elapsed(t0, t1);
     tm.ms_slice = ms_slice;
-    tm.ms_tf = ms_tf;
+    tm.ms_tf = J.ms_tf;
     tm.probe_slices = ps;
+    J.finish(out);
+}
 
-    if (out->image_tot) {
-        check(hipMemcpyAsync(out->image_tot, image.p, image.n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-        check(hipStreamSynchronize(s), "sync");
+// Several STEM specimens, possibly of different grid sizes, advance
+// together: each round gives every specimen with work left its next work
+// item, and the round's device work -- one branch per specimen, each with its
+// own tuned kernels -- is one hipGraph launch (branches forked onto
+// per-specimen streams during capture, joined at the end).  Graphs are kept
+// per set of active specimens.
+void run_stem_group(const std::vector<const Model *> &ms, const std::vector<mt_output_multislice *> &outs)
+{
+    Timing &tm = last_timing();
+    tm = Timing{};
+    const size_t n = ms.size();
+    if (n == 0 || n > 16) throw Error{MT_ERR_INVALID_ARGUMENT, "group of 1..16 specimens"};
+    // memory: each specimen gets a share of the budget proportional to its
+    // transmission stack plus a full batch
+    std::vector<double> want(n);
+    double tot = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const double nxy = (double)ms[i]->nx * ms[i]->ny;
+        want[i] = nxy * 8.0 * (ms[i]->n_slice() + 256.0);
+        tot += want[i];
     }
-    if (out->image_coh) throw Error{MT_ERR_UNSUPPORTED, "STEM coherent image (pn_coh_contrib)"};
+    const size_t budget = budget_bytes();
+    std::vector<std::unique_ptr<StemJob>> jobs;
+    std::ostringstream plan, tune;
+    int rounds = 0, niter = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const size_t share = (size_t)((double)budget * want[i] / tot);
+        jobs.emplace_back(new StemJob(*ms[i], share, run_control().batch_override));
+        plan << (i ? " | " : "") << jobs.back()->plan();
+        tune << (i ? " | " : "") << jobs.back()->choice.log;
+        rounds = std::max(rounds, jobs.back()->n_work());
+        niter += ms[i]->n_conf_run() * jobs.back()->nscan;
+    }
+    tm.plan = "group " + plan.str();
+    tm.tune_log = tune.str();
+    run_control().niter = niter;
+    run_control().iter = 0;
+
+    Stream main;
+    hipStream_t s = main.s;
+    std::vector<std::unique_ptr<Stream>> bs;
+    std::vector<std::unique_ptr<Event>> join;
+    for (size_t i = 0; i < n; ++i) {
+        bs.emplace_back(new Stream());
+        join.emplace_back(new Event());
+    }
+    Event fork;
+    std::map<unsigned, std::unique_ptr<GraphExec>> graphs;
+    auto enqueue = [&](unsigned mask) {
+        check(hipEventRecord(fork.e, s), "hipEventRecord");
+        for (size_t i = 0; i < n; ++i) {
+            if (!(mask >> i & 1u)) continue;
+            check(hipStreamWaitEvent(bs[i]->s, fork.e, 0), "hipStreamWaitEvent");
+            jobs[i]->pipeline(bs[i]->s, nullptr);
+            check(hipEventRecord(join[i]->e, bs[i]->s), "hipEventRecord");
+            check(hipStreamWaitEvent(s, join[i]->e, 0), "hipStreamWaitEvent");
+        }
+    };
+
+    Event t0, t1, sl0, sl1;
+    double ms_slice = 0, ps = 0;
+    record(t0, s);
+    for (int k = 0; k < rounds; ++k) {
+        stop_check();
+        unsigned mask = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (k < jobs[i]->n_work()) {
+                jobs[i]->prepare(k);
+                mask |= 1u << i;
+            }
+        record(sl0, s);
+        if (run_control().use_graph) {
+            auto it = graphs.find(mask);
+            if (it == graphs.end()) {
+                std::unique_ptr<GraphExec> G(new GraphExec());
+                check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+                enqueue(mask);
+                check(hipStreamEndCapture(s, &G->g), "end capture");
+                check(hipGraphInstantiate(&G->e, G->g, nullptr, nullptr, 0), "graph instantiate");
+                it = graphs.emplace(mask, std::move(G)).first;
+            }
+            check(hipGraphLaunch(it->second->e, s), "graph launch");
+        } else {
+            enqueue(mask);
+        }
+        record(sl1, s);
+        wait(sl1, s);   // host tables are reused by the next round
+        ms_slice += elapsed(sl0, sl1);
+        for (size_t i = 0; i < n; ++i)
+            if (mask >> i & 1u) {
+                ps += (double)jobs[i]->batch_size(k) * jobs[i]->ns_run;
+                progress_add(jobs[i]->batch_size(k));
+            }
+    }
+    record(t1, s);
+    wait(t1, s);
+    tm.ms_total = elapsed(t0, t1);
+    tm.ms_slice = ms_slice;
+    tm.probe_slices = ps;
+    for (size_t i = 0; i < n; ++i) {
+        tm.ms_tf += jobs[i]->ms_tf;
+        jobs[i]->finish(outs[i]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -664,6 +900,8 @@ void run_ew(const Model &m, mt_output_multislice *out)
     for (const ThickInfo &t : m.thk) zl.push_back((double)t.z_back_prop);
     DBuf<float2> px, py;
     S.prop_tables(zl, px, py);
+    DBuf<float2> rx, ry;
+    if (m.tilted()) S.ramp_tables(rx, ry);
     ProbeParams P{};
     if (m.iw_type == 2) {
         P = S.probe_params(S.count_aperture());
@@ -733,8 +971,17 @@ void run_ew(const Model &m, mt_output_multislice *out)
             // O = F_x(L)/nx * bwl * exp(i phi_back)  (get_psi, src/wave_function.cuh:96-105)
             const float zb = m.thk[ithk].z_back_prop;
             const float2 *osrc = psi.p;
-            if (fs || zb != 0.f || m.bwl) {
-                ColArgs co = S.col_base(psi.p, scratch.p, nb);
+            if (m.tilted()) {
+                // psi exp(-2 pi i g0.r) in real space first (get_psi, src/wave_function.cuh:96-105)
+                RowArgs ro = r;
+                ro.src = psi.p; ro.dst = scratch.p;
+                ro.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_RAMP | ROW_FFT | ROW_STORE;
+                ro.ramp_x = rx.p; ro.ramp_y = ry.p;
+                check(launch_row_pass(m.ny, ro, s), "output ramp pass");
+                osrc = scratch.p;
+            }
+            if (fs || zb != 0.f || m.bwl || m.tilted()) {
+                ColArgs co = S.col_base(osrc, scratch.p, nb);
                 co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) | COL_STORE;
                 co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
                 co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
@@ -798,6 +1045,19 @@ Timing &last_timing()
 {
     static Timing t;
     return t;
+}
+
+void run_multislice_group(const std::vector<const Model *> &ms, int device, const std::vector<mt_output_multislice *> &outs)
+{
+    select_device(device);
+    for (size_t i = 0; i < ms.size(); ++i) {
+        const Model &m = *ms[i];
+        if (!m.is_stem()) throw Error{MT_ERR_UNSUPPORTED, "group runs take STEM inputs only"};
+        if (m.cond.phi_required) throw Error{MT_ERR_UNSUPPORTED, "azimuthal condenser aberrations"};
+        m.output_dims(outs[i]);
+        m.fill_axes(outs[i]);
+    }
+    run_stem_group(ms, outs);
 }
 
 void run_multislice(const Model &m, int device, mt_output_multislice *out)
@@ -865,7 +1125,11 @@ void incident_wave(const Model &m, int device, double *psi0)
         std::vector<float2> one(S.nxy, make_float2(1.f, 0.f));
         w.upload(one.data(), S.nxy, st.s);
     } else {
-        ProbeParams P = S.probe_params(S.count_aperture());
+        // the stand-alone incident wave evaluates chi at g + g0 (Incident_Wave::operator()
+        // with gx_0, gy_0, src/incident_wave.cuh:116-131); inside a simulation the probe is untilted
+        ProbeParams P = S.probe_params(S.count_aperture(m.gx0, m.gy0));
+        P.gxu = (double)m.gx0; P.gyu = (double)m.gy0;
+        P.gxu_f = m.gx0; P.gyu_f = m.gy0;
         std::vector<Beam> hb;
         for (size_t i = 0; i < m.iw_x.size(); ++i) hb.push_back(S.beam(m.iw_x[i], m.iw_y[i]));
         if (hb.empty()) throw Error{MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y"};

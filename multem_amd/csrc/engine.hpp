@@ -47,6 +47,10 @@ Timing &last_timing();
 // Full simulation into caller-owned output arrays.
 void run_multislice(const Model &m, int device, mt_output_multislice *out);
 
+// Several STEM specimens (grid sizes may differ) run together, one hipGraph
+// launch per round of batches.
+void run_multislice_group(const std::vector<const Model *> &ms, int device, const std::vector<mt_output_multislice *> &outs);
+
 // Per-piece entries (natural, unshifted layouts).
 void projected_potential(const Model &m, int device, int iconf, int islice, double *V);
 void transmission_function(const Model &m, int device, int iconf, int islice, double *trans);

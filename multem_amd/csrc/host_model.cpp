@@ -347,7 +347,6 @@ Model::Model(const mt_input_multislice &in)
     if (!(in.potential_slicing == 1 || in.potential_slicing == 2))
         fail(MT_ERR_UNSUPPORTED, "potential_slicing 3/4 (sub-slicing)");
     if (std::fabs(in.spec_rot_theta) > 0) fail(MT_ERR_UNSUPPORTED, "specimen rotation");
-    if (std::fabs(in.theta) > 0) fail(MT_ERR_UNSUPPORTED, "tilted illumination");
     if (in.illumination_model == 4) fail(MT_ERR_UNSUPPORTED, "illumination_model 4 (full integration)");
 
     pn_model = in.pn_model;
@@ -392,8 +391,20 @@ Model::Model(const mt_input_multislice &in)
 
     E_0 = (float)in.E_0;
     if (!(E_0 > 0)) fail(MT_ERR_INVALID_ARGUMENT, "E_0 must be positive");
-    theta = 0.f; phi = (float)(in.phi * kDegToRad);
+    theta = (float)(in.theta * kDegToRad);
+    phi = (float)(in.phi * kDegToRad);
+    if (std::fabs(theta) >= 0.5f * (float)kPi) fail(MT_ERR_INVALID_ARGUMENT, "theta must be below 90 degrees");
     lambda = lambda_f(E_0);
+    // Input_Multislice::set_incident_angle (src/input_multislice.cuh:891-895):
+    // sin(theta) quantised to a multiple of lambda*dg_min
+    if (theta != 0.f) {
+        const float dgm = std::fmin(dgx, dgy);
+        const float nq = std::round(std::sin(theta) / (lambda * dgm));
+        theta = std::asin(nq * lambda * dgm);
+    }
+    // gx_0, gy_0 (src/input_multislice.cuh:919-927)
+    gx0 = std::sin(theta) * std::cos(phi) / lambda;
+    gy0 = std::sin(theta) * std::sin(phi) / lambda;
     {
         double e = in.E_0, emass = 510.99906, hc = 12.3984244;
         lambda_d = hc / std::sqrt(e * (2 * emass + e));

@@ -80,6 +80,8 @@ public:
     bool pn_x = true, pn_y = true, pn_z = false;
     int mul_sign = 1;
     float E_0 = 300, lambda = 0, theta = 0, phi = 0;
+    float gx0 = 0, gy0 = 0;         // tilt of the incident beam in reciprocal space (1/A)
+    bool tilted() const { return gx0 != 0.f || gy0 != 0.f; }
     double lambda_d = 0;            // FP64 wavelength for phase arguments
     float Vr_factor = 0;            // mul_sign * gamma*lambda/(c_Potf*cos(theta)), float as the FP32 path
     int iw_type = 1;

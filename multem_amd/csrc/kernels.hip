@@ -32,6 +32,15 @@ __device__ __forceinline__ float g2_float(int igx, int igy, float dgx, float dgy
     return gx * gx + gy * gy;
 }
 
+// |g + u|^2 on the float grid with the rounding of exp_i_chi's float path
+// (src/cgpu_fcns.cuh:1949-1956): (igx dgx + ux)^2 + (igy dgy + uy)^2
+__device__ __forceinline__ float g2_float_shift(int igx, int igy, float dgx, float dgy, float ux, float uy)
+{
+#pragma clang fp contract(off)
+    const float gx = (float)igx * dgx + ux, gy = (float)igy * dgy + uy;
+    return gx * gx + gy * gy;
+}
+
 // bandwidth-limit factor 1/(1+exp(alpha (g^2 - gl2_max))) (Grid_2d::bwl_factor_shift)
 __device__ __forceinline__ float bwl_factor(float g2, float alpha, float gl2_max)
 {
@@ -101,6 +110,11 @@ __global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgs a)
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = cmul(tv[e], v[e]);
     }
+    if (flags & ROW_MUL_RAMP) {
+        const float2 rxv = a.ramp_x[row];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cmul(cmul(rxv, a.ramp_y[t + T * e]), v[e]);
+    }
     if (flags & ROW_FFT) fft_run<S, false>(v, t, tw, lp, idx, sync);
     if (flags & ROW_STORE) {
 #pragma unroll
@@ -146,15 +160,15 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
         // float aperture mask, FP64 phase reduced mod 2 pi, FP32 sincos.
         const Member m = a.members[member];
         const ProbeParams &P = a.probe;
-        const double gy = igy * P.dgy;
+        const double gy = igy * P.dgy + P.gyu;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int ix = t + T * e;
             const int igx = signed_freq(ix, a.nx);
-            const float g2f = g2_float(igx, igy, P.dgx_f, P.dgy_f);
+            const float g2f = g2_float_shift(igx, igy, P.dgx_f, P.dgy_f, P.gxu_f, P.gyu_f);
             float2 acc = make_float2(0.f, 0.f);
             if (P.g2_min <= g2f && g2f < P.g2_max) {
-                const double gx = igx * P.dgx;
+                const double gx = igx * P.dgx + P.gxu;
                 const double g2 = gx * gx + gy * gy;
                 const double aber = (P.c10 + (P.c30 + P.c50 * g2) * g2) * g2;
                 for (int b = 0; b < m.nbeam; ++b) {
@@ -186,6 +200,20 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
         }
     }
     if constexpr (DET) {
+        if (flags & COL_COH) {
+            // coherent sums over configurations (Output_Multislice::add_scale_psi_coh)
+            const int iscan = a.members[member].iscan;
+            if (iscan >= 0) {
+                double2 *cp = a.coh + ((long long)iscan * a.n_thk + a.ithk) * ((long long)a.nx * a.ny) + col;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    double2 q = cp[(long long)(t + T * e) * a.ncols];
+                    q.x += a.coh_w * (double)v[e].x;
+                    q.y += a.coh_w * (double)v[e].y;
+                    cp[(long long)(t + T * e) * a.ncols] = q;
+                }
+            }
+        }
         // Annular detector sums (sum_square_over_Det, src/cgpu_fcns.cuh:1794-1805):
         // FP64 per-thread sums, wave and workgroup reduction in fixed order.
         double acc[kMaxDet];
@@ -196,7 +224,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
             const int ix = t + T * e;
             const float g2f = g2_float(signed_freq(ix, a.nx), igy, a.dgx_f, a.dgy_f);
             float m2 = cnorm2(v[e]);
-            if (flags & COL_BWL) {
+            if (flags & COL_DET_BWL) {
                 const float f = bwl_factor(g2f, a.bwl_alpha, a.gl2_max);
                 m2 *= f * f;
             }
@@ -358,9 +386,10 @@ __global__ void k_det_reduce(const double *part, int n_members, int n_thk, int n
 
 // Fresnel propagator factors, separable in gx and gy, with the FP64 phase
 // reduced mod 2 pi before the FP32 sincos:
-// polar(1/(nx ny), w g^2), w = -mul_sign pi lambda z  (propagate, src/cgpu_fcns.cuh:1835-1845).
+// polar(1/(nx ny), w |g - g0|^2), w = -mul_sign pi lambda z
+// (propagate, src/cgpu_fcns.cuh:1835-1845; g0 the beam tilt).
 __global__ void k_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
-                              double lx, double ly, float2 *px, float2 *py)
+                              double lx, double ly, double gx0, double gy0, float2 *px, float2 *py)
 {
     const int n = nx + ny;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -368,8 +397,8 @@ __global__ void k_prop_tables(int nx, int ny, const double *z, int nz, double la
     const int zi = (int)(i / n), j = (int)(i % n);
     const double w = -mul_sign * 3.141592653589793238463 * lambda * z[zi];
     double g;
-    if (j < nx) g = signed_freq(j, nx) / lx;
-    else g = signed_freq(j - nx, ny) / ly;
+    if (j < nx) g = signed_freq(j, nx) / lx - gx0;
+    else g = signed_freq(j - nx, ny) / ly - gy0;
     double arg = w * g * g;
     arg -= kTwoPiD * rint(arg * (1.0 / kTwoPiD));
     float s, c;
@@ -382,13 +411,34 @@ __global__ void k_prop_tables(int nx, int ny, const double *z, int nz, double la
     }
 }
 
-__global__ void k_count_aperture(int nx, int ny, float dgx, float dgy, float g2_min, float g2_max, unsigned int *count)
+// Tilt phase ramp exp(-2 pi i (gx0 (Rx - Rx_c) + gy0 (Ry - Ry_c))) of the
+// output (Wave_Function::phase_multiplication with -g0, src/wave_function.cuh:77-105;
+// exp_r_factor_2d, src/cgpu_fcns.cuh:1867-1877) as separable tables over
+// the shifted layout, rx with the extra factor `scale`; FP64 argument
+// reduced mod 2 pi before the FP32 sincos.
+__global__ void k_ramp_tables(int nx, int ny, double gx0, double gy0, double dRx, double dRy, double Rx_c,
+                              double Ry_c, float scale, float2 *rx, float2 *ry)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nx + ny) return;
+    double arg;
+    if (j < nx) arg = -kTwoPiD * gx0 * (shift_idx(j, nx >> 1) * dRx - Rx_c);
+    else arg = -kTwoPiD * gy0 * (shift_idx(j - nx, ny >> 1) * dRy - Ry_c);
+    arg -= kTwoPiD * rint(arg * (1.0 / kTwoPiD));
+    float sn, cs;
+    sincosf((float)arg, &sn, &cs);
+    if (j < nx) rx[j] = make_float2(cs * scale, sn * scale);
+    else ry[j - nx] = make_float2(cs, sn);
+}
+
+__global__ void k_count_aperture(int nx, int ny, float dgx, float dgy, float ux, float uy, float g2_min, float g2_max,
+                                 unsigned int *count)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool in = false;
     if (i < (long long)nx * ny) {
         const int ix = (int)(i / ny), iy = (int)(i % ny);
-        const float g2 = g2_float(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy);
+        const float g2 = g2_float_shift(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy, ux, uy);
         in = (g2_min <= g2 && g2 < g2_max);
     }
     const unsigned long long m = __ballot(in);
@@ -426,6 +476,43 @@ __global__ void k_out_accum(OutArgs a)
             }
         }
     }
+}
+
+// Coherent STEM image (Wave_Function::set_m2psi_coh, src/wave_function.cuh:188-205):
+// one workgroup per (scan point, thickness), FP64 sums reduced in fixed order.
+struct CohDet {
+    float g2_min[kMaxDet], g2_max[kMaxDet];
+};
+
+__global__ __launch_bounds__(256) void k_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny,
+                                                   float dgx, float dgy, CohDet det, int extra_bwl, float bwl_alpha,
+                                                   float gl2_max, double *image_coh)
+{
+    __shared__ double red[kMaxDet][256];
+    const int iscan = blockIdx.x % nscan, ithk = blockIdx.x / nscan;
+    const double2 *c = coh + ((long long)iscan * n_thk + ithk) * ((long long)nx * ny);
+    double acc[kMaxDet];
+    for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.0;
+    for (long long i = threadIdx.x; i < (long long)nx * ny; i += 256) {
+        const int ix = (int)(i / ny), iy = (int)(i % ny);
+        const float g2 = g2_float(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy);
+        const double2 v = c[i];
+        double m2 = v.x * v.x + v.y * v.y;
+        if (extra_bwl) {
+            const double f = (double)bwl_factor(g2, bwl_alpha, gl2_max);
+            m2 *= f * f;
+        }
+        for (int d = 0; d < n_det; ++d)
+            if (det.g2_min[d] <= g2 && g2 < det.g2_max[d]) acc[d] += m2;
+    }
+    for (int d = 0; d < n_det; ++d) red[d][threadIdx.x] = acc[d];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int d = 0; d < n_det; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < n_det) image_coh[((long long)ithk * n_det + threadIdx.x) * nscan + iscan] = red[threadIdx.x][0];
 }
 
 __global__ void k_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o, double *m2)
@@ -578,20 +665,28 @@ hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n
 }
 
 hipError_t launch_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
-                              double lx, double ly, float2 *px, float2 *py, hipStream_t s)
+                              double lx, double ly, double gx0, double gy0, float2 *px, float2 *py, hipStream_t s)
 {
     const long long n = (long long)(nx + ny) * nz;
     hipLaunchKernelGGL(k_prop_tables, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, z, nz, lambda,
-                       mul_sign, lx, ly, px, py);
+                       mul_sign, lx, ly, gx0, gy0, px, py);
     return hipGetLastError();
 }
 
-hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float g2_min, float g2_max,
+hipError_t launch_ramp_tables(int nx, int ny, double gx0, double gy0, double dRx, double dRy, double Rx_c,
+                              double Ry_c, float scale, float2 *rx, float2 *ry, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ramp_tables, dim3((unsigned)((nx + ny + 255) / 256)), dim3(256), 0, s, nx, ny, gx0, gy0,
+                       dRx, dRy, Rx_c, Ry_c, scale, rx, ry);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float ux, float uy, float g2_min, float g2_max,
                                  unsigned int *count, hipStream_t s)
 {
     const long long n = (long long)nx * ny;
-    hipLaunchKernelGGL(k_count_aperture, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, dgx, dgy,
-                       g2_min, g2_max, count);
+    hipLaunchKernelGGL(k_count_aperture, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, dgx, dgy, ux,
+                       uy, g2_min, g2_max, count);
     return hipGetLastError();
 }
 
@@ -608,6 +703,18 @@ hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, 
     const long long n = (long long)nx_o * ny_o;
     hipLaunchKernelGGL(k_m2_from_psi, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, psi_full, nx, ny, ix_0,
                        iy_0, nx_o, ny_o, m2);
+    return hipGetLastError();
+}
+
+hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, float dgx, float dgy,
+                            const float *det_g2_min, const float *det_g2_max, int extra_bwl, float bwl_alpha,
+                            float gl2_max, double *image_coh, hipStream_t s)
+{
+    if (n_det > kMaxDet || n_det < 0) return hipErrorInvalidValue;
+    CohDet d{};
+    for (int k = 0; k < n_det; ++k) { d.g2_min[k] = det_g2_min[k]; d.g2_max[k] = det_g2_max[k]; }
+    hipLaunchKernelGGL(k_coh_image, dim3((unsigned)(nscan * n_thk)), dim3(256), 0, s, coh, nscan, n_thk, n_det, nx, ny,
+                       dgx, dgy, d, extra_bwl, bwl_alpha, gl2_max, image_coh);
     return hipGetLastError();
 }
 

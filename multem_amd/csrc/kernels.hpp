@@ -27,6 +27,8 @@ struct ProbeParams {
     double c10, c30, c50;         // -pi*lambda*f, -pi*Cs3*lambda^3/2, -pi*Cs5*lambda^5/3 (FP64)
     double dgx, dgy;              // FP64 reciprocal sampling (1/lx, 1/ly)
     float dgx_f, dgy_f;           // float sampling, for the aperture mask as the FP32 path evaluates it
+    double gxu, gyu;              // frequency shift of chi (tilted per-piece incident wave; 0 in simulations)
+    float gxu_f, gyu_f;
     float g2_min, g2_max;         // aperture
     float norm;                   // 1/sqrt(number of aperture pixels)
 };
@@ -37,6 +39,7 @@ enum RowFlags : int {
     ROW_MUL_T = 4,      // multiply by the transmission function of slice `slice`
     ROW_FFT = 8,        // forward transform along y
     ROW_STORE = 16,
+    ROW_MUL_RAMP = 32,  // multiply by ramp_x[row] * ramp_y[iy] (after the transmission)
 };
 
 struct RowArgs {
@@ -52,6 +55,7 @@ struct RowArgs {
     int slice;
     const Member *members;
     int variant;                  // launch variant of the full slice step (kRowSlice flags)
+    const float2 *ramp_x, *ramp_y;   // ROW_MUL_RAMP tables [nrows], [row length]
 };
 
 enum ColFlags : int {
@@ -61,6 +65,8 @@ enum ColFlags : int {
     COL_BWL = 8,        // multiply by the bandwidth-limit factor (and 1/(nx ny) when !COL_MUL_P)
     COL_IFFT = 16,
     COL_STORE = 32,
+    COL_DET_BWL = 64,   // detector sums take one more bandwidth-limit factor (the back-propagation's)
+    COL_COH = 128,      // detector passes: coh[iscan][ithk] += coh_w * psi(g) (coherent STEM image)
 };
 
 struct ColArgs {
@@ -83,6 +89,9 @@ struct ColArgs {
     float det_g2_min[kMaxDet], det_g2_max[kMaxDet];
     double *det_part;             // [member][n_thk][n_det][ntile]
     int n_thk, ithk;
+    // coherent STEM accumulators [scan][n_thk][nx*ny], FP64
+    double2 *coh;
+    double coh_w;
     // probe generation
     const Member *members;
     const Beam *beams;
@@ -131,12 +140,18 @@ hipError_t launch_tf_build(const TfArgs &a, hipStream_t s);
 hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
                              const Member *members, double w, double *image, int nscan, hipStream_t s);
 hipError_t launch_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
-                              double lx, double ly, float2 *px, float2 *py, hipStream_t s);
-hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float g2_min, float g2_max,
+                              double lx, double ly, double gx0, double gy0, float2 *px, float2 *py, hipStream_t s);
+hipError_t launch_ramp_tables(int nx, int ny, double gx0, double gy0, double dRx, double dRy, double Rx_c,
+                              double Ry_c, float scale, float2 *rx, float2 *ry, hipStream_t s);
+hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float ux, float uy, float g2_min, float g2_max,
                                  unsigned int *count, hipStream_t s);
 hipError_t launch_out_accum(const OutArgs &a, hipStream_t s);
 hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o,
                               double *m2, hipStream_t s);
+// image_coh[ithk][det][iscan] = sum over the ring of f |coh|^2 (f = bwl^2 if extra_bwl, else 1)
+hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, float dgx, float dgy,
+                            const float *det_g2_min, const float *det_g2_max, int extra_bwl, float bwl_alpha,
+                            float gl2_max, double *image_coh, hipStream_t s);
 hipError_t launch_shift_complex(const float2 *src, float2 *dst, int nx, int ny, hipStream_t s);
 
 // Launch variants of the slice passes.  Variant 0 is the default; the tuner

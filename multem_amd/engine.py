@@ -36,6 +36,7 @@ def lib():
         L.mt_hip_output_dims.argtypes = [sysp, inp, outp]
         L.mt_hip_run_multislice.argtypes = [sysp, inp, outp]
         L.mt_hip_run_multislice_f32.argtypes = [sysp, inp, outp]
+        L.mt_hip_run_multislice_group.argtypes = [sysp, P(inp), C.c_int, outp]
         L.mt_hip_spec_slicing.argtypes = [sysp, inp, C.c_int, P(capi.Slice), C.c_int, P(C.c_int)]
         L.mt_hip_projected_potential.argtypes = [sysp, inp, C.c_int, C.c_int, dp]
         L.mt_hip_transmission_function.argtypes = [sysp, inp, C.c_int, C.c_int, dp]
@@ -83,6 +84,26 @@ def run_multislice(params):
     o = out.as_struct()
     _check(L.mt_hip_run_multislice(C.byref(sysc), C.byref(s), C.byref(o)))
     return out
+
+
+def run_multislice_group(params_list):
+    """Several STEM inputs (grid sizes may differ) in one run; returns one
+    capi.Output per input, as run_multislice would for each alone."""
+    L = lib()
+    n = len(params_list)
+    structs, keeps, outs = [], [], []
+    sysc = params_list[0].system_conf.as_struct()
+    for p in params_list:
+        st, keep = p.to_struct()
+        structs.append(st)
+        keeps.append(keep)
+        d = capi.OutputMultislice()
+        _check(L.mt_hip_output_dims(C.byref(sysc), C.byref(st), C.byref(d)))
+        outs.append(capi.Output(d))
+    ins = (C.POINTER(capi.InputMultislice) * n)(*[C.pointer(st) for st in structs])
+    os_ = (capi.OutputMultislice * n)(*[o.as_struct() for o in outs])
+    _check(L.mt_hip_run_multislice_group(C.byref(sysc), ins, n, os_))
+    return outs
 
 
 def spec_slicing(params, iconf=0):

@@ -1253,7 +1253,6 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     s->E_0 = (real)in->E_0;
     s->theta = (real)(in->theta * c_deg_2_rad);
     s->phi = (real)(in->phi * c_deg_2_rad);
-    if (fabs(in->theta) > 0) FAIL(MT_ERR_UNSUPPORTED, "tilted illumination");
     s->illumination_model = in->illumination_model;
     if (s->illumination_model == 4) FAIL(MT_ERR_UNSUPPORTED, "illumination_model 4 (full integration)");
 
@@ -1529,6 +1528,30 @@ static void propagate_k(const sim_t *s, real z, cplx *M)
         }
 }
 
+/* Output phase multiplication with -g0 before the back-propagation:
+   Wave_Function::get_psi / phase_multiplication (src/wave_function.cuh:77-105),
+   exp_r_factor_2d (src/cgpu_fcns.cuh:1867-1877) without scaling; Rx_c() and
+   Ry_c() are ints (src/types.cuh:2284-2287).  Nothing to do without tilt. */
+static void tilt_ramp_out(const sim_t *s, cplx *psi)
+{
+    const grid_t *g = &s->g;
+    real gxu = sin(s->theta) * cos(s->phi) / s->lambda, gyu = sin(s->theta) * sin(s->phi) / s->lambda;
+    if (is_zero_r(gxu) && is_zero_r(gyu)) return;
+    real gx = (real)(-6.283185307179586476925) * gxu, gy = (real)(-6.283185307179586476925) * gyu;
+    int Rx_c = (int)((real)0.5 * g->lx), Ry_c = (int)((real)0.5 * g->ly);
+    for (int ix = 0; ix < g->nx; ix++) {
+        real Rx = ((ix < g->nxh) ? ix + g->nxh : ix - g->nxh) * g->dRx - Rx_c;
+        for (int iy = 0; iy < g->ny; iy++) {
+            real Ry = ((iy < g->nyh) ? iy + g->nyh : iy - g->nyh) * g->dRy - Ry_c;
+            real th = gx * Rx + gy * Ry;
+            cplx e = { cos(th), sin(th) };
+            cplx *v = &psi[(size_t)ix * g->ny + iy];
+            cplx r = { v->re * e.re - v->im * e.im, v->re * e.im + v->im * e.re };
+            *v = r;
+        }
+    }
+}
+
 /* Propagator::operator() (src/propagator.cuh:52-87); space 1 real, 2 reciprocal */
 static void propagator(const sim_t *s, int space, real z, cplx *psi, const fft2 *F, cplx *work)
 {
@@ -1553,7 +1576,7 @@ static void propagator(const sim_t *s, int space, real z, cplx *psi, const fft2 
 
 /* Convergent probe: Incident_Wave::operator() (src/incident_wave.cuh:72-91),
    exp_i_chi (src/cgpu_fcns.cuh:1949-1979); psi in shifted real space */
-static void incident_wave(sim_t *s, const real *xb, const real *yb, int nb, cplx *psi,
+static void incident_wave(sim_t *s, const real *xb, const real *yb, int nb, real gxu, real gyu, cplx *psi,
                           const fft2 *F, cplx *work, cplx *fpsi)
 {
     const grid_t *g = &s->g;
@@ -1578,7 +1601,7 @@ static void incident_wave(sim_t *s, const real *xb, const real *yb, int nb, cplx
             for (int iy = 0; iy < g->ny; iy++) {
                 int igx = (ix < g->nxh) ? ix : ix - g->nx;
                 int igy = (iy < g->nyh) ? iy : iy - g->ny;
-                real gx = igx * g->dgx + 0, gy = igy * g->dgy + 0;
+                real gx = igx * g->dgx + gxu, gy = igy * g->dgy + gyu;
                 real g2 = gx * gx + gy * gy;
                 cplx v = { 0, 0 };
                 if ((lens.g2_min <= g2) && (g2 < lens.g2_max)) {
@@ -1623,6 +1646,8 @@ typedef struct {
     int n_work;             /* number of (probe) work items */
     const int *items;       /* scan indices */
     double *result;         /* [item][n_thk][n_det] */
+    cplx *coh;              /* pn_coh_contrib: [scan][n_thk][nxy] sum of w psi(g), or NULL */
+    real w;                 /* configuration weight of the coherent sums */
     int next;
     pthread_mutex_t lock;
     int status;
@@ -1646,7 +1671,7 @@ static void *stem_worker(void *arg)
         if (it >= J->n_work) break;
         int iscan = J->items[it];
         real xb = s->sc_x[iscan], yb = s->sc_y[iscan];
-        incident_wave(s, &xb, &yb, 1, psi, &F, work, fpsi);
+        incident_wave(s, &xb, &yb, 1, 0, 0, psi, &F, work, fpsi);
         for (int is = 0; is < s->n_slice; is++) {
             const cplx *T = J->trans + (size_t)is * nxy;
             for (size_t i = 0; i < nxy; i++) {
@@ -1659,7 +1684,14 @@ static void *stem_worker(void *arg)
             int ithk = s->slice[is].ithk;
             if (ithk >= 0) {
                 memcpy(po, psi, sizeof(cplx) * nxy);
+                tilt_ramp_out(s, po);
                 propagator(s, 2, s->thk[ithk].z_back_prop, po, &F, work);
+                if (J->coh) {
+                    /* Output_Multislice::add_scale_psi_coh: psi_coh += w psi (one probe's
+                       configurations are never processed concurrently) */
+                    cplx *c = J->coh + ((size_t)iscan * s->n_thk + ithk) * nxy;
+                    for (size_t i = 0; i < nxy; i++) { c[i].re += J->w * po[i].re; c[i].im += J->w * po[i].im; }
+                }
                 for (int id = 0; id < s->n_det; id++) {
                     real g2_min = (real)pow(s->det_g_inner[id], 2), g2_max = (real)pow(s->det_g_outer[id], 2);
                     real sum = 0;
@@ -1759,12 +1791,17 @@ static int run_stem(sim_t *s, mt_output_multislice *out, int nthreads)
     for (int i = 0; i < nscan; i++) items[i] = i;
     int nconf_w = (!is_frozen(s) || s->pn_single_conf) ? 1 : s->pn_nconf;
     real w = (real)(1.0 / (real)(nconf_w * 1));
+    cplx *coh = NULL;
+    if (s->pn_coh_contrib) {
+        coh = calloc((size_t)nscan * s->n_thk * nxy, sizeof(cplx));
+        if (!coh) { free(img); free(trans); free(res); free(items); set_err("out of memory"); return MT_ERR_OUT_OF_MEMORY; }
+    }
     int st = MT_OK;
     for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
         move_atoms(s, iconf);
         st = build_trans_stack(s, trans);
         if (st != MT_OK) break;
-        stem_job J = { s, trans, nscan, items, res, 0, PTHREAD_MUTEX_INITIALIZER, 0 };
+        stem_job J = { s, trans, nscan, items, res, coh, w, 0, PTHREAD_MUTEX_INITIALIZER, 0 };
         int nt = nthreads > 0 ? nthreads : 1;
         pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nt);
         for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, stem_worker, &J);
@@ -1780,7 +1817,28 @@ static int run_stem(sim_t *s, mt_output_multislice *out, int nthreads)
     }
     if (st == MT_OK && out->image_tot)
         for (size_t i = 0; i < nimg; i++) out->image_tot[i] = img[i];
-    free(img); free(trans); free(res); free(items);
+    if (st == MT_OK && coh && out->image_coh) {
+        /* Wave_Function::set_m2psi_coh for STEM (wave_function.cuh:188-205):
+           image_coh = integral over the detector of |psi_coh|^2 */
+        for (int iscan = 0; iscan < nscan; iscan++)
+            for (int ithk = 0; ithk < s->n_thk; ithk++) {
+                const cplx *c = coh + ((size_t)iscan * s->n_thk + ithk) * nxy;
+                for (int id = 0; id < s->n_det; id++) {
+                    real g2_min = (real)pow(s->det_g_inner[id], 2), g2_max = (real)pow(s->det_g_outer[id], 2);
+                    real sum = 0;
+                    for (int ix = 0; ix < g->nx; ix++)
+                        for (int iy = 0; iy < g->ny; iy++) {
+                            real g2 = g2_shift(g, ix, iy, 0, 0);
+                            if ((g2_min <= g2) && (g2 < g2_max)) {
+                                cplx v = c[(size_t)ix * g->ny + iy];
+                                sum += v.re * v.re + v.im * v.im;
+                            }
+                        }
+                    out->image_coh[((size_t)ithk * s->n_det + id) * nscan + iscan] = (double)sum;
+                }
+            }
+    }
+    free(coh); free(img); free(trans); free(res); free(items);
     return st;
 }
 
@@ -1813,22 +1871,27 @@ typedef struct {
     int status;
 } ew_conf;
 
-/* EWFS_EWRS (tem_simulation.cuh:303-327) for one configuration */
+/* EWFS_EWRS (tem_simulation.cuh:303-327) for one configuration.  Each slice
+   is used once, so its transmission function is built just before use
+   (the reference builds a stack when memory allows; same values). */
 static int run_ew_conf(sim_t *s, cplx **po_thk)
 {
     const grid_t *g = &s->g;
     size_t nxy = (size_t)g->nx * g->ny;
     fft2 F;
     fft2_init(&F, g->nx, g->ny);
-    cplx *trans = malloc(sizeof(cplx) * nxy * (size_t)s->n_slice);
+    cplx *T = malloc(sizeof(cplx) * nxy);
+    real *V = malloc(sizeof(real) * nxy);
     cplx *psi = malloc(sizeof(cplx) * nxy), *fpsi = malloc(sizeof(cplx) * nxy);
     cplx *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
-    int st = build_trans_stack(s, trans);
+    int st = (T && V && psi && fpsi && work) ? MT_OK : MT_ERR_OUT_OF_MEMORY;
     if (st == MT_OK) {
-        incident_wave(s, s->iw_x, s->iw_y, s->n_iw, psi, &F, work, fpsi);
+        incident_wave(s, s->iw_x, s->iw_y, s->n_iw, 0, 0, psi, &F, work, fpsi);
         int space = is_ewrs(s) ? 1 : 2;
-        for (int is = 0; is < s->n_slice; is++) {
-            const cplx *T = trans + (size_t)is * nxy;
+        for (int is = 0; is < s->n_slice && st == MT_OK; is++) {
+            st = slice_potential(s, is, V);
+            if (st != MT_OK) break;
+            slice_transmission(s, V, T, &F, work);
             for (size_t i = 0; i < nxy; i++) {
                 cplx a = psi[i], b = T[i];
                 psi[i].re = b.re * a.re - b.im * a.im;
@@ -1839,70 +1902,156 @@ static int run_ew_conf(sim_t *s, cplx **po_thk)
             int ithk = s->slice[is].ithk;
             if (ithk >= 0) {
                 memcpy(po_thk[ithk], psi, sizeof(cplx) * nxy);
+                tilt_ramp_out(s, po_thk[ithk]);
                 propagator(s, space, s->thk[ithk].z_back_prop, po_thk[ithk], &F, work);
             }
         }
     }
-    free(trans); free(psi); free(fpsi); free(work);
+    free(T); free(V); free(psi); free(fpsi); free(work);
     fft2_free(&F);
     return st;
 }
 
-static int run_ew(sim_t *s, mt_output_multislice *out)
+/* Accumulators of the exit-wave / CBED outputs over configurations. */
+typedef struct {
+    double *m2, *pc, *pcs;
+    size_t no, nxy;
+    int ot;
+    double w;
+} ew_acc;
+
+static void ew_accumulate(const sim_t *s, ew_acc *A, cplx **po)
+{
+    for (int k = 0; k < s->n_thk; k++) {
+        if (A->ot == MT_OT_PSI_COH) {
+            accumulate_output(s, po[k], 1.0, 0, 1, A->pcs + 2 * A->no * k, 1);
+        } else if (A->ot == MT_OT_M2PSI_TOT_PSI_COH) {
+            accumulate_output(s, po[k], A->w, 1, 0, A->m2 + A->no * k, 0);
+            accumulate_output(s, po[k], A->w, 0, 0, A->pcs + 2 * A->no * k, 1);
+        } else {
+            accumulate_output(s, po[k], A->w, 1, 0, A->m2 + A->no * k, 0);
+            if (s->pn_coh_contrib)
+                for (size_t i = 0; i < A->nxy; i++) {
+                    A->pc[2 * (A->nxy * k + i)] += A->w * po[k][i].re;
+                    A->pc[2 * (A->nxy * k + i) + 1] += A->w * po[k][i].im;
+                }
+        }
+    }
+}
+
+/* Configurations are computed in parallel, each thread with its own
+   simulation state (displaced atoms, slicing and tables are per
+   configuration), and added to the accumulators strictly in configuration
+   order, so the sums equal the sequential run's. */
+typedef struct {
+    const mt_input_multislice *in;
+    ew_acc *A;
+    int iconf_0, iconf_e, nt;
+    int next;                  /* configuration whose turn it is to accumulate */
+    int status;
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+} ew_job;
+
+typedef struct { ew_job *J; int t; } ew_arg;
+
+static void *ew_worker(void *arg)
+{
+    ew_arg *a = (ew_arg *)arg;
+    ew_job *J = a->J;
+    int st;
+    sim_t *s = sim_create(J->in, &st);
+    size_t nxy = s ? (size_t)s->g.nx * s->g.ny : 0;
+    int nthk = s ? s->n_thk : 0;
+    cplx **po = s ? malloc(sizeof(cplx *) * (size_t)(nthk > 0 ? nthk : 1)) : NULL;
+    for (int k = 0; po && k < nthk; k++) po[k] = malloc(sizeof(cplx) * nxy);
+    for (int iconf = J->iconf_0 + a->t; iconf <= J->iconf_e; iconf += J->nt) {
+        int ok = 0;
+        if (s && po) {
+            move_atoms(s, iconf);
+            ok = run_ew_conf(s, po) == MT_OK;
+        }
+        pthread_mutex_lock(&J->mu);
+        while (J->next != iconf) pthread_cond_wait(&J->cv, &J->mu);
+        if (ok && J->status == MT_OK) ew_accumulate(s, J->A, po);
+        else if (J->status == MT_OK) J->status = s ? MT_ERR_OUT_OF_MEMORY : st;
+        J->next++;
+        pthread_cond_broadcast(&J->cv);
+        pthread_mutex_unlock(&J->mu);
+    }
+    for (int k = 0; po && k < nthk; k++) free(po[k]);
+    free(po);
+    if (s) sim_free(s);
+    return NULL;
+}
+
+static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice *out, int nthreads)
 {
     const grid_t *g = &s->g;
     size_t nxy = (size_t)g->nx * g->ny;
     int nxo = s->oa_ix_e - s->oa_ix_0, nyo = s->oa_iy_e - s->oa_iy_0;
     size_t no = (size_t)nxo * nyo;
     int nconf_w = (!is_frozen(s) || s->pn_single_conf) ? 1 : s->pn_nconf;
-    double w = (double)(real)(1.0 / (real)nconf_w);
     mt_output_multislice dims;
     memset(&dims, 0, sizeof dims);
     output_dims_from_sim(s, &dims);
-    int ot = dims.output_type;
-    double *m2 = calloc(no * (size_t)s->n_thk, sizeof(double));
-    double *pc = calloc(2 * nxy * (size_t)s->n_thk, sizeof(double));   /* psi_coh, uncropped for CBED */
-    double *pcs = calloc(2 * no * (size_t)s->n_thk, sizeof(double));
-    cplx **po = malloc(sizeof(cplx *) * (size_t)s->n_thk);
-    for (int k = 0; k < s->n_thk; k++) po[k] = malloc(sizeof(cplx) * nxy);
+    ew_acc A;
+    A.ot = dims.output_type;
+    A.w = (double)(real)(1.0 / (real)nconf_w);
+    A.no = no;
+    A.nxy = nxy;
+    A.m2 = calloc(no * (size_t)s->n_thk, sizeof(double));
+    A.pc = calloc(2 * nxy * (size_t)s->n_thk, sizeof(double));   /* psi_coh, uncropped for CBED */
+    A.pcs = calloc(2 * no * (size_t)s->n_thk, sizeof(double));
     int st = MT_OK;
-    for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
-        move_atoms(s, iconf);
-        st = run_ew_conf(s, po);
-        if (st != MT_OK) break;
-        for (int k = 0; k < s->n_thk; k++) {
-            if (ot == MT_OT_PSI_COH) {
-                accumulate_output(s, po[k], 1.0, 0, 1, pcs + 2 * no * k, 1);
-            } else if (ot == MT_OT_M2PSI_TOT_PSI_COH) {
-                accumulate_output(s, po[k], w, 1, 0, m2 + no * k, 0);
-                accumulate_output(s, po[k], w, 0, 0, pcs + 2 * no * k, 1);
-            } else {
-                accumulate_output(s, po[k], w, 1, 0, m2 + no * k, 0);
-                if (s->pn_coh_contrib)
-                    for (size_t i = 0; i < nxy; i++) {
-                        pc[2 * (nxy * k + i)] += w * po[k][i].re;
-                        pc[2 * (nxy * k + i) + 1] += w * po[k][i].im;
-                    }
-            }
+    int nconf = s->pn_nconf - s->fp_iconf_0 + 1;
+    int nt = nthreads > 1 ? (nthreads < nconf ? nthreads : nconf) : 1;
+    if (nt <= 1) {
+        cplx **po = malloc(sizeof(cplx *) * (size_t)s->n_thk);
+        for (int k = 0; k < s->n_thk; k++) po[k] = malloc(sizeof(cplx) * nxy);
+        for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
+            move_atoms(s, iconf);
+            st = run_ew_conf(s, po);
+            if (st == MT_OK) ew_accumulate(s, &A, po);
         }
+        for (int k = 0; k < s->n_thk; k++) free(po[k]);
+        free(po);
+    } else {
+        ew_job J;
+        J.in = in; J.A = &A; J.iconf_0 = s->fp_iconf_0; J.iconf_e = s->pn_nconf; J.nt = nt;
+        J.next = s->fp_iconf_0; J.status = MT_OK;
+        pthread_mutex_init(&J.mu, NULL);
+        pthread_cond_init(&J.cv, NULL);
+        pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nt);
+        ew_arg *args = malloc(sizeof(ew_arg) * (size_t)nt);
+        for (int t = 0; t < nt; t++) {
+            args[t].J = &J;
+            args[t].t = t;
+            pthread_create(&th[t], NULL, ew_worker, &args[t]);
+        }
+        for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+        free(th); free(args);
+        pthread_mutex_destroy(&J.mu);
+        pthread_cond_destroy(&J.cv);
+        st = J.status;
     }
+    int ot = A.ot;
     if (st == MT_OK) {
         if (ot == MT_OT_M2PSI_TOT_COH && out->m2psi_coh) {
             /* set_m2psi_coh (wave_function.cuh:188-227): |psi_coh|^2 */
             cplx *tmp = malloc(sizeof(cplx) * nxy);
             for (int k = 0; k < s->n_thk; k++) {
-                for (size_t i = 0; i < nxy; i++) { tmp[i].re = (real)pc[2 * (nxy * k + i)]; tmp[i].im = (real)pc[2 * (nxy * k + i) + 1]; }
+                for (size_t i = 0; i < nxy; i++) { tmp[i].re = (real)A.pc[2 * (nxy * k + i)]; tmp[i].im = (real)A.pc[2 * (nxy * k + i) + 1]; }
                 accumulate_output(s, tmp, 1.0, 1, 1, out->m2psi_coh + no * k, 0);
             }
             free(tmp);
         }
         if (out->m2psi_tot && (ot == MT_OT_M2PSI_TOT || ot == MT_OT_M2PSI_TOT_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
-            memcpy(out->m2psi_tot, m2, sizeof(double) * no * (size_t)s->n_thk);
+            memcpy(out->m2psi_tot, A.m2, sizeof(double) * no * (size_t)s->n_thk);
         if (out->psi_coh && (ot == MT_OT_PSI_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
-            memcpy(out->psi_coh, pcs, sizeof(double) * 2 * no * (size_t)s->n_thk);
+            memcpy(out->psi_coh, A.pcs, sizeof(double) * 2 * no * (size_t)s->n_thk);
     }
-    for (int k = 0; k < s->n_thk; k++) free(po[k]);
-    free(po); free(m2); free(pc); free(pcs);
+    free(A.m2); free(A.pc); free(A.pcs);
     return st;
 }
 
@@ -1918,7 +2067,7 @@ int orc_run(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_
     out->output_type = d.output_type; out->nx = d.nx; out->ny = d.ny; out->n_thk = d.n_thk;
     out->n_det = d.n_det; out->dx = d.dx; out->dy = d.dy;
     fill_axes(s, out);
-    st = is_stem(s) ? run_stem(s, out, nthreads) : run_ew(s, out);
+    st = is_stem(s) ? run_stem(s, out, nthreads) : run_ew(s, in, out, nthreads);
     sim_free(s);
     return st;
 }
@@ -2025,7 +2174,11 @@ int orc_incident_wave(const mt_system_conf *sys, const mt_input_multislice *in, 
     cplx *work = malloc(sizeof(cplx) * (size_t)s->g.nx * 16);
     fft2 F;
     fft2_init(&F, s->g.nx, s->g.ny);
-    incident_wave(s, s->iw_x, s->iw_y, s->n_iw, p, &F, work, f);
+    /* the stand-alone incident wave evaluates chi at g + g0 (Incident_Wave::operator(),
+       src/incident_wave.cuh:116-131); inside a simulation the probe is untilted (gxu = gyu = 0,
+       Wave_Function::set_incident_wave, src/wave_function.cuh:300-307) */
+    incident_wave(s, s->iw_x, s->iw_y, s->n_iw, sin(s->theta) * cos(s->phi) / s->lambda,
+                  sin(s->theta) * sin(s->phi) / s->lambda, p, &F, work, f);
     unshift_cplx(&s->g, p, psi0);
     fft2_free(&F);
     free(p); free(f); free(work);

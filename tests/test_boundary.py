@@ -83,7 +83,7 @@ def test_scan_grid_square_region():
     ("nx", 0, 1),
     ("E_0", 0.0, 1),
     ("iw_type", 3, 2),                # user-defined wave
-    ("theta", 0.5, 2),                # tilt
+    ("theta", 95.0, 1),               # tilt beyond 90 degrees
 ])
 def test_errors_are_reported(field, value, status):
     p = configs.si_ewrs_256()

@@ -212,3 +212,98 @@ def test_cbed_small():
         e = rel_l2(out.m2psi_tot[k], r64.m2psi_tot[k])
         print("CBED thickness", k, "rel L2 vs FP64 %.3e" % e)
         assert e < 1e-4
+
+
+def _group_inputs():
+    return [configs.sto_stem(n=256, ncells=6, nc=3, ns=6, nconf=2),
+            configs.sto_stem(n=512, ncells=12, nc=2, ns=5, nconf=1),
+            small_stem(n=256, ns=4, nconf=3)]
+
+
+@pytest.mark.parametrize("batch", [0, 3])
+def test_group_uneven_sizes_matches_solo_runs(batch):
+    """Specimens of different grid sizes in one group run give, bit for bit,
+    what each gives alone (fixed launch variants; batch 3 makes the
+    specimens run out of work in different rounds)."""
+    ps = _group_inputs()
+    engine.set_pass_variants(0, 0, 0)
+    engine.set_batch(batch, True)
+    try:
+        solo = [engine.run_multislice(p).image_tot.copy() for p in ps]
+        grp = engine.run_multislice_group(ps)
+        plan = engine.last_plan()
+    finally:
+        engine.set_batch(0, True)
+        engine.set_pass_variants(-1, -1, -1)
+    assert plan.startswith("group ") and plan.count("stem nx=") == 3
+    for a, o in zip(solo, grp):
+        assert np.array_equal(a, o.image_tot)
+
+
+def test_group_vs_oracle():
+    ps = _group_inputs()[:2]
+    outs = engine.run_multislice_group(ps)
+    for p, o in zip(ps, outs):
+        e = rel_l2(o.image_tot, oracle.run(p, "f64").image_tot)
+        print("group member %d^2: rel L2 vs FP64 %.3e" % (p.nx, e))
+        assert e < 1e-4
+
+
+def test_group_refuses_non_stem():
+    with pytest.raises(engine.EngineError) as ei:
+        engine.run_multislice_group([small_stem(), configs.si_ewrs_256()])
+    assert ei.value.status == 2
+
+
+def test_tilted_exit_wave():
+    p = configs.si_ewrs_256()
+    p.theta, p.phi = 0.6, 30.0                    # degrees
+    out = engine.run_multislice(p)
+    e32 = rel_l2(out.psi_coh, oracle.run(p, "f32").psi_coh)
+    e64 = rel_l2(out.psi_coh, oracle.run(p, "f64").psi_coh)
+    print("tilted exit wave: rel L2 vs FP32 restatement %.3e, vs FP64 %.3e" % (e32, e64))
+    assert e32 < 1e-5 and e64 < 1e-4
+
+
+def test_tilted_stem_and_cbed():
+    p = small_stem(ns=3, nconf=2)
+    p.theta, p.phi = 0.5, 0.0
+    e = rel_l2(engine.run_multislice(p).image_tot, oracle.run(p, "f64").image_tot)
+    print("tilted STEM rel L2 vs FP64 %.3e" % e)
+    assert e < 1e-4
+    q = configs.pt_cbed_1024(nconf=2, n=256, na=4, nb=6, nc=12, thick=[15.0, 30.0])
+    q.theta, q.phi = 0.3, 45.0
+    out, ref = engine.run_multislice(q), oracle.run(q, "f64")
+    for k in range(out.n_thk):
+        e = rel_l2(out.m2psi_tot[k], ref.m2psi_tot[k])
+        print("tilted CBED thickness %d rel L2 vs FP64 %.3e" % (k, e))
+        assert e < 1e-4
+
+
+def test_tilted_pieces():
+    p = small_stem()
+    p.simulation_type = 21
+    p.iw_type = 2
+    p.cond_lens_c_10 = 20.0
+    p.iw_x = np.array([0.4 * p.spec_lx])
+    p.iw_y = np.array([0.5 * p.spec_ly])
+    p.theta, p.phi = 0.4, 60.0
+    assert rel_l2(engine.incident_wave(p), oracle.incident_wave(p, precision="f64")) < 1e-5
+    rng = np.random.default_rng(3)
+    psi = rng.standard_normal((p.nx, p.ny)) + 1j * rng.standard_normal((p.nx, p.ny))
+    for space in (1, 2):
+        a = engine.propagate(p, psi, 37.0, space)
+        b = oracle.propagate(p, psi, 37.0, space, precision="f64")
+        assert rel_l2(a, b) < 1e-5, space
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5])
+def test_stem_coherent_image(theta):
+    p = small_stem(ns=3, nconf=3)
+    p.pn_coh_contrib = 1
+    p.theta = theta
+    out, ref = engine.run_multislice(p), oracle.run(p, "f64")
+    e_tot = rel_l2(out.image_tot, ref.image_tot)
+    e_coh = rel_l2(out.image_coh, ref.image_coh)
+    print("coherent STEM theta=%.1f: image_tot %.3e, image_coh %.3e" % (theta, e_tot, e_coh))
+    assert e_tot < 1e-4 and e_coh < 1e-4

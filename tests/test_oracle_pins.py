@@ -243,3 +243,49 @@ def test_phonon_displacements():
         assert abs(np.std(dx) - 0.085) < 0.01 and abs(np.std(dy) - 0.085) < 0.01
         assert abs(np.mean(dx)) < 0.01
     assert not np.array_equal(oracle.displaced_atoms(p, 1), oracle.displaced_atoms(p, 2))
+
+
+def test_threaded_cbed_equals_sequential():
+    # configurations run in parallel but are accumulated in configuration order
+    p = configs.pt_cbed_1024(nconf=5, n=128, na=3, nb=4, nc=6, thick=[4.0, 8.0])
+    a = oracle.run(p, "f64", nthreads=1).m2psi_tot.copy()
+    b = oracle.run(p, "f64", nthreads=4).m2psi_tot.copy()
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_tilted_vacuum_gives_one_spot(precision):
+    # Through an empty specimen a tilted plane wave keeps unit modulus; the
+    # output ramp exp(-2 pi i g0.r) then puts all intensity in one Fourier
+    # pixel, -g0 / dg, where sin(theta) is quantised to n lambda dg.
+    p = configs.si_ewrs_256(nc=2, n=128)
+    p.spec_atoms = p.spec_atoms.copy()
+    p.spec_atoms[:, 5] = 0.0                      # occupancy 0: no potential
+    p.simulation_type = 51                        # EWFS
+    p.obj_lens_zero_defocus_type = 1
+    lam = 0.0196875                               # 300 kV
+    dg = 1.0 / p.spec_lx
+    n_tilt = 3
+    p.theta = np.degrees(np.arcsin(n_tilt * lam * dg * 1.2))   # quantised to n = round(3.6) = 4
+    p.phi = 0.0
+    out = oracle.run(p, precision)
+    I = np.abs(out.psi_coh[0]) ** 2              # [nx, ny], centred (fftshift) layout
+    k = np.unravel_index(np.argmax(I), I.shape)
+    assert I.sum() == pytest.approx(1.0, rel=1e-5)
+    assert I[k] == pytest.approx(1.0, rel=1e-5)
+    assert (k[0] - p.nx // 2, k[1] - p.ny // 2) == (-4, 0)
+
+
+def test_stem_coherent_image_bounds():
+    # one configuration: the coherent image is the total image; several:
+    # |sum w psi|^2 <= sum w |psi|^2 (sum w = 1) at every pixel
+    p = configs.au_stem_512(ns=3, nconf=1, n=128, ncells=2, nc=3)
+    p.pn_coh_contrib = 1
+    p.pn_model = 1
+    o = oracle.run(p, "f64")
+    assert np.allclose(o.image_coh, o.image_tot, rtol=1e-12, atol=0)
+    p = configs.au_stem_512(ns=3, nconf=4, n=128, ncells=2, nc=3)
+    p.pn_coh_contrib = 1
+    o = oracle.run(p, "f64")
+    assert np.all(o.image_coh <= o.image_tot * (1 + 1e-12))
+    assert np.all(o.image_coh < o.image_tot)     # phonons remove coherent intensity from the ADF ring
