@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ns", type=int, default=16)
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-size group measurement")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM counters")
     return ap.parse_args()
@@ -101,6 +102,26 @@ def group_measurement():
             "sequential_value": work / t_seq, "plan": plan}
 
 
+SURROGATE_WEIGHTS = os.path.join(ROOT, "multem_amd", "data", "surrogate_6x32.npz")
+
+
+def surrogate_measurement():
+    """Frozen-phonon surrogate on 4 held-out random specimens (32x32 scans,
+    256^2 grid): GPU time of plain 64-configuration multislice over that of
+    the surrogate path (2-configuration run + potential map + network), and
+    relative RMS errors against the 64-configuration image."""
+    from multem_amd import surrogate
+    model = surrogate.Surrogate.load(SURROGATE_WEIGHTS)
+    ev = surrogate.evaluate(model, n_specimens=4, seed=2024)
+    return {"metric": "surrogate ADF: plain 64-config GPU time / surrogate GPU time",
+            "gpu_time_ratio": ev["gpu_time_ratio"],
+            "rel_rms_surrogate_vs_64conf": ev["rel_rms_surrogate_mean"],
+            "rel_rms_2conf_vs_64conf": ev["rel_rms_small_mean"],
+            "ms_plain": sum(r["ms_plain"] for r in ev["specimens"]),
+            "ms_surrogate": sum(r["ms_surrogate"] for r in ev["specimens"]),
+            "specimens": [r["desc"] + " %.0f A" % r["thick"] for r in ev["specimens"]]}
+
+
 def copy_calibration():
     """Device-to-device copy of 1 GiB on the engine's HIP runtime: the known-size
     reference for rocprofv3's FETCH_SIZE / WRITE_SIZE, and the achievable copy
@@ -122,7 +143,8 @@ def copy_calibration():
     dt = (time.perf_counter() - t0) / reps
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"copy_bytes_per_launch": n, "read_plus_write_GBs": 2 * n / dt / 1e9, "ms": dt * 1e3}))
+    print(json.dumps({"copy_bytes_per_launch": n, "read_plus_write_GBs": 2 * n / dt / 1e9, "ms": dt * 1e3,
+                      "stream_kernel_read_plus_write_GBs": engine.stream_copy_bandwidth(n, reps)}))
 
 
 def traffic_from_profiles(kernel, n):
@@ -224,6 +246,8 @@ def main():
     }
     if not args.no_group:
         out["uneven_group"] = group_measurement()
+    if not args.no_surrogate and os.path.exists(SURROGATE_WEIGHTS):
+        out["surrogate"] = surrogate_measurement()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))

@@ -218,6 +218,13 @@ mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multisli
 mt_status mt_hip_projected_potential(const mt_system_conf *sys, const mt_input_multislice *in,
                                      int iconf, int islice, double *V);
 
+/* Sum over all slices of the projected potential of configuration iconf
+   (natural layout, nx*ny doubles, accumulated in FP64 in slice order): the
+   potential map input of the surrogate network.  No counterpart in the
+   reference (sum of ilc_projected_potential over islice). */
+mt_status mt_hip_projected_potential_sum(const mt_system_conf *sys, const mt_input_multislice *in, int iconf,
+                                         double *V);
+
 /* Transmission function exp(i sigma V) of slice islice, ny x nx complex,
    natural layout.  Replaces ilc_transmission_function
    (ilc_transmission_function.cu:142-168). */
@@ -280,6 +287,11 @@ void mt_hip_set_batch(int batch, int use_graph);
    launch counts; ms_tf is the transmission-function build time. */
 void mt_hip_kernel_timing(double *ms_row, int *n_row, double *ms_col, int *n_col, double *ms_tf);
 
+/* Read + write bandwidth (GB/s) of a plain streaming copy kernel (16-byte
+   accesses) over `bytes` bytes, `reps` launches: the achievable-bandwidth
+   reference for the slice passes on this GPU. */
+mt_status mt_hip_stream_copy_bandwidth(int device, size_t bytes, int reps, double *gbs);
+
 /* Autotuner of the slice step.  Each pass has MT_PASS_VARIANTS launch
    variants (radix schedule, rows/columns per workgroup, twiddles in registers
    or read from the cached table).  Variants with the default schedule give
@@ -289,7 +301,7 @@ void mt_hip_kernel_timing(double *ms_row, int *n_row, double *ms_col, int *n_col
    epilogue) and the choice kept for the process (default); 2: timed on
    every run.  The choice is the lowest-numbered variant within 2 % of the
    fastest. */
-#define MT_PASS_VARIANTS 7
+#define MT_PASS_VARIANTS 8
 void mt_hip_set_autotune(int mode);
 
 /* Forces the variants of the row pass, the column pass and the last-slice

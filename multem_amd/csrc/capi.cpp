@@ -139,6 +139,13 @@ mt_status mt_hip_projected_potential(const mt_system_conf *sys, const mt_input_m
     return guarded(sys, in, [&](const mtg::Model &m) { mtg::projected_potential(m, sys->gpu_device, iconf, islice, V); });
 }
 
+mt_status mt_hip_projected_potential_sum(const mt_system_conf *sys, const mt_input_multislice *in, int iconf,
+                                         double *V)
+{
+    if (!V) return fail(MT_ERR_INVALID_ARGUMENT, "null V");
+    return guarded(sys, in, [&](const mtg::Model &m) { mtg::projected_potential_sum(m, sys->gpu_device, iconf, V); });
+}
+
 mt_status mt_hip_transmission_function(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, int islice,
                                        double *trans)
 {
@@ -248,6 +255,23 @@ mt_status mt_hip_tune_log(char *buf, size_t n)
     if (!buf || n == 0) return fail(MT_ERR_INVALID_ARGUMENT, "null buffer");
     std::snprintf(buf, n, "%s", mtg::last_timing().tune_log.c_str());
     return MT_OK;
+}
+
+// Read + write bandwidth of a plain streaming copy kernel (16-byte accesses)
+// over `bytes` bytes, `reps` launches, on GPU `device`: the reference for
+// the achievable HBM bandwidth of the slice passes.
+mt_status mt_hip_stream_copy_bandwidth(int device, size_t bytes, int reps, double *gbs)
+{
+    if (!gbs || bytes < 16 || bytes % 16 || reps < 1) return fail(MT_ERR_INVALID_ARGUMENT, "bytes (multiple of 16), reps >= 1");
+    std::lock_guard<std::mutex> lock(g_engine_mutex);
+    try {
+        *gbs = mtg::stream_copy_gbs(device, bytes, reps);
+        return MT_OK;
+    } catch (const mtg::Error &e) {
+        return fail(e.code, e.msg);
+    } catch (const std::exception &e) {
+        return fail(MT_ERR_DEVICE, e.what());
+    }
 }
 
 // Per-kernel timing of the last run: summed row / column pass durations and launch counts.

@@ -232,6 +232,23 @@ struct Setup {
         P.c30 = (std::fabs(m.cond.c_30) <= 1e-5f) ? 0.0 : -kPi * (double)m.cond.c_30 * std::pow(lam, 3) / 2.0;
         P.c50 = (std::fabs(m.cond.c_50) <= 1e-5f) ? 0.0 : -kPi * (double)m.cond.c_50 * std::pow(lam, 5) / 3.0;
         P.dgx = 1.0 / m.lx_d; P.dgy = 1.0 / m.ly_d;
+        // azimuthal coefficients c_c_* of Lens::set_input_data (src/types.cuh:2924-2976)
+        const LensModel &L = m.cond;
+        if (L.phi_required) {
+            P.azim = 1;
+            P.az_m = (double)L.m;
+            const double l1 = lam, l2 = lam * lam, l3 = l2 * lam, l4 = l3 * lam, l5 = l4 * lam;
+            const double cf[11] = {-kPi * l1, -2.0 * kPi * l2 / 3.0, -2.0 * kPi * l2 / 3.0, -kPi * l3 / 2.0,
+                                   -kPi * l3 / 2.0, -2.0 * kPi * l4 / 5.0, -2.0 * kPi * l4 / 5.0,
+                                   -2.0 * kPi * l4 / 5.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0};
+            const float cv[11] = {L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
+            const float pv[11] = {L.phi_12, L.phi_21, L.phi_23, L.phi_32, L.phi_34, L.phi_41, L.phi_43, L.phi_45,
+                                  L.phi_52, L.phi_54, L.phi_56};
+            for (int k = 0; k < 11; ++k) {
+                P.az_c[k] = (std::fabs(cv[k]) <= 1e-5f) ? 0.0 : cf[k] * (double)cv[k];
+                P.az_phi[k] = (double)pv[k];
+            }
+        }
         P.dgx_f = m.dgx; P.dgy_f = m.dgy;
         P.g2_min = m.cond.g2_min; P.g2_max = m.cond.g2_max;
         P.norm = n_aperture ? (float)std::sqrt(1.0 / (double)n_aperture) : 0.f;
@@ -473,6 +490,7 @@ public:
     std::vector<Beam> hb;
     int cur_conf = -1;
     double ms_tf = 0;
+    std::string batch_log;       // batch-width selection log
 
     // budget: bytes this job may use for its transmission stack and waves
     StemJob(const Model &mm, size_t budget, int batch_override)
@@ -493,21 +511,30 @@ public:
             throw Error{MT_ERR_OUT_OF_MEMORY, m.pn_coh_contrib ? "transmission stack and coherent accumulators "
                                                                   "exceed the memory budget"
                                                                 : "transmission stack exceeds the memory budget"};
+        const size_t per_wave = (m.tilted() ? 2 : 1) * nxy * sizeof(float2);
+        const int B_cap = (int)std::min<size_t>((size_t)std::max(1, nscan), (budget - stack_bytes) / per_wave);
+        if (B_cap < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for a wave batch"};
+        ntile = m.ny / col_tile_width(m.nx);
+        trans.alloc(m.n_slice() * nxy);
+        std::vector<double> zl;
+        for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
+        for (const ThickInfo &t : m.thk) zl.push_back((double)t.z_back_prop);
+        S.prop_tables(zl, px, py);
+
         B = batch_override;
         if (B <= 0) {
             // enough waves to fill 256 CUs in the column pass, and a working set
-            // that stays near the 256 MB last-level cache
+            // that stays near the 256 MB last-level cache; the tuner may double
+            // or quadruple it
             const int nt = m.ny / col_tile_width(m.nx);
             B = std::max(1, (int)std::min<size_t>(256, (size_t)(160u << 20) / (nxy * sizeof(float2))));
             B = std::max(B, (2048 + nt - 1) / nt);
+            B = std::min(B, B_cap);
+            if (run_control().autotune != 0) B = choose_batch(B, std::min(B_cap, std::max(B, 256)));
         }
-        B = std::min(B, std::max(1, nscan));
-        B = (int)std::min<size_t>(B, (budget - stack_bytes) / ((m.tilted() ? 2 : 1) * nxy * sizeof(float2)));
-        if (B < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for a wave batch"};
-        ntile = m.ny / col_tile_width(m.nx);
+        B = std::min(B, B_cap);
         nbatch = (nscan + B - 1) / B;
 
-        trans.alloc(m.n_slice() * nxy);
         psi.alloc((size_t)B * nxy);
         members.alloc(B);
         beams.alloc(B);
@@ -517,10 +544,6 @@ public:
             std::vector<double> zero(image.n, 0.0);
             image.upload(zero.data(), zero.size(), s);
         }
-        std::vector<double> zl;
-        for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
-        for (const ThickInfo &t : m.thk) zl.push_back((double)t.z_back_prop);
-        S.prop_tables(zl, px, py);
         if (m.tilted()) {
             S.ramp_tables(rx, ry);
             scratch.alloc((size_t)B * nxy);
@@ -561,6 +584,59 @@ public:
         check(hipMemsetAsync(psi.p, 0, psi.n * sizeof(float2), s), "hipMemsetAsync");
         choice = choose_variants(m.nx, m.ny, B, rt, ct, cl.ithk >= 0 ? &cl : nullptr, s);
         r.variant = choice.row;
+    }
+
+    // Batch width: B0, 2 B0 and 4 B0 (up to B_max) timed on the full slice
+    // step (variant 0 of both passes) per wave; the smallest within 2 % of
+    // the best wins.  Cached per (grid, B0, B_max).
+    int choose_batch(int B0, int B_max)
+    {
+        std::vector<int> cand;
+        for (int b = B0; b <= B_max && cand.size() < 3; b *= 2) cand.push_back(b);
+        if (cand.size() < 2) return B0;
+        static std::map<std::tuple<int, int, int, int>, int> cache;
+        const auto key = std::make_tuple(m.nx, m.ny, B0, B_max);
+        const bool reuse = run_control().autotune == 1;
+        if (reuse) {
+            auto it = cache.find(key);
+            if (it != cache.end()) {
+                batch_log = "batch=" + std::to_string(it->second) + "(cached)";
+                return it->second;
+            }
+        }
+        const int bm = cand.back();
+        DBuf<float2> w((size_t)bm * nxy);
+        DBuf<Member> mem(bm);
+        std::vector<Member> hmv(bm, Member{0, -1, 0, 1});
+        mem.upload(hmv.data(), bm, s);
+        check(hipMemsetAsync(w.p, 0, w.n * sizeof(float2), s), "hipMemsetAsync");
+        std::ostringstream log;
+        log << "batch[";
+        double best = -1;
+        std::vector<double> per(cand.size());
+        for (size_t i = 0; i < cand.size(); ++i) {
+            RowArgs rr{};
+            rr.src = w.p; rr.dst = w.p; rr.member_stride = (long long)nxy; rr.nrows = m.nx; rr.n_members = cand[i];
+            rr.tw = S.tw_y.p; rr.trans = trans.p; rr.members = mem.p; rr.slice = 0;
+            rr.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
+            ColArgs cc = S.col_base(w.p, w.p, cand[i]);
+            cc.flags = col_flags(false);
+            cc.px = px.p; cc.py = py.p; cc.ithk = -1;
+            const double t = time_variant([&] {
+                hipError_t e = launch_row_pass(m.ny, rr, s);
+                return e != hipSuccess ? e : launch_col_pass(m.nx, cc, false, false, s);
+            }, s);
+            per[i] = t / cand[i];
+            if (per[i] > 0 && (best < 0 || per[i] < best)) best = per[i];
+            log << (i ? "," : "") << cand[i] << ":" << (int)std::lround(t * 1000.0) << "us";
+        }
+        int pick = B0;
+        for (size_t i = 0; i < cand.size(); ++i)
+            if (per[i] > 0 && per[i] <= best * 1.02) { pick = cand[i]; break; }
+        log << "]=" << pick;
+        batch_log = log.str();
+        cache[key] = pick;
+        return pick;
     }
 
     int col_flags(bool last) const
@@ -695,7 +771,7 @@ void run_stem(const Model &m, mt_output_multislice *out)
     StemJob J(m, budget_bytes(), run_control().batch_override);
     hipStream_t s = J.s;
     tm.plan = J.plan();
-    tm.tune_log = J.choice.log;
+    tm.tune_log = (J.batch_log.empty() ? "" : J.batch_log + " ") + J.choice.log;
     run_control().niter = m.n_conf_run() * J.nscan;
     run_control().iter = 0;
 
@@ -774,7 +850,8 @@ void run_stem_group(const std::vector<const Model *> &ms, const std::vector<mt_o
         const size_t share = (size_t)((double)budget * want[i] / tot);
         jobs.emplace_back(new StemJob(*ms[i], share, run_control().batch_override));
         plan << (i ? " | " : "") << jobs.back()->plan();
-        tune << (i ? " | " : "") << jobs.back()->choice.log;
+        tune << (i ? " | " : "") << jobs.back()->batch_log << (jobs.back()->batch_log.empty() ? "" : " ")
+             << jobs.back()->choice.log;
         rounds = std::max(rounds, jobs.back()->n_work());
         niter += ms[i]->n_conf_run() * jobs.back()->nscan;
     }
@@ -1053,8 +1130,7 @@ void run_multislice_group(const std::vector<const Model *> &ms, int device, cons
     for (size_t i = 0; i < ms.size(); ++i) {
         const Model &m = *ms[i];
         if (!m.is_stem()) throw Error{MT_ERR_UNSUPPORTED, "group runs take STEM inputs only"};
-        if (m.cond.phi_required) throw Error{MT_ERR_UNSUPPORTED, "azimuthal condenser aberrations"};
-        m.output_dims(outs[i]);
+            m.output_dims(outs[i]);
         m.fill_axes(outs[i]);
     }
     run_stem_group(ms, outs);
@@ -1063,7 +1139,6 @@ void run_multislice_group(const std::vector<const Model *> &ms, int device, cons
 void run_multislice(const Model &m, int device, mt_output_multislice *out)
 {
     select_device(device);
-    if (m.cond.phi_required) throw Error{MT_ERR_UNSUPPORTED, "azimuthal condenser aberrations"};
     m.output_dims(out);
     m.fill_axes(out);
     if (m.is_stem()) run_stem(m, out);
@@ -1088,6 +1163,48 @@ void projected_potential(const Model &m, int device, int iconf, int islice, doub
     check(hipMemcpyAsync(h.data(), dv.p, S.nxy * sizeof(float), hipMemcpyDeviceToHost, st.s), "D2H");
     check(hipStreamSynchronize(st.s), "sync");
     for (size_t i = 0; i < S.nxy; ++i) V[i] = h[i];
+}
+
+double stream_copy_gbs(int device, size_t bytes, int reps)
+{
+    select_device(device);
+    Stream st;
+    DBuf<char> a(bytes), b(bytes);
+    check(hipMemsetAsync(a.p, 1, bytes, st.s), "hipMemsetAsync");
+    check(launch_stream_copy(a.p, b.p, bytes, st.s), "stream copy");
+    Event t0, t1;
+    check(hipEventRecord(t0.e, st.s), "hipEventRecord");
+    for (int i = 0; i < reps; ++i) check(launch_stream_copy(a.p, b.p, bytes, st.s), "stream copy");
+    check(hipEventRecord(t1.e, st.s), "hipEventRecord");
+    check(hipEventSynchronize(t1.e), "hipEventSynchronize");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, t0.e, t1.e), "hipEventElapsedTime");
+    return 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+}
+
+// Sum over all slices of the projected potential of configuration iconf
+// (natural layout), accumulated in FP64 in slice order; the device time of
+// the build and the sum is reported as the run's ms_total.
+void projected_potential_sum(const Model &m, int device, int iconf, double *V)
+{
+    select_device(device);
+    Timing &tm = last_timing();
+    tm = Timing{};
+    Stream st;
+    Setup S(m, st.s);
+    auto a = S.atoms_of(iconf);
+    DBuf<float4> atoms(a.size());
+    atoms.upload(a.data(), a.size(), st.s);
+    DBuf<float> dv((size_t)m.n_slice() * S.nxy);
+    DBuf<double> sum(S.nxy);
+    Event t0, t1;
+    record(t0, st.s);
+    S.build_tf(atoms, 0, m.n_slice(), nullptr, dv.p, nullptr);
+    check(launch_slice_sum(dv.p, m.n_slice(), (long long)S.nxy, sum.p, st.s), "slice sum");
+    record(t1, st.s);
+    check(hipMemcpyAsync(V, sum.p, S.nxy * sizeof(double), hipMemcpyDeviceToHost, st.s), "D2H");
+    check(hipStreamSynchronize(st.s), "sync");
+    tm.ms_total = elapsed(t0, t1);
 }
 
 static void download_unshifted(Setup &S, const float2 *d, double *out)
@@ -1117,7 +1234,6 @@ void transmission_function(const Model &m, int device, int iconf, int islice, do
 void incident_wave(const Model &m, int device, double *psi0)
 {
     select_device(device);
-    if (m.cond.phi_required) throw Error{MT_ERR_UNSUPPORTED, "azimuthal condenser aberrations"};
     Stream st;
     Setup S(m, st.s);
     DBuf<float2> w(S.nxy);

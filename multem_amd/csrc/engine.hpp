@@ -53,6 +53,9 @@ void run_multislice_group(const std::vector<const Model *> &ms, int device, cons
 
 // Per-piece entries (natural, unshifted layouts).
 void projected_potential(const Model &m, int device, int iconf, int islice, double *V);
+// read + write bandwidth (GB/s) of a streaming copy of `bytes` bytes, reps times
+double stream_copy_gbs(int device, size_t bytes, int reps);
+void projected_potential_sum(const Model &m, int device, int iconf, double *V);
 void transmission_function(const Model &m, int device, int iconf, int islice, double *trans);
 void incident_wave(const Model &m, int device, double *psi0);
 void propagate(const Model &m, int device, const double *psi_in, double z, int space, double *psi_out);

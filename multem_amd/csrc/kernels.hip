@@ -12,6 +12,7 @@
 #include "fft_core.hpp"
 
 #include <type_traits>
+#include <algorithm>
 #include <utility>
 
 namespace mtg {
@@ -170,7 +171,19 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
             if (P.g2_min <= g2f && g2f < P.g2_max) {
                 const double gx = igx * P.dgx + P.gxu;
                 const double g2 = gx * gx + gy * gy;
-                const double aber = (P.c10 + (P.c30 + P.c50 * g2) * g2) * g2;
+                double aber = (P.c10 + (P.c30 + P.c50 * g2) * g2) * g2;
+                if (P.azim) {
+                    // powers of |g| and multiplicities of the 11 azimuthal terms
+                    constexpr int pw[11] = {2, 3, 3, 4, 4, 5, 5, 5, 6, 6, 6};
+                    constexpr int mu[11] = {2, 1, 3, 2, 4, 1, 3, 5, 2, 4, 6};
+                    const double g1 = sqrt(g2), ph = atan2(gy, gx);
+                    double gp[7];
+                    gp[0] = 1.0;
+                    for (int k = 1; k < 7; ++k) gp[k] = gp[k - 1] * g1;
+                    if (ph != 0.0) aber += P.az_m * ph;
+                    for (int k = 0; k < 11; ++k)
+                        if (P.az_c[k] != 0.0) aber += P.az_c[k] * gp[pw[k]] * sin(mu[k] * (ph - P.az_phi[k]));
+                }
                 for (int b = 0; b < m.nbeam; ++b) {
                     const Beam bm = a.beams[m.beam0 + b];
                     double chi = bm.xph * gx + bm.yph * gy + aber;
@@ -515,6 +528,22 @@ __global__ __launch_bounds__(256) void k_coh_image(const double2 *coh, int nscan
     if (threadIdx.x < n_det) image_coh[((long long)ithk * n_det + threadIdx.x) * nscan + iscan] = red[threadIdx.x][0];
 }
 
+__global__ void k_slice_sum(const float *V, int n_slices, long long nxy, double *out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxy) return;
+    double s = 0.0;
+    for (int k = 0; k < n_slices; ++k) s += (double)V[(long long)k * nxy + i];
+    out[i] = s;
+}
+
+// streaming copy, 16 B per thread per iteration (HBM bandwidth reference)
+__global__ __launch_bounds__(256) void k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 __global__ void k_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o, double *m2)
 {
     const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -715,6 +744,21 @@ hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det,
     for (int k = 0; k < n_det; ++k) { d.g2_min[k] = det_g2_min[k]; d.g2_max[k] = det_g2_max[k]; }
     hipLaunchKernelGGL(k_coh_image, dim3((unsigned)(nscan * n_thk)), dim3(256), 0, s, coh, nscan, n_thk, n_det, nx, ny,
                        dgx, dgy, d, extra_bwl, bwl_alpha, gl2_max, image_coh);
+    return hipGetLastError();
+}
+
+hipError_t launch_slice_sum(const float *V, int n_slices, long long nxy, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, s, V, n_slices, nxy, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(const void *a, void *b, size_t bytes, hipStream_t s)
+{
+    if (bytes % 16) return hipErrorInvalidValue;
+    const long long n = (long long)(bytes / 16);
+    const long long blocks = std::min<long long>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)a, (float4 *)b, n);
     return hipGetLastError();
 }
 

@@ -27,6 +27,11 @@ struct ProbeParams {
     double c10, c30, c50;         // -pi*lambda*f, -pi*Cs3*lambda^3/2, -pi*Cs5*lambda^5/3 (FP64)
     double dgx, dgy;              // FP64 reciprocal sampling (1/lx, 1/ly)
     float dgx_f, dgy_f;           // float sampling, for the aperture mask as the FP32 path evaluates it
+    // azimuthal aberrations (Lens::eval_m ... eval_c_52_c_54_c_56, src/types.cuh:3180-3300):
+    // chi += m phi + sum_k az_c[k] g^az_p[k] sin(az_n[k] (phi - az_phi[k])), FP64
+    int azim;                     // nonzero: the terms below are present
+    double az_m;
+    double az_c[11], az_phi[11];  // order 12, 21, 23, 32, 34, 41, 43, 45, 52, 54, 56
     double gxu, gyu;              // frequency shift of chi (tilted per-piece incident wave; 0 in simulations)
     float gxu_f, gyu_f;
     float g2_min, g2_max;         // aperture
@@ -152,6 +157,8 @@ hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, 
 hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, float dgx, float dgy,
                             const float *det_g2_min, const float *det_g2_max, int extra_bwl, float bwl_alpha,
                             float gl2_max, double *image_coh, hipStream_t s);
+hipError_t launch_stream_copy(const void *a, void *b, size_t bytes, hipStream_t s);
+hipError_t launch_slice_sum(const float *V, int n_slices, long long nxy, double *out, hipStream_t s);
 hipError_t launch_shift_complex(const float2 *src, float2 *dst, int nx, int ny, hipStream_t s);
 
 // Launch variants of the slice passes.  Variant 0 is the default; the tuner
@@ -164,7 +171,7 @@ struct PassVariant {
     int tile;       // rows (row pass) or columns (column pass) per workgroup
     int tw_table;   // 1: twiddles read from the table at each use
 };
-constexpr int kPassVariants = 7;
+constexpr int kPassVariants = 8;
 
 // elements per thread of schedule sc at length n (Sched<n, sc>::E)
 constexpr int sched_elems(int n, int sc)
@@ -176,11 +183,14 @@ constexpr int col_tile0(int n) { return n >= 2048 ? 8 : 16; }
 
 // Row pass, v: 0 default; 1 table twiddles; 2 twice the rows; 3 alternative
 // schedule at the same thread count; 4 = 3 with table twiddles; 5 half the
-// rows; 6 = 2 with table twiddles.
+// rows; 6 = 2 with table twiddles; 7 alternative schedule, twice the rows
+// (capped at 1024 threads).
 constexpr PassVariant row_variant(int n, int v)
 {
     const int r = row_tile0(n), alt = r * sched_elems(n, 1) / sched_elems(n, 0);
+    const int t1 = n / sched_elems(n, 1);
     switch (v) {
+    case 7: return PassVariant{1, 2 * alt * t1 > 1024 ? 1024 / t1 : 2 * alt, 0};
     case 1: return PassVariant{0, r, 1};
     case 2: return PassVariant{0, 2 * r, 0};
     case 3: return PassVariant{1, alt, 0};
@@ -193,13 +203,16 @@ constexpr PassVariant row_variant(int n, int v)
 // Column pass, v: 0 default; 1 table twiddles; 2 half the columns;
 // 3 alternative schedule (columns capped at 1024 threads per workgroup);
 // 4 = 3 with table twiddles; 5 a quarter of the columns; 6 = 2 with table
-// twiddles.  Detector epilogues use only variants with the default width,
-// which fixes the layout of the detector partials.
+// twiddles; 7 alternative schedule, twice the columns (capped at 1024
+// threads: 256-byte row segments at 512^2).  Detector epilogues use only
+// variants with the default width, which fixes the layout of the detector
+// partials.
 constexpr PassVariant col_variant(int n, int v)
 {
     const int c = col_tile0(n), t1 = n / sched_elems(n, 1);
     const int c1 = c * t1 > 1024 ? 1024 / t1 : c;
     switch (v) {
+    case 7: return PassVariant{1, 2 * c * t1 > 1024 ? 1024 / t1 : 2 * c, 0};
     case 1: return PassVariant{0, c, 1};
     case 2: return PassVariant{0, c / 2, 0};
     case 3: return PassVariant{1, c1, 0};

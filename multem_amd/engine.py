@@ -40,6 +40,7 @@ def lib():
         L.mt_hip_spec_slicing.argtypes = [sysp, inp, C.c_int, P(capi.Slice), C.c_int, P(C.c_int)]
         L.mt_hip_projected_potential.argtypes = [sysp, inp, C.c_int, C.c_int, dp]
         L.mt_hip_transmission_function.argtypes = [sysp, inp, C.c_int, C.c_int, dp]
+        L.mt_hip_projected_potential_sum.argtypes = [sysp, inp, C.c_int, dp]
         L.mt_hip_incident_wave.argtypes = [sysp, inp, dp]
         L.mt_hip_propagate.argtypes = [sysp, inp, dp, C.c_double, C.c_int, dp]
         L.mt_hip_request_stop.argtypes = []
@@ -52,6 +53,7 @@ def lib():
         L.mt_hip_synchronize.argtypes = [C.c_int]
         L.mt_hip_atom_table.argtypes = [C.c_int, C.c_double] + [dp] * 6
         L.mt_hip_set_autotune.argtypes = [C.c_int]
+        L.mt_hip_stream_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, P(C.c_double)]
         L.mt_hip_set_pass_variants.argtypes = [C.c_int, C.c_int, C.c_int]
         L.mt_hip_tune_log.argtypes = [C.c_char_p, C.c_size_t]
         if L.mt_hip_abi_version() != capi.MT_ABI_VERSION:
@@ -125,6 +127,15 @@ def projected_potential(params, islice, iconf=1):
     return V.reshape(params.nx, params.ny)
 
 
+def projected_potential_sum(params, iconf=1):
+    """Sum over all slices of the projected potential, [nx, ny] (natural layout)."""
+    s, keep = params.to_struct()
+    sysc = params.system_conf.as_struct()
+    V = np.zeros(params.nx * params.ny)
+    _check(lib().mt_hip_projected_potential_sum(C.byref(sysc), C.byref(s), iconf, capi.dptr(V)))
+    return V.reshape(params.nx, params.ny)
+
+
 def transmission_function(params, islice, iconf=1):
     s, keep = params.to_struct()
     sysc = params.system_conf.as_struct()
@@ -166,7 +177,14 @@ def set_batch(batch=0, use_graph=True):
     lib().mt_hip_set_batch(int(batch), int(bool(use_graph)))
 
 
-PASS_VARIANTS = 7
+PASS_VARIANTS = 8
+
+
+def stream_copy_bandwidth(nbytes=1 << 30, reps=10, device=0):
+    """Read + write GB/s of the library's streaming copy kernel."""
+    g = C.c_double(0)
+    _check(lib().mt_hip_stream_copy_bandwidth(int(device), int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def set_autotune(mode=1):

@@ -149,3 +149,43 @@ class Surrogate:
         y = self.net.forward(x[None] if single else x)
         y = y.astype(np.float64) * (s if not single else s[..., 0, 0])
         return y[0] if single else y
+
+
+def _rel_rms(pred, ref):
+    pred, ref = np.asarray(pred, np.float64), np.asarray(ref, np.float64)
+    return float(np.sqrt(np.mean((pred - ref) ** 2) / np.mean(ref ** 2)))
+
+
+def evaluate(model, n_specimens=4, seed=2024, ns=32, n_small=2, n_big=64):
+    """Surrogate against plain multislice on held-out random specimens.
+
+    GPU time: plain = the n_big-configuration run; surrogate = the
+    n_small-configuration run + the potential-sum build + the network
+    forward (device times from HIP events).  Errors are relative RMS
+    against the n_big-configuration image, for the surrogate and for the
+    n_small-configuration image it starts from."""
+    from . import engine, surrogate_data
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(n_specimens):
+        p, meta = surrogate_data.random_specimen(rng, ns)
+        q = p.copy()
+        q.pn_nconf = n_big
+        big = engine.run_multislice(q).image_tot[0, 0].copy()
+        t_big = engine.last_timing()["ms_total"]
+        q = p.copy()
+        q.pn_nconf = n_small
+        small = engine.run_multislice(q).image_tot[0, 0].copy()
+        t_small = engine.last_timing()["ms_total"]
+        vmap = surrogate_data.potential_map(p)
+        t_v = engine.last_timing()["ms_total"]
+        pred = model.predict(small, vmap)
+        t_f = model.net.last_forward_ms()
+        rows.append(dict(desc=meta["desc"], thick=meta["thick"], ms_plain=t_big, ms_surrogate=t_small + t_v + t_f,
+                         ms_small=t_small, ms_vmap=t_v, ms_forward=t_f,
+                         rel_rms_surrogate=_rel_rms(pred, big), rel_rms_small=_rel_rms(small, big)))
+    ms_p = sum(r["ms_plain"] for r in rows)
+    ms_s = sum(r["ms_surrogate"] for r in rows)
+    return dict(specimens=rows, gpu_time_ratio=ms_p / ms_s,
+                rel_rms_surrogate_mean=float(np.mean([r["rel_rms_surrogate"] for r in rows])),
+                rel_rms_small_mean=float(np.mean([r["rel_rms_small"] for r in rows])))

@@ -13,7 +13,8 @@ Specimens: fcc, diamond-cubic or perovskite [001], a in [3.5, 5.5] A, Z in
 20-200 A; 300 kV, 20-25 mrad probe, zero defocus, Cs = 0, 60-200 mrad ring;
 ns x ns periodic scan over one unit cell; the lateral box is the largest
 number of cells that keeps the 256^2 grid's Nyquist frequency beyond the
-200 mrad ring.  Illumination is untilted (the engine does not provide tilt).
+200 mrad ring.  Illumination tilt theta is uniform in [0, tilt_max] degrees
+with a uniform azimuth (tilt_max = 0 by default: untilted).
 
     python -m multem_amd.surrogate_data --n 200 --seed 1 --out pairs.npz
 """
@@ -31,7 +32,7 @@ GRID = 256
 BOX_MAX = 0.5 * GRID / (0.200 / 0.0196875)   # Angstrom: Nyquist 1/(2 dR) at 200 mrad, 300 kV
 
 
-def random_specimen(rng, ns=32, nconf=64):
+def random_specimen(rng, ns=32, nconf=64, tilt_max=0.0):
     """Parameters of one random specimen (STEM, `nconf` frozen-phonon configurations)."""
     kind = int(rng.integers(0, 3))
     a = float(rng.uniform(3.5, 5.5))
@@ -72,6 +73,9 @@ def random_specimen(rng, ns=32, nconf=64):
     p.scanning_xe = p.scanning_ye = (c0 + 1) * a
     p.detector_type = 1
     p.detector_cir = [(60.0, 200.0)]
+    if tilt_max > 0:
+        p.theta = float(rng.uniform(0.0, tilt_max))
+        p.phi = float(rng.uniform(0.0, 360.0))
     meta = dict(kind=kind, a=a, sigma=sigma, thick=nc * a, cells=nl, desc=desc)
     return p, meta
 
@@ -82,8 +86,7 @@ def potential_map(p):
     q = p.copy()
     q.pn_model = 1
     q.pn_nconf = 1
-    ns_sl = len(engine.spec_slicing(q))
-    V = sum(engine.projected_potential(q, i, 1) for i in range(ns_sl))   # [nx, ny], pixel ix at x = ix*dx
+    V = engine.projected_potential_sum(q, 1)   # [nx, ny], pixel ix at x = ix*dx
     n = p.scanning_ns
     xs = p.scanning_x0 + (p.scanning_xe - p.scanning_x0) * np.arange(n) / n
     ys = p.scanning_y0 + (p.scanning_ye - p.scanning_y0) * np.arange(n) / n
@@ -113,12 +116,12 @@ def make_pair(p, n_small=2, n_big=64):
     return small, vmap, big, dict(t_small=t1 - t0, t_vmap=t2 - t1, t_big=t3 - t2)
 
 
-def generate(n, seed, ns=32, t_max=None, log=sys.stderr):
+def generate(n, seed, ns=32, t_max=None, tilt_max=0.0, log=sys.stderr):
     rng = np.random.default_rng(seed)
     S, Vm, T, metas = [], [], [], []
     t_start = time.perf_counter()
     for k in range(n):
-        p, meta = random_specimen(rng, ns)
+        p, meta = random_specimen(rng, ns, tilt_max=tilt_max)
         small, vmap, big, tm = make_pair(p)
         meta.update(tm)
         S.append(small)
@@ -140,9 +143,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--ns", type=int, default=32)
     ap.add_argument("--t-max", type=float, default=None, help="stop after this many seconds")
+    ap.add_argument("--tilt-max", type=float, default=0.0, help="largest illumination tilt (degrees)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
-    S, Vm, T, metas = generate(a.n, a.seed, a.ns, a.t_max)
+    S, Vm, T, metas = generate(a.n, a.seed, a.ns, a.t_max, a.tilt_max)
     np.savez_compressed(a.out, adf_small=S.astype(np.float32), vmap=Vm.astype(np.float32),
                         adf_big=T.astype(np.float32), a=np.array([m["a"] for m in metas]),
                         thick=np.array([m["thick"] for m in metas]), kind=np.array([m["kind"] for m in metas]),

@@ -307,3 +307,30 @@ def test_stem_coherent_image(theta):
     e_coh = rel_l2(out.image_coh, ref.image_coh)
     print("coherent STEM theta=%.1f: image_tot %.3e, image_coh %.3e" % (theta, e_tot, e_coh))
     assert e_tot < 1e-4 and e_coh < 1e-4
+
+
+def test_projected_potential_sum():
+    p = configs.si_ewrs_256(nc=3)
+    ns = len(oracle.spec_slicing(p))
+    want = np.zeros((p.nx, p.ny))
+    for i in range(ns):
+        want += oracle.projected_potential(p, i, precision="f32")
+    assert np.array_equal(engine.projected_potential_sum(p), want)
+
+
+def test_azimuthal_aberrations():
+    # two-fold and three-fold astigmatism, axial coma, a vortex term
+    p = small_stem(ns=3, nconf=1)
+    p.cond_lens_c_12, p.cond_lens_phi_12 = 15.0, 30.0
+    p.cond_lens_c_23, p.cond_lens_phi_23 = 300.0, 70.0
+    p.cond_lens_c_21, p.cond_lens_phi_21 = 200.0, 10.0
+    p.cond_lens_m = 1
+    e = rel_l2(engine.run_multislice(p).image_tot, oracle.run(p, "f64").image_tot)
+    print("STEM with azimuthal aberrations: rel L2 vs FP64 %.3e" % e)
+    assert e < 1e-4
+    q = p.copy()
+    q.simulation_type = 21
+    q.iw_type = 2
+    q.iw_x = np.array([0.45 * q.spec_lx])
+    q.iw_y = np.array([0.55 * q.spec_ly])
+    assert rel_l2(engine.incident_wave(q), oracle.incident_wave(q, precision="f64")) < 1e-5
