@@ -147,15 +147,16 @@ def copy_calibration():
                       "stream_kernel_read_plus_write_GBs": engine.stream_copy_bandwidth(n, reps)}))
 
 
-def traffic_from_profiles(kernel, n):
+def traffic_from_profiles(kernel, n, batch):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), if a summary has been recorded."""
+    scaled by the copy-kernel calibration), if a summary has been recorded:
+    the recorded bytes per wave times this run's batch width."""
     path = os.path.join(ROOT, "profiles", "traffic_%s_au_stem_%d.txt" % (kernel, n))
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("bytes_per_launch"):
-            return float(line.split("=")[1])
+        if line.startswith("bytes_per_wave"):
+            return float(line.split("=")[1].split()[0]) * batch
     return None
 
 
@@ -236,7 +237,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": traffic_from_profiles(kern, args.n),
+            "traffic": traffic_from_profiles(kern, args.n, B),
             "bytes_alg_per_launch": kb,
             "mean_launch_ms": kms,
             "row_pass_ms": row_ms,

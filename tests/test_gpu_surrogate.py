@@ -92,3 +92,43 @@ def test_training_reduces_loss_deterministically(net):
     print("training loss %.4f -> %.4f" % (l1[:5].mean(), l1[-10:].mean()))
     assert np.array_equal(l1, l2) and np.array_equal(p1, p2)
     assert l1[-10:].mean() < 0.3 * l1[:5].mean()
+
+
+import os
+
+_DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "multem_amd", "data")
+WEIGHTS = os.path.join(_DATA, "surrogate_6x32.npz")
+EVAL = os.path.join(_DATA, "surrogate_eval.npz")
+
+
+def _rel_rms(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.sqrt(np.mean((a - b) ** 2, axis=(-2, -1)) / np.mean(b ** 2, axis=(-2, -1)))
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained surrogate weights")
+def test_trained_surrogate_gain():
+    """Held-out pairs (never seen in training): the network's error against
+    the 64-configuration image, next to that of the 2-configuration image it
+    starts from."""
+    from multem_amd.surrogate import Surrogate
+    model = Surrogate.load(WEIGHTS)
+    d = np.load(EVAL)
+    pred = model.predict(d["adf_small"], d["vmap"])
+    e_net = _rel_rms(pred, d["adf_big"])
+    e_base = _rel_rms(d["adf_small"], d["adf_big"])
+    print("held-out pairs %d: rel RMS vs 64 configs: network %.4f (max %.4f), 2 configs %.4f (max %.4f)"
+          % (len(e_net), e_net.mean(), e_net.max(), e_base.mean(), e_base.max()))
+    assert e_net.mean() < 0.5 * e_base.mean()
+    assert np.all(e_net < e_base)
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained surrogate weights")
+def test_surrogate_gpu_time_ratio():
+    from multem_amd import surrogate
+    model = surrogate.Surrogate.load(WEIGHTS)
+    ev = surrogate.evaluate(model, n_specimens=2, seed=2024)
+    print("GPU time plain 64 configs / surrogate: %.1f; rel RMS surrogate %.4f, 2 configs %.4f"
+          % (ev["gpu_time_ratio"], ev["rel_rms_surrogate_mean"], ev["rel_rms_small_mean"]))
+    assert ev["gpu_time_ratio"] >= 8.0
+    assert ev["rel_rms_surrogate_mean"] < ev["rel_rms_small_mean"]
