@@ -6,6 +6,8 @@ can move an intermediate value across a bf16 rounding boundary, which then
 propagates); the loss 1e-4.  The oracle's own backward pass is pinned by
 finite differences in tests/test_surrogate_oracle.py.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -93,8 +95,6 @@ def test_training_reduces_loss_deterministically(net):
     assert np.array_equal(l1, l2) and np.array_equal(p1, p2)
     assert l1[-10:].mean() < 0.3 * l1[:5].mean()
 
-
-import os
 
 _DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "multem_amd", "data")
 WEIGHTS = os.path.join(_DATA, "surrogate_6x32.npz")
