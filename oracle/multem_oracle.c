@@ -625,6 +625,7 @@ typedef struct {
     real c_c_10, c_c_12, c_c_21, c_c_23, c_c_30, c_c_32, c_c_34, c_c_41, c_c_43, c_c_45;
     real c_c_50, c_c_52, c_c_54, c_c_56;
     real g2_min, g2_max;
+    real ti_iehwgd, si_theta_c, lambda2;   /* partial-coherence envelope (objective lens) */
 } lens_t;
 
 typedef struct {
@@ -641,7 +642,7 @@ typedef struct {
     int simulation_type, interaction_model, potential_type;
     int pn_model, pn_coh_contrib, pn_single_conf, pn_nconf, fp_iconf_0, pn_seed;
     int pn_dim_x, pn_dim_y, pn_dim_z;
-    int thick_type, potential_slicing, illumination_model;
+    int thick_type, potential_slicing, illumination_model, tsi;
     int n_thick; real *thick;
     grid_t g;
     int iw_type; int n_iw; real *iw_x, *iw_y;
@@ -1017,6 +1018,7 @@ static int is_stem(const sim_t *s) { return s->simulation_type == 11; }
 static int is_cbed(const sim_t *s) { return s->simulation_type == 21; }
 static int is_ewfs(const sim_t *s) { return s->simulation_type == 51; }
 static int is_ewrs(const sim_t *s) { return s->simulation_type == 52; }
+static int is_hrtem(const sim_t *s) { return s->simulation_type == 32; }
 static int is_frozen(const sim_t *s) { return s->pn_model == 3; }
 static int is_scanning_t(int st) { return st == 11 || st == 12 || st == 61; }
 
@@ -1169,8 +1171,8 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     *status = MT_OK;
 #define FAIL(code, msg) do { set_err(msg); *status = code; sim_free(s); return NULL; } while (0)
     s->simulation_type = in->simulation_type;
-    if (!(is_stem(s) || is_cbed(s) || is_ewfs(s) || is_ewrs(s)))
-        FAIL(MT_ERR_UNSUPPORTED, "simulation_type not provided (STEM 11, CBED 21, EWFS 51, EWRS 52)");
+    if (!(is_stem(s) || is_cbed(s) || is_hrtem(s) || is_ewfs(s) || is_ewrs(s)))
+        FAIL(MT_ERR_UNSUPPORTED, "simulation_type not provided (STEM 11, CBED 21, HRTEM 32, EWFS 51, EWRS 52)");
     s->interaction_model = in->interaction_model;
     if (s->interaction_model != 1) FAIL(MT_ERR_UNSUPPORTED, "only interaction_model 1 (multislice)");
     s->potential_type = in->potential_type;
@@ -1260,6 +1262,12 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     lens_set(&s->cond, s->E_0, &s->g);
     lens_read(&s->obj, &in->obj_lens);
     lens_set(&s->obj, s->E_0, &s->g);
+    /* Lens::set_ti_sigma / set_si_sigma (src/types.cuh:2978-2989); the
+       objective lens takes si_sigma from cond_lens_si_sigma (ilc_multem.cu:247) */
+    s->obj.ti_iehwgd = (real)1.414213562373095048802 * (real)fmax(0.0, in->obj_lens.ti_sigma);
+    s->obj.si_theta_c = asin((real)1.414213562373095048802 * (real)fmax(0.0, in->cond_lens.si_sigma) * s->obj.lambda);
+    s->obj.lambda2 = pow(s->obj.lambda, 2);
+    s->tsi = in->temporal_spatial_incoh;
 
     if (is_scanning_t(s->simulation_type)) {
         s->sc_type = in->scanning_type; s->sc_pbc = in->scanning_periodic != 0;
@@ -1740,10 +1748,10 @@ static void output_dims_from_sim(const sim_t *s, mt_output_multislice *out)
         out->dx = s->sc_dRx; out->dy = s->sc_dRy;
     } else {
         out->nx = s->oa_ix_e - s->oa_ix_0; out->ny = s->oa_iy_e - s->oa_iy_0; out->n_det = 0;
-        int rs = is_ewrs(s);
+        int rs = is_ewrs(s) || is_hrtem(s);
         out->dx = rs ? s->g.dRx : s->g.dgx;
         out->dy = rs ? s->g.dRy : s->g.dgy;
-        if (is_cbed(s)) out->output_type = s->pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
+        if (is_cbed(s) || is_hrtem(s)) out->output_type = s->pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
         else {
             int sc = !is_frozen(s) || s->pn_single_conf;
             out->output_type = sc ? MT_OT_PSI_COH : MT_OT_M2PSI_TOT_PSI_COH;
@@ -1769,7 +1777,7 @@ static void fill_axes(const sim_t *s, mt_output_multislice *out)
         if (out->x) for (int i = 0; i < s->sc_nx; i++) out->x[i] = s->sc_x0 + i * s->sc_dRx;
         if (out->y) for (int i = 0; i < s->sc_ny; i++) out->y[i] = s->sc_y0 + i * s->sc_dRy;
     } else {
-        int rs = is_ewrs(s);
+        int rs = is_ewrs(s) || is_hrtem(s);
         if (out->x) for (int ix = s->oa_ix_0; ix < s->oa_ix_e; ix++)
             out->x[ix - s->oa_ix_0] = rs ? ix * s->g.dRx : (ix - s->g.nxh) * s->g.dgx;
         if (out->y) for (int iy = s->oa_iy_0; iy < s->oa_iy_e; iy++)
@@ -1912,17 +1920,94 @@ static int run_ew_conf(sim_t *s, cplx **po_thk)
     return st;
 }
 
+/* Objective-lens image formation on a reciprocal-space wave (natural order),
+   in place: MicroscopeEffects::CTF_TEM / PCTF_LI_WPO_TEM
+   (src/microscope_effects.cuh:57-145) with apply_CTF / apply_PCTF
+   (src/cgpu_fcns.cuh:1993-2040), then the unnormalised inverse FFT. */
+static void microscope_effects(const sim_t *s, cplx *f, const fft2 *F, cplx *work)
+{
+    const grid_t *g = &s->g;
+    const lens_t *L = &s->obj;
+    const real c_Pi_ = (real)3.141592653589793238463;
+    int phireq = lens_phi_required(L);
+    real ti = L->ti_iehwgd, sth = L->si_theta_c;
+    if (s->tsi == 2) sth = 0;          /* temporal only */
+    else if (s->tsi == 3) ti = 0;      /* spatial only */
+    for (int ix = 0; ix < g->nx; ix++)
+        for (int iy = 0; iy < g->ny; iy++) {
+            int igx = (ix < g->nxh) ? ix : ix - g->nx;
+            int igy = (iy < g->nyh) ? iy : iy - g->ny;
+            real gx = igx * g->dgx, gy = igy * g->dgy;
+            real g2 = gx * gx + gy * gy;
+            cplx v = { 0, 0 };
+            if ((L->g2_min <= g2) && (g2 < L->g2_max)) {
+                if (s->illumination_model == 1) {
+                    /* exp_i_chi with x = y = 0 (apply_CTF) */
+                    real g4 = g2 * g2, g6 = g4 * g2;
+                    real chi = (is_zero_r(L->c_10) ? 0 : L->c_c_10 * g2) + (is_zero_r(L->c_30) ? 0 : L->c_c_30 * g4)
+                               + (is_zero_r(L->c_50) ? 0 : L->c_c_50 * g6);
+                    if (phireq) {
+                        real gg = sqrt(g2), g3 = g2 * gg, g5 = g4 * gg;
+                        real ph = atan2(gy, gx);
+                        chi += (is_zero_r(ph) ? 0 : L->m * ph);
+                        chi += is_zero_r(L->c_12) ? 0 : L->c_c_12 * g2 * sin(2 * (ph - L->phi_12));
+                        chi += (is_zero_r(L->c_21) ? 0 : L->c_c_21 * g3 * sin(ph - L->phi_21))
+                               + (is_zero_r(L->c_23) ? 0 : L->c_c_23 * g3 * sin(3 * (ph - L->phi_23)));
+                        chi += (is_zero_r(L->c_32) ? 0 : L->c_c_32 * g4 * sin(2 * (ph - L->phi_32)))
+                               + (is_zero_r(L->c_34) ? 0 : L->c_c_34 * g4 * sin(4 * (ph - L->phi_34)));
+                        chi += (is_zero_r(L->c_41) ? 0 : L->c_c_41 * g5 * sin(ph - L->phi_41))
+                               + (is_zero_r(L->c_43) ? 0 : L->c_c_43 * g5 * sin(3 * (ph - L->phi_43)))
+                               + (is_zero_r(L->c_45) ? 0 : L->c_c_45 * g5 * sin(5 * (ph - L->phi_45)));
+                        chi += (is_zero_r(L->c_52) ? 0 : L->c_c_52 * g6 * sin(2 * (ph - L->phi_52)))
+                               + (is_zero_r(L->c_54) ? 0 : L->c_c_54 * g6 * sin(4 * (ph - L->phi_54)))
+                               + (is_zero_r(L->c_56) ? 0 : L->c_c_56 * g6 * sin(6 * (ph - L->phi_56)));
+                    }
+                    v.re = cos(chi); v.im = sin(chi);
+                } else {
+                    /* apply_PCTF: chi of c_10, c_30 only, Gaussian temporal/spatial envelope */
+                    real chi = g2 * (L->c_c_30 * g2 + L->c_c_10);
+                    real c = c_Pi_ * sth * ti;
+                    real u = 1 + c * c * g2;
+                    c = c_Pi_ * ti * L->lambda * g2;
+                    real temp_inc = (real)0.25 * c * c;
+                    c = c_Pi_ * sth * (L->c_30 * L->lambda2 * g2 + L->c_10);
+                    real spa_inc = c * c * g2;
+                    real st_inc = exp(-(spa_inc + temp_inc) / u) / sqrt(u);
+                    v.re = st_inc * cos(chi); v.im = st_inc * sin(chi);
+                }
+            }
+            cplx *p = &f[(size_t)ix * g->ny + iy];
+            cplx r = { v.re * p->re - v.im * p->im, v.re * p->im + v.im * p->re };
+            *p = r;
+        }
+    fft2_exec(F, f, 1, work);
+}
+
 /* Accumulators of the exit-wave / CBED outputs over configurations. */
 typedef struct {
     double *m2, *pc, *pcs;
     size_t no, nxy;
     int ot;
     double w;
+    fft2 F;            /* HRTEM image formation */
+    cplx *tmp, *work;
 } ew_acc;
 
 static void ew_accumulate(const sim_t *s, ew_acc *A, cplx **po)
 {
     for (int k = 0; k < s->n_thk; k++) {
+        if (is_hrtem(s)) {
+            /* add_scale_crop_shift_m2psi_tot_from_m2psi of the image; psi_coh in k-space */
+            memcpy(A->tmp, po[k], sizeof(cplx) * A->nxy);
+            microscope_effects(s, A->tmp, &A->F, A->work);
+            accumulate_output(s, A->tmp, A->w, 1, 0, A->m2 + A->no * k, 0);
+            if (s->pn_coh_contrib)
+                for (size_t i = 0; i < A->nxy; i++) {
+                    A->pc[2 * (A->nxy * k + i)] += A->w * po[k][i].re;
+                    A->pc[2 * (A->nxy * k + i) + 1] += A->w * po[k][i].im;
+                }
+            continue;
+        }
         if (A->ot == MT_OT_PSI_COH) {
             accumulate_output(s, po[k], 1.0, 0, 1, A->pcs + 2 * A->no * k, 1);
         } else if (A->ot == MT_OT_M2PSI_TOT_PSI_COH) {
@@ -2003,6 +2088,9 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
     A.m2 = calloc(no * (size_t)s->n_thk, sizeof(double));
     A.pc = calloc(2 * nxy * (size_t)s->n_thk, sizeof(double));   /* psi_coh, uncropped for CBED */
     A.pcs = calloc(2 * no * (size_t)s->n_thk, sizeof(double));
+    fft2_init(&A.F, g->nx, g->ny);
+    A.tmp = malloc(sizeof(cplx) * nxy);
+    A.work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
     int st = MT_OK;
     int nconf = s->pn_nconf - s->fp_iconf_0 + 1;
     int nt = nthreads > 1 ? (nthreads < nconf ? nthreads : nconf) : 1;
@@ -2038,10 +2126,12 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
     int ot = A.ot;
     if (st == MT_OK) {
         if (ot == MT_OT_M2PSI_TOT_COH && out->m2psi_coh) {
-            /* set_m2psi_coh (wave_function.cuh:188-227): |psi_coh|^2 */
+            /* set_m2psi_coh (wave_function.cuh:188-227): |psi_coh|^2, for HRTEM
+               after the objective-lens image formation */
             cplx *tmp = malloc(sizeof(cplx) * nxy);
             for (int k = 0; k < s->n_thk; k++) {
                 for (size_t i = 0; i < nxy; i++) { tmp[i].re = (real)A.pc[2 * (nxy * k + i)]; tmp[i].im = (real)A.pc[2 * (nxy * k + i) + 1]; }
+                if (is_hrtem(s)) microscope_effects(s, tmp, &A.F, A.work);
                 accumulate_output(s, tmp, 1.0, 1, 1, out->m2psi_coh + no * k, 0);
             }
             free(tmp);
@@ -2051,7 +2141,8 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
         if (out->psi_coh && (ot == MT_OT_PSI_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
             memcpy(out->psi_coh, A.pcs, sizeof(double) * 2 * no * (size_t)s->n_thk);
     }
-    free(A.m2); free(A.pc); free(A.pcs);
+    free(A.m2); free(A.pc); free(A.pcs); free(A.tmp); free(A.work);
+    fft2_free(&A.F);
     return st;
 }
 

@@ -289,3 +289,44 @@ def test_stem_coherent_image_bounds():
     o = oracle.run(p, "f64")
     assert np.all(o.image_coh <= o.image_tot * (1 + 1e-12))
     assert np.all(o.image_coh < o.image_tot)     # phonons remove coherent intensity from the ADF ring
+
+
+def _hrtem_case(**kw):
+    p = configs.si_ewrs_256(nc=3, n=128)
+    p.simulation_type = 32                  # HRTEM
+    p.obj_lens_zero_defocus_type = 3
+    p.obj_lens_c_10, p.obj_lens_c_30 = 0.0, 0.0
+    p.obj_lens_outer_aper_ang = 20.0
+    p.illumination_model = 1
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_hrtem_coherent_is_aperture_filtered_exit_wave():
+    # aberration-free objective: image = |IFFT(aperture * O(g))|^2 with O the
+    # reciprocal-space exit wave (EWFS output, centred, scaled by 1/N^2)
+    p = _hrtem_case()
+    img = oracle.run(p, "f64").m2psi_tot[0]
+    q = p.copy()
+    q.simulation_type = 51
+    O = oracle.run(q, "f64").psi_coh[0]                 # [nx, ny], centred k-space
+    nx, ny = p.nx, p.ny
+    lam = 0.0196875
+    gx = (np.arange(nx) - nx // 2) / p.spec_lx
+    gy = (np.arange(ny) - ny // 2) / p.spec_ly
+    g2 = gx[:, None] ** 2 + gy[None, :] ** 2
+    mask = g2 < (np.sin(20e-3) / lam) ** 2
+    psi = np.fft.ifft2(np.fft.ifftshift(O * mask)) * (nx * ny)
+    want = np.fft.fftshift(np.abs(psi) ** 2)
+    assert rel_l2(img, want) < 1e-9
+
+
+def test_hrtem_partial_coherence_without_spread_equals_coherent():
+    kw = dict(obj_lens_c_10=-60.0, obj_lens_c_30=1.2, obj_lens_ti_sigma=0.0, cond_lens_si_sigma=0.0)
+    a = oracle.run(_hrtem_case(illumination_model=1, **kw), "f64").m2psi_tot[0]
+    b = oracle.run(_hrtem_case(illumination_model=2, **kw), "f64").m2psi_tot[0]
+    assert rel_l2(a, b) < 1e-12
+    # a finite focal spread damps the high frequencies: less image contrast
+    c = oracle.run(_hrtem_case(illumination_model=2, **dict(kw, obj_lens_ti_sigma=40.0)), "f64").m2psi_tot[0]
+    assert np.std(c) < np.std(b)
