@@ -275,6 +275,47 @@ struct Setup {
         return h;
     }
 
+    // objective-lens transfer function of HRTEM (MicroscopeEffects,
+    // src/microscope_effects.cuh:57-145)
+    void ctf_table(DBuf<float2> &ctf)
+    {
+        const LensModel &L = m.obj;
+        const double lam = m.lambda_d;
+        CtfParams P{};
+        P.mode = m.illumination_model == 2 ? 2 : 1;
+        P.dgx = 1.0 / m.lx_d; P.dgy = 1.0 / m.ly_d;
+        P.dgx_f = m.dgx; P.dgy_f = m.dgy;
+        P.g2_min = L.g2_min; P.g2_max = L.g2_max;
+        auto nz = [](float v) { return std::fabs(v) > 1e-5f; };
+        P.cc[0] = nz(L.c_10) ? -kPi * (double)L.c_10 * lam : 0.0;
+        P.cc[1] = nz(L.c_30) ? -kPi * (double)L.c_30 * std::pow(lam, 3) / 2.0 : 0.0;
+        P.cc[2] = nz(L.c_50) ? -kPi * (double)L.c_50 * std::pow(lam, 5) / 3.0 : 0.0;
+        P.cc[3] = (double)L.c_10;
+        P.cc[4] = (double)L.c_30;
+        if (L.phi_required && P.mode == 1) {
+            P.azim = 1;
+            P.az_m = (double)L.m;
+            const double l1 = lam, l2 = lam * lam, l3 = l2 * lam, l4 = l3 * lam, l5 = l4 * lam;
+            const double cf[11] = {-kPi * l1, -2.0 * kPi * l2 / 3.0, -2.0 * kPi * l2 / 3.0, -kPi * l3 / 2.0,
+                                   -kPi * l3 / 2.0, -2.0 * kPi * l4 / 5.0, -2.0 * kPi * l4 / 5.0,
+                                   -2.0 * kPi * l4 / 5.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0};
+            const float cv[11] = {L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
+            const float pv[11] = {L.phi_12, L.phi_21, L.phi_23, L.phi_32, L.phi_34, L.phi_41, L.phi_43, L.phi_45,
+                                  L.phi_52, L.phi_54, L.phi_56};
+            for (int k = 0; k < 11; ++k) {
+                P.az_c[k] = nz(cv[k]) ? cf[k] * (double)cv[k] : 0.0;
+                P.az_phi[k] = (double)pv[k];
+            }
+        }
+        // temporal / spatial selection of PCTF_LI_WPO_TEM
+        P.ti = m.temporal_spatial_incoh == 3 ? 0.0 : (double)L.ti_iehwgd;
+        P.sth = m.temporal_spatial_incoh == 2 ? 0.0 : (double)L.si_theta_c;
+        P.lambda = lam;
+        ctf.alloc(nxy);
+        check(launch_ctf_table(nx, ny, P, ctf.p, s), "ctf table");
+        check(hipStreamSynchronize(s), "sync");
+    }
+
     // tilt ramp of the outputs, exp(-2 pi i g0.(r - r_c)), with 1/ny folded
     // into rx (it follows an unnormalised inverse transform along y)
     void ramp_tables(DBuf<float2> &rx, DBuf<float2> &ry)
@@ -943,7 +984,7 @@ void run_ew(const Model &m, mt_output_multislice *out)
     const int ot = dims.output_type;
     const int nxo = m.oa_ix_e - m.oa_ix_0, nyo = m.oa_iy_e - m.oa_iy_0;
     const size_t no = (size_t)nxo * nyo;
-    const bool fs = !m.is_ewrs();     // Fourier-space output (EWFS, CBED)
+    const bool fs = !m.is_ewrs() && !m.is_hrtem();     // Fourier-space output (EWFS, CBED)
     const int ns_run = [&] {
         int last = 0;
         for (const ThickInfo &t : m.thk) last = std::max(last, t.islice);
@@ -979,6 +1020,8 @@ void run_ew(const Model &m, mt_output_multislice *out)
     S.prop_tables(zl, px, py);
     DBuf<float2> rx, ry;
     if (m.tilted()) S.ramp_tables(rx, ry);
+    DBuf<float2> ctf;
+    if (m.is_hrtem()) S.ctf_table(ctf);
     ProbeParams P{};
     if (m.iw_type == 2) {
         P = S.probe_params(S.count_aperture());
@@ -1057,9 +1100,13 @@ void run_ew(const Model &m, mt_output_multislice *out)
                 check(launch_row_pass(m.ny, ro, s), "output ramp pass");
                 osrc = scratch.p;
             }
-            if (fs || zb != 0.f || m.bwl || m.tilted()) {
+            if (fs || zb != 0.f || m.bwl || m.tilted() || m.is_hrtem()) {
+                // HRTEM: the objective lens acts on O(g) before the inverse
+                // transforms (microscope_effects, src/wave_function.cuh:172-180)
                 ColArgs co = S.col_base(osrc, scratch.p, nb);
-                co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) | COL_STORE;
+                co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) | COL_STORE |
+                           (m.is_hrtem() ? COL_MUL_CTF : 0);
+                co.ctf = ctf.p;
                 co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
                 co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
                 co.p_scale = (float)m.ny;

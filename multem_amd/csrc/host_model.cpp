@@ -17,6 +17,7 @@ namespace mtg {
 namespace {
 
 constexpr double kPi = 3.141592653589793238463;
+constexpr double kSqrt2 = 1.414213562373095048802;
 constexpr double kTwoPi = 6.283185307179586476925;
 constexpr double kPi2 = 9.869604401089358618834;      // c_Pi2 (src/types.cuh:88)
 constexpr double kPotf = 47.877645145863056;      // c_Potf (src/types.cuh:66)
@@ -338,8 +339,8 @@ std::vector<float> slice_bounds(const std::vector<float> &p, float dz, float z_i
 Model::Model(const mt_input_multislice &in)
 {
     sim_type = in.simulation_type;
-    if (!(is_stem() || is_cbed() || is_ewfs() || is_ewrs()))
-        fail(MT_ERR_UNSUPPORTED, "simulation_type must be 11 (STEM), 21 (CBED), 51 (EWFS) or 52 (EWRS)");
+    if (!(is_stem() || is_cbed() || is_hrtem() || is_ewfs() || is_ewrs()))
+        fail(MT_ERR_UNSUPPORTED, "simulation_type must be 11 (STEM), 21 (CBED), 32 (HRTEM), 51 (EWFS) or 52 (EWRS)");
     if (in.interaction_model != 1) fail(MT_ERR_UNSUPPORTED, "only interaction_model 1 (multislice)");
     if (in.potential_type != 6) fail(MT_ERR_UNSUPPORTED, "only potential_type 6 (Lobato)");
     if (in.pn_model == 2) fail(MT_ERR_UNSUPPORTED, "absorptive potential model");
@@ -348,6 +349,12 @@ Model::Model(const mt_input_multislice &in)
         fail(MT_ERR_UNSUPPORTED, "potential_slicing 3/4 (sub-slicing)");
     if (std::fabs(in.spec_rot_theta) > 0) fail(MT_ERR_UNSUPPORTED, "specimen rotation");
     if (in.illumination_model == 4) fail(MT_ERR_UNSUPPORTED, "illumination_model 4 (full integration)");
+    illumination_model = (is_ewfs() || is_ewrs() || is_cbed()) ? 1 : in.illumination_model;
+    temporal_spatial_incoh = in.temporal_spatial_incoh;
+    if (is_hrtem() && illumination_model == 3)
+        fail(MT_ERR_UNSUPPORTED, "HRTEM with illumination_model 3 (transmission cross coefficient)");
+    if (is_hrtem() && in.pn_coh_contrib)
+        fail(MT_ERR_UNSUPPORTED, "HRTEM coherent image (pn_coh_contrib)");
 
     pn_model = in.pn_model;
     pn_coh_contrib = in.pn_coh_contrib;
@@ -420,6 +427,10 @@ Model::Model(const mt_input_multislice &in)
     read_lens(in.cond_lens, cond);
     set_lens(cond, E_0, g_max);
     read_lens(in.obj_lens, obj);
+    // Lens::set_ti_sigma / set_si_sigma (src/types.cuh:2978-2989); the objective
+    // lens reads si_sigma from cond_lens_si_sigma (ilc_multem.cu:247)
+    obj.ti_iehwgd = (float)kSqrt2 * std::fmax(0.f, (float)in.obj_lens.ti_sigma);
+    obj.si_theta_c = std::asin((float)kSqrt2 * std::fmax(0.f, (float)in.cond_lens.si_sigma) * lambda);
     set_lens(obj, E_0, g_max);
 
     // scanning (Scanning::set_grid, src/types.cuh:4294-4389)
@@ -683,9 +694,9 @@ void Model::output_dims(mt_output_multislice *out) const
         return;
     }
     out->nx = oa_ix_e - oa_ix_0; out->ny = oa_iy_e - oa_iy_0; out->n_det = 0;
-    out->dx = is_ewrs() ? dRx : dgx;
-    out->dy = is_ewrs() ? dRy : dgy;
-    if (is_cbed()) out->output_type = pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
+    out->dx = (is_ewrs() || is_hrtem()) ? dRx : dgx;
+    out->dy = (is_ewrs() || is_hrtem()) ? dRy : dgy;
+    if (is_cbed() || is_hrtem()) out->output_type = pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
     else out->output_type = (!frozen() || pn_single_conf) ? MT_OT_PSI_COH : MT_OT_M2PSI_TOT_PSI_COH;
 }
 
@@ -697,7 +708,7 @@ void Model::fill_axes(mt_output_multislice *out) const
         if (out->y) for (int i = 0; i < sc_ny; ++i) out->y[i] = sc_y0 + i * sc_dRy;
         return;
     }
-    bool rs = is_ewrs();
+    bool rs = is_ewrs() || is_hrtem();
     if (out->x) for (int ix = oa_ix_0; ix < oa_ix_e; ++ix) out->x[ix - oa_ix_0] = rs ? ix * dRx : (ix - nxh) * dgx;
     if (out->y) for (int iy = oa_iy_0; iy < oa_iy_e; ++iy) out->y[iy - oa_iy_0] = rs ? iy * dRy : (iy - nyh) * dgy;
 }

@@ -41,6 +41,7 @@ struct LensModel {
     float lambda = 0;
     float g2_min = 0, g2_max = 0;      // aperture (float, as evaluated by the reference FP32 path)
     bool phi_required = false;
+    float ti_iehwgd = 0, si_theta_c = 0;   // partial-coherence envelope (objective lens)
 };
 
 struct SliceInfo {
@@ -73,6 +74,9 @@ public:
     bool is_cbed() const { return sim_type == 21; }
     bool is_ewfs() const { return sim_type == 51; }
     bool is_ewrs() const { return sim_type == 52; }
+    bool is_hrtem() const { return sim_type == 32; }
+    int illumination_model = 1;     // 1 coherent, 2 partially coherent (objective-lens envelope)
+    int temporal_spatial_incoh = 1; // 1 both, 2 temporal only, 3 spatial only
     bool frozen() const { return pn_model == 3; }
 
     // electron / phonon

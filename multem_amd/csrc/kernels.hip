@@ -212,6 +212,10 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
             v[e] = cmul(p, v[e]);
         }
     }
+    if (flags & COL_MUL_CTF) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cmul(a.ctf[(long long)(t + T * e) * a.ncols + col], v[e]);
+    }
     if constexpr (DET) {
         if (flags & COL_COH) {
             // coherent sums over configurations (Output_Multislice::add_scale_psi_coh)
@@ -491,6 +495,49 @@ __global__ void k_out_accum(OutArgs a)
     }
 }
 
+__global__ void k_ctf_table(int nx, int ny, CtfParams P, float2 *ctf)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nx * ny) return;
+    const int ix = (int)(i / ny), iy = (int)(i % ny);
+    const int igx = signed_freq(ix, nx), igy = signed_freq(iy, ny);
+    const float g2f = g2_float(igx, igy, P.dgx_f, P.dgy_f);
+    float2 v = make_float2(0.f, 0.f);
+    if (P.g2_min <= g2f && g2f < P.g2_max) {
+        const double gx = igx * P.dgx, gy = igy * P.dgy, g2 = gx * gx + gy * gy;
+        double chi, amp = 1.0;
+        if (P.mode == 1) {
+            chi = (P.cc[0] + (P.cc[1] + P.cc[2] * g2) * g2) * g2;
+            if (P.azim) {
+                constexpr int pw[11] = {2, 3, 3, 4, 4, 5, 5, 5, 6, 6, 6};
+                constexpr int mu[11] = {2, 1, 3, 2, 4, 1, 3, 5, 2, 4, 6};
+                const double g1 = sqrt(g2), ph = atan2(gy, gx);
+                double gp[7];
+                gp[0] = 1.0;
+                for (int k = 1; k < 7; ++k) gp[k] = gp[k - 1] * g1;
+                if (ph != 0.0) chi += P.az_m * ph;
+                for (int k = 0; k < 11; ++k)
+                    if (P.az_c[k] != 0.0) chi += P.az_c[k] * gp[pw[k]] * sin(mu[k] * (ph - P.az_phi[k]));
+            }
+        } else {
+            const double pi = 3.141592653589793238463;
+            chi = g2 * (P.cc[1] * g2 + P.cc[0]);
+            double c = pi * P.sth * P.ti;
+            const double u = 1.0 + c * c * g2;
+            c = pi * P.ti * P.lambda * g2;
+            const double temp_inc = 0.25 * c * c;
+            c = pi * P.sth * (P.cc[4] * P.lambda * P.lambda * g2 + P.cc[3]);
+            const double spa_inc = c * c * g2;
+            amp = exp(-(spa_inc + temp_inc) / u) / sqrt(u);
+        }
+        chi -= kTwoPiD * rint(chi * (1.0 / kTwoPiD));
+        float sn, cs;
+        sincosf((float)chi, &sn, &cs);
+        v = make_float2((float)amp * cs, (float)amp * sn);
+    }
+    ctf[i] = v;
+}
+
 // Coherent STEM image (Wave_Function::set_m2psi_coh, src/wave_function.cuh:188-205):
 // one workgroup per (scan point, thickness), FP64 sums reduced in fixed order.
 struct CohDet {
@@ -744,6 +791,13 @@ hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det,
     for (int k = 0; k < n_det; ++k) { d.g2_min[k] = det_g2_min[k]; d.g2_max[k] = det_g2_max[k]; }
     hipLaunchKernelGGL(k_coh_image, dim3((unsigned)(nscan * n_thk)), dim3(256), 0, s, coh, nscan, n_thk, n_det, nx, ny,
                        dgx, dgy, d, extra_bwl, bwl_alpha, gl2_max, image_coh);
+    return hipGetLastError();
+}
+
+hipError_t launch_ctf_table(int nx, int ny, const CtfParams &p, float2 *ctf, hipStream_t s)
+{
+    const long long n = (long long)nx * ny;
+    hipLaunchKernelGGL(k_ctf_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, p, ctf);
     return hipGetLastError();
 }
 

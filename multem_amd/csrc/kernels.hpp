@@ -72,6 +72,7 @@ enum ColFlags : int {
     COL_STORE = 32,
     COL_DET_BWL = 64,   // detector sums take one more bandwidth-limit factor (the back-propagation's)
     COL_COH = 128,      // detector passes: coh[iscan][ithk] += coh_w * psi(g) (coherent STEM image)
+    COL_MUL_CTF = 256,  // multiply by ctf[ix*ny + iy] after the propagator (objective-lens image formation)
 };
 
 struct ColArgs {
@@ -94,6 +95,7 @@ struct ColArgs {
     float det_g2_min[kMaxDet], det_g2_max[kMaxDet];
     double *det_part;             // [member][n_thk][n_det][ntile]
     int n_thk, ithk;
+    const float2 *ctf;            // COL_MUL_CTF table [nx*ny], natural order
     // coherent STEM accumulators [scan][n_thk][nx*ny], FP64
     double2 *coh;
     double coh_w;
@@ -153,6 +155,22 @@ hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float ux,
 hipError_t launch_out_accum(const OutArgs &a, hipStream_t s);
 hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o,
                               double *m2, hipStream_t s);
+// Objective-lens transfer function on the natural reciprocal grid
+// (apply_CTF / apply_PCTF, src/cgpu_fcns.cuh:1993-2040): mode 1 coherent
+// exp(i chi(g)), mode 2 partially coherent polar(E(g), chi_10,30(g));
+// zero outside g2_min <= g^2 < g2_max.
+struct CtfParams {
+    int mode;
+    double dgx, dgy;              // FP64 sampling for chi
+    float dgx_f, dgy_f;           // float sampling for the aperture test
+    float g2_min, g2_max;
+    double cc[5];                 // c_c_10, c_c_30, c_c_50 (indices 0..2); raw c_10 (3), c_30 (4) in A
+    int azim;
+    double az_m, az_c[11], az_phi[11];
+    double ti, sth, lambda;       // ti_iehwgd, si_theta_c (after the temporal/spatial selection), lambda
+};
+hipError_t launch_ctf_table(int nx, int ny, const CtfParams &p, float2 *ctf, hipStream_t s);
+
 // image_coh[ithk][det][iscan] = sum over the ring of f |coh|^2 (f = bwl^2 if extra_bwl, else 1)
 hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, float dgx, float dgy,
                             const float *det_g2_min, const float *det_g2_max, int extra_bwl, float bwl_alpha,

@@ -334,3 +334,19 @@ def test_azimuthal_aberrations():
     q.iw_x = np.array([0.45 * q.spec_lx])
     q.iw_y = np.array([0.55 * q.spec_ly])
     assert rel_l2(engine.incident_wave(q), oracle.incident_wave(q, precision="f64")) < 1e-5
+
+
+@pytest.mark.parametrize("illum", [1, 2])
+def test_hrtem(illum):
+    # objective lens with defocus, Cs and (partially coherent) focal/angular spread
+    p = configs.si_ewrs_256(nc=4)
+    p.simulation_type = 32
+    p.illumination_model = illum
+    p.obj_lens_c_10, p.obj_lens_c_30 = -55.0, 0.8
+    p.obj_lens_outer_aper_ang = 25.0
+    p.obj_lens_ti_sigma, p.cond_lens_si_sigma = 30.0, 0.005
+    out = engine.run_multislice(p)
+    e64 = rel_l2(out.m2psi_tot, oracle.run(p, "f64").m2psi_tot)
+    e32 = rel_l2(out.m2psi_tot, oracle.run(p, "f32").m2psi_tot)
+    print("HRTEM illumination %d: rel L2 vs FP64 %.3e, vs FP32 restatement %.3e" % (illum, e64, e32))
+    assert e64 < 1e-4 and e32 < 1e-4
