@@ -571,7 +571,7 @@ public:
             B = std::max(1, (int)std::min<size_t>(256, (size_t)(160u << 20) / (nxy * sizeof(float2))));
             B = std::max(B, (2048 + nt - 1) / nt);
             B = std::min(B, B_cap);
-            if (run_control().autotune != 0) B = choose_batch(B, std::min(B_cap, std::max(B, 256)));
+            if (run_control().autotune != 0) B = choose_batch(B, std::min(B_cap, std::max(B, 256)));   // B0..2 B0
         }
         B = std::min(B, B_cap);
         nbatch = (nscan + B - 1) / B;
@@ -627,13 +627,14 @@ public:
         r.variant = choice.row;
     }
 
-    // Batch width: B0, 2 B0 and 4 B0 (up to B_max) timed on the full slice
+    // Batch width: B0, 1.5 B0 and 2 B0 (up to B_max) timed on the full slice
     // step (variant 0 of both passes) per wave; the smallest within 2 % of
     // the best wins.  Cached per (grid, B0, B_max).
     int choose_batch(int B0, int B_max)
     {
         std::vector<int> cand;
-        for (int b = B0; b <= B_max && cand.size() < 3; b *= 2) cand.push_back(b);
+        for (int b : {B0, B0 + B0 / 2, 2 * B0})
+            if (b <= B_max && (cand.empty() || b > cand.back())) cand.push_back(b);
         if (cand.size() < 2) return B0;
         static std::map<std::tuple<int, int, int, int>, int> cache;
         const auto key = std::make_tuple(m.nx, m.ny, B0, B_max);
@@ -761,6 +762,8 @@ public:
                 cd.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
                 cd.ithk = ithk;
                 cd.variant = 0;
+                // the stored wave lacks the 1/ny that the next row pass would supply
+                cd.p_scale = (float)m.ny;
                 check(launch_col_pass(m.nx, cd, false, true, st), "output detector pass");
             } else {
                 c.flags = col_flags(is == ns_run - 1) | ((ithk >= 0 && m.pn_coh_contrib) ? COL_COH : 0);
