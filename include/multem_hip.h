@@ -192,6 +192,11 @@ mt_status mt_hip_run_multislice(const mt_system_conf *sys, const mt_input_multis
                                 mt_output_multislice *out);
 mt_status mt_hip_run_multislice_f32(const mt_system_conf *sys, const mt_input_multislice *in,
                                     mt_output_multislice *out);
+/* run_multislice<double, e_device> (ilc_multem.cu:483-518, chosen by
+   system_conf.is_double_device() at ilc_multem.cu:520-546): the wave,
+   transmission stack, propagator and FFTs in FP64. */
+mt_status mt_hip_run_multislice_f64(const mt_system_conf *sys, const mt_input_multislice *in,
+                                    mt_output_multislice *out);
 
 /* Several STEM inputs run together: grid sizes, slice counts, scans and
    configuration counts may differ.  Each round advances every input with

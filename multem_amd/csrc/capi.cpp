@@ -23,15 +23,30 @@ mt_status fail(mt_status c, const std::string &m)
     return c;
 }
 
+// precision of a call: 1 float, 2 double (system_conf.precision, eP_float /
+// eP_double of the reference); prec_forced > 0 overrides it
+int precision_of(const mt_system_conf *sys, int prec_forced)
+{
+    if (prec_forced > 0) return prec_forced;
+    return sys->precision == 2 ? 2 : 1;
+}
+
+// Builds the model in the call's precision and runs f on it; f is generic in
+// the model type (ModelT<float> or ModelT<double>).
 template <class F>
-mt_status guarded(const mt_system_conf *sys, const mt_input_multislice *in, F &&f)
+mt_status guarded(const mt_system_conf *sys, const mt_input_multislice *in, F &&f, int prec_forced = 0)
 {
     if (!sys || !in) return fail(MT_ERR_INVALID_ARGUMENT, "null argument");
     if (sys->device == 1) return fail(MT_ERR_UNSUPPORTED, "system_conf.device = 1 (CPU): this library runs on the GPU");
     std::lock_guard<std::mutex> lock(g_engine_mutex);
     try {
-        mtg::Model m(*in);
-        f(m);
+        if (precision_of(sys, prec_forced) == 2) {
+            mtg::Model64 m(*in);
+            f(m);
+        } else {
+            mtg::Model m(*in);
+            f(m);
+        }
         g_last_error.clear();
         return MT_OK;
     } catch (const mtg::Error &e) {
@@ -41,6 +56,21 @@ mt_status guarded(const mt_system_conf *sys, const mt_input_multislice *in, F &&
     } catch (const std::exception &e) {
         return fail(MT_ERR_DEVICE, std::string(e.what()) + " (after " + mtg::last_successful_call() + ")");
     }
+}
+
+template <class M>
+void run_group_as(const mt_system_conf *sys, const mt_input_multislice *const *in, int n,
+                         mt_output_multislice *out)
+{
+    std::vector<std::unique_ptr<M>> models;
+    std::vector<const M *> mp;
+    std::vector<mt_output_multislice *> op;
+    for (int i = 0; i < n; ++i) {
+        models.emplace_back(new M(*in[i]));
+        mp.push_back(models.back().get());
+        op.push_back(&out[i]);
+    }
+    mtg::run_multislice_group(mp, sys->gpu_device, op);
 }
 
 } // namespace
@@ -61,21 +91,30 @@ const char *mt_hip_last_error(void) { return g_last_error.c_str(); }
 mt_status mt_hip_output_dims(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
 {
     if (!out) return fail(MT_ERR_INVALID_ARGUMENT, "null output");
-    return guarded(sys, in, [&](const mtg::Model &m) { m.output_dims(out); });
+    return guarded(sys, in, [&](const auto &m) { m.output_dims(out); });
+}
+
+static mt_status run_multislice_as(const mt_system_conf *sys, const mt_input_multislice *in,
+                                   mt_output_multislice *out, int prec)
+{
+    if (!out) return fail(MT_ERR_INVALID_ARGUMENT, "null output");
+    mtg::run_control().stop = false;
+    return guarded(sys, in, [&](const auto &m) { mtg::run_multislice(m, sys->gpu_device, out); }, prec);
 }
 
 mt_status mt_hip_run_multislice_f32(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
 {
-    if (!out) return fail(MT_ERR_INVALID_ARGUMENT, "null output");
-    mtg::run_control().stop = false;
-    return guarded(sys, in, [&](const mtg::Model &m) { mtg::run_multislice(m, sys->gpu_device, out); });
+    return run_multislice_as(sys, in, out, 1);
+}
+
+mt_status mt_hip_run_multislice_f64(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
+{
+    return run_multislice_as(sys, in, out, 2);
 }
 
 mt_status mt_hip_run_multislice(const mt_system_conf *sys, const mt_input_multislice *in, mt_output_multislice *out)
 {
-    if (sys && sys->precision == 2)
-        return fail(MT_ERR_UNSUPPORTED, "precision 2 (double): the engine runs the FP32 wave path");
-    return mt_hip_run_multislice_f32(sys, in, out);
+    return run_multislice_as(sys, in, out, 0);
 }
 
 mt_status mt_hip_run_multislice_group(const mt_system_conf *sys, const mt_input_multislice *const *in, int n,
@@ -88,15 +127,8 @@ mt_status mt_hip_run_multislice_group(const mt_system_conf *sys, const mt_input_
     mtg::run_control().stop = false;
     std::lock_guard<std::mutex> lock(g_engine_mutex);
     try {
-        std::vector<std::unique_ptr<mtg::Model>> models;
-        std::vector<const mtg::Model *> mp;
-        std::vector<mt_output_multislice *> op;
-        for (int i = 0; i < n; ++i) {
-            models.emplace_back(new mtg::Model(*in[i]));
-            mp.push_back(models.back().get());
-            op.push_back(&out[i]);
-        }
-        mtg::run_multislice_group(mp, sys->gpu_device, op);
+        if (precision_of(sys, 0) == 2) run_group_as<mtg::Model64>(sys, in, n, out);
+        else run_group_as<mtg::Model>(sys, in, n, out);
         g_last_error.clear();
         return MT_OK;
     } catch (const mtg::Error &e) {
@@ -115,8 +147,7 @@ mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multisli
     (void)iconf;   // plane slicing without z displacement: identical for every configuration
     if (!sys || !in) return fail(MT_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lock(g_engine_mutex);
-    try {
-        mtg::Model m(*in);
+    auto fill = [&](const auto &m) {
         *n_slice = m.n_slice();
         for (int i = 0; slices && i < m.n_slice() && i < max_slice; ++i) {
             const auto &s = m.slices[i];
@@ -125,6 +156,10 @@ mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multisli
             slices[i].iatom_0 = s.iatom_0; slices[i].iatom_e = s.iatom_e;
             slices[i].ithk = s.ithk;
         }
+    };
+    try {
+        if (precision_of(sys, 0) == 2) fill(mtg::Model64(*in));
+        else fill(mtg::Model(*in));
         g_last_error.clear();
         return MT_OK;
     } catch (const mtg::Error &e) {
@@ -136,14 +171,14 @@ mt_status mt_hip_projected_potential(const mt_system_conf *sys, const mt_input_m
                                      double *V)
 {
     if (!V) return fail(MT_ERR_INVALID_ARGUMENT, "null V");
-    return guarded(sys, in, [&](const mtg::Model &m) { mtg::projected_potential(m, sys->gpu_device, iconf, islice, V); });
+    return guarded(sys, in, [&](const auto &m) { mtg::projected_potential(m, sys->gpu_device, iconf, islice, V); });
 }
 
 mt_status mt_hip_projected_potential_sum(const mt_system_conf *sys, const mt_input_multislice *in, int iconf,
                                          double *V)
 {
     if (!V) return fail(MT_ERR_INVALID_ARGUMENT, "null V");
-    return guarded(sys, in, [&](const mtg::Model &m) { mtg::projected_potential_sum(m, sys->gpu_device, iconf, V); });
+    return guarded(sys, in, [&](const auto &m) { mtg::projected_potential_sum(m, sys->gpu_device, iconf, V); });
 }
 
 mt_status mt_hip_transmission_function(const mt_system_conf *sys, const mt_input_multislice *in, int iconf, int islice,
@@ -151,13 +186,13 @@ mt_status mt_hip_transmission_function(const mt_system_conf *sys, const mt_input
 {
     if (!trans) return fail(MT_ERR_INVALID_ARGUMENT, "null trans");
     return guarded(sys, in,
-                   [&](const mtg::Model &m) { mtg::transmission_function(m, sys->gpu_device, iconf, islice, trans); });
+                   [&](const auto &m) { mtg::transmission_function(m, sys->gpu_device, iconf, islice, trans); });
 }
 
 mt_status mt_hip_incident_wave(const mt_system_conf *sys, const mt_input_multislice *in, double *psi0)
 {
     if (!psi0) return fail(MT_ERR_INVALID_ARGUMENT, "null psi0");
-    return guarded(sys, in, [&](const mtg::Model &m) { mtg::incident_wave(m, sys->gpu_device, psi0); });
+    return guarded(sys, in, [&](const auto &m) { mtg::incident_wave(m, sys->gpu_device, psi0); });
 }
 
 mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice *in, const double *psi_in, double z,
@@ -165,7 +200,7 @@ mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice 
 {
     if (!psi_in || !psi_out) return fail(MT_ERR_INVALID_ARGUMENT, "null wave");
     return guarded(sys, in,
-                   [&](const mtg::Model &m) { mtg::propagate(m, sys->gpu_device, psi_in, z, space, psi_out); });
+                   [&](const auto &m) { mtg::propagate(m, sys->gpu_device, psi_in, z, space, psi_out); });
 }
 
 mt_status mt_hip_atom_table(int Z, double dR_min, double *R2, double *c0, double *c1, double *c2, double *c3,
@@ -173,7 +208,7 @@ mt_status mt_hip_atom_table(int Z, double dR_min, double *R2, double *c0, double
 {
     if (Z < 1 || Z > 103) return fail(MT_ERR_INVALID_ARGUMENT, "Z out of range 1..103");
     if (!(dR_min > 0)) return fail(MT_ERR_INVALID_ARGUMENT, "dR_min must be positive");
-    mtg::AtomTable t = mtg::build_atom_table(Z, (double)(float)dR_min, (double)0.015f);
+    mtg::AtomTable t = mtg::build_atom_table<float>(Z, (double)(float)dR_min, (double)0.015f);
     for (int i = 0; i < 128; ++i) {
         if (R2) R2[i] = t.R2[i];
         if (c0) c0[i] = t.c0[i];

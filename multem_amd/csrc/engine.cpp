@@ -26,6 +26,30 @@ namespace {
 
 constexpr double kPi = 3.141592653589793238463;
 
+// isZero of the reference: |x| <= epsilon_abs<T>, 1e-5 for float and 1e-13
+// for double (src/types.cuh:383-391)
+template <class R>
+bool is_zero(R x)
+{
+    return std::fabs(x) <= (sizeof(R) == 4 ? R(1e-5) : R(1e-13));
+}
+
+template <class Z>
+Z cz(double x, double y)
+{
+    Z z;
+    z.x = (real_t<Z>)x;
+    z.y = (real_t<Z>)y;
+    return z;
+}
+
+// columns per column-pass workgroup of the precision (the detector partial layout)
+template <class R>
+int col_tile_w(int nx)
+{
+    return sizeof(R) == 8 ? col_tile_width_fp64(nx) : col_tile_width(nx);
+}
+
 thread_local const char *t_last_ok = "none";
 
 void check(hipError_t e, const char *what)
@@ -108,39 +132,44 @@ float elapsed(const Event &a, const Event &b)
     return ms;
 }
 
-std::vector<float2> twiddles(int n)
+template <class Z>
+std::vector<Z> twiddles(int n)
 {
-    std::vector<float2> w(n);
+    std::vector<Z> w(n);
     for (int m = 0; m < n; ++m) {
         double a = -2.0 * kPi * (double)m / (double)n;
-        w[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        w[m] = cz<Z>(std::cos(a), std::sin(a));
     }
     return w;
 }
 
-// Device-resident data shared by all entries of one model.
+// Device-resident data shared by all entries of one model; R is the real
+// type of the wave path (float or double).
+template <class R>
 struct Setup {
-    const Model &m;
+    using Z = cplx_t<R>;
+    using V4 = vec4_t<R>;
+    const ModelT<R> &m;
     hipStream_t s;
     int nx, ny;
     size_t nxy;
-    DBuf<float2> tw_x, tw_y;
-    DBuf<float> tables;
-    DBuf<float2> rmax;
+    DBuf<Z> tw_x, tw_y;
+    DBuf<R> tables;
+    DBuf<Z> rmax;
     DBuf<int2> ranges;
     std::vector<int2> h_ranges;
 
-    Setup(const Model &mm, hipStream_t st) : m(mm), s(st), nx(mm.nx), ny(mm.ny), nxy((size_t)mm.nx * mm.ny)
+    Setup(const ModelT<R> &mm, hipStream_t st) : m(mm), s(st), nx(mm.nx), ny(mm.ny), nxy((size_t)mm.nx * mm.ny)
     {
-        auto wx = twiddles(nx), wy = twiddles(ny);
+        auto wx = twiddles<Z>(nx), wy = twiddles<Z>(ny);
         tw_x.alloc(nx); tw_x.upload(wx.data(), nx, s);
         tw_y.alloc(ny); tw_y.upload(wy.data(), ny, s);
         const int nt = (int)m.tables.size();
         if (nt > kMaxTypes) throw Error{MT_ERR_UNSUPPORTED, "more than 16 distinct elements"};
-        std::vector<float> tb((size_t)nt * 5 * 128);
-        std::vector<float2> rm(nt);
+        std::vector<R> tb((size_t)nt * 5 * 128);
+        std::vector<Z> rm(nt);
         for (int i = 0; i < nt; ++i) {
-            const AtomTable &t = m.tables[i];
+            const AtomTableT<R> &t = m.tables[i];
             for (int k = 0; k < 128; ++k) {
                 tb[(size_t)i * 640 + k] = t.R2[k];
                 tb[(size_t)i * 640 + 128 + k] = t.c0[k];
@@ -148,35 +177,36 @@ struct Setup {
                 tb[(size_t)i * 640 + 384 + k] = t.c2[k];
                 tb[(size_t)i * 640 + 512 + k] = t.c3[k];
             }
-            rm[i] = make_float2(t.R_max_f, t.R2_max_f);
+            rm[i] = cz<Z>(t.R_max_f, t.R2_max_f);
         }
         tables.alloc(tb.size()); tables.upload(tb.data(), tb.size(), s);
         rmax.alloc(nt); rmax.upload(rm.data(), nt, s);
-        for (const SliceInfo &sl : m.slices) h_ranges.push_back(make_int2(sl.iatom_0, sl.iatom_e));
+        for (const auto &sl : m.slices) h_ranges.push_back(make_int2(sl.iatom_0, sl.iatom_e));
         ranges.alloc(h_ranges.size()); ranges.upload(h_ranges.data(), h_ranges.size(), s);
     }
 
-    // float4 atoms (x, y, occ, type) of configuration iconf
-    std::vector<float4> atoms_of(int iconf) const
+    // atoms (x, y, occ, type) of configuration iconf: the type is the bit
+    // pattern of an int in .w for float, its value for double
+    std::vector<V4> atoms_of(int iconf) const
     {
-        std::vector<float> x, y;
+        std::vector<R> x, y;
         m.displaced(iconf, x, y);
-        std::vector<float4> a(x.size());
+        std::vector<V4> a(x.size());
         for (size_t i = 0; i < x.size(); ++i) {
-            int ty = m.type_of_atom[i];
-            float tyf;
-            std::memcpy(&tyf, &ty, sizeof tyf);
-            a[i] = make_float4(x[i], y[i], m.atoms_u.occ[i], tyf);
+            const int ty = m.type_of_atom[i];
+            R tyr;
+            if constexpr (sizeof(R) == 4) std::memcpy(&tyr, &ty, sizeof tyr);
+            else tyr = (R)ty;
+            a[i].x = x[i]; a[i].y = y[i]; a[i].z = (R)m.atoms_u.occ[i]; a[i].w = tyr;
         }
         return a;
     }
 
     // Transmission stack of one configuration into trans (n_slice * nxy),
     // optionally the potential V (natural layout) of the same slices.
-    void build_tf(const DBuf<float4> &atoms, int slice0, int n_slices, float2 *trans, float *V,
-                  DBuf<float2> *bwl_work)
+    void build_tf(const DBuf<V4> &atoms, int slice0, int n_slices, Z *trans, R *V)
     {
-        TfArgs a{};
+        TfArgsT<R> a{};
         a.nx = nx; a.ny = ny; a.nxh = m.nxh; a.nyh = m.nyh;
         a.dRx = m.dRx; a.dRy = m.dRy;
         a.atoms = atoms.p;
@@ -187,53 +217,50 @@ struct Setup {
         a.trans = trans; a.V = V;
         check(launch_tf_build(a, s), "tf_build");
         if (m.bwl && trans) bandwidth_limit(trans, n_slices);
-        (void)bwl_work;
     }
 
     // FFT -> x bwl/(nx ny) -> IFFT on n waves in place (Transmission_Function::trans,
     // src/transmission_function.cuh:72-82)
-    void bandwidth_limit(float2 *w, int n)
+    void bandwidth_limit(Z *w, int n)
     {
-        std::vector<Member> hm(n);
-        for (int i = 0; i < n; ++i) hm[i] = Member{0, -1, 0, 0};
-        RowArgs r{};
+        RowArgsT<Z> r{};
         r.src = w; r.dst = w; r.member_stride = (long long)nxy; r.nrows = nx; r.n_members = n;
         r.tw = tw_y.p;
         r.flags = ROW_LOAD | ROW_FFT | ROW_STORE;
         check(launch_row_pass(ny, r, s), "row pass");
-        ColArgs c = col_base(w, w, n);
+        ColArgsT<Z> c = col_base(w, w, n);
         c.flags = COL_LOAD | COL_FFT | COL_BWL | COL_IFFT | COL_STORE;
         check(launch_col_pass(nx, c, false, false, s), "col pass");
         r.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
         check(launch_row_pass(ny, r, s), "row pass");
     }
 
-    ColArgs col_base(const float2 *src, float2 *dst, int n_members) const
+    ColArgsT<Z> col_base(const Z *src, Z *dst, int n_members) const
     {
-        ColArgs c{};
+        ColArgsT<Z> c{};
         c.src = src; c.dst = dst; c.member_stride = (long long)nxy;
         c.ncols = ny; c.n_members = n_members;
         c.tw = tw_x.p;
         c.dgx_f = m.dgx; c.dgy_f = m.dgy;
         c.nx = nx; c.ny = ny;
         c.bwl_alpha = m.bwl_alpha; c.gl2_max = m.gl2_max;
-        c.inv_nxy = 1.0f / (float)nxy;
-        c.p_scale = 1.0f;
+        c.inv_nxy = R(1) / (R)nxy;
+        c.p_scale = R(1);
         return c;
     }
 
-    ProbeParams probe_params(unsigned int n_aperture) const
+    ProbeParamsT<R> probe_params(unsigned int n_aperture) const
     {
-        ProbeParams P{};
+        ProbeParamsT<R> P{};
         const double lam = m.lambda_d;
         // defocus referred to the first slice (Incident_Wave::operator(), src/incident_wave.cuh:72-91)
-        const float f_s = m.cond.c_10 - (m.cond.zero_defocus_plane - m.z_init());
-        P.c10 = (std::fabs(f_s) <= 1e-5f) ? 0.0 : -kPi * (double)f_s * lam;
-        P.c30 = (std::fabs(m.cond.c_30) <= 1e-5f) ? 0.0 : -kPi * (double)m.cond.c_30 * std::pow(lam, 3) / 2.0;
-        P.c50 = (std::fabs(m.cond.c_50) <= 1e-5f) ? 0.0 : -kPi * (double)m.cond.c_50 * std::pow(lam, 5) / 3.0;
+        const R f_s = m.cond.c_10 - (m.cond.zero_defocus_plane - m.z_init());
+        P.c10 = is_zero(f_s) ? 0.0 : -kPi * (double)f_s * lam;
+        P.c30 = is_zero(m.cond.c_30) ? 0.0 : -kPi * (double)m.cond.c_30 * std::pow(lam, 3) / 2.0;
+        P.c50 = is_zero(m.cond.c_50) ? 0.0 : -kPi * (double)m.cond.c_50 * std::pow(lam, 5) / 3.0;
         P.dgx = 1.0 / m.lx_d; P.dgy = 1.0 / m.ly_d;
         // azimuthal coefficients c_c_* of Lens::set_input_data (src/types.cuh:2924-2976)
-        const LensModel &L = m.cond;
+        const LensModelT<R> &L = m.cond;
         if (L.phi_required) {
             P.azim = 1;
             P.az_m = (double)L.m;
@@ -241,28 +268,28 @@ struct Setup {
             const double cf[11] = {-kPi * l1, -2.0 * kPi * l2 / 3.0, -2.0 * kPi * l2 / 3.0, -kPi * l3 / 2.0,
                                    -kPi * l3 / 2.0, -2.0 * kPi * l4 / 5.0, -2.0 * kPi * l4 / 5.0,
                                    -2.0 * kPi * l4 / 5.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0};
-            const float cv[11] = {L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
-            const float pv[11] = {L.phi_12, L.phi_21, L.phi_23, L.phi_32, L.phi_34, L.phi_41, L.phi_43, L.phi_45,
-                                  L.phi_52, L.phi_54, L.phi_56};
+            const R cv[11] = {L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
+            const R pv[11] = {L.phi_12, L.phi_21, L.phi_23, L.phi_32, L.phi_34, L.phi_41, L.phi_43, L.phi_45,
+                              L.phi_52, L.phi_54, L.phi_56};
             for (int k = 0; k < 11; ++k) {
-                P.az_c[k] = (std::fabs(cv[k]) <= 1e-5f) ? 0.0 : cf[k] * (double)cv[k];
+                P.az_c[k] = is_zero(cv[k]) ? 0.0 : cf[k] * (double)cv[k];
                 P.az_phi[k] = (double)pv[k];
             }
         }
         P.dgx_f = m.dgx; P.dgy_f = m.dgy;
         P.g2_min = m.cond.g2_min; P.g2_max = m.cond.g2_max;
-        P.norm = n_aperture ? (float)std::sqrt(1.0 / (double)n_aperture) : 0.f;
+        P.norm = n_aperture ? (R)std::sqrt(1.0 / (double)n_aperture) : R(0);
         return P;
     }
 
-    Beam beam(float xb, float yb) const
+    Beam beam(R xb, R yb) const
     {
         // x = -2 pi (x_b - lx/2)  (Grid_2d::exp_factor_Rx, src/types.cuh:2599-2611)
         return Beam{-2.0 * kPi * ((double)xb - 0.5 * m.lx_d), -2.0 * kPi * ((double)yb - 0.5 * m.ly_d)};
     }
 
     // pixels inside the condenser aperture, |g + u| evaluated as the probe does
-    unsigned int count_aperture(float ux = 0.f, float uy = 0.f)
+    unsigned int count_aperture(R ux = 0, R uy = 0)
     {
         DBuf<unsigned int> cnt(1);
         unsigned int zero = 0;
@@ -277,16 +304,16 @@ struct Setup {
 
     // objective-lens transfer function of HRTEM (MicroscopeEffects,
     // src/microscope_effects.cuh:57-145)
-    void ctf_table(DBuf<float2> &ctf)
+    void ctf_table(DBuf<Z> &ctf)
     {
-        const LensModel &L = m.obj;
+        const LensModelT<R> &L = m.obj;
         const double lam = m.lambda_d;
-        CtfParams P{};
+        CtfParamsT<R> P{};
         P.mode = m.illumination_model == 2 ? 2 : 1;
         P.dgx = 1.0 / m.lx_d; P.dgy = 1.0 / m.ly_d;
         P.dgx_f = m.dgx; P.dgy_f = m.dgy;
         P.g2_min = L.g2_min; P.g2_max = L.g2_max;
-        auto nz = [](float v) { return std::fabs(v) > 1e-5f; };
+        auto nz = [](R v) { return !is_zero(v); };
         P.cc[0] = nz(L.c_10) ? -kPi * (double)L.c_10 * lam : 0.0;
         P.cc[1] = nz(L.c_30) ? -kPi * (double)L.c_30 * std::pow(lam, 3) / 2.0 : 0.0;
         P.cc[2] = nz(L.c_50) ? -kPi * (double)L.c_50 * std::pow(lam, 5) / 3.0 : 0.0;
@@ -299,9 +326,9 @@ struct Setup {
             const double cf[11] = {-kPi * l1, -2.0 * kPi * l2 / 3.0, -2.0 * kPi * l2 / 3.0, -kPi * l3 / 2.0,
                                    -kPi * l3 / 2.0, -2.0 * kPi * l4 / 5.0, -2.0 * kPi * l4 / 5.0,
                                    -2.0 * kPi * l4 / 5.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0, -kPi * l5 / 3.0};
-            const float cv[11] = {L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
-            const float pv[11] = {L.phi_12, L.phi_21, L.phi_23, L.phi_32, L.phi_34, L.phi_41, L.phi_43, L.phi_45,
-                                  L.phi_52, L.phi_54, L.phi_56};
+            const R cv[11] = {L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
+            const R pv[11] = {L.phi_12, L.phi_21, L.phi_23, L.phi_32, L.phi_34, L.phi_41, L.phi_43, L.phi_45,
+                              L.phi_52, L.phi_54, L.phi_56};
             for (int k = 0; k < 11; ++k) {
                 P.az_c[k] = nz(cv[k]) ? cf[k] * (double)cv[k] : 0.0;
                 P.az_phi[k] = (double)pv[k];
@@ -318,20 +345,20 @@ struct Setup {
 
     // tilt ramp of the outputs, exp(-2 pi i g0.(r - r_c)), with 1/ny folded
     // into rx (it follows an unnormalised inverse transform along y)
-    void ramp_tables(DBuf<float2> &rx, DBuf<float2> &ry)
+    void ramp_tables(DBuf<Z> &rx, DBuf<Z> &ry)
     {
         rx.alloc(nx);
         ry.alloc(ny);
         // Grid_2d::Rx_c() is an int (src/types.cuh:2284-2287)
-        const double Rx_c = (double)(int)(0.5f * m.lx), Ry_c = (double)(int)(0.5f * m.ly);
+        const double Rx_c = (double)(int)(R(0.5) * m.lx), Ry_c = (double)(int)(R(0.5) * m.ly);
         check(launch_ramp_tables(nx, ny, (double)m.gx0, (double)m.gy0, m.lx_d / nx, m.ly_d / ny, Rx_c, Ry_c,
-                                 1.0f / (float)ny, rx.p, ry.p, s),
+                                 R(1) / (R)ny, rx.p, ry.p, s),
               "ramp tables");
         check(hipStreamSynchronize(s), "sync");
     }
 
     // propagator tables for the given distances: px [nz][nx], py [nz][ny]
-    void prop_tables(const std::vector<double> &z, DBuf<float2> &px, DBuf<float2> &py)
+    void prop_tables(const std::vector<double> &z, DBuf<Z> &px, DBuf<Z> &py)
     {
         DBuf<double> dz(z.size());
         dz.upload(z.data(), z.size(), s);
@@ -383,10 +410,10 @@ void stop_check()
 // Tuning launches overwrite the wave buffer and the detector partials; the
 // caller runs it before the first batch, which rewrites both.
 struct TuneKey {
-    int nx, ny, B, det_last;
+    int nx, ny, B, det_last, real_bytes;
     bool operator<(const TuneKey &o) const
     {
-        return std::tie(nx, ny, B, det_last) < std::tie(o.nx, o.ny, o.B, o.det_last);
+        return std::tie(nx, ny, B, det_last, real_bytes) < std::tie(o.nx, o.ny, o.B, o.det_last, o.real_bytes);
     }
 };
 
@@ -464,12 +491,14 @@ int pick_variant(const char *name, int forced, bool tune, F &&launch, hipStream_
 // r: a full row slice step over the batch; c: a full column slice step;
 // c_last: the last-slice column pass with detector sums (nullptr if the run
 // has none).
-PassChoice choose_variants(int nx, int ny, int B, RowArgs r, ColArgs c, const ColArgs *c_last, hipStream_t s)
+template <class Z>
+PassChoice choose_variants(int nx, int ny, int B, RowArgsT<Z> r, ColArgsT<Z> c, const ColArgsT<Z> *c_last,
+                           hipStream_t s)
 {
     const RunControl &rc = run_control();
     const bool forced = rc.force_row >= 0 || rc.force_col >= 0 || rc.force_col_det >= 0;
     const bool tune = rc.autotune != 0;
-    const TuneKey key{nx, ny, B, c_last ? 1 : 0};
+    const TuneKey key{nx, ny, B, c_last ? 1 : 0, (int)sizeof(real_t<Z>)};
     if (tune && rc.autotune == 1 && !forced) {
         auto it = tune_cache().find(key);
         if (it != tune_cache().end()) {
@@ -486,7 +515,7 @@ PassChoice choose_variants(int nx, int ny, int B, RowArgs r, ColArgs c, const Co
     ch.col = pick_variant("col", rc.force_col, tune,
                           [&](int v) { c.variant = v; return launch_col_pass(nx, c, false, false, s); }, s, log);
     if (c_last) {
-        ColArgs cl = *c_last;
+        ColArgsT<Z> cl = *c_last;
         ch.col_det = pick_variant("col_det", rc.force_col_det, tune,
                                   [&](int v) { cl.variant = v; return launch_col_pass(nx, cl, false, true, s); }, s,
                                   log);
@@ -505,27 +534,29 @@ PassChoice choose_variants(int nx, int ny, int B, RowArgs r, ColArgs c, const Co
 // resident (transmission stack of the configuration, the batch's member and
 // beam tables) and pipeline(st) enqueues the item's whole device work on
 // stream st: probe generation, the slice loop, detector partials -> image.
+template <class R>
 class StemJob {
 public:
-    const Model &m;
+    using Z = cplx_t<R>;
+    const ModelT<R> &m;
     Stream own;
     hipStream_t s;            // uploads and transmission builds
-    Setup S;
+    Setup<R> S;
     int ns_run, nscan, n_thk, n_det, B, ntile, nbatch;
     size_t nxy;
-    DBuf<float2> trans, psi;
+    DBuf<Z> trans, psi;
     DBuf<Member> members;
     DBuf<Beam> beams;
     DBuf<double> part, image;
-    DBuf<float2> px, py;           // [n_slice + n_thk]: slice propagators, then back-propagations
-    DBuf<float2> rx, ry, scratch;   // tilted illumination: output ramp and the output copy of the batch
+    DBuf<Z> px, py;           // [n_slice + n_thk]: slice propagators, then back-propagations
+    DBuf<Z> rx, ry, scratch;   // tilted illumination: output ramp and the output copy of the batch
     DBuf<double2> coh;              // pn_coh_contrib: sum over configurations of w psi(g), [scan][n_thk][nxy]
-    DBuf<float4> atoms;
-    ProbeParams P{};
+    DBuf<vec4_t<R>> atoms;
+    ProbeParamsT<R> P{};
     unsigned int nap = 0;
-    float w = 0;
-    RowArgs r{};
-    ColArgs c{};
+    R w = 0;
+    RowArgsT<Z> r{};
+    ColArgsT<Z> c{};
     PassChoice choice;
     std::vector<Member> hm;
     std::vector<Beam> hb;
@@ -534,11 +565,11 @@ public:
     std::string batch_log;       // batch-width selection log
 
     // budget: bytes this job may use for its transmission stack and waves
-    StemJob(const Model &mm, size_t budget, int batch_override)
+    StemJob(const ModelT<R> &mm, size_t budget, int batch_override)
         : m(mm), s(own.s), S(mm, own.s), nxy((size_t)mm.nx * mm.ny)
     {
         ns_run = 0;
-        for (const ThickInfo &t : m.thk) ns_run = std::max(ns_run, t.islice + 1);   // deeper slices add nothing
+        for (const auto &t : m.thk) ns_run = std::max(ns_run, t.islice + 1);   // deeper slices add nothing
         nscan = (int)m.sc_x.size();
         n_thk = (int)m.thk.size();
         n_det = (int)m.det_g2_min.size();
@@ -547,19 +578,19 @@ public:
         // memory plan: one transmission stack + B waves (+ the coherent
         // accumulators of every scan point, all resident)
         const size_t coh_bytes = m.pn_coh_contrib ? (size_t)nscan * n_thk * nxy * sizeof(double2) : 0;
-        const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(float2) + coh_bytes;
-        if (stack_bytes + 2 * nxy * sizeof(float2) > budget)
+        const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(Z) + coh_bytes;
+        if (stack_bytes + 2 * nxy * sizeof(Z) > budget)
             throw Error{MT_ERR_OUT_OF_MEMORY, m.pn_coh_contrib ? "transmission stack and coherent accumulators "
                                                                   "exceed the memory budget"
                                                                 : "transmission stack exceeds the memory budget"};
-        const size_t per_wave = (m.tilted() ? 2 : 1) * nxy * sizeof(float2);
+        const size_t per_wave = (m.tilted() ? 2 : 1) * nxy * sizeof(Z);
         const int B_cap = (int)std::min<size_t>((size_t)std::max(1, nscan), (budget - stack_bytes) / per_wave);
         if (B_cap < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for a wave batch"};
-        ntile = m.ny / col_tile_width(m.nx);
+        ntile = m.ny / col_tile_w<R>(m.nx);
         trans.alloc(m.n_slice() * nxy);
         std::vector<double> zl;
         for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
-        for (const ThickInfo &t : m.thk) zl.push_back((double)t.z_back_prop);
+        for (const auto &t : m.thk) zl.push_back((double)t.z_back_prop);
         S.prop_tables(zl, px, py);
 
         B = batch_override;
@@ -567,8 +598,8 @@ public:
             // enough waves to fill 256 CUs in the column pass, and a working set
             // that stays near the 256 MB last-level cache; the tuner may double
             // or quadruple it
-            const int nt = m.ny / col_tile_width(m.nx);
-            B = std::max(1, (int)std::min<size_t>(256, (size_t)(160u << 20) / (nxy * sizeof(float2))));
+            const int nt = m.ny / col_tile_w<R>(m.nx);
+            B = std::max(1, (int)std::min<size_t>(256, (size_t)(160u << 20) / (nxy * sizeof(Z))));
             B = std::max(B, (2048 + nt - 1) / nt);
             B = std::min(B, B_cap);
             if (run_control().autotune != 0) B = choose_batch(B, std::min(B_cap, std::max(B, 256)));   // B0..2 B0
@@ -612,17 +643,17 @@ public:
         c.coh = coh.p; c.coh_w = (double)w;
 
         // launch variants, picked before any pipeline is captured
-        RowArgs rt = r;
+        RowArgsT<Z> rt = r;
         rt.slice = 0;
-        ColArgs ct = c;
+        ColArgsT<Z> ct = c;
         ct.flags = col_flags(false);
         ct.px = px.p; ct.py = py.p; ct.ithk = -1;
-        ColArgs cl = c;
+        ColArgsT<Z> cl = c;
         cl.flags = col_flags(true);
         cl.px = px.p; cl.py = py.p; cl.ithk = m.slices[ns_run - 1].ithk;
         for (int j = 0; j < B; ++j) hm[j] = Member{0, -1, j, 1};   // valid entries while tuning
         members.upload(hm.data(), B, s);
-        check(hipMemsetAsync(psi.p, 0, psi.n * sizeof(float2), s), "hipMemsetAsync");
+        check(hipMemsetAsync(psi.p, 0, psi.n * sizeof(Z), s), "hipMemsetAsync");
         choice = choose_variants(m.nx, m.ny, B, rt, ct, cl.ithk >= 0 ? &cl : nullptr, s);
         r.variant = choice.row;
     }
@@ -636,7 +667,7 @@ public:
         for (int b : {B0, B0 + B0 / 2, 2 * B0})
             if (b <= B_max && (cand.empty() || b > cand.back())) cand.push_back(b);
         if (cand.size() < 2) return B0;
-        static std::map<std::tuple<int, int, int, int>, int> cache;
+        static std::map<std::tuple<int, int, int, int>, int> cache;   // one per precision
         const auto key = std::make_tuple(m.nx, m.ny, B0, B_max);
         const bool reuse = run_control().autotune == 1;
         if (reuse) {
@@ -647,21 +678,21 @@ public:
             }
         }
         const int bm = cand.back();
-        DBuf<float2> w((size_t)bm * nxy);
+        DBuf<Z> w((size_t)bm * nxy);
         DBuf<Member> mem(bm);
         std::vector<Member> hmv(bm, Member{0, -1, 0, 1});
         mem.upload(hmv.data(), bm, s);
-        check(hipMemsetAsync(w.p, 0, w.n * sizeof(float2), s), "hipMemsetAsync");
+        check(hipMemsetAsync(w.p, 0, w.n * sizeof(Z), s), "hipMemsetAsync");
         std::ostringstream log;
         log << "batch[";
         double best = -1;
         std::vector<double> per(cand.size());
         for (size_t i = 0; i < cand.size(); ++i) {
-            RowArgs rr{};
+            RowArgsT<Z> rr{};
             rr.src = w.p; rr.dst = w.p; rr.member_stride = (long long)nxy; rr.nrows = m.nx; rr.n_members = cand[i];
             rr.tw = S.tw_y.p; rr.trans = trans.p; rr.members = mem.p; rr.slice = 0;
             rr.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
-            ColArgs cc = S.col_base(w.p, w.p, cand[i]);
+            ColArgsT<Z> cc = S.col_base(w.p, w.p, cand[i]);
             cc.flags = col_flags(false);
             cc.px = px.p; cc.py = py.p; cc.ithk = -1;
             const double t = time_variant([&] {
@@ -693,7 +724,7 @@ public:
     std::string plan() const
     {
         std::ostringstream o;
-        o << "stem nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
+        o << "stem " << (sizeof(R) == 8 ? "fp64" : "fp32") << " nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
           << " row_variant=" << choice.row << " col_variant=" << choice.col << " col_det_variant=" << choice.col_det
           << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
         return o.str();
@@ -708,7 +739,7 @@ public:
             atoms.upload(a.data(), a.size(), s);
             Event t0, t1;
             record(t0, s);
-            S.build_tf(atoms, 0, m.n_slice(), trans.p, nullptr, nullptr);
+            S.build_tf(atoms, 0, m.n_slice(), trans.p, nullptr);
             record(t1, s);
             wait(t1, s);
             ms_tf += elapsed(t0, t1);
@@ -729,7 +760,7 @@ public:
     // bracketing the row and column passes
     void pipeline(hipStream_t st, std::vector<Event> *ev)
     {
-        ColArgs c0 = S.col_base(nullptr, psi.p, B);
+        ColArgsT<Z> c0 = S.col_base(nullptr, psi.p, B);
         c0.flags = COL_IFFT | COL_STORE;
         c0.members = members.p; c0.beams = beams.p; c0.probe = P;
         check(launch_col_pass(m.nx, c0, true, false, st), "probe pass");
@@ -750,12 +781,12 @@ public:
                 c.ithk = -1;
                 c.variant = choice.col;
                 check(launch_col_pass(m.nx, c, false, false, st), "col pass");
-                RowArgs ro = r;
+                RowArgsT<Z> ro = r;
                 ro.dst = scratch.p;
                 ro.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_RAMP | ROW_FFT | ROW_STORE;
                 ro.ramp_x = rx.p; ro.ramp_y = ry.p;
                 check(launch_row_pass(m.ny, ro, st), "output ramp pass");
-                ColArgs cd = c;
+                ColArgsT<Z> cd = c;
                 cd.src = scratch.p;
                 cd.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (m.pn_coh_contrib ? COL_COH : 0);
                 cd.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
@@ -763,7 +794,7 @@ public:
                 cd.ithk = ithk;
                 cd.variant = 0;
                 // the stored wave lacks the 1/ny that the next row pass would supply
-                cd.p_scale = (float)m.ny;
+                cd.p_scale = (R)m.ny;
                 check(launch_col_pass(m.nx, cd, false, true, st), "output detector pass");
             } else {
                 c.flags = col_flags(is == ns_run - 1) | ((ithk >= 0 && m.pn_coh_contrib) ? COL_COH : 0);
@@ -808,11 +839,12 @@ struct GraphExec {
     }
 };
 
-void run_stem(const Model &m, mt_output_multislice *out)
+template <class R>
+void run_stem(const ModelT<R> &m, mt_output_multislice *out)
 {
     Timing &tm = last_timing();
     tm = Timing{};
-    StemJob J(m, budget_bytes(), run_control().batch_override);
+    StemJob<R> J(m, budget_bytes(), run_control().batch_override);
     hipStream_t s = J.s;
     tm.plan = J.plan();
     tm.tune_log = (J.batch_log.empty() ? "" : J.batch_log + " ") + J.choice.log;
@@ -871,7 +903,8 @@ void run_stem(const Model &m, mt_output_multislice *out)
 // own tuned kernels -- is one hipGraph launch (branches forked onto
 // per-specimen streams during capture, joined at the end).  Graphs are kept
 // per set of active specimens.
-void run_stem_group(const std::vector<const Model *> &ms, const std::vector<mt_output_multislice *> &outs)
+template <class R>
+void run_stem_group(const std::vector<const ModelT<R> *> &ms, const std::vector<mt_output_multislice *> &outs)
 {
     Timing &tm = last_timing();
     tm = Timing{};
@@ -883,16 +916,16 @@ void run_stem_group(const std::vector<const Model *> &ms, const std::vector<mt_o
     double tot = 0;
     for (size_t i = 0; i < n; ++i) {
         const double nxy = (double)ms[i]->nx * ms[i]->ny;
-        want[i] = nxy * 8.0 * (ms[i]->n_slice() + 256.0);
+        want[i] = nxy * (double)sizeof(cplx_t<R>) * (ms[i]->n_slice() + 256.0);
         tot += want[i];
     }
     const size_t budget = budget_bytes();
-    std::vector<std::unique_ptr<StemJob>> jobs;
+    std::vector<std::unique_ptr<StemJob<R>>> jobs;
     std::ostringstream plan, tune;
     int rounds = 0, niter = 0;
     for (size_t i = 0; i < n; ++i) {
         const size_t share = (size_t)((double)budget * want[i] / tot);
-        jobs.emplace_back(new StemJob(*ms[i], share, run_control().batch_override));
+        jobs.emplace_back(new StemJob<R>(*ms[i], share, run_control().batch_override));
         plan << (i ? " | " : "") << jobs.back()->plan();
         tune << (i ? " | " : "") << jobs.back()->batch_log << (jobs.back()->batch_log.empty() ? "" : " ")
              << jobs.back()->choice.log;
@@ -973,11 +1006,13 @@ void run_stem_group(const std::vector<const Model *> &ms, const std::vector<mt_o
 
 // ---------------------------------------------------------------------------
 // Exit wave / CBED: the members of a batch are phonon configurations.
-void run_ew(const Model &m, mt_output_multislice *out)
+template <class R>
+void run_ew(const ModelT<R> &m, mt_output_multislice *out)
 {
+    using Z = cplx_t<R>;
     Stream stream;
     hipStream_t s = stream.s;
-    Setup S(m, s);
+    Setup<R> S(m, s);
     Timing &tm = last_timing();
     tm = Timing{};
     const size_t nxy = S.nxy;
@@ -990,13 +1025,13 @@ void run_ew(const Model &m, mt_output_multislice *out)
     const bool fs = !m.is_ewrs() && !m.is_hrtem();     // Fourier-space output (EWFS, CBED)
     const int ns_run = [&] {
         int last = 0;
-        for (const ThickInfo &t : m.thk) last = std::max(last, t.islice);
+        for (const auto &t : m.thk) last = std::max(last, t.islice);
         return last + 1;
     }();
 
     const int nconf = m.n_conf_run();
-    const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(float2);
-    const size_t per_member = stack_bytes + 2 * nxy * sizeof(float2);
+    const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(Z);
+    const size_t per_member = stack_bytes + 2 * nxy * sizeof(Z);
     const size_t fixed = (size_t)n_thk * (no * 3 + 2 * nxy) * sizeof(double);
     const size_t budget = budget_bytes();
     if (fixed + per_member > budget) throw Error{MT_ERR_OUT_OF_MEMORY, "one configuration exceeds the memory budget"};
@@ -1004,7 +1039,7 @@ void run_ew(const Model &m, mt_output_multislice *out)
     if (run_control().batch_override > 0) K = std::min(K, run_control().batch_override);
     K = std::max(K, 1);
 
-    DBuf<float2> trans((size_t)K * m.n_slice() * nxy), psi((size_t)K * nxy), scratch((size_t)K * nxy);
+    DBuf<Z> trans((size_t)K * m.n_slice() * nxy), psi((size_t)K * nxy), scratch((size_t)K * nxy);
     DBuf<Member> members(K);
     DBuf<Beam> beams(std::max<size_t>(1, m.iw_x.size()));
     DBuf<double> m2(std::max<size_t>(1, (size_t)n_thk * no)), pc(std::max<size_t>(1, (size_t)n_thk * no * 2));
@@ -1018,14 +1053,14 @@ void run_ew(const Model &m, mt_output_multislice *out)
     }
     std::vector<double> zl;
     for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
-    for (const ThickInfo &t : m.thk) zl.push_back((double)t.z_back_prop);
-    DBuf<float2> px, py;
+    for (const auto &t : m.thk) zl.push_back((double)t.z_back_prop);
+    DBuf<Z> px, py;
     S.prop_tables(zl, px, py);
-    DBuf<float2> rx, ry;
+    DBuf<Z> rx, ry;
     if (m.tilted()) S.ramp_tables(rx, ry);
-    DBuf<float2> ctf;
+    DBuf<Z> ctf;
     if (m.is_hrtem()) S.ctf_table(ctf);
-    ProbeParams P{};
+    ProbeParamsT<R> P{};
     if (m.iw_type == 2) {
         P = S.probe_params(S.count_aperture());
         std::vector<Beam> hb;
@@ -1033,7 +1068,8 @@ void run_ew(const Model &m, mt_output_multislice *out)
         beams.upload(hb.data(), hb.size(), s);
     }
     std::ostringstream plan;
-    plan << (m.is_cbed() ? "cbed" : (m.is_ewrs() ? "ewrs" : "ewfs")) << " nx=" << m.nx << " ny=" << m.ny
+    plan << (m.is_cbed() ? "cbed" : (m.is_hrtem() ? "hrtem" : (m.is_ewrs() ? "ewrs" : "ewfs")))
+         << (sizeof(R) == 8 ? " fp64" : " fp32") << " nx=" << m.nx << " ny=" << m.ny
          << " configs_per_batch=" << K << " slices=" << ns_run;
     tm.plan = plan.str();
 
@@ -1042,7 +1078,7 @@ void run_ew(const Model &m, mt_output_multislice *out)
     run_control().iter = 0;
     Event t0, t1;
     record(t0, s);
-    DBuf<float4> atoms(m.atoms_u.x.size());
+    DBuf<vec4_t<R>> atoms(m.atoms_u.x.size());
     PassChoice choice;
     for (int c0 = 0; c0 < nconf; c0 += K) {
         stop_check();
@@ -1052,29 +1088,29 @@ void run_ew(const Model &m, mt_output_multislice *out)
             hm[j] = Member{j, -1, 0, m.iw_type == 2 ? (int)m.iw_x.size() : 0};
             auto a = S.atoms_of(m.fp_iconf_0 + c0 + j);
             atoms.upload(a.data(), a.size(), s);
-            S.build_tf(atoms, 0, m.n_slice(), trans.p + (size_t)j * m.n_slice() * nxy, nullptr, nullptr);
+            S.build_tf(atoms, 0, m.n_slice(), trans.p + (size_t)j * m.n_slice() * nxy, nullptr);
             check(hipStreamSynchronize(s), "sync");   // atoms buffer is reused by the next configuration
         }
         members.upload(hm.data(), nb, s);
 
-        RowArgs r{};
+        RowArgsT<Z> r{};
         r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nb;
         r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = (long long)m.n_slice() * nxy; r.members = members.p;
         if (c0 == 0) {   // pick the slice-step launch variants on this batch's buffers
-            RowArgs rt = r;
+            RowArgsT<Z> rt = r;
             rt.slice = 0;
             rt.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
-            ColArgs ct = S.col_base(psi.p, psi.p, nb);
+            ColArgsT<Z> ct = S.col_base(psi.p, psi.p, nb);
             ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
             ct.px = px.p; ct.py = py.p;
-            check(hipMemsetAsync(psi.p, 0, (size_t)nb * nxy * sizeof(float2), s), "hipMemsetAsync");
-            choice = choose_variants(m.nx, m.ny, nb, rt, ct, nullptr, s);
+            check(hipMemsetAsync(psi.p, 0, (size_t)nb * nxy * sizeof(Z), s), "hipMemsetAsync");
+            choice = choose_variants(m.nx, m.ny, nb, rt, ct, (const ColArgsT<Z> *)nullptr, s);
             tm.tune_log = choice.log;
             tm.plan += " row_variant=" + std::to_string(choice.row) + " col_variant=" + std::to_string(choice.col);
         }
         r.variant = choice.row;
         if (m.iw_type == 2) {
-            ColArgs cp = S.col_base(nullptr, psi.p, nb);
+            ColArgsT<Z> cp = S.col_base(nullptr, psi.p, nb);
             cp.flags = COL_IFFT | COL_STORE;
             cp.members = members.p; cp.beams = beams.p; cp.probe = P;
             check(launch_col_pass(m.nx, cp, true, false, s), "probe pass");
@@ -1083,7 +1119,7 @@ void run_ew(const Model &m, mt_output_multislice *out)
             r.slice = is;
             r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
             check(launch_row_pass(m.ny, r, s), "row pass");
-            ColArgs c = S.col_base(psi.p, psi.p, nb);
+            ColArgsT<Z> c = S.col_base(psi.p, psi.p, nb);
             c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
             c.px = px.p + (size_t)is * m.nx;
             c.py = py.p + (size_t)is * m.ny;
@@ -1092,38 +1128,38 @@ void run_ew(const Model &m, mt_output_multislice *out)
             const int ithk = m.slices[is].ithk;
             if (ithk < 0) continue;
             // O = F_x(L)/nx * bwl * exp(i phi_back)  (get_psi, src/wave_function.cuh:96-105)
-            const float zb = m.thk[ithk].z_back_prop;
-            const float2 *osrc = psi.p;
+            const R zb = m.thk[ithk].z_back_prop;
+            const Z *osrc = psi.p;
             if (m.tilted()) {
                 // psi exp(-2 pi i g0.r) in real space first (get_psi, src/wave_function.cuh:96-105)
-                RowArgs ro = r;
+                RowArgsT<Z> ro = r;
                 ro.src = psi.p; ro.dst = scratch.p;
                 ro.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_RAMP | ROW_FFT | ROW_STORE;
                 ro.ramp_x = rx.p; ro.ramp_y = ry.p;
                 check(launch_row_pass(m.ny, ro, s), "output ramp pass");
                 osrc = scratch.p;
             }
-            if (fs || zb != 0.f || m.bwl || m.tilted() || m.is_hrtem()) {
+            if (fs || zb != R(0) || m.bwl || m.tilted() || m.is_hrtem()) {
                 // HRTEM: the objective lens acts on O(g) before the inverse
                 // transforms (microscope_effects, src/wave_function.cuh:172-180)
-                ColArgs co = S.col_base(osrc, scratch.p, nb);
+                ColArgsT<Z> co = S.col_base(osrc, scratch.p, nb);
                 co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) | COL_STORE |
                            (m.is_hrtem() ? COL_MUL_CTF : 0);
                 co.ctf = ctf.p;
                 co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
                 co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
-                co.p_scale = (float)m.ny;
+                co.p_scale = (R)m.ny;
                 check(launch_col_pass(m.nx, co, false, false, s), "output col pass");
                 osrc = scratch.p;
             }
             if (!fs) {
-                RowArgs ro = r;
+                RowArgsT<Z> ro = r;
                 ro.src = osrc; ro.dst = scratch.p;
                 ro.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
                 check(launch_row_pass(m.ny, ro, s), "output row pass");
                 osrc = scratch.p;
             }
-            OutArgs oa{};
+            OutArgsT<Z> oa{};
             oa.src = osrc; oa.member_stride = (long long)nxy; oa.n_members = nb;
             oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
             oa.w = w;
@@ -1174,19 +1210,22 @@ Timing &last_timing()
     return t;
 }
 
-void run_multislice_group(const std::vector<const Model *> &ms, int device, const std::vector<mt_output_multislice *> &outs)
+template <class R>
+void run_multislice_group(const std::vector<const ModelT<R> *> &ms, int device,
+                          const std::vector<mt_output_multislice *> &outs)
 {
     select_device(device);
     for (size_t i = 0; i < ms.size(); ++i) {
-        const Model &m = *ms[i];
+        const ModelT<R> &m = *ms[i];
         if (!m.is_stem()) throw Error{MT_ERR_UNSUPPORTED, "group runs take STEM inputs only"};
-            m.output_dims(outs[i]);
+        m.output_dims(outs[i]);
         m.fill_axes(outs[i]);
     }
     run_stem_group(ms, outs);
 }
 
-void run_multislice(const Model &m, int device, mt_output_multislice *out)
+template <class R>
+void run_multislice(const ModelT<R> &m, int device, mt_output_multislice *out)
 {
     select_device(device);
     m.output_dims(out);
@@ -1198,19 +1237,20 @@ void run_multislice(const Model &m, int device, mt_output_multislice *out)
 // ---------------------------------------------------------------------------
 // per-piece entries
 
-void projected_potential(const Model &m, int device, int iconf, int islice, double *V)
+template <class R>
+void projected_potential(const ModelT<R> &m, int device, int iconf, int islice, double *V)
 {
     select_device(device);
     if (islice < 0 || islice >= m.n_slice()) throw Error{MT_ERR_INVALID_ARGUMENT, "islice out of range"};
     Stream st;
-    Setup S(m, st.s);
+    Setup<R> S(m, st.s);
     auto a = S.atoms_of(iconf);
-    DBuf<float4> atoms(a.size());
+    DBuf<vec4_t<R>> atoms(a.size());
     atoms.upload(a.data(), a.size(), st.s);
-    DBuf<float> dv(S.nxy);
-    S.build_tf(atoms, islice, 1, nullptr, dv.p, nullptr);
-    std::vector<float> h(S.nxy);
-    check(hipMemcpyAsync(h.data(), dv.p, S.nxy * sizeof(float), hipMemcpyDeviceToHost, st.s), "D2H");
+    DBuf<R> dv(S.nxy);
+    S.build_tf(atoms, islice, 1, nullptr, dv.p);
+    std::vector<R> h(S.nxy);
+    check(hipMemcpyAsync(h.data(), dv.p, S.nxy * sizeof(R), hipMemcpyDeviceToHost, st.s), "D2H");
     check(hipStreamSynchronize(st.s), "sync");
     for (size_t i = 0; i < S.nxy; ++i) V[i] = h[i];
 }
@@ -1235,21 +1275,22 @@ double stream_copy_gbs(int device, size_t bytes, int reps)
 // Sum over all slices of the projected potential of configuration iconf
 // (natural layout), accumulated in FP64 in slice order; the device time of
 // the build and the sum is reported as the run's ms_total.
-void projected_potential_sum(const Model &m, int device, int iconf, double *V)
+template <class R>
+void projected_potential_sum(const ModelT<R> &m, int device, int iconf, double *V)
 {
     select_device(device);
     Timing &tm = last_timing();
     tm = Timing{};
     Stream st;
-    Setup S(m, st.s);
+    Setup<R> S(m, st.s);
     auto a = S.atoms_of(iconf);
-    DBuf<float4> atoms(a.size());
+    DBuf<vec4_t<R>> atoms(a.size());
     atoms.upload(a.data(), a.size(), st.s);
-    DBuf<float> dv((size_t)m.n_slice() * S.nxy);
+    DBuf<R> dv((size_t)m.n_slice() * S.nxy);
     DBuf<double> sum(S.nxy);
     Event t0, t1;
     record(t0, st.s);
-    S.build_tf(atoms, 0, m.n_slice(), nullptr, dv.p, nullptr);
+    S.build_tf(atoms, 0, m.n_slice(), nullptr, dv.p);
     check(launch_slice_sum(dv.p, m.n_slice(), (long long)S.nxy, sum.p, st.s), "slice sum");
     record(t1, st.s);
     check(hipMemcpyAsync(V, sum.p, S.nxy * sizeof(double), hipMemcpyDeviceToHost, st.s), "D2H");
@@ -1257,43 +1298,48 @@ void projected_potential_sum(const Model &m, int device, int iconf, double *V)
     tm.ms_total = elapsed(t0, t1);
 }
 
-static void download_unshifted(Setup &S, const float2 *d, double *out)
+template <class R>
+static void download_unshifted(Setup<R> &S, const cplx_t<R> *d, double *out)
 {
-    DBuf<float2> tmp(S.nxy);
+    using Z = cplx_t<R>;
+    DBuf<Z> tmp(S.nxy);
     check(launch_shift_complex(d, tmp.p, S.nx, S.ny, S.s), "shift");
-    std::vector<float2> h(S.nxy);
-    check(hipMemcpyAsync(h.data(), tmp.p, S.nxy * sizeof(float2), hipMemcpyDeviceToHost, S.s), "D2H");
+    std::vector<Z> h(S.nxy);
+    check(hipMemcpyAsync(h.data(), tmp.p, S.nxy * sizeof(Z), hipMemcpyDeviceToHost, S.s), "D2H");
     check(hipStreamSynchronize(S.s), "sync");
     for (size_t i = 0; i < S.nxy; ++i) { out[2 * i] = h[i].x; out[2 * i + 1] = h[i].y; }
 }
 
-void transmission_function(const Model &m, int device, int iconf, int islice, double *trans)
+template <class R>
+void transmission_function(const ModelT<R> &m, int device, int iconf, int islice, double *trans)
 {
     select_device(device);
     if (islice < 0 || islice >= m.n_slice()) throw Error{MT_ERR_INVALID_ARGUMENT, "islice out of range"};
     Stream st;
-    Setup S(m, st.s);
+    Setup<R> S(m, st.s);
     auto a = S.atoms_of(iconf);
-    DBuf<float4> atoms(a.size());
+    DBuf<vec4_t<R>> atoms(a.size());
     atoms.upload(a.data(), a.size(), st.s);
-    DBuf<float2> t(S.nxy);
-    S.build_tf(atoms, islice, 1, t.p, nullptr, nullptr);
+    DBuf<cplx_t<R>> t(S.nxy);
+    S.build_tf(atoms, islice, 1, t.p, nullptr);
     download_unshifted(S, t.p, trans);
 }
 
-void incident_wave(const Model &m, int device, double *psi0)
+template <class R>
+void incident_wave(const ModelT<R> &m, int device, double *psi0)
 {
+    using Z = cplx_t<R>;
     select_device(device);
     Stream st;
-    Setup S(m, st.s);
-    DBuf<float2> w(S.nxy);
+    Setup<R> S(m, st.s);
+    DBuf<Z> w(S.nxy);
     if (m.iw_type == 1) {
-        std::vector<float2> one(S.nxy, make_float2(1.f, 0.f));
+        std::vector<Z> one(S.nxy, cz<Z>(1.0, 0.0));
         w.upload(one.data(), S.nxy, st.s);
     } else {
         // the stand-alone incident wave evaluates chi at g + g0 (Incident_Wave::operator()
         // with gx_0, gy_0, src/incident_wave.cuh:116-131); inside a simulation the probe is untilted
-        ProbeParams P = S.probe_params(S.count_aperture(m.gx0, m.gy0));
+        ProbeParamsT<R> P = S.probe_params(S.count_aperture(m.gx0, m.gy0));
         P.gxu = (double)m.gx0; P.gyu = (double)m.gy0;
         P.gxu_f = m.gx0; P.gyu_f = m.gy0;
         std::vector<Beam> hb;
@@ -1304,11 +1350,11 @@ void incident_wave(const Model &m, int device, double *psi0)
         Member hm{0, -1, 0, (int)hb.size()};
         DBuf<Member> mem(1);
         mem.upload(&hm, 1, st.s);
-        ColArgs c = S.col_base(nullptr, w.p, 1);
+        ColArgsT<Z> c = S.col_base(nullptr, w.p, 1);
         c.flags = COL_IFFT | COL_STORE;
         c.members = mem.p; c.beams = beams.p; c.probe = P;
         check(launch_col_pass(m.nx, c, true, false, st.s), "probe pass");
-        RowArgs r{};
+        RowArgsT<Z> r{};
         r.src = w.p; r.dst = w.p; r.member_stride = (long long)S.nxy; r.nrows = m.nx; r.n_members = 1;
         r.tw = S.tw_y.p; r.members = mem.p;
         r.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
@@ -1319,32 +1365,34 @@ void incident_wave(const Model &m, int device, double *psi0)
 }
 
 // Propagator::operator()(space, ..., output) (src/propagator.cuh:95-104)
-void propagate(const Model &m, int device, const double *psi_in, double z, int space, double *psi_out)
+template <class R>
+void propagate(const ModelT<R> &m, int device, const double *psi_in, double z, int space, double *psi_out)
 {
+    using Z = cplx_t<R>;
     select_device(device);
     if (space != 1 && space != 2) throw Error{MT_ERR_INVALID_ARGUMENT, "space must be 1 (real) or 2 (reciprocal)"};
     Stream st;
-    Setup S(m, st.s);
-    std::vector<float2> h(S.nxy);
-    for (size_t i = 0; i < S.nxy; ++i) h[i] = make_float2((float)psi_in[2 * i], (float)psi_in[2 * i + 1]);
-    DBuf<float2> a(S.nxy), w(S.nxy);
+    Setup<R> S(m, st.s);
+    std::vector<Z> h(S.nxy);
+    for (size_t i = 0; i < S.nxy; ++i) h[i] = cz<Z>((R)psi_in[2 * i], (R)psi_in[2 * i + 1]);
+    DBuf<Z> a(S.nxy), w(S.nxy);
     a.upload(h.data(), S.nxy, st.s);
     check(launch_shift_complex(a.p, w.p, S.nx, S.ny, st.s), "shift");
-    DBuf<float2> px, py;
+    DBuf<Z> px, py;
     std::vector<double> zl{z};
     S.prop_tables(zl, px, py);
-    const bool zero = std::fabs((float)z) <= 1e-5f;
+    const bool zero = is_zero((R)z);
     Member hm{0, -1, 0, 0};
     DBuf<Member> mem(1);
     mem.upload(&hm, 1, st.s);
-    RowArgs r{};
+    RowArgsT<Z> r{};
     r.src = w.p; r.dst = w.p; r.member_stride = (long long)S.nxy; r.nrows = m.nx; r.n_members = 1;
     r.tw = S.tw_y.p; r.members = mem.p;
     bool transformed = false;
     if (!zero || m.bwl || space == 2) {
         r.flags = ROW_LOAD | ROW_FFT | ROW_STORE;
         check(launch_row_pass(m.ny, r, st.s), "row pass");
-        ColArgs c = S.col_base(w.p, w.p, 1);
+        ColArgsT<Z> c = S.col_base(w.p, w.p, 1);
         if (!zero) c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE;
         else c.flags = COL_LOAD | COL_FFT | (m.bwl ? COL_BWL : 0) | COL_STORE;
         if (zero && !m.bwl) {
@@ -1362,5 +1410,19 @@ void propagate(const Model &m, int device, const double *psi_in, double z, int s
     check(hipStreamSynchronize(st.s), "sync");
     download_unshifted(S, w.p, psi_out);
 }
+
+#define MTG_ENGINE_INSTANTIATE(R)                                                                                      \
+    template void run_multislice<R>(const ModelT<R> &, int, mt_output_multislice *);                                   \
+    template void run_multislice_group<R>(const std::vector<const ModelT<R> *> &, int,                                 \
+                                          const std::vector<mt_output_multislice *> &);                               \
+    template void projected_potential<R>(const ModelT<R> &, int, int, int, double *);                                  \
+    template void projected_potential_sum<R>(const ModelT<R> &, int, int, double *);                                   \
+    template void transmission_function<R>(const ModelT<R> &, int, int, int, double *);                                \
+    template void incident_wave<R>(const ModelT<R> &, int, double *);                                                  \
+    template void propagate<R>(const ModelT<R> &, int, const double *, double, int, double *);
+
+MTG_ENGINE_INSTANTIATE(float)
+MTG_ENGINE_INSTANTIATE(double)
+#undef MTG_ENGINE_INSTANTIATE
 
 } // namespace mtg

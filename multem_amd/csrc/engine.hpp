@@ -44,20 +44,30 @@ RunControl &run_control();
 const char *last_successful_call();   // name of the last HIP call that succeeded on this thread
 Timing &last_timing();
 
-// Full simulation into caller-owned output arrays.
-void run_multislice(const Model &m, int device, mt_output_multislice *out);
+// Full simulation into caller-owned output arrays; R = float runs the FP32
+// wave path, R = double the FP64 one.
+template <class R>
+void run_multislice(const ModelT<R> &m, int device, mt_output_multislice *out);
 
 // Several STEM specimens (grid sizes may differ) run together, one hipGraph
 // launch per round of batches.
-void run_multislice_group(const std::vector<const Model *> &ms, int device, const std::vector<mt_output_multislice *> &outs);
+template <class R>
+void run_multislice_group(const std::vector<const ModelT<R> *> &ms, int device,
+                          const std::vector<mt_output_multislice *> &outs);
 
 // Per-piece entries (natural, unshifted layouts).
-void projected_potential(const Model &m, int device, int iconf, int islice, double *V);
+template <class R>
+void projected_potential(const ModelT<R> &m, int device, int iconf, int islice, double *V);
+template <class R>
+void projected_potential_sum(const ModelT<R> &m, int device, int iconf, double *V);
+template <class R>
+void transmission_function(const ModelT<R> &m, int device, int iconf, int islice, double *trans);
+template <class R>
+void incident_wave(const ModelT<R> &m, int device, double *psi0);
+template <class R>
+void propagate(const ModelT<R> &m, int device, const double *psi_in, double z, int space, double *psi_out);
+
 // read + write bandwidth (GB/s) of a streaming copy of `bytes` bytes, reps times
 double stream_copy_gbs(int device, size_t bytes, int reps);
-void projected_potential_sum(const Model &m, int device, int iconf, double *V);
-void transmission_function(const Model &m, int device, int iconf, int islice, double *trans);
-void incident_wave(const Model &m, int device, double *psi0);
-void propagate(const Model &m, int device, const double *psi_in, double z, int space, double *psi_out);
 
 } // namespace mtg

@@ -14,10 +14,17 @@
 
 #include <hip/hip_runtime.h>
 
+#include "complex_types.hpp"
+
 namespace mtg {
+
+__device__ __forceinline__ float2 cmake(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ double2 cmake(double x, double y) { return make_double2(x, y); }
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 // Complex product with a fixed rounding pattern (the a.y terms rounded, the
 // a.x terms fused), so that every kernel instance rounds alike whatever the
 // compiler's contraction choices.
@@ -27,6 +34,12 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     const float yy = a.y * b.y, yx = a.y * b.x;
     return make_float2(__builtin_fmaf(a.x, b.x, -yy), __builtin_fmaf(a.x, b.y, yx));
 }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+#pragma clang fp contract(off)
+    const double yy = a.y * b.y, yx = a.y * b.x;
+    return make_double2(__builtin_fma(a.x, b.x, -yy), __builtin_fma(a.x, b.y, yx));
+}
 // |a|^2 with a fixed rounding pattern
 __device__ __forceinline__ float cnorm2(float2 a)
 {
@@ -34,29 +47,48 @@ __device__ __forceinline__ float cnorm2(float2 a)
     const float yy = a.y * a.y;
     return __builtin_fmaf(a.x, a.x, yy);
 }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-// multiply by -i (forward) or +i (inverse)
-template <bool INV>
-__device__ __forceinline__ float2 mul_mi(float2 a)
+__device__ __forceinline__ double cnorm2(double2 a)
 {
-    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+#pragma clang fp contract(off)
+    const double yy = a.y * a.y;
+    return __builtin_fma(a.x, a.x, yy);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, class C>
+__device__ __forceinline__ C mul_mi(C a)
+{
+    return INV ? cmake(-a.y, a.x) : cmake(a.y, -a.x);
 }
 
-constexpr float kSqrtHalf = 0.70710678118654752440f;
-
-template <bool INV>
-__device__ __forceinline__ void dft2(float2 &a0, float2 &a1)
+// radix constants written in the precision they are used in
+template <class C> struct DftConst;
+template <> struct DftConst<float2> {
+    static constexpr float h = 0.70710678118654752440f, c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+};
+template <> struct DftConst<double2> {
+    static constexpr double h = 0.70710678118654752440, c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;
+};
+template <class C>
+__device__ __forceinline__ real_t<C> sqrt_half()
 {
-    float2 t = csub(a0, a1);
+    return DftConst<C>::h;
+}
+
+template <bool INV, class C>
+__device__ __forceinline__ void dft2(C &a0, C &a1)
+{
+    C t = csub(a0, a1);
     a0 = cadd(a0, a1);
     a1 = t;
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3)
+template <bool INV, class C>
+__device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3)
 {
-    float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
-    float2 s13 = cadd(a1, a3), d13 = mul_mi<INV>(csub(a1, a3));
+    C s02 = cadd(a0, a2), d02 = csub(a0, a2);
+    C s13 = cadd(a1, a3), d13 = mul_mi<INV>(csub(a1, a3));
     a0 = cadd(s02, s13);
     a2 = csub(s02, s13);
     a1 = cadd(d02, d13);
@@ -64,27 +96,27 @@ __device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 
 }
 
 // W8^k for k = 1, 3 (forward) or their conjugates (inverse)
-template <bool INV>
-__device__ __forceinline__ float2 w8_1(float2 a)
+template <bool INV, class C>
+__device__ __forceinline__ C w8_1(C a)
 {
     // (1 -/+ i)/sqrt2 * a
-    return INV ? make_float2((a.x - a.y) * kSqrtHalf, (a.x + a.y) * kSqrtHalf)
-               : make_float2((a.x + a.y) * kSqrtHalf, (a.y - a.x) * kSqrtHalf);
+    const real_t<C> h = sqrt_half<C>();
+    return INV ? cmake((a.x - a.y) * h, (a.x + a.y) * h) : cmake((a.x + a.y) * h, (a.y - a.x) * h);
 }
-template <bool INV>
-__device__ __forceinline__ float2 w8_3(float2 a)
+template <bool INV, class C>
+__device__ __forceinline__ C w8_3(C a)
 {
     // (-1 -/+ i)/sqrt2 * a
-    return INV ? make_float2((-a.x - a.y) * kSqrtHalf, (a.x - a.y) * kSqrtHalf)
-               : make_float2((a.y - a.x) * kSqrtHalf, (-a.x - a.y) * kSqrtHalf);
+    const real_t<C> h = sqrt_half<C>();
+    return INV ? cmake((-a.x - a.y) * h, (a.x - a.y) * h) : cmake((a.y - a.x) * h, (-a.x - a.y) * h);
 }
 
 // 8-point DFT of v[0], v[S], ..., v[7S], result in natural order
-template <bool INV, int S, int E>
-__device__ __forceinline__ void dft8(float2 (&v)[E], int o)
+template <bool INV, int S, int E, class C>
+__device__ __forceinline__ void dft8(C (&v)[E], int o)
 {
-    float2 e0 = v[o], e1 = v[o + 2 * S], e2 = v[o + 4 * S], e3 = v[o + 6 * S];
-    float2 d0 = v[o + S], d1 = v[o + 3 * S], d2 = v[o + 5 * S], d3 = v[o + 7 * S];
+    C e0 = v[o], e1 = v[o + 2 * S], e2 = v[o + 4 * S], e3 = v[o + 6 * S];
+    C d0 = v[o + S], d1 = v[o + 3 * S], d2 = v[o + 5 * S], d3 = v[o + 7 * S];
     dft4<INV>(e0, e1, e2, e3);
     dft4<INV>(d0, d1, d2, d3);
     d1 = w8_1<INV>(d1);
@@ -97,10 +129,11 @@ __device__ __forceinline__ void dft8(float2 (&v)[E], int o)
 }
 
 // 16-point DFT as 4 x 4 with internal twiddles W16^(q*kp)
-template <bool INV, int S, int E>
-__device__ __forceinline__ void dft16(float2 (&v)[E], int o)
+template <bool INV, int S, int E, class C>
+__device__ __forceinline__ void dft16(C (&v)[E], int o)
 {
-    float2 y[4][4];
+    using R = real_t<C>;
+    C y[4][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         y[q][0] = v[o + q * S];
@@ -110,19 +143,19 @@ __device__ __forceinline__ void dft16(float2 (&v)[E], int o)
         dft4<INV>(y[q][0], y[q][1], y[q][2], y[q][3]);
     }
     // twiddles W16^(q*kp), q, kp in 1..3
-    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
-    const float sg = INV ? 1.f : -1.f;
-    const float2 w1 = make_float2(c1, sg * s1);                // W16^1
-    const float2 w2 = make_float2(kSqrtHalf, sg * kSqrtHalf);  // W16^2
-    const float2 w3 = make_float2(s1, sg * c1);                // W16^3
-    const float2 w6 = make_float2(-kSqrtHalf, sg * kSqrtHalf); // W16^6
-    const float2 w9 = make_float2(-c1, -sg * s1);              // W16^9
+    const R c1 = DftConst<C>::c1, s1 = DftConst<C>::s1, h = DftConst<C>::h;
+    const R sg = INV ? R(1) : R(-1);
+    const C w1 = cmake(c1, sg * s1);     // W16^1
+    const C w2 = cmake(h, sg * h);       // W16^2
+    const C w3 = cmake(s1, sg * c1);     // W16^3
+    const C w6 = cmake(-h, sg * h);      // W16^6
+    const C w9 = cmake(-c1, -sg * s1);   // W16^9
     y[1][1] = cmul(y[1][1], w1); y[1][2] = cmul(y[1][2], w2); y[1][3] = cmul(y[1][3], w3);
     y[2][1] = cmul(y[2][1], w2); y[2][2] = mul_mi<INV>(y[2][2]); y[2][3] = cmul(y[2][3], w6);
     y[3][1] = cmul(y[3][1], w3); y[3][2] = cmul(y[3][2], w6); y[3][3] = cmul(y[3][3], w9);
 #pragma unroll
     for (int kp = 0; kp < 4; ++kp) {
-        float2 a0 = y[0][kp], a1 = y[1][kp], a2 = y[2][kp], a3 = y[3][kp];
+        C a0 = y[0][kp], a1 = y[1][kp], a2 = y[2][kp], a3 = y[3][kp];
         dft4<INV>(a0, a1, a2, a3);
         v[o + kp * S] = a0;          // X[4*kq + kp], kq = 0..3
         v[o + (kp + 4) * S] = a1;
@@ -131,8 +164,8 @@ __device__ __forceinline__ void dft16(float2 (&v)[E], int o)
     }
 }
 
-template <int R, bool INV, int S, int E>
-__device__ __forceinline__ void dftR(float2 (&v)[E], int o)
+template <int R, bool INV, int S, int E, class C>
+__device__ __forceinline__ void dftR(C (&v)[E], int o)
 {
     if constexpr (R == 2) dft2<INV>(v[o], v[o + S]);
     else if constexpr (R == 4) dft4<INV>(v[o], v[o + S], v[o + 2 * S], v[o + 3 * S]);
@@ -177,8 +210,8 @@ struct StageNS<S, 0> { static constexpr int value = 1; };
 // One Stockham stage ST of schedule S.  tw.get<ST>(q, r) is this thread's
 // twiddle W_{NS R}^{r k} of butterfly q (forward sign; conjugated here for
 // the inverse).
-template <class S, int ST, bool INV, class TW>
-__device__ __forceinline__ void fft_stage(float2 (&v)[S::E], const TW &tw)
+template <class S, int ST, bool INV, class TW, class C>
+__device__ __forceinline__ void fft_stage(C (&v)[S::E], const TW &tw)
 {
     constexpr int R = S::R[ST], Q = S::E / R;
     constexpr int NS = StageNS<S, ST>::value;
@@ -187,7 +220,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[S::E], const TW &tw)
         if constexpr (NS > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw.template get<ST>(q, r);
+                C w = tw.template get<ST>(q, r);
                 if (INV) w.y = -w.y;
                 v[q + r * Q] = cmul(v[q + r * Q], w);
             }
@@ -217,14 +250,14 @@ struct TwOffset<S, 0> { static constexpr int value = 0; };
 
 // Twiddles held in registers: loaded once per thread, reused by the forward
 // and the inverse transform.
-template <class S>
+template <class S, class C = float2>
 struct TwReg {
     static constexpr int COUNT = TwOffset<S, S::NST>::value;
-    float2 w[COUNT > 0 ? COUNT : 1];
+    C w[COUNT > 0 ? COUNT : 1];
 
     // tw: W_N^m = exp(-2 pi i m / N), m in [0, N); t: thread index in the sequence
     template <int ST = 0>
-    __device__ __forceinline__ void load(int t, const float2 *__restrict__ tw)
+    __device__ __forceinline__ void load(int t, const C *__restrict__ tw)
     {
         constexpr int R = S::R[ST], NS = StageNS<S, ST>::value;
         if constexpr (NS > 1) {
@@ -239,24 +272,24 @@ struct TwReg {
         if constexpr (ST + 1 < S::NST) load<ST + 1>(t, tw);
     }
     template <int ST>
-    __device__ __forceinline__ float2 get(int q, int r) const
+    __device__ __forceinline__ C get(int q, int r) const
     {
         return w[TwOffset<S, ST>::value + q * (S::R[ST] - 1) + r - 1];
     }
 };
 
 // Twiddles read from the cache-resident table at each use: fewer registers.
-template <class S>
+template <class S, class C = float2>
 struct TwTable {
-    const float2 *__restrict__ tab;
+    const C *__restrict__ tab;
     int t;
-    __device__ __forceinline__ void load(int t_, const float2 *__restrict__ tw)
+    __device__ __forceinline__ void load(int t_, const C *__restrict__ tw)
     {
         t = t_;
         tab = tw;
     }
     template <int ST>
-    __device__ __forceinline__ float2 get(int q, int r) const
+    __device__ __forceinline__ C get(int q, int r) const
     {
         constexpr int R = S::R[ST], NS = StageNS<S, ST>::value;
         return tab[r * ((t + S::T * q) & (NS - 1)) * (S::N / (NS * R))];
@@ -266,9 +299,8 @@ struct TwTable {
 // Full transform.  lds_idx(i) maps a sequence index to this sequence's LDS
 // slot; sync() is the barrier that orders LDS writes and reads of all threads
 // sharing the buffer.  The data starts and ends in the t + T*e layout.
-template <class S, bool INV, int ST = 0, class TW, class LdsIdx, class Sync>
-__device__ __forceinline__ void fft_run(float2 (&v)[S::E], int t, const TW &tw, float2 *lds, LdsIdx lds_idx,
-                                        Sync sync)
+template <class S, bool INV, int ST = 0, class TW, class LdsIdx, class Sync, class C>
+__device__ __forceinline__ void fft_run(C (&v)[S::E], int t, const TW &tw, C *lds, LdsIdx lds_idx, Sync sync)
 {
     fft_stage<S, ST, INV>(v, tw);
     if constexpr (ST + 1 < S::NST) {

@@ -1,7 +1,8 @@
 // Host model: input reading and validation, spline tables, slicing, phonons.
 //
-// Follows the reference's FP32 path (Input_Multislice<float>): quantities the
-// reference keeps in T are float here, tables are built in double and cast.
+// Follows Input_Multislice<T> of the reference: quantities the reference keeps
+// in T are R here (float for the FP32 path, double for the FP64 path); tables
+// are built in double and cast.
 #include "host_model.hpp"
 
 #include <algorithm>
@@ -30,10 +31,13 @@ constexpr double kMmToAngs = 1.0e7;
 // isEqual<float> / isEqual<double> (src/types.cuh:562-592)
 bool near_zero(float a) { return std::fabs(a) <= 1e-5f; }
 bool near_zero(double a) { return std::fabs(a) <= 1e-13; }
-bool same_value(float a, float b)
+template <class R>
+bool same_value(R a, R b)
 {
-    float d = std::fabs(a - b);
-    return d <= 1e-5f || d <= std::max(std::fabs(a), std::fabs(b)) * 1e-4f;
+    // isEqual<T> (src/types.cuh:562-592)
+    const R eps_abs = sizeof(R) == 4 ? R(1e-5) : R(1e-13), eps_rel = sizeof(R) == 4 ? R(1e-4) : R(1e-8);
+    R d = std::fabs(a - b);
+    return d <= eps_abs || d <= std::max(std::fabs(a), std::fabs(b)) * eps_rel;
 }
 
 // Polynomial approximations of I0, I1, K0, K1: Abramowitz & Stegun 9.8.1-9.8.8.
@@ -110,9 +114,10 @@ double vr3(const Terms &t, double r)
 
 } // namespace
 
-AtomTable build_atom_table(int Z, double dR_min, double Vrl, int nR)
+template <class T>
+AtomTableT<T> build_atom_table(int Z, double dR_min, double Vrl, int nR)
 {
-    AtomTable tb;
+    AtomTableT<T> tb;
     tb.Z = Z;
     Terms vr, VR;
     lobato_terms(Z, vr, VR);
@@ -164,70 +169,77 @@ AtomTable build_atom_table(int Z, double dR_min, double Vrl, int nR)
         y[i] = v;
         dy[i] = d;
     }
-    tb.R2.resize(nR); tb.c0.resize(nR); tb.c1.resize(nR); tb.c2.assign(nR, 0.f); tb.c3.assign(nR, 0.f);
+    tb.R2.resize(nR); tb.c0.resize(nR); tb.c1.resize(nR); tb.c2.assign(nR, T(0)); tb.c3.assign(nR, T(0));
     for (int i = 0; i < nR; ++i) {
-        tb.R2[i] = (float)R2[i];
-        tb.c0[i] = (float)y[i];
-        tb.c1[i] = (float)dy[i];
+        tb.R2[i] = (T)R2[i];
+        tb.c0[i] = (T)y[i];
+        tb.c1[i] = (T)dy[i];
     }
     for (int i = 0; i + 1 < nR; ++i) {   // Hermite cubic on [R2_i, R2_i+1]
         double h = 1.0 / (R2[i + 1] - R2[i]);
         double m = (y[i + 1] - y[i]) * h, n = dy[i] + dy[i + 1];
-        tb.c2[i] = (float)((3.0 * m - n - dy[i]) * h);
-        tb.c3[i] = (float)((n - 2.0 * m) * h * h);
+        tb.c2[i] = (T)((3.0 * m - n - dy[i]) * h);
+        tb.c3[i] = (T)((n - 2.0 * m) * h * h);
     }
-    tb.R_max_f = (float)R_max;
+    tb.R_max_f = (T)R_max;
     tb.R2_max_f = tb.R_max_f * tb.R_max_f;
     return tb;
 }
 
+template AtomTableT<float> build_atom_table<float>(int, double, double, int);
+template AtomTableT<double> build_atom_table<double>(int, double, double, int);
+
 // ---------------------------------------------------------------------------
 namespace {
 
-float lambda_f(float E_0)                 // get_lambda<float> (src/cgpu_fcns.cuh:51-58)
+template <class R>
+R lambda_of(R E_0)                         // get_lambda<T> (src/cgpu_fcns.cuh:51-58)
 {
-    float emass = 510.99906f, hc = 12.3984244f;
+    R emass = R(510.99906), hc = R(12.3984244);
     return hc / std::sqrt(E_0 * (2 * emass + E_0));
 }
 
-void read_lens(const mt_lens &m, LensModel &L)
+template <class R>
+void read_lens(const mt_lens &m, LensModelT<R> &L)
 {
     L.m = (int)m.m;
-    L.c_10 = (float)m.c_10; L.c_12 = (float)m.c_12; L.phi_12 = (float)(m.phi_12 * kDegToRad);
-    L.c_21 = (float)m.c_21; L.phi_21 = (float)(m.phi_21 * kDegToRad);
-    L.c_23 = (float)m.c_23; L.phi_23 = (float)(m.phi_23 * kDegToRad);
-    L.c_30 = (float)(m.c_30 * kMmToAngs); L.c_32 = (float)m.c_32; L.phi_32 = (float)(m.phi_32 * kDegToRad);
-    L.c_34 = (float)m.c_34; L.phi_34 = (float)(m.phi_34 * kDegToRad);
-    L.c_41 = (float)m.c_41; L.phi_41 = (float)(m.phi_41 * kDegToRad);
-    L.c_43 = (float)m.c_43; L.phi_43 = (float)(m.phi_43 * kDegToRad);
-    L.c_45 = (float)m.c_45; L.phi_45 = (float)(m.phi_45 * kDegToRad);
-    L.c_50 = (float)(m.c_50 * kMmToAngs); L.c_52 = (float)m.c_52; L.phi_52 = (float)(m.phi_52 * kDegToRad);
-    L.c_54 = (float)m.c_54; L.phi_54 = (float)(m.phi_54 * kDegToRad);
-    L.c_56 = (float)m.c_56; L.phi_56 = (float)(m.phi_56 * kDegToRad);
-    L.inner_aper_ang = (float)(m.inner_aper_ang * kMradToRad);
-    L.outer_aper_ang = (float)(m.outer_aper_ang * kMradToRad);
+    L.c_10 = (R)m.c_10; L.c_12 = (R)m.c_12; L.phi_12 = (R)(m.phi_12 * kDegToRad);
+    L.c_21 = (R)m.c_21; L.phi_21 = (R)(m.phi_21 * kDegToRad);
+    L.c_23 = (R)m.c_23; L.phi_23 = (R)(m.phi_23 * kDegToRad);
+    L.c_30 = (R)(m.c_30 * kMmToAngs); L.c_32 = (R)m.c_32; L.phi_32 = (R)(m.phi_32 * kDegToRad);
+    L.c_34 = (R)m.c_34; L.phi_34 = (R)(m.phi_34 * kDegToRad);
+    L.c_41 = (R)m.c_41; L.phi_41 = (R)(m.phi_41 * kDegToRad);
+    L.c_43 = (R)m.c_43; L.phi_43 = (R)(m.phi_43 * kDegToRad);
+    L.c_45 = (R)m.c_45; L.phi_45 = (R)(m.phi_45 * kDegToRad);
+    L.c_50 = (R)(m.c_50 * kMmToAngs); L.c_52 = (R)m.c_52; L.phi_52 = (R)(m.phi_52 * kDegToRad);
+    L.c_54 = (R)m.c_54; L.phi_54 = (R)(m.phi_54 * kDegToRad);
+    L.c_56 = (R)m.c_56; L.phi_56 = (R)(m.phi_56 * kDegToRad);
+    L.inner_aper_ang = (R)(m.inner_aper_ang * kMradToRad);
+    L.outer_aper_ang = (R)(m.outer_aper_ang * kMradToRad);
     L.zero_defocus_type = m.zero_defocus_type;
-    L.zero_defocus_plane = (float)m.zero_defocus_plane;
+    L.zero_defocus_plane = (R)m.zero_defocus_plane;
 }
 
 // aperture bounds of Lens::set_input_data (src/types.cuh:2924-2976)
-void set_lens(LensModel &L, float E_0, float g_max)
+template <class R>
+void set_lens(LensModelT<R> &L, R E_0, R g_max)
 {
-    L.lambda = lambda_f(E_0);
+    L.lambda = lambda_of<R>(E_0);
     L.g2_min = (near_zero(L.inner_aper_ang) || L.inner_aper_ang < 0)
-                   ? 0.f : (float)std::pow(std::sin(L.inner_aper_ang) / L.lambda, 2);
+                   ? R(0.0) : (R)std::pow(std::sin(L.inner_aper_ang) / L.lambda, 2);
     L.g2_max = (near_zero(L.outer_aper_ang) || L.outer_aper_ang < 0)
-                   ? g_max * g_max : (float)std::pow(std::sin(L.outer_aper_ang) / L.lambda, 2);
-    const float c[] = {(float)L.m, L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
+                   ? g_max * g_max : (R)std::pow(std::sin(L.outer_aper_ang) / L.lambda, 2);
+    const R c[] = {(R)L.m, L.c_12, L.c_21, L.c_23, L.c_32, L.c_34, L.c_41, L.c_43, L.c_45, L.c_52, L.c_54, L.c_56};
     L.phi_required = false;
-    for (float v : c) L.phi_required = L.phi_required || !near_zero(v);
+    for (R v : c) L.phi_required = L.phi_required || !near_zero(v);
 }
 
-float zero_defocus_plane(const LensModel &L, float z_min, float z_max)
+template <class R>
+R zero_defocus_plane(const LensModelT<R> &L, R z_min, R z_max)
 {
     switch (L.zero_defocus_type) {
     case 1: return z_min;
-    case 2: return 0.5f * (z_min + z_max);
+    case 2: return R(0.5) * (z_min + z_max);
     case 3: return z_max;
     default: return L.zero_defocus_plane;
     }
@@ -235,16 +247,17 @@ float zero_defocus_plane(const LensModel &L, float z_min, float z_max)
 
 // Plane finder of Identify_Planes (src/types.cuh:1166-1372): a 0.1 A histogram
 // whose one-bin gaps inside a layer are closed, then Kahan means per layer.
-std::vector<float> find_planes(std::vector<float> v)
+template <class R>
+std::vector<R> find_planes(std::vector<R> v)
 {
-    std::vector<float> planes;
+    std::vector<R> planes;
     if (v.empty()) return planes;
     std::sort(v.begin(), v.end());
     const double dv = 0.1;
     const double v_min = v.front(), v_max = v.back();
     int nb = (int)std::ceil(std::max(v_max - v_min, dv) / dv);
     std::vector<int> h(nb, 0);
-    for (float f : v) {
+    for (R f : v) {
         double x = f;
         int ih = (int)std::floor((x - v_min) / dv);
         // move to the bin [v_min + ih dv, v_min + (ih+1) dv) that holds x
@@ -269,21 +282,21 @@ std::vector<float> find_planes(std::vector<float> v)
         if (h[i] == 0 && h[i - 1] > 0 && (h[i + 1] > 0 || ahead)) h[i] = 1;
     }
     if (nb == 1) {
-        float s = 0;
-        for (float f : v) s += f;
-        planes.push_back(s / float(v.size()));
+        R s = 0;
+        for (R f : v) s += f;
+        planes.push_back(s / R(v.size()));
         return planes;
     }
-    std::vector<float> lim;
+    std::vector<R> lim;
     for (int i = 0; i + 1 < nb; ++i)
-        if (h[i] > 0 && h[i + 1] == 0) lim.push_back((float)(v.front() + (i + 1) * dv));
-    lim.push_back((float)(v.back() + dv));
-    float mean = v.front(), carry = 0;
+        if (h[i] > 0 && h[i + 1] == 0) lim.push_back((R)(v.front() + (i + 1) * dv));
+    lim.push_back((R)(v.back() + dv));
+    R mean = v.front(), carry = 0;
     int cnt = 1;
     size_t il = 0;
-    for (float f : v) {
+    for (R f : v) {
         if (f < lim[il]) {
-            float y = f - carry, t = mean + y;   // Kahan summation
+            R y = f - carry, t = mean + y;   // Kahan summation
             carry = (t - mean) - y;
             mean = t;
             ++cnt;
@@ -297,46 +310,51 @@ std::vector<float> find_planes(std::vector<float> v)
 }
 
 // evenly spaced planes with matched borders (Identify_Planes(v_min, v_max, dv), eMB_MinMax)
-std::vector<float> even_planes(float v_min, float v_max, float dv)
+template <class R>
+std::vector<R> even_planes(R v_min, R v_max, R dv)
 {
-    const float eps = 1e-4f;
-    if (std::fabs(v_max - v_min) < eps) return {0.5f * (v_min + v_max)};
+    const R eps = R(1e-4);
+    if (std::fabs(v_max - v_min) < eps) return {R(0.5) * (v_min + v_max)};
     if (v_max < v_min) return {};
-    float s = v_max - v_min;
+    R s = v_max - v_min;
     int nv = std::max(1, (int)std::floor(s / dv + eps)) + 1;
-    float db = dv + 0.5f * (s - (nv - 1) * dv);
-    std::vector<float> p(nv);
+    R db = dv + R(0.5) * (s - (nv - 1) * dv);
+    std::vector<R> p(nv);
     p[0] = v_min;
     for (int i = 1; i < nv; ++i) p[i] = p[i - 1] + ((i == 1 || i == nv - 1) ? db : dv);
     return p;
 }
 
-void unique_sorted(std::vector<float> &v)
+template <class R>
+void unique_sorted(std::vector<R> &v)
 {
     std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end(), same_value), v.end());
+    v.erase(std::unique(v.begin(), v.end(), same_value<R>), v.end());
 }
 
-float spacing(const std::vector<float> &p, int i, float dz)
+template <class R>
+R spacing(const std::vector<R> &p, int i, R dz)
 {
     if (p.size() == 1) return dz;
     i = std::max(1, std::min(i, (int)p.size() - 1));
     return p[i] - p[i - 1];
 }
 
-std::vector<float> slice_bounds(const std::vector<float> &p, float dz, float z_int_min, float z_int_max)
+template <class R>
+std::vector<R> slice_bounds(const std::vector<R> &p, R dz, R z_int_min, R z_int_max)
 {
     if (p.size() == 1) return {z_int_min, z_int_max};
-    std::vector<float> zs(p.size() + 1);
-    zs[0] = p[0] - 0.5f * spacing(p, 0, dz);
-    for (size_t i = 1; i + 1 < zs.size(); ++i) zs[i] = 0.5f * (p[i] + p[i - 1]);
+    std::vector<R> zs(p.size() + 1);
+    zs[0] = p[0] - R(0.5) * spacing(p, 0, dz);
+    for (size_t i = 1; i + 1 < zs.size(); ++i) zs[i] = R(0.5) * (p[i] + p[i - 1]);
     zs.back() = zs[zs.size() - 2] + spacing(p, (int)p.size(), dz);
     return zs;
 }
 
 } // namespace
 
-Model::Model(const mt_input_multislice &in)
+template <class R>
+ModelT<R>::ModelT(const mt_input_multislice &in)
 {
     sim_type = in.simulation_type;
     if (!(is_stem() || is_cbed() || is_hrtem() || is_ewfs() || is_ewrs()))
@@ -378,35 +396,35 @@ Model::Model(const mt_input_multislice &in)
     if (nx <= 0 || ny <= 0) fail(MT_ERR_INVALID_ARGUMENT, "nx, ny must be positive");
     if (!pow2(nx) || !pow2(ny)) fail(MT_ERR_UNSUPPORTED, "nx, ny must be powers of two in [64, 2048]");
     nxh = nx / 2; nyh = ny / 2;
-    lx = (float)in.spec_lx; ly = (float)in.spec_ly;
+    lx = (R)in.spec_lx; ly = (R)in.spec_ly;
     lx_d = in.spec_lx; ly_d = in.spec_ly;
     if (!(lx > 0 && ly > 0)) fail(MT_ERR_INVALID_ARGUMENT, "spec_lx, spec_ly must be positive");
     dRx = lx / nx; dRy = ly / ny;
-    dgx = 1.0f / lx; dgy = 1.0f / ly;
+    dgx = R(1.0) / lx; dgy = R(1.0) / ly;
     bwl = in.bwl != 0;
-    const float g_max = std::fmin((float)nxh * dgx, (float)nyh * dgy);
-    const float gl_max = 2.0f * g_max / 3.0f;
+    const R g_max = std::fmin((R)nxh * dgx, (R)nyh * dgy);
+    const R gl_max = R(2.0) * g_max / R(3.0);
     gl2_max = gl_max * gl_max;
-    bwl_alpha = (float)(std::log(1.0 / 1e-2 - 1.0) / ((gl_max + 0.25f) * (gl_max + 0.25f) - gl2_max));
+    bwl_alpha = (R)(std::log(1.0 / 1e-2 - 1.0) / ((gl_max + R(0.25)) * (gl_max + R(0.25)) - gl2_max));
 
     iw_type = in.iw_type;
     if (iw_type == 4) iw_type = (is_stem() || is_cbed()) ? 2 : 1;
     if (iw_type == 3) fail(MT_ERR_UNSUPPORTED, "user-defined incident wave");
     if (iw_type != 1 && iw_type != 2) fail(MT_ERR_INVALID_ARGUMENT, "iw_type must be 1..4");
-    for (int i = 0; i < in.n_iw_xy; ++i) { iw_x.push_back((float)in.iw_x[i]); iw_y.push_back((float)in.iw_y[i]); }
+    for (int i = 0; i < in.n_iw_xy; ++i) { iw_x.push_back((R)in.iw_x[i]); iw_y.push_back((R)in.iw_y[i]); }
     if (!is_stem() && iw_type == 2 && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y");
 
-    E_0 = (float)in.E_0;
+    E_0 = (R)in.E_0;
     if (!(E_0 > 0)) fail(MT_ERR_INVALID_ARGUMENT, "E_0 must be positive");
-    theta = (float)(in.theta * kDegToRad);
-    phi = (float)(in.phi * kDegToRad);
-    if (std::fabs(theta) >= 0.5f * (float)kPi) fail(MT_ERR_INVALID_ARGUMENT, "theta must be below 90 degrees");
-    lambda = lambda_f(E_0);
+    theta = (R)(in.theta * kDegToRad);
+    phi = (R)(in.phi * kDegToRad);
+    if (std::fabs(theta) >= R(0.5) * (R)kPi) fail(MT_ERR_INVALID_ARGUMENT, "theta must be below 90 degrees");
+    lambda = lambda_of<R>(E_0);
     // Input_Multislice::set_incident_angle (src/input_multislice.cuh:891-895):
     // sin(theta) quantised to a multiple of lambda*dg_min
-    if (theta != 0.f) {
-        const float dgm = std::fmin(dgx, dgy);
-        const float nq = std::round(std::sin(theta) / (lambda * dgm));
+    if (theta != R(0.0)) {
+        const R dgm = std::fmin(dgx, dgy);
+        const R nq = std::round(std::sin(theta) / (lambda * dgm));
         theta = std::asin(nq * lambda * dgm);
     }
     // gx_0, gy_0 (src/input_multislice.cuh:919-927)
@@ -418,9 +436,9 @@ Model::Model(const mt_input_multislice &in)
     }
     mul_sign = in.reverse_multislice ? -1 : 1;
     {
-        float emass = 510.99906f;
-        float gamma = 1 + E_0 / emass;
-        float cpotf = (float)kPotf;
+        R emass = R(510.99906);
+        R gamma = 1 + E_0 / emass;
+        R cpotf = (R)kPotf;
         Vr_factor = mul_sign * (gamma * lambda / (cpotf * std::cos(theta)));
     }
 
@@ -429,8 +447,8 @@ Model::Model(const mt_input_multislice &in)
     read_lens(in.obj_lens, obj);
     // Lens::set_ti_sigma / set_si_sigma (src/types.cuh:2978-2989); the objective
     // lens reads si_sigma from cond_lens_si_sigma (ilc_multem.cu:247)
-    obj.ti_iehwgd = (float)kSqrt2 * std::fmax(0.f, (float)in.obj_lens.ti_sigma);
-    obj.si_theta_c = std::asin((float)kSqrt2 * std::fmax(0.f, (float)in.cond_lens.si_sigma) * lambda);
+    obj.ti_iehwgd = (R)kSqrt2 * std::fmax(R(0.0), (R)in.obj_lens.ti_sigma);
+    obj.si_theta_c = std::asin((R)kSqrt2 * std::fmax(R(0.0), (R)in.cond_lens.si_sigma) * lambda);
     set_lens(obj, E_0, g_max);
 
     // scanning (Scanning::set_grid, src/types.cuh:4294-4389)
@@ -438,14 +456,14 @@ Model::Model(const mt_input_multislice &in)
         int ns = in.scanning_ns;
         bool pbc = in.scanning_periodic != 0, spxs = in.scanning_square_pxs != 0;
         if (ns <= 0) fail(MT_ERR_INVALID_ARGUMENT, "scanning_ns must be positive");
-        float x0 = (float)in.scanning_x0, y0 = (float)in.scanning_y0;
-        float xe = (float)in.scanning_xe, ye = (float)in.scanning_ye;
-        float xu = xe - x0, yu = ye - y0;
+        R x0 = (R)in.scanning_x0, y0 = (R)in.scanning_y0;
+        R xe = (R)in.scanning_xe, ye = (R)in.scanning_ye;
+        R xu = xe - x0, yu = ye - y0;
         if (in.scanning_type == 1) {
-            float ds = std::sqrt(yu * yu + xu * xu), th = std::atan2(yu, xu);
-            float ct = std::cos(th);
-            ct = near_zero(ct) ? 0.f : ct;
-            float st = std::sin(th);
+            R ds = std::sqrt(yu * yu + xu * xu), th = std::atan2(yu, xu);
+            R ct = std::cos(th);
+            ct = near_zero(ct) ? R(0.0) : ct;
+            R st = std::sin(th);
             sc_dRx = ds * ct / (pbc ? ns : (ns - 1));
             sc_dRy = ds * st / (pbc ? ns : (ns - 1));
             sc_nx = ns; sc_ny = 1;
@@ -455,12 +473,12 @@ Model::Model(const mt_input_multislice &in)
             if (std::fabs(xu) > std::fabs(yu)) {
                 sc_dRx = xu / (pbc ? ns : (ns - 1));
                 sc_dRy = std::copysign(sc_dRx, yu);
-                sc_ny = (int)std::floor(yu / sc_dRy + 1e-4f + 0.5f) + (pbc ? 0 : 1);
+                sc_ny = (int)std::floor(yu / sc_dRy + R(1e-4) + R(0.5)) + (pbc ? 0 : 1);
                 if (!spxs) sc_dRy = yu / (pbc ? sc_ny : (sc_ny - 1));
             } else {
                 sc_dRy = yu / (pbc ? ns : (ns - 1));
                 sc_dRx = std::copysign(sc_dRy, xu);
-                sc_nx = (int)std::floor(xu / sc_dRx + 1e-4f + 0.5f) + (pbc ? 0 : 1);
+                sc_nx = (int)std::floor(xu / sc_dRx + R(1e-4) + R(0.5)) + (pbc ? 0 : 1);
                 if (!spxs) sc_dRx = xu / (pbc ? sc_nx : (sc_nx - 1));
             }
             if (sc_nx <= 0 || sc_ny <= 0) fail(MT_ERR_INVALID_ARGUMENT, "empty scan grid");
@@ -474,17 +492,17 @@ Model::Model(const mt_input_multislice &in)
         if (in.detector_type != 1) fail(MT_ERR_UNSUPPORTED, "only circular detectors");
         if (in.n_detector <= 0 || !in.detector_cir) fail(MT_ERR_INVALID_ARGUMENT, "STEM needs at least one detector");
         for (int i = 0; i < in.n_detector; ++i) {
-            float gi = std::sin((float)(in.detector_cir[i].inner_ang * kMradToRad)) / lambda;
-            float go = std::sin((float)(in.detector_cir[i].outer_ang * kMradToRad)) / lambda;
-            det_g2_min.push_back((float)std::pow(gi, 2));
-            det_g2_max.push_back((float)std::pow(go, 2));
+            R gi = std::sin((R)(in.detector_cir[i].inner_ang * kMradToRad)) / lambda;
+            R go = std::sin((R)(in.detector_cir[i].outer_ang * kMradToRad)) / lambda;
+            det_g2_min.push_back((R)std::pow(gi, 2));
+            det_g2_max.push_back((R)std::pow(go, 2));
         }
     }
 
     // atoms (Atom_Data::set_atoms, src/atomic_data_mt.hpp:320-350)
     const int na = in.spec_atoms_rows, nc = in.spec_atoms_cols;
     if (na <= 0 || !in.spec_atoms || nc < 4) fail(MT_ERR_INVALID_ARGUMENT, "spec_atoms must be n x (4..8), n > 0");
-    Specimen in_atoms;
+    SpecimenT<R> in_atoms;
     for (int i = 0; i < na; ++i) {
         const double *A = in.spec_atoms;
         int Zi = (int)A[i];
@@ -492,14 +510,14 @@ Model::Model(const mt_input_multislice &in)
         if (Zi % 1000 < 1 || Zi % 1000 > 103) fail(MT_ERR_INVALID_ARGUMENT, "atomic number out of range 1..103");
         if (charge != 0) fail(MT_ERR_UNSUPPORTED, "ionic charges");
         in_atoms.Z.push_back(Zi);
-        in_atoms.x.push_back((float)A[na + i]);
-        in_atoms.y.push_back((float)A[2 * na + i]);
-        in_atoms.z.push_back((float)A[3 * na + i]);
-        in_atoms.sigma.push_back((float)(nc > 4 ? A[4 * na + i] : 0.085));
+        in_atoms.x.push_back((R)A[na + i]);
+        in_atoms.y.push_back((R)A[2 * na + i]);
+        in_atoms.z.push_back((R)A[3 * na + i]);
+        in_atoms.sigma.push_back((float)(nc > 4 ? A[4 * na + i] : 0.085));  // float in Atom_Data
         in_atoms.occ.push_back((float)(nc > 5 ? A[5 * na + i] : 1.0));
         in_atoms.region.push_back(std::abs((int)(nc > 6 ? A[6 * na + i] : 0)));
     }
-    const float spec_dz = (float)in.spec_dz;
+    const R spec_dz = (R)in.spec_dz;
 
     // validate_parameters (src/input_multislice.cuh:622-796)
     pn_seed = std::max(1, pn_seed);
@@ -508,11 +526,11 @@ Model::Model(const mt_input_multislice &in)
     if (is_ewfs() || is_ewrs()) pn_coh_contrib = 1;
     pn_z = false;
 
-    auto zminmax = [](const Specimen &s, float &lo, float &hi) {
+    auto zminmax = [](const SpecimenT<R> &s, R &lo, R &hi) {
         auto mm = std::minmax_element(s.z.begin(), s.z.end());
         lo = *mm.first; hi = *mm.second;
     };
-    float zmin, zmax;
+    R zmin, zmax;
     zminmax(in_atoms, zmin, zmax);
 
     // thicknesses matched to planes (Slicing::match_thickness, src/slicing.hpp:53-71)
@@ -521,36 +539,36 @@ Model::Model(const mt_input_multislice &in)
         thick = {zmax};
     } else {
         if (thick_type != 2 && thick_type != 3) fail(MT_ERR_INVALID_ARGUMENT, "thick_type must be 1..3");
-        std::vector<float> zc;
+        std::vector<R> zc;
         for (int i = 0; i < na; ++i) if (in_atoms.region[i] == 0) zc.push_back(in_atoms.z[i]);
         std::sort(zc.begin(), zc.end());
-        std::vector<float> zp = (in.potential_slicing == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
+        std::vector<R> zp = (in.potential_slicing == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
         unique_sorted(zp);
-        if (thick_type == 3) zp = slice_bounds(zp, spec_dz, zmin - 2.5f, zmax + 2.5f);
-        std::vector<float> req;
-        for (int i = 0; i < in.n_thick; ++i) req.push_back((float)in.thick[i]);
+        if (thick_type == 3) zp = slice_bounds(zp, spec_dz, zmin - R(2.5), zmax + R(2.5));
+        std::vector<R> req;
+        for (int i = 0; i < in.n_thick; ++i) req.push_back((R)in.thick[i]);
         // match_vectors (src/cpu_fcns.hpp:3613-3654): each request snaps to the
         // nearest plane (ties to the lower), each plane used once
-        float dv = std::fmax(0.25f * spec_dz, 0.1f);
-        float lo = *std::min_element(zp.begin(), zp.end()) - dv, hi = *std::max_element(zp.begin(), zp.end()) + dv;
+        R dv = std::fmax(R(0.25) * spec_dz, R(0.1));
+        R lo = *std::min_element(zp.begin(), zp.end()) - dv, hi = *std::max_element(zp.begin(), zp.end()) + dv;
         std::vector<bool> used(zp.size(), false);
-        for (float r : req) {
+        for (R r : req) {
             if (r < lo || r > hi) continue;
             size_t best = 0;
             for (size_t k = 1; k < zp.size(); ++k)
-                if (std::fabs(zp[k] - r - 1e-4f) < std::fabs(zp[best] - r)) best = k;
+                if (std::fabs(zp[k] - r - R(1e-4)) < std::fabs(zp[best] - r)) best = k;
             if (!used[best]) { used[best] = true; thick.push_back(zp[best]); }
         }
         if (thick.empty()) fail(MT_ERR_INVALID_ARGUMENT, "no requested thickness lies inside the specimen");
     }
     // drop atoms beyond the last thickness
     {
-        float z_e = thick.back() + 0.1f;
+        R z_e = thick.back() + R(0.1);
         if (zmax > z_e) {
-            float z_0 = zmin - 0.1f;
-            Specimen t;
+            R z_0 = zmin - R(0.1);
+            SpecimenT<R> t;
             for (int i = 0; i < (int)in_atoms.Z.size(); ++i) {
-                float z = in_atoms.z[i];
+                R z = in_atoms.z[i];
                 if (z_0 < z && z < z_e) {
                     t.Z.push_back(in_atoms.Z[i]); t.x.push_back(in_atoms.x[i]); t.y.push_back(in_atoms.y[i]);
                     t.z.push_back(z); t.sigma.push_back(in_atoms.sigma[i]); t.occ.push_back(in_atoms.occ[i]);
@@ -578,7 +596,7 @@ Model::Model(const mt_input_multislice &in)
 
     // Spec::set_input_data (src/spec.hpp:44-82): periodic de-duplication,
     // stable z sort, tables of the elements present
-    const float lx_b = lx - 1e-4f, ly_b = ly - 1e-4f;
+    const R lx_b = lx - R(1e-4), ly_b = ly - R(1e-4);
     std::vector<int> keep;
     for (int i = 0; i < (int)in_atoms.Z.size(); ++i)
         if (in_atoms.x[i] < lx_b && in_atoms.y[i] < ly_b) keep.push_back(i);
@@ -592,11 +610,11 @@ Model::Model(const mt_input_multislice &in)
     z_types = atoms_u.Z;
     std::sort(z_types.begin(), z_types.end());
     z_types.erase(std::unique(z_types.begin(), z_types.end()), z_types.end());
-    // The FP32 path stores Vrl and dR_min as float and widens them for the
+    // Vrl and dR_min are T in the reference and widened for the
     // double-precision table builder (Spec::set_input_data, src/spec.hpp:51-55).
     const double dR_min = std::fmin(dRx, dRy);
-    const double Vrl = (double)0.015f;
-    for (int Zt : z_types) tables.push_back(build_atom_table(Zt, dR_min, Vrl));
+    const double Vrl = (double)R(0.015);
+    for (int Zt : z_types) tables.push_back(build_atom_table<R>(Zt, dR_min, Vrl));
     for (int Zi : atoms_u.Z)
         type_of_atom.push_back((int)(std::lower_bound(z_types.begin(), z_types.end(), Zi) - z_types.begin()));
     R_int_max = 0;
@@ -604,24 +622,24 @@ Model::Model(const mt_input_multislice &in)
 
     // planes of the undisplaced crystal
     {
-        std::vector<float> zc;
+        std::vector<R> zc;
         for (size_t i = 0; i < atoms_u.z.size(); ++i) if (atoms_u.region[i] == 0) zc.push_back(atoms_u.z[i]);
         z_plane = (in.potential_slicing == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
         unique_sorted(z_plane);
     }
-    float uzmin, uzmax;
+    R uzmin, uzmax;
     zminmax(atoms_u, uzmin, uzmax);
-    const float s_z_int = (uzmax + R_int_max) - (uzmin - R_int_max);
-    if (s_z_int < 2.0f * spec_dz || (z_plane.size() == 1 && in.potential_slicing == 1))
+    const R s_z_int = (uzmax + R_int_max) - (uzmin - R_int_max);
+    if (s_z_int < R(2.0) * spec_dz || (z_plane.size() == 1 && in.potential_slicing == 1))
         fail(MT_ERR_UNSUPPORTED, "specimen reduces to a single phase object");
 
     // slices and thicknesses (Slicing::calculate, src/slicing.hpp:93-100 / 253-403)
-    std::vector<float> zs = slice_bounds(z_plane, spec_dz, uzmin - R_int_max, uzmax + R_int_max);
-    const std::vector<float> &az = atoms_u.z;
-    auto lower = [&](float v) { return (int)(std::lower_bound(az.begin(), az.end(), v) - az.begin()); };
+    std::vector<R> zs = slice_bounds(z_plane, spec_dz, uzmin - R_int_max, uzmax + R_int_max);
+    const std::vector<R> &az = atoms_u.z;
+    auto lower = [&](R v) { return (int)(std::lower_bound(az.begin(), az.end(), v) - az.begin()); };
     for (size_t is = 0; is + 1 < zs.size(); ++is) {
-        SliceInfo s{zs[is], zs[is + 1], 1, 0, -1};
-        float z0 = s.z_0, ze = s.z_e;
+        SliceInfoT<R> s{zs[is], zs[is + 1], 1, 0, -1};
+        R z0 = s.z_0, ze = s.z_e;
         if (!(z0 > ze || az.back() < z0 || ze < az.front())) {
             int i0 = (z0 <= az.front()) ? 0 : lower(z0);
             int ie = (az.back() <= ze) ? (int)az.size() - 1 : lower(ze) - 1;
@@ -629,37 +647,39 @@ Model::Model(const mt_input_multislice &in)
         }
         slices.push_back(s);
     }
-    for (float t : thick) {
-        ThickInfo ti{t, 0, 0.f};
+    for (R t : thick) {
+        ThickInfoT<R> ti{t, 0, R(0.0)};
         if (thick_type == 3) {
-            for (size_t i = 0; i + 1 < zs.size(); ++i) if (std::fabs(zs[i + 1] - t) < 1e-4f) { ti.islice = (int)i; break; }
+            for (size_t i = 0; i + 1 < zs.size(); ++i) if (std::fabs(zs[i + 1] - t) < R(1e-4)) { ti.islice = (int)i; break; }
         } else {
             for (size_t i = 0; i + 1 < zs.size(); ++i) if (zs[i] < t && t <= zs[i + 1]) { ti.islice = (int)i; break; }
         }
-        float ze = zs[ti.islice + 1];
+        R ze = zs[ti.islice + 1];
         int iatom_e = (az.back() <= ze) ? (int)az.size() - 1 : lower(ze) - 1;
         if (iatom_e < 0) iatom_e = 0;
-        float zdp = zero_defocus_plane(obj, az[0], az[iatom_e]);
-        if (thick_type == 3) ti.z_back_prop = 0.f;
+        R zdp = zero_defocus_plane(obj, az[0], az[iatom_e]);
+        if (thick_type == 3) ti.z_back_prop = R(0.0);
         else {
-            ti.z_back_prop = zdp - 0.5f * (zs[ti.islice] + zs[ti.islice + 1]);
-            if (std::fabs(ti.z_back_prop) < 1e-4f) ti.z_back_prop = 0.f;
+            ti.z_back_prop = zdp - R(0.5) * (zs[ti.islice] + zs[ti.islice + 1]);
+            if (std::fabs(ti.z_back_prop) < R(1e-4)) ti.z_back_prop = R(0.0);
         }
         thk.push_back(ti);
     }
     for (size_t k = 0; k < thk.size(); ++k) slices[thk[k].islice].ithk = (int)k;
 }
 
-float Model::phonon_weight() const
+template <class R>
+R ModelT<R>::phonon_weight() const
 {
     int nconf = (!frozen() || pn_single_conf) ? 1 : pn_nconf;
-    return 1.0f / float(nconf);
+    return R(1.0) / R(nconf);
 }
 
 // Randn_3d<T, e_host> (src/cgpu_rand.cuh:303-402): a 64-bit Mersenne Twister
 // seeded with pn_seed draws three seeds per configuration index; each axis
-// then has its own generator and a float normal distribution.
-void Model::displaced(int iconf, std::vector<float> &x, std::vector<float> &y) const
+// then has its own generator and a R normal distribution.
+template <class R>
+void ModelT<R>::displaced(int iconf, std::vector<R> &x, std::vector<R> &y) const
 {
     x = atoms_u.x;
     y = atoms_u.y;
@@ -674,17 +694,18 @@ void Model::displaced(int iconf, std::vector<float> &x, std::vector<float> &y) c
     }
     (void)sz;
     std::mt19937_64 gx(sx), gy(sy);
-    std::normal_distribution<float> nx_(0.f, 1.f), ny_(0.f, 1.f);
+    std::normal_distribution<R> nx_(R(0), R(1)), ny_(R(0), R(1));
     for (size_t i = 0; i < x.size(); ++i) {
-        float s = atoms_u.sigma[i];
-        float dx = pn_x ? s * nx_(gx) : 0.f;
-        float dy = pn_y ? s * ny_(gy) : 0.f;
+        R s = (R)atoms_u.sigma[i];
+        R dx = pn_x ? s * nx_(gx) : R(0.0);
+        R dy = pn_y ? s * ny_(gy) : R(0.0);
         x[i] = atoms_u.x[i] + dx;
         y[i] = atoms_u.y[i] + dy;
     }
 }
 
-void Model::output_dims(mt_output_multislice *out) const
+template <class R>
+void ModelT<R>::output_dims(mt_output_multislice *out) const
 {
     out->n_thk = (int)thk.size();
     if (is_stem()) {
@@ -700,7 +721,8 @@ void Model::output_dims(mt_output_multislice *out) const
     else out->output_type = (!frozen() || pn_single_conf) ? MT_OT_PSI_COH : MT_OT_M2PSI_TOT_PSI_COH;
 }
 
-void Model::fill_axes(mt_output_multislice *out) const
+template <class R>
+void ModelT<R>::fill_axes(mt_output_multislice *out) const
 {
     if (out->thick) for (size_t i = 0; i < thk.size(); ++i) out->thick[i] = thk[i].z;
     if (is_stem()) {
@@ -712,5 +734,8 @@ void Model::fill_axes(mt_output_multislice *out) const
     if (out->x) for (int ix = oa_ix_0; ix < oa_ix_e; ++ix) out->x[ix - oa_ix_0] = rs ? ix * dRx : (ix - nxh) * dgx;
     if (out->y) for (int iy = oa_iy_0; iy < oa_iy_e; ++iy) out->y[iy - oa_iy_0] = rs ? iy * dRy : (iy - nyh) * dgy;
 }
+
+template class ModelT<float>;
+template class ModelT<double>;
 
 } // namespace mtg

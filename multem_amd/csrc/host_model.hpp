@@ -1,7 +1,8 @@
-// Host-side model of one simulation: the validated input (Input_Multislice
-// of the reference's FP32 path), the specimen, its slicing and the
-// per-element spline tables.  Everything here runs once per call or once per
-// phonon configuration; the per-probe work is on the GPU.
+// Host-side model of one simulation: the validated input (Input_Multislice<T>
+// of the reference, T = float for the FP32 path, double for the FP64 path),
+// the specimen, its slicing and the per-element spline tables.  Everything
+// here runs once per call or once per phonon configuration; the per-probe
+// work is on the GPU.
 #pragma once
 
 #include <cstdint>
@@ -18,55 +19,72 @@ struct Error {
 };
 
 // Projected-potential spline of one element: 128 knots log-spaced in R,
-// cubic Hermite in R^2 (Atomic_Data::get_atom_coef, src/atomic_data.hpp:1546-1668).
-struct AtomTable {
+// cubic Hermite in R^2 (Atomic_Data::get_atom_coef, src/atomic_data.hpp:1546-1668),
+// built in double and cast to R.
+template <class R>
+struct AtomTableT {
     int Z = 0;
     double R_min = 0, R_max = 0, R_tap = 0, tap_cf = 0;   // as built (double)
-    float R_max_f = 0, R2_max_f = 0;
-    std::vector<float> R2, c0, c1, c2, c3;                 // 128 each
+    R R_max_f = 0, R2_max_f = 0;
+    std::vector<R> R2, c0, c1, c2, c3;                     // 128 each
 };
 
-AtomTable build_atom_table(int Z, double dR_min, double Vrl = 0.015, int nR = 128);
+template <class R>
+AtomTableT<R> build_atom_table(int Z, double dR_min, double Vrl = 0.015, int nR = 128);
 
-struct LensModel {
+template <class R>
+struct LensModelT {
     // MATLAB-unit fields converted as ilc_multem.cu:147-259 does
     int m = 0;
-    float c_10 = 0, c_12 = 0, phi_12 = 0, c_21 = 0, phi_21 = 0, c_23 = 0, phi_23 = 0;
-    float c_30 = 0, c_32 = 0, phi_32 = 0, c_34 = 0, phi_34 = 0;
-    float c_41 = 0, phi_41 = 0, c_43 = 0, phi_43 = 0, c_45 = 0, phi_45 = 0;
-    float c_50 = 0, c_52 = 0, phi_52 = 0, c_54 = 0, phi_54 = 0, c_56 = 0, phi_56 = 0;
-    float inner_aper_ang = 0, outer_aper_ang = 0;
+    R c_10 = 0, c_12 = 0, phi_12 = 0, c_21 = 0, phi_21 = 0, c_23 = 0, phi_23 = 0;
+    R c_30 = 0, c_32 = 0, phi_32 = 0, c_34 = 0, phi_34 = 0;
+    R c_41 = 0, phi_41 = 0, c_43 = 0, phi_43 = 0, c_45 = 0, phi_45 = 0;
+    R c_50 = 0, c_52 = 0, phi_52 = 0, c_54 = 0, phi_54 = 0, c_56 = 0, phi_56 = 0;
+    R inner_aper_ang = 0, outer_aper_ang = 0;
     int zero_defocus_type = 1;
-    float zero_defocus_plane = 0;
-    float lambda = 0;
-    float g2_min = 0, g2_max = 0;      // aperture (float, as evaluated by the reference FP32 path)
+    R zero_defocus_plane = 0;
+    R lambda = 0;
+    R g2_min = 0, g2_max = 0;      // aperture (in R, as the reference evaluates it)
     bool phi_required = false;
-    float ti_iehwgd = 0, si_theta_c = 0;   // partial-coherence envelope (objective lens)
+    R ti_iehwgd = 0, si_theta_c = 0;   // partial-coherence envelope (objective lens)
 };
 
-struct SliceInfo {
-    float z_0, z_e;
+template <class R>
+struct SliceInfoT {
+    R z_0, z_e;
     int iatom_0, iatom_e;   // inclusive range into the z-sorted atoms; empty if iatom_0 > iatom_e
     int ithk;
 };
 
-struct ThickInfo {
-    float z;
+template <class R>
+struct ThickInfoT {
+    R z;
     int islice;
-    float z_back_prop;
+    R z_back_prop;
 };
 
-struct Specimen {
-    // SoA, z-sorted (Spec::atoms_u)
+template <class R>
+struct SpecimenT {
+    // SoA, z-sorted (Spec::atoms_u); sigma and occupancy are float in the
+    // reference's Atom_Data whatever T is
     std::vector<int> Z;
-    std::vector<float> x, y, z, sigma, occ;
+    std::vector<R> x, y, z;
+    std::vector<float> sigma, occ;
     std::vector<int> region;
 };
 
-class Model {
+template <class R>
+class ModelT {
 public:
+    using real = R;
+    using LensModel = LensModelT<R>;
+    using SliceInfo = SliceInfoT<R>;
+    using ThickInfo = ThickInfoT<R>;
+    using Specimen = SpecimenT<R>;
+    using AtomTable = AtomTableT<R>;
+
     // Reads and validates; throws Error.
-    Model(const mt_input_multislice &in);
+    ModelT(const mt_input_multislice &in);
 
     // simulation kind
     int sim_type = 0;
@@ -83,63 +101,70 @@ public:
     int pn_model = 1, pn_coh_contrib = 0, pn_single_conf = 0, pn_nconf = 1, fp_iconf_0 = 1, pn_seed = 300183;
     bool pn_x = true, pn_y = true, pn_z = false;
     int mul_sign = 1;
-    float E_0 = 300, lambda = 0, theta = 0, phi = 0;
-    float gx0 = 0, gy0 = 0;         // tilt of the incident beam in reciprocal space (1/A)
-    bool tilted() const { return gx0 != 0.f || gy0 != 0.f; }
+    R E_0 = 300, lambda = 0, theta = 0, phi = 0;
+    R gx0 = 0, gy0 = 0;         // tilt of the incident beam in reciprocal space (1/A)
+    bool tilted() const { return gx0 != R(0) || gy0 != R(0); }
     double lambda_d = 0;            // FP64 wavelength for phase arguments
-    float Vr_factor = 0;            // mul_sign * gamma*lambda/(c_Potf*cos(theta)), float as the FP32 path
+    R Vr_factor = 0;            // mul_sign * gamma*lambda/(c_Potf*cos(theta)), evaluated in R
     int iw_type = 1;
-    std::vector<float> iw_x, iw_y;
+    std::vector<R> iw_x, iw_y;
 
     // grid
     int nx = 0, ny = 0, nxh = 0, nyh = 0;
-    float lx = 0, ly = 0, dRx = 0, dRy = 0, dgx = 0, dgy = 0;
+    R lx = 0, ly = 0, dRx = 0, dRy = 0, dgx = 0, dgy = 0;
     double lx_d = 0, ly_d = 0;      // box lengths as given, for FP64 phase arguments
     bool bwl = false;
-    float gl2_max = 0, bwl_alpha = 0;
+    R gl2_max = 0, bwl_alpha = 0;
 
     LensModel cond, obj;
 
     // scanning
     int sc_nx = 0, sc_ny = 0;
-    float sc_x0 = 0, sc_y0 = 0, sc_dRx = 0, sc_dRy = 0;
-    std::vector<float> sc_x, sc_y;
+    R sc_x0 = 0, sc_y0 = 0, sc_dRx = 0, sc_dRy = 0;
+    std::vector<R> sc_x, sc_y;
 
-    // detectors: g^2 bounds (float, as pow(g,2) in the FP32 path)
-    std::vector<float> det_g2_min, det_g2_max;
+    // detectors: g^2 bounds (pow(g, 2) in R)
+    std::vector<R> det_g2_min, det_g2_max;
 
     // output area (0-based, half open)
     int oa_ix_0 = 0, oa_ix_e = 0, oa_iy_0 = 0, oa_iy_e = 0;
 
     // thicknesses requested (validated)
-    std::vector<float> thick;
+    std::vector<R> thick;
 
     // specimen
     Specimen atoms_u;                 // undisplaced, pbc-deduped, z-sorted
     std::vector<int> z_types;         // distinct atomic numbers, ascending
     std::vector<int> type_of_atom;    // index into z_types per atom of atoms_u
     std::vector<AtomTable> tables;    // per z_types entry
-    std::vector<float> z_plane;
+    std::vector<R> z_plane;
     std::vector<SliceInfo> slices;    // of the undisplaced specimen
     std::vector<ThickInfo> thk;
 
     int n_slice() const { return (int)slices.size(); }
     int n_conf_run() const { return pn_nconf - fp_iconf_0 + 1; }
-    float phonon_weight() const;      // get_phonon_rot_weight (input_multislice.cuh:885-889)
-    float dz_slice(int is) const { return (slices[is].z_e - slices[is].z_0) / std::cos(theta); }
-    float z_init() const { return 0.5f * (slices[0].z_e + slices[0].z_0); }
+    R phonon_weight() const;      // get_phonon_rot_weight (input_multislice.cuh:885-889)
+    R dz_slice(int is) const { return (slices[is].z_e - slices[is].z_0) / std::cos(theta); }
+    R z_init() const { return R(0.5) * (slices[0].z_e + slices[0].z_0); }
 
     // Displaced positions of configuration iconf (Spec::move_atoms, src/spec.hpp:85-127);
     // returns x, y (z unchanged for plane slicing).
-    void displaced(int iconf, std::vector<float> &x, std::vector<float> &y) const;
+    void displaced(int iconf, std::vector<R> &x, std::vector<R> &y) const;
 
     // output dims
     void output_dims(mt_output_multislice *out) const;
     void fill_axes(mt_output_multislice *out) const;
 
 private:
-    float R_int_max = 2.5f;
+    R R_int_max = R(2.5);
     void slice_specimen();
 };
+
+using Model = ModelT<float>;      // FP32 path
+using Model64 = ModelT<double>;   // FP64 path
+using AtomTable = AtomTableT<float>;
+using LensModel = LensModelT<float>;
+using SliceInfo = SliceInfoT<float>;
+using ThickInfo = ThickInfoT<float>;
 
 } // namespace mtg

@@ -24,29 +24,42 @@ constexpr double kTwoPiD = 6.283185307179586476925;
 __device__ __forceinline__ int shift_idx(int i, int nh) { return i < nh ? i + nh : i - nh; }
 __device__ __forceinline__ int signed_freq(int i, int n) { return i < (n >> 1) ? i : i - n; }
 
-// g^2 on the float grid exactly as the FP32 path rounds it (no contraction):
-// Grid_2d::g2_shift (src/types.cuh:2549-2552)
-__device__ __forceinline__ float g2_float(int igx, int igy, float dgx, float dgy)
+// exp / sincos of the precision in use
+__device__ __forceinline__ float rexp(float x) { return expf(x); }
+__device__ __forceinline__ double rexp(double x) { return exp(x); }
+__device__ __forceinline__ void rsincos(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ void rsincos(double x, double *s, double *c) { sincos(x, s, c); }
+
+// g^2 on the grid exactly as the reference rounds it in precision R (no
+// contraction): Grid_2d::g2_shift (src/types.cuh:2549-2552)
+template <class R>
+__device__ __forceinline__ R g2_grid(int igx, int igy, R dgx, R dgy)
 {
 #pragma clang fp contract(off)
-    const float gx = (float)igx * dgx, gy = (float)igy * dgy;
+    const R gx = (R)igx * dgx, gy = (R)igy * dgy;
     return gx * gx + gy * gy;
 }
 
-// |g + u|^2 on the float grid with the rounding of exp_i_chi's float path
-// (src/cgpu_fcns.cuh:1949-1956): (igx dgx + ux)^2 + (igy dgy + uy)^2
-__device__ __forceinline__ float g2_float_shift(int igx, int igy, float dgx, float dgy, float ux, float uy)
+// |g + u|^2 with the rounding of exp_i_chi (src/cgpu_fcns.cuh:1949-1956):
+// (igx dgx + ux)^2 + (igy dgy + uy)^2
+template <class R>
+__device__ __forceinline__ R g2_grid_shift(int igx, int igy, R dgx, R dgy, R ux, R uy)
 {
 #pragma clang fp contract(off)
-    const float gx = (float)igx * dgx + ux, gy = (float)igy * dgy + uy;
+    const R gx = (R)igx * dgx + ux, gy = (R)igy * dgy + uy;
     return gx * gx + gy * gy;
 }
 
 // bandwidth-limit factor 1/(1+exp(alpha (g^2 - gl2_max))) (Grid_2d::bwl_factor_shift)
-__device__ __forceinline__ float bwl_factor(float g2, float alpha, float gl2_max)
+template <class R>
+__device__ __forceinline__ R bwl_factor(R g2, R alpha, R gl2_max)
 {
-    return 1.0f / (1.0f + expf(alpha * (g2 - gl2_max)));
+    return R(1) / (R(1) + rexp(alpha * (g2 - gl2_max)));
 }
+
+// atom type stored in the fourth component of an atom record
+__device__ __forceinline__ int atom_type(float4 a) { return __float_as_int(a.w); }
+__device__ __forceinline__ int atom_type(double4 a) { return (int)a.w; }
 
 // blockIdx -> (row group, member) such that one row group of every member
 // runs on the same XCD (blockIdx % 8): the transmission rows it multiplies
@@ -69,13 +82,14 @@ __device__ __forceinline__ void map_rows(int wg, int G, int M, int &rg, int &mem
 // Row pass: RW rows of length N per workgroup, T = N/E threads per row.
 // S: radix schedule; F: compile-time RowFlags of the hot paths, or -1 for
 // a.flags at run time; TWT: twiddles from the table (1) or in registers (0).
-template <class S, int RW, int F, int TWT>
-__global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgs a)
+template <class S, int RW, int F, int TWT, class Z>
+__global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgsT<Z> a)
 {
     constexpr int N = S::N, E = S::E, T = S::T;
-    using TW = std::conditional_t<TWT != 0, TwTable<S>, TwReg<S>>;
+    using TW = std::conditional_t<TWT != 0, TwTable<S, Z>, TwReg<S, Z>>;
+    using R = real_t<Z>;
     constexpr int PADN = N + N / 8;
-    __shared__ float2 lds[RW * PADN];
+    __shared__ Z lds[RW * PADN];
     const int flags = (F < 0) ? a.flags : F;
 
     const int tid = threadIdx.x;
@@ -83,23 +97,23 @@ __global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgs a)
     int rg, member;
     map_rows(blockIdx.x, a.nrows / RW, a.n_members, rg, member);
     const int row = rg * RW + r;
-    float2 *lp = lds + r * PADN;
+    Z *lp = lds + r * PADN;
     auto idx = [](int i) { return i + (i >> 3); };
     auto sync = [] { __syncthreads(); };
 
     const long long off = (long long)member * a.member_stride + (long long)row * N;
-    float2 v[E], tv[E];
+    Z v[E], tv[E];
     // issue the wave and transmission loads together, then the twiddles
     if (flags & ROW_LOAD) {
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = a.src[off + t + T * e];
     } else {
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = make_float2(1.f, 0.f);
+        for (int e = 0; e < E; ++e) v[e] = cmake(R(1), R(0));
     }
     if (flags & ROW_MUL_T) {
         const int slot = a.members[member].tslot;
-        const float2 *T_s = a.trans + (long long)slot * a.tslot_stride + (long long)a.slice * a.nrows * N
+        const Z *T_s = a.trans + (long long)slot * a.tslot_stride + (long long)a.slice * a.nrows * N
                             + (long long)row * N;
 #pragma unroll
         for (int e = 0; e < E; ++e) tv[e] = T_s[t + T * e];
@@ -112,7 +126,7 @@ __global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgs a)
         for (int e = 0; e < E; ++e) v[e] = cmul(tv[e], v[e]);
     }
     if (flags & ROW_MUL_RAMP) {
-        const float2 rxv = a.ramp_x[row];
+        const Z rxv = a.ramp_x[row];
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = cmul(cmul(rxv, a.ramp_y[t + T * e]), v[e]);
     }
@@ -127,13 +141,14 @@ __global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgs a)
 // Column pass: C adjacent columns of length N per workgroup.  Thread (c, t)
 // holds elements ix = t + T*e of column tile*C + c.
 // S, F, TWT as for the row pass.
-template <class S, int C, int F, bool PROBE, bool DET, int TWT>
-__global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
+template <class S, int C, int F, bool PROBE, bool DET, int TWT, class Z>
+__global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
 {
     constexpr int N = S::N, E = S::E, T = S::T;
-    using TW = std::conditional_t<TWT != 0, TwTable<S>, TwReg<S>>;
+    using TW = std::conditional_t<TWT != 0, TwTable<S, Z>, TwReg<S, Z>>;
+    using R = real_t<Z>;
     constexpr int NWAVE = (C * T + 63) / 64;
-    __shared__ float2 lds[N * C];
+    __shared__ Z lds[N * C];
     __shared__ double red[DET ? NWAVE * kMaxDet : 1];
     const int flags = (F < 0) ? a.flags : F;
 
@@ -147,7 +162,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
     const long long base = (long long)member * a.member_stride + col;
     const int igy = signed_freq(col, a.ny);
 
-    float2 v[E];
+    Z v[E];
     if constexpr (!PROBE) {
         if (flags & COL_LOAD) {
 #pragma unroll
@@ -158,16 +173,16 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
     tw.load(t, a.tw);
     if constexpr (PROBE) {
         // Convergent probe in Fourier space (exp_i_chi, src/cgpu_fcns.cuh:1949-1979):
-        // float aperture mask, FP64 phase reduced mod 2 pi, FP32 sincos.
+        // aperture mask in R, FP64 phase reduced mod 2 pi, sincos in R.
         const Member m = a.members[member];
-        const ProbeParams &P = a.probe;
+        const ProbeParamsT<R> &P = a.probe;
         const double gy = igy * P.dgy + P.gyu;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int ix = t + T * e;
             const int igx = signed_freq(ix, a.nx);
-            const float g2f = g2_float_shift(igx, igy, P.dgx_f, P.dgy_f, P.gxu_f, P.gyu_f);
-            float2 acc = make_float2(0.f, 0.f);
+            const R g2f = g2_grid_shift(igx, igy, P.dgx_f, P.dgy_f, P.gxu_f, P.gyu_f);
+            Z acc = cmake(R(0), R(0));
             if (P.g2_min <= g2f && g2f < P.g2_max) {
                 const double gx = igx * P.dgx + P.gxu;
                 const double g2 = gx * gx + gy * gy;
@@ -188,8 +203,8 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
                     const Beam bm = a.beams[m.beam0 + b];
                     double chi = bm.xph * gx + bm.yph * gy + aber;
                     chi -= kTwoPiD * rint(chi * (1.0 / kTwoPiD));
-                    float s, co;
-                    sincosf((float)chi, &s, &co);
+                    R s, co;
+                    rsincos((R)chi, &s, &co);
                     acc.x += co * P.norm;
                     acc.y += s * P.norm;
                 }
@@ -200,12 +215,12 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
     if (flags & COL_FFT) fft_run<S, false>(v, t, tw, lds, idx, sync);
     if (flags & (COL_MUL_P | COL_BWL)) {
         // the propagator factors are tiny tables, cache-resident across the batch
-        const float2 pyv = (flags & COL_MUL_P) ? cscale(a.py[col], a.p_scale) : make_float2(a.inv_nxy, 0.f);
+        const Z pyv = (flags & COL_MUL_P) ? cscale(a.py[col], a.p_scale) : cmake(a.inv_nxy, R(0));
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            float2 p = (flags & COL_MUL_P) ? cmul(a.px[t + T * e], pyv) : pyv;
+            Z p = (flags & COL_MUL_P) ? cmul(a.px[t + T * e], pyv) : pyv;
             if (flags & COL_BWL) {
-                const float f = bwl_factor(g2_float(signed_freq(t + T * e, a.nx), igy, a.dgx_f, a.dgy_f),
+                const R f = bwl_factor(g2_grid(signed_freq(t + T * e, a.nx), igy, a.dgx_f, a.dgy_f),
                                            a.bwl_alpha, a.gl2_max);
                 p = cscale(p, f);
             }
@@ -239,10 +254,10 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int ix = t + T * e;
-            const float g2f = g2_float(signed_freq(ix, a.nx), igy, a.dgx_f, a.dgy_f);
-            float m2 = cnorm2(v[e]);
+            const R g2f = g2_grid(signed_freq(ix, a.nx), igy, a.dgx_f, a.dgy_f);
+            R m2 = cnorm2(v[e]);
             if (flags & COL_DET_BWL) {
-                const float f = bwl_factor(g2f, a.bwl_alpha, a.gl2_max);
+                const R f = bwl_factor(g2f, a.bwl_alpha, a.gl2_max);
                 m2 *= f * f;
             }
             for (int d = 0; d < a.n_det; ++d)
@@ -278,13 +293,16 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgs a)
 // with periodic images; T = exp(i w V) written fft-shifted.
 constexpr int kTfTy = 64, kTfTx = 8, kTfThreads = 256;
 
-__global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
+template <class R>
+__global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
 {
-    // Every product and sum rounds separately, as on the reference's FP32 host
+    // Every product and sum rounds separately, as on the reference's host
     // path: the pixel values then depend only on the summation order.
 #pragma clang fp contract(off)
-    __shared__ float tab[kMaxTypes * 5 * 128];
-    __shared__ float2 rmx[kMaxTypes];
+    using R2v = cplx_t<R>;
+    using A4 = vec4_t<R>;
+    __shared__ R tab[kMaxTypes * 5 * 128];
+    __shared__ R2v rmx[kMaxTypes];
     __shared__ int list[kTfThreads];
     __shared__ int wave_cnt[kTfThreads / 64];
 
@@ -301,7 +319,7 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
     const int ty = tid % kTfTy, tx = tid / kTfTy;   // pixels (px0+tx, py) and (px0+tx+4, py)
     const int py = py0 + ty;
     const bool py_ok = py < a.ny;
-    float vacc[2] = {0.f, 0.f};
+    R vacc[2] = {R(0), R(0)};
 
     const int2 rng = a.slice_range[is];
     __syncthreads();
@@ -310,10 +328,10 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
         const int ia = c0 + tid;
         bool hit = false;
         if (ia <= rng.y) {
-            const float4 at = a.atoms[ia];
-            const float R_max = rmx[__float_as_int(at.w)].x;
-            int ix0 = (int)floorf((at.x - R_max) / a.dRx), ixe = (int)ceilf((at.x + R_max) / a.dRx);
-            int iy0 = (int)floorf((at.y - R_max) / a.dRy), iye = (int)ceilf((at.y + R_max) / a.dRy);
+            const A4 at = a.atoms[ia];
+            const R R_max = rmx[atom_type(at)].x;
+            int ix0 = (int)floor((at.x - R_max) / a.dRx), ixe = (int)ceil((at.x + R_max) / a.dRx);
+            int iy0 = (int)floor((at.y - R_max) / a.dRy), iye = (int)ceil((at.y + R_max) / a.dRy);
             if (ix0 != ixe && iy0 != iye) {
                 // smallest image of the tile's last column/row not left of the window start
                 auto fdiv = [](int p, int q) { return p >= 0 ? p / q : -((-p + q - 1) / q); };
@@ -334,23 +352,23 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
         if (hit) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = ia;
         __syncthreads();
         for (int li = 0; li < total; ++li) {
-            const float4 at = a.atoms[list[li]];
-            const int ty_ = __float_as_int(at.w);
-            const float *tb = tab + ty_ * 5 * 128;
-            const float R_max = rmx[ty_].x, R2_max = rmx[ty_].y;
-            const int ix0 = (int)floorf((at.x - R_max) / a.dRx), ixe = (int)ceilf((at.x + R_max) / a.dRx);
-            const int iy0 = (int)floorf((at.y - R_max) / a.dRy), iye = (int)ceilf((at.y + R_max) / a.dRy);
+            const A4 at = a.atoms[list[li]];
+            const int ty_ = atom_type(at);
+            const R *tb = tab + ty_ * 5 * 128;
+            const R R_max = rmx[ty_].x, R2_max = rmx[ty_].y;
+            const int ix0 = (int)floor((at.x - R_max) / a.dRx), ixe = (int)ceil((at.x + R_max) / a.dRx);
+            const int iy0 = (int)floor((at.y - R_max) / a.dRy), iye = (int)ceil((at.y + R_max) / a.dRy);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int px = px0 + tx + 4 * h;
                 // images ix = px + k*nx inside [ix0, ixe]
                 int kx = (ix0 - px >= 0) ? (ix0 - px + a.nx - 1) / a.nx : -((px - ix0) / a.nx);
                 for (int ix = px + kx * a.nx; ix <= ixe; ix += a.nx) {
-                    const float rx = (float)ix * a.dRx - at.x;
+                    const R rx = (R)ix * a.dRx - at.x;
                     int ky = (iy0 - py >= 0) ? (iy0 - py + a.ny - 1) / a.ny : -((py - iy0) / a.ny);
                     for (int iy = py + ky * a.ny; iy <= iye; iy += a.ny) {
-                        const float ry = (float)iy * a.dRy - at.y;
-                        const float R2 = rx * rx + ry * ry;
+                        const R ry = (R)iy * a.dRy - at.y;
+                        const R R2 = rx * rx + ry * ry;
                         if (R2 < R2_max) {
                             // 7-step binary search over the 128 knots, then the cubic
                             int i0 = 0, ie = 127;
@@ -359,8 +377,8 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
                                 const int im = (i0 + ie) >> 1;
                                 if (R2 < tb[im]) ie = im; else i0 = im;
                             }
-                            const float dx = R2 - tb[i0];
-                            const float p = ((tb[4 * 128 + i0] * dx + tb[3 * 128 + i0]) * dx + tb[2 * 128 + i0]) * dx
+                            const R dx = R2 - tb[i0];
+                            const R p = ((tb[4 * 128 + i0] * dx + tb[3 * 128 + i0]) * dx + tb[2 * 128 + i0]) * dx
                                             + tb[1 * 128 + i0];
                             vacc[h] = vacc[h] + at.z * p;
                         }
@@ -378,9 +396,9 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgs a)
         const long long sl = is - a.slice0;
         if (a.V) a.V[sl * nxy + (long long)px * a.ny + py] = vacc[h];
         if (a.trans) {
-            float s, c;
-            sincosf(a.w * vacc[h], &s, &c);
-            a.trans[sl * nxy + (long long)shift_idx(px, a.nxh) * a.ny + shift_idx(py, a.nyh)] = make_float2(c, s);
+            R s, c;
+            rsincos(a.w * vacc[h], &s, &c);
+            a.trans[sl * nxy + (long long)shift_idx(px, a.nxh) * a.ny + shift_idx(py, a.nyh)] = cmake(c, s);
         }
     }
 }
@@ -405,9 +423,11 @@ __global__ void k_det_reduce(const double *part, int n_members, int n_thk, int n
 // reduced mod 2 pi before the FP32 sincos:
 // polar(1/(nx ny), w |g - g0|^2), w = -mul_sign pi lambda z
 // (propagate, src/cgpu_fcns.cuh:1835-1845; g0 the beam tilt).
+template <class Z>
 __global__ void k_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
-                              double lx, double ly, double gx0, double gy0, float2 *px, float2 *py)
+                              double lx, double ly, double gx0, double gy0, Z *px, Z *py)
 {
+    using R = real_t<Z>;
     const int n = nx + ny;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)n * nz) return;
@@ -418,13 +438,13 @@ __global__ void k_prop_tables(int nx, int ny, const double *z, int nz, double la
     else g = signed_freq(j - nx, ny) / ly - gy0;
     double arg = w * g * g;
     arg -= kTwoPiD * rint(arg * (1.0 / kTwoPiD));
-    float s, c;
-    sincosf((float)arg, &s, &c);
+    R s, c;
+    rsincos((R)arg, &s, &c);
     if (j < nx) {
-        const float inv = 1.0f / (float)(nx * ny);
-        px[(long long)zi * nx + j] = make_float2(c * inv, s * inv);
+        const R inv = R(1) / (R)(nx * ny);
+        px[(long long)zi * nx + j] = cmake(c * inv, s * inv);
     } else {
-        py[(long long)zi * ny + (j - nx)] = make_float2(c, s);
+        py[(long long)zi * ny + (j - nx)] = cmake(c, s);
     }
 }
 
@@ -433,29 +453,31 @@ __global__ void k_prop_tables(int nx, int ny, const double *z, int nz, double la
 // exp_r_factor_2d, src/cgpu_fcns.cuh:1867-1877) as separable tables over
 // the shifted layout, rx with the extra factor `scale`; FP64 argument
 // reduced mod 2 pi before the FP32 sincos.
+template <class Z>
 __global__ void k_ramp_tables(int nx, int ny, double gx0, double gy0, double dRx, double dRy, double Rx_c,
-                              double Ry_c, float scale, float2 *rx, float2 *ry)
+                              double Ry_c, real_t<Z> scale, Z *rx, Z *ry)
 {
+    using R = real_t<Z>;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nx + ny) return;
     double arg;
     if (j < nx) arg = -kTwoPiD * gx0 * (shift_idx(j, nx >> 1) * dRx - Rx_c);
     else arg = -kTwoPiD * gy0 * (shift_idx(j - nx, ny >> 1) * dRy - Ry_c);
     arg -= kTwoPiD * rint(arg * (1.0 / kTwoPiD));
-    float sn, cs;
-    sincosf((float)arg, &sn, &cs);
-    if (j < nx) rx[j] = make_float2(cs * scale, sn * scale);
-    else ry[j - nx] = make_float2(cs, sn);
+    R sn, cs;
+    rsincos((R)arg, &sn, &cs);
+    if (j < nx) rx[j] = cmake(cs * scale, sn * scale);
+    else ry[j - nx] = cmake(cs, sn);
 }
 
-__global__ void k_count_aperture(int nx, int ny, float dgx, float dgy, float ux, float uy, float g2_min, float g2_max,
-                                 unsigned int *count)
+template <class R>
+__global__ void k_count_aperture(int nx, int ny, R dgx, R dgy, R ux, R uy, R g2_min, R g2_max, unsigned int *count)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool in = false;
     if (i < (long long)nx * ny) {
         const int ix = (int)(i / ny), iy = (int)(i % ny);
-        const float g2 = g2_float_shift(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy, ux, uy);
+        const R g2 = g2_grid_shift(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy, ux, uy);
         in = (g2_min <= g2 && g2 < g2_max);
     }
     const unsigned long long m = __ballot(in);
@@ -464,7 +486,8 @@ __global__ void k_count_aperture(int nx, int ny, float dgx, float dgy, float ux,
 
 // Output accumulation with fftshift + crop (add_scale_*_crop_shift_2d,
 // src/cgpu_fcns.cuh:1664-1778); members summed in order.
-__global__ void k_out_accum(OutArgs a)
+template <class Z>
+__global__ void k_out_accum(OutArgsT<Z> a)
 {
     const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= (long long)a.nx_o * a.ny_o) return;
@@ -472,7 +495,7 @@ __global__ void k_out_accum(OutArgs a)
     const int sx = shift_idx(ixo + a.ix_0, a.nx >> 1), sy = shift_idx(iyo + a.iy_0, a.ny >> 1);
     const long long si = (long long)sx * a.ny + sy;
     for (int m = 0; m < a.n_members; ++m) {
-        const float2 v = a.src[(long long)m * a.member_stride + si];
+        const Z v = a.src[(long long)m * a.member_stride + si];
         if (a.mode & OUT_SET_PSI) {
             a.psi[2 * o] = v.x;
             a.psi[2 * o + 1] = v.y;
@@ -487,7 +510,7 @@ __global__ void k_out_accum(OutArgs a)
         // natural index o maps to the same pixel of the uncropped wave
         for (long long k = o; k < (long long)a.nx * a.ny; k += (long long)a.nx_o * a.ny_o) {
             for (int m = 0; m < a.n_members; ++m) {
-                const float2 v = a.src[(long long)m * a.member_stride + k];
+                const Z v = a.src[(long long)m * a.member_stride + k];
                 a.psi_full[2 * k] += a.w * (double)v.x;
                 a.psi_full[2 * k + 1] += a.w * (double)v.y;
             }
@@ -495,14 +518,15 @@ __global__ void k_out_accum(OutArgs a)
     }
 }
 
-__global__ void k_ctf_table(int nx, int ny, CtfParams P, float2 *ctf)
+template <class R>
+__global__ void k_ctf_table(int nx, int ny, CtfParamsT<R> P, cplx_t<R> *ctf)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)nx * ny) return;
     const int ix = (int)(i / ny), iy = (int)(i % ny);
     const int igx = signed_freq(ix, nx), igy = signed_freq(iy, ny);
-    const float g2f = g2_float(igx, igy, P.dgx_f, P.dgy_f);
-    float2 v = make_float2(0.f, 0.f);
+    const R g2f = g2_grid(igx, igy, P.dgx_f, P.dgy_f);
+    cplx_t<R> v = cmake(R(0), R(0));
     if (P.g2_min <= g2f && g2f < P.g2_max) {
         const double gx = igx * P.dgx, gy = igy * P.dgy, g2 = gx * gx + gy * gy;
         double chi, amp = 1.0;
@@ -531,22 +555,24 @@ __global__ void k_ctf_table(int nx, int ny, CtfParams P, float2 *ctf)
             amp = exp(-(spa_inc + temp_inc) / u) / sqrt(u);
         }
         chi -= kTwoPiD * rint(chi * (1.0 / kTwoPiD));
-        float sn, cs;
-        sincosf((float)chi, &sn, &cs);
-        v = make_float2((float)amp * cs, (float)amp * sn);
+        R sn, cs;
+        rsincos((R)chi, &sn, &cs);
+        v = cmake((R)amp * cs, (R)amp * sn);
     }
     ctf[i] = v;
 }
 
 // Coherent STEM image (Wave_Function::set_m2psi_coh, src/wave_function.cuh:188-205):
 // one workgroup per (scan point, thickness), FP64 sums reduced in fixed order.
+template <class R>
 struct CohDet {
-    float g2_min[kMaxDet], g2_max[kMaxDet];
+    R g2_min[kMaxDet], g2_max[kMaxDet];
 };
 
+template <class R>
 __global__ __launch_bounds__(256) void k_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny,
-                                                   float dgx, float dgy, CohDet det, int extra_bwl, float bwl_alpha,
-                                                   float gl2_max, double *image_coh)
+                                                   R dgx, R dgy, CohDet<R> det, int extra_bwl, R bwl_alpha, R gl2_max,
+                                                   double *image_coh)
 {
     __shared__ double red[kMaxDet][256];
     const int iscan = blockIdx.x % nscan, ithk = blockIdx.x / nscan;
@@ -555,7 +581,7 @@ __global__ __launch_bounds__(256) void k_coh_image(const double2 *coh, int nscan
     for (int d = 0; d < kMaxDet; ++d) acc[d] = 0.0;
     for (long long i = threadIdx.x; i < (long long)nx * ny; i += 256) {
         const int ix = (int)(i / ny), iy = (int)(i % ny);
-        const float g2 = g2_float(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy);
+        const R g2 = g2_grid(signed_freq(ix, nx), signed_freq(iy, ny), dgx, dgy);
         const double2 v = c[i];
         double m2 = v.x * v.x + v.y * v.y;
         if (extra_bwl) {
@@ -575,7 +601,8 @@ __global__ __launch_bounds__(256) void k_coh_image(const double2 *coh, int nscan
     if (threadIdx.x < n_det) image_coh[((long long)ithk * n_det + threadIdx.x) * nscan + iscan] = red[threadIdx.x][0];
 }
 
-__global__ void k_slice_sum(const float *V, int n_slices, long long nxy, double *out)
+template <class R>
+__global__ void k_slice_sum(const R *V, int n_slices, long long nxy, double *out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nxy) return;
@@ -601,7 +628,8 @@ __global__ void k_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, 
     m2[o] = re * re + im * im;
 }
 
-__global__ void k_shift_complex(const float2 *src, float2 *dst, int nx, int ny)
+template <class Z>
+__global__ void k_shift_complex(const Z *src, Z *dst, int nx, int ny)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)nx * ny) return;
@@ -614,6 +642,7 @@ __global__ void k_shift_complex(const float2 *src, float2 *dst, int nx, int ny)
 
 int col_tile_width(int nx) { return col_variant(nx, 0).tile; }
 int row_group(int ny) { return row_variant(ny, 0).tile; }
+int col_tile_width_fp64(int nx) { return col_tile0_fp64(nx); }
 
 namespace {
 
@@ -623,81 +652,111 @@ constexpr int kColSlice = COL_LOAD | COL_FFT | COL_MUL_P | COL_IFFT | COL_STORE;
 constexpr int kColLast = COL_LOAD | COL_FFT | COL_MUL_P;
 constexpr int kColProbe = COL_IFFT | COL_STORE;
 
-// variant V of the row pass at length N with compile-time flags F
-template <int N, int V, int F>
-hipError_t row_go(const RowArgs &a, hipStream_t s)
+template <class Z> constexpr bool kFp64 = std::is_same<Z, double2>::value;
+
+// variant V of the row pass at length N with compile-time flags F.  The FP64
+// passes have one variant: the default schedule with table twiddles (the
+// double registers do not leave room for register twiddles).
+template <int N, int V, int F, class Z>
+hipError_t row_go(const RowArgsT<Z> &a, hipStream_t s)
 {
     constexpr PassVariant pv = row_variant(N, V);
     using S = Sched<N, pv.sched>;
     static_assert(S::E == sched_elems(N, pv.sched) && pv.tile * S::T <= 1024, "row variant");
     if (a.nrows % pv.tile) return hipErrorInvalidValue;
     const int grid = (a.nrows / pv.tile) * a.n_members;
-    hipLaunchKernelGGL((k_row_pass<S, pv.tile, F, pv.tw_table>), dim3(grid), dim3(pv.tile * S::T), 0, s, a);
+    constexpr int twt = kFp64<Z> ? 1 : pv.tw_table;
+    hipLaunchKernelGGL((k_row_pass<S, pv.tile, F, twt, Z>), dim3(grid), dim3(pv.tile * S::T), 0, s, a);
     return hipGetLastError();
 }
 
-template <int N, int F, int... V>
-hipError_t row_pick(const RowArgs &a, hipStream_t s, std::integer_sequence<int, V...>)
+template <int N, int F, class Z, int... V>
+hipError_t row_pick(const RowArgsT<Z> &a, hipStream_t s, std::integer_sequence<int, V...>)
 {
     hipError_t e = hipErrorInvalidValue;
     ((a.variant == V ? (e = row_go<N, V, F>(a, s), 0) : 0), ...);
     return e;
 }
 
-template <int N>
-hipError_t row_launch(const RowArgs &a, hipStream_t s)
+template <int N, class Z>
+hipError_t row_launch(const RowArgsT<Z> &a, hipStream_t s)
 {
-    if (a.flags == kRowSlice)
-        return row_pick<N, kRowSlice>(a, s, std::make_integer_sequence<int, kPassVariants>{});
+    if constexpr (kFp64<Z>) {
+        if (a.variant != 0) return hipErrorInvalidValue;
+        if (a.flags == kRowSlice) return row_go<N, 0, kRowSlice>(a, s);
+    } else {
+        if (a.flags == kRowSlice)
+            return row_pick<N, kRowSlice>(a, s, std::make_integer_sequence<int, kPassVariants>{});
+    }
     if (a.flags == kRowStart) return row_go<N, 0, kRowStart>(a, s);
     return row_go<N, 0, -1>(a, s);
 }
 
-template <int N, int V, int F, bool PROBE, bool DET>
-hipError_t col_go(const ColArgs &a, hipStream_t s)
+template <int N, int V, int F, bool PROBE, bool DET, class Z>
+hipError_t col_go(const ColArgsT<Z> &a, hipStream_t s)
 {
-    constexpr PassVariant pv = col_variant(N, V);
-    using S = Sched<N, pv.sched>;
-    static_assert(S::E == sched_elems(N, pv.sched) && pv.tile * S::T <= 1024, "column variant");
-    if constexpr (DET && pv.tile != col_tile0(N)) {
-        return hipErrorInvalidValue;   // the detector partials assume the default width
-    } else {
-        if (a.ncols % pv.tile) return hipErrorInvalidValue;
-        const int grid = (a.ncols / pv.tile) * a.n_members;
-        hipLaunchKernelGGL((k_col_pass<S, pv.tile, F, PROBE, DET, pv.tw_table>), dim3(grid), dim3(pv.tile * S::T), 0,
-                           s, a);
+    if constexpr (kFp64<Z>) {
+        // FP64: default schedule, table twiddles, columns narrowed so the
+        // double LDS tile fits (col_tile0_fp64)
+        using S = Sched<N, 0>;
+        constexpr int C = col_tile0_fp64(N);
+        static_assert(C * S::T <= 1024 && N * C * 16 <= 160 * 1024, "FP64 column tile");
+        if (a.ncols % C) return hipErrorInvalidValue;
+        const int grid = (a.ncols / C) * a.n_members;
+        hipLaunchKernelGGL((k_col_pass<S, C, F, PROBE, DET, 1, Z>), dim3(grid), dim3(C * S::T), 0, s, a);
         return hipGetLastError();
+    } else {
+        constexpr PassVariant pv = col_variant(N, V);
+        using S = Sched<N, pv.sched>;
+        static_assert(S::E == sched_elems(N, pv.sched) && pv.tile * S::T <= 1024, "column variant");
+        if constexpr (DET && pv.tile != col_tile0(N)) {
+            return hipErrorInvalidValue;   // the detector partials assume the default width
+        } else {
+            if (a.ncols % pv.tile) return hipErrorInvalidValue;
+            const int grid = (a.ncols / pv.tile) * a.n_members;
+            hipLaunchKernelGGL((k_col_pass<S, pv.tile, F, PROBE, DET, pv.tw_table, Z>), dim3(grid),
+                               dim3(pv.tile * S::T), 0, s, a);
+            return hipGetLastError();
+        }
     }
 }
 
-template <int N, int F, bool DET, int... V>
-hipError_t col_pick(const ColArgs &a, hipStream_t s, std::integer_sequence<int, V...>)
+template <int N, int F, bool DET, class Z, int... V>
+hipError_t col_pick(const ColArgsT<Z> &a, hipStream_t s, std::integer_sequence<int, V...>)
 {
     hipError_t e = hipErrorInvalidValue;
     ((a.variant == V ? (e = col_go<N, V, F, false, DET>(a, s), 0) : 0), ...);
     return e;
 }
 
-template <int N>
-hipError_t col_launch(const ColArgs &a, bool probe, bool det, hipStream_t s)
+template <int N, class Z>
+hipError_t col_launch(const ColArgsT<Z> &a, bool probe, bool det, hipStream_t s)
 {
     constexpr auto all = std::make_integer_sequence<int, kPassVariants>{};
+    if (kFp64<Z> && a.variant != 0) return hipErrorInvalidValue;
     if (probe) {
         if (a.flags == kColProbe) return col_go<N, 0, kColProbe, true, false>(a, s);
         return col_go<N, 0, -1, true, false>(a, s);
     }
     if (det) {
-        if (a.flags == kColLast) return col_pick<N, kColLast, true>(a, s, all);
+        if (a.flags == kColLast) {
+            if constexpr (kFp64<Z>) return col_go<N, 0, kColLast, false, true>(a, s);
+            else return col_pick<N, kColLast, true>(a, s, all);
+        }
         if (a.flags == kColSlice) return col_go<N, 0, kColSlice, false, true>(a, s);
         return col_go<N, 0, -1, false, true>(a, s);
     }
-    if (a.flags == kColSlice) return col_pick<N, kColSlice, false>(a, s, all);
+    if (a.flags == kColSlice) {
+        if constexpr (kFp64<Z>) return col_go<N, 0, kColSlice, false, false>(a, s);
+        else return col_pick<N, kColSlice, false>(a, s, all);
+    }
     return col_go<N, 0, -1, false, false>(a, s);
 }
 
 } // namespace
 
-hipError_t launch_row_pass(int n, const RowArgs &a, hipStream_t s)
+template <class Z>
+hipError_t launch_row_pass(int n, const RowArgsT<Z> &a, hipStream_t s)
 {
     switch (n) {
     case 64: return row_launch<64>(a, s);
@@ -710,7 +769,8 @@ hipError_t launch_row_pass(int n, const RowArgs &a, hipStream_t s)
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_col_pass(int n, const ColArgs &a, bool probe, bool det, hipStream_t s)
+template <class Z>
+hipError_t launch_col_pass(int n, const ColArgsT<Z> &a, bool probe, bool det, hipStream_t s)
 {
     if (probe && det) return hipErrorInvalidValue;
     switch (n) {
@@ -724,11 +784,12 @@ hipError_t launch_col_pass(int n, const ColArgs &a, bool probe, bool det, hipStr
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_tf_build(const TfArgs &a, hipStream_t s)
+template <class R>
+hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s)
 {
     if (a.nx % kTfTx || a.n_types > kMaxTypes || a.n_slices <= 0) return hipErrorInvalidValue;
     const int tiles = (a.nx / kTfTx) * ((a.ny + kTfTy - 1) / kTfTy);
-    hipLaunchKernelGGL(k_tf_build, dim3(tiles, a.n_slices), dim3(kTfThreads), 0, s, a);
+    hipLaunchKernelGGL((k_tf_build<R>), dim3(tiles, a.n_slices), dim3(kTfThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -740,36 +801,40 @@ hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n
     return hipGetLastError();
 }
 
+template <class Z>
 hipError_t launch_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
-                              double lx, double ly, double gx0, double gy0, float2 *px, float2 *py, hipStream_t s)
+                              double lx, double ly, double gx0, double gy0, Z *px, Z *py, hipStream_t s)
 {
     const long long n = (long long)(nx + ny) * nz;
-    hipLaunchKernelGGL(k_prop_tables, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, z, nz, lambda,
+    hipLaunchKernelGGL((k_prop_tables<Z>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, z, nz, lambda,
                        mul_sign, lx, ly, gx0, gy0, px, py);
     return hipGetLastError();
 }
 
+template <class Z>
 hipError_t launch_ramp_tables(int nx, int ny, double gx0, double gy0, double dRx, double dRy, double Rx_c,
-                              double Ry_c, float scale, float2 *rx, float2 *ry, hipStream_t s)
+                              double Ry_c, real_t<Z> scale, Z *rx, Z *ry, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ramp_tables, dim3((unsigned)((nx + ny + 255) / 256)), dim3(256), 0, s, nx, ny, gx0, gy0,
+    hipLaunchKernelGGL((k_ramp_tables<Z>), dim3((unsigned)((nx + ny + 255) / 256)), dim3(256), 0, s, nx, ny, gx0, gy0,
                        dRx, dRy, Rx_c, Ry_c, scale, rx, ry);
     return hipGetLastError();
 }
 
-hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float ux, float uy, float g2_min, float g2_max,
-                                 unsigned int *count, hipStream_t s)
+template <class R>
+hipError_t launch_count_aperture(int nx, int ny, R dgx, R dgy, R ux, R uy, R g2_min, R g2_max, unsigned int *count,
+                                 hipStream_t s)
 {
     const long long n = (long long)nx * ny;
-    hipLaunchKernelGGL(k_count_aperture, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, dgx, dgy, ux,
+    hipLaunchKernelGGL((k_count_aperture<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, dgx, dgy, ux,
                        uy, g2_min, g2_max, count);
     return hipGetLastError();
 }
 
-hipError_t launch_out_accum(const OutArgs &a, hipStream_t s)
+template <class Z>
+hipError_t launch_out_accum(const OutArgsT<Z> &a, hipStream_t s)
 {
     const long long n = (long long)a.nx_o * a.ny_o;
-    hipLaunchKernelGGL(k_out_accum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_out_accum<Z>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -782,28 +847,31 @@ hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, 
     return hipGetLastError();
 }
 
-hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, float dgx, float dgy,
-                            const float *det_g2_min, const float *det_g2_max, int extra_bwl, float bwl_alpha,
-                            float gl2_max, double *image_coh, hipStream_t s)
+template <class R>
+hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, R dgx, R dgy,
+                            const R *det_g2_min, const R *det_g2_max, int extra_bwl, R bwl_alpha, R gl2_max,
+                            double *image_coh, hipStream_t s)
 {
     if (n_det > kMaxDet || n_det < 0) return hipErrorInvalidValue;
-    CohDet d{};
+    CohDet<R> d{};
     for (int k = 0; k < n_det; ++k) { d.g2_min[k] = det_g2_min[k]; d.g2_max[k] = det_g2_max[k]; }
-    hipLaunchKernelGGL(k_coh_image, dim3((unsigned)(nscan * n_thk)), dim3(256), 0, s, coh, nscan, n_thk, n_det, nx, ny,
-                       dgx, dgy, d, extra_bwl, bwl_alpha, gl2_max, image_coh);
+    hipLaunchKernelGGL((k_coh_image<R>), dim3((unsigned)(nscan * n_thk)), dim3(256), 0, s, coh, nscan, n_thk, n_det,
+                       nx, ny, dgx, dgy, d, extra_bwl, bwl_alpha, gl2_max, image_coh);
     return hipGetLastError();
 }
 
-hipError_t launch_ctf_table(int nx, int ny, const CtfParams &p, float2 *ctf, hipStream_t s)
+template <class R>
+hipError_t launch_ctf_table(int nx, int ny, const CtfParamsT<R> &p, cplx_t<R> *ctf, hipStream_t s)
 {
     const long long n = (long long)nx * ny;
-    hipLaunchKernelGGL(k_ctf_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, p, ctf);
+    hipLaunchKernelGGL((k_ctf_table<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, p, ctf);
     return hipGetLastError();
 }
 
-hipError_t launch_slice_sum(const float *V, int n_slices, long long nxy, double *out, hipStream_t s)
+template <class R>
+hipError_t launch_slice_sum(const R *V, int n_slices, long long nxy, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, s, V, n_slices, nxy, out);
+    hipLaunchKernelGGL((k_slice_sum<R>), dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, s, V, n_slices, nxy, out);
     return hipGetLastError();
 }
 
@@ -816,11 +884,33 @@ hipError_t launch_stream_copy(const void *a, void *b, size_t bytes, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_shift_complex(const float2 *src, float2 *dst, int nx, int ny, hipStream_t s)
+template <class Z>
+hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_t s)
 {
     const long long n = (long long)nx * ny;
-    hipLaunchKernelGGL(k_shift_complex, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, nx, ny);
+    hipLaunchKernelGGL((k_shift_complex<Z>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, nx, ny);
     return hipGetLastError();
 }
+
+// both precisions
+#define MTG_INSTANTIATE(R, Z)                                                                                          \
+    template hipError_t launch_row_pass<Z>(int, const RowArgsT<Z> &, hipStream_t);                                     \
+    template hipError_t launch_col_pass<Z>(int, const ColArgsT<Z> &, bool, bool, hipStream_t);                         \
+    template hipError_t launch_tf_build<R>(const TfArgsT<R> &, hipStream_t);                                           \
+    template hipError_t launch_prop_tables<Z>(int, int, const double *, int, double, int, double, double, double,      \
+                                              double, Z *, Z *, hipStream_t);                                          \
+    template hipError_t launch_ramp_tables<Z>(int, int, double, double, double, double, double, double, R, Z *, Z *,   \
+                                              hipStream_t);                                                            \
+    template hipError_t launch_count_aperture<R>(int, int, R, R, R, R, R, R, unsigned int *, hipStream_t);            \
+    template hipError_t launch_out_accum<Z>(const OutArgsT<Z> &, hipStream_t);                                         \
+    template hipError_t launch_coh_image<R>(const double2 *, int, int, int, int, int, R, R, const R *, const R *, int, \
+                                            R, R, double *, hipStream_t);                                              \
+    template hipError_t launch_ctf_table<R>(int, int, const CtfParamsT<R> &, Z *, hipStream_t);                        \
+    template hipError_t launch_slice_sum<R>(const R *, int, long long, double *, hipStream_t);                         \
+    template hipError_t launch_shift_complex<Z>(const Z *, Z *, int, int, hipStream_t);
+
+MTG_INSTANTIATE(float, float2)
+MTG_INSTANTIATE(double, double2)
+#undef MTG_INSTANTIATE
 
 } // namespace mtg

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "complex_types.hpp"
+
 namespace mtg {
 
 constexpr int kMaxDet = 8;
@@ -23,20 +25,22 @@ struct Beam {
     double xph, yph;
 };
 
-struct ProbeParams {
+template <class R>
+struct ProbeParamsT {
     double c10, c30, c50;         // -pi*lambda*f, -pi*Cs3*lambda^3/2, -pi*Cs5*lambda^5/3 (FP64)
     double dgx, dgy;              // FP64 reciprocal sampling (1/lx, 1/ly)
-    float dgx_f, dgy_f;           // float sampling, for the aperture mask as the FP32 path evaluates it
+    R dgx_f, dgy_f;               // sampling in R, for the aperture mask as the reference evaluates it
     // azimuthal aberrations (Lens::eval_m ... eval_c_52_c_54_c_56, src/types.cuh:3180-3300):
     // chi += m phi + sum_k az_c[k] g^az_p[k] sin(az_n[k] (phi - az_phi[k])), FP64
     int azim;                     // nonzero: the terms below are present
     double az_m;
     double az_c[11], az_phi[11];  // order 12, 21, 23, 32, 34, 41, 43, 45, 52, 54, 56
     double gxu, gyu;              // frequency shift of chi (tilted per-piece incident wave; 0 in simulations)
-    float gxu_f, gyu_f;
-    float g2_min, g2_max;         // aperture
-    float norm;                   // 1/sqrt(number of aperture pixels)
+    R gxu_f, gyu_f;
+    R g2_min, g2_max;             // aperture
+    R norm;                       // 1/sqrt(number of aperture pixels)
 };
+using ProbeParams = ProbeParamsT<float>;
 
 enum RowFlags : int {
     ROW_LOAD = 1,       // load the wave (else start from psi = 1, a plane wave)
@@ -47,21 +51,23 @@ enum RowFlags : int {
     ROW_MUL_RAMP = 32,  // multiply by ramp_x[row] * ramp_y[iy] (after the transmission)
 };
 
-struct RowArgs {
-    const float2 *src;
-    float2 *dst;
+template <class Z>
+struct RowArgsT {
+    const Z *src;
+    Z *dst;
     long long member_stride;      // complex elements between members
     int nrows;                    // rows per member (= nx)
     int n_members;
     int flags;
-    const float2 *tw;             // twiddles W_ny
-    const float2 *trans;          // transmission stacks: [tslot][n_slice][nx*ny]
+    const Z *tw;             // twiddles W_ny
+    const Z *trans;          // transmission stacks: [tslot][n_slice][nx*ny]
     long long tslot_stride;       // complex elements between stacks
     int slice;
     const Member *members;
     int variant;                  // launch variant of the full slice step (kRowSlice flags)
-    const float2 *ramp_x, *ramp_y;   // ROW_MUL_RAMP tables [nrows], [row length]
+    const Z *ramp_x, *ramp_y;   // ROW_MUL_RAMP tables [nrows], [row length]
 };
+using RowArgs = RowArgsT<float2>;
 
 enum ColFlags : int {
     COL_LOAD = 1,
@@ -75,50 +81,55 @@ enum ColFlags : int {
     COL_MUL_CTF = 256,  // multiply by ctf[ix*ny + iy] after the propagator (objective-lens image formation)
 };
 
-struct ColArgs {
-    const float2 *src;
-    float2 *dst;
+template <class Z>
+struct ColArgsT {
+    using R = real_t<Z>;
+    const Z *src;
+    Z *dst;
     long long member_stride;
     int ncols;                    // = ny (row length, element stride of x)
     int n_members;
     int flags;
-    const float2 *tw;             // twiddles W_nx
-    const float2 *px;             // [nx]
-    const float2 *py;             // [ny]
-    float dgx_f, dgy_f;           // float sampling for masks
+    const Z *tw;             // twiddles W_nx
+    const Z *px;             // [nx]
+    const Z *py;             // [ny]
+    R dgx_f, dgy_f;               // sampling in R for masks
     int nx, ny;
-    float bwl_alpha, gl2_max;
-    float inv_nxy;
-    float p_scale;                // extra factor on px*py (1 for a slice step)
+    R bwl_alpha, gl2_max;
+    R inv_nxy;
+    R p_scale;                // extra factor on px*py (1 for a slice step)
     // detector epilogue
     int n_det;
-    float det_g2_min[kMaxDet], det_g2_max[kMaxDet];
+    R det_g2_min[kMaxDet], det_g2_max[kMaxDet];
     double *det_part;             // [member][n_thk][n_det][ntile]
     int n_thk, ithk;
-    const float2 *ctf;            // COL_MUL_CTF table [nx*ny], natural order
+    const Z *ctf;            // COL_MUL_CTF table [nx*ny], natural order
     // coherent STEM accumulators [scan][n_thk][nx*ny], FP64
     double2 *coh;
     double coh_w;
     // probe generation
     const Member *members;
     const Beam *beams;
-    ProbeParams probe;
+    ProbeParamsT<R> probe;
     int variant;                  // launch variant of the slice step / last slice
 };
+using ColArgs = ColArgsT<float2>;
 
-struct TfArgs {
+template <class R>
+struct TfArgsT {
     int nx, ny, nxh, nyh;
-    float dRx, dRy;
-    const float4 *atoms;          // x, y, occ, type (bit pattern of an int)
+    R dRx, dRy;
+    const vec4_t<R> *atoms;       // x, y, occ, type (bit pattern of an int in the low 32 bits of .w)
     const int2 *slice_range;      // per slice: first, last atom (inclusive); empty if first > last
     int slice0, n_slices;         // slices built by this launch
-    const float *tables;          // [type][5][128]: R2, c0, c1, c2, c3
-    const float2 *rmax;           // [type]: R_max, R2_max
+    const R *tables;              // [type][5][128]: R2, c0, c1, c2, c3
+    const cplx_t<R> *rmax;        // [type]: R_max, R2_max
     int n_types;
-    float w;                      // sigma factor (Vr_factor)
-    float2 *trans;                // out, shifted layout: [slice][nx*ny] (relative to slice0), may be null
-    float *V;                     // out, natural layout: [slice][nx*ny], may be null
+    R w;                          // sigma factor (Vr_factor)
+    cplx_t<R> *trans;             // out, shifted layout: [slice][nx*ny] (relative to slice0), may be null
+    R *V;                         // out, natural layout: [slice][nx*ny], may be null
 };
+using TfArgs = TfArgsT<float>;
 
 enum OutMode : int {
     OUT_SET_PSI = 1,       // psi_coh = O                     (single configuration)
@@ -127,8 +138,9 @@ enum OutMode : int {
     OUT_ADD_PSI_FULL = 8,  // psi_full += w O (uncropped, unshifted; for m2psi_coh)
 };
 
-struct OutArgs {
-    const float2 *src;
+template <class Z>
+struct OutArgsT {
+    const Z *src;
     long long member_stride;
     int n_members;
     int nx, ny;
@@ -139,45 +151,54 @@ struct OutArgs {
     double *psi;           // complex [nx_o*ny_o]
     double *psi_full;      // complex [nx*ny]
 };
+using OutArgs = OutArgsT<float2>;
 
-// launchers (return hipError_t of the launch)
-hipError_t launch_row_pass(int n, const RowArgs &a, hipStream_t s);
-hipError_t launch_col_pass(int n, const ColArgs &a, bool probe, bool det, hipStream_t s);
-hipError_t launch_tf_build(const TfArgs &a, hipStream_t s);
+// launchers (return hipError_t of the launch); Z = float2 or double2, R its real type
+template <class Z> hipError_t launch_row_pass(int n, const RowArgsT<Z> &a, hipStream_t s);
+template <class Z> hipError_t launch_col_pass(int n, const ColArgsT<Z> &a, bool probe, bool det, hipStream_t s);
+template <class R> hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s);
 hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
                              const Member *members, double w, double *image, int nscan, hipStream_t s);
+template <class Z>
 hipError_t launch_prop_tables(int nx, int ny, const double *z, int nz, double lambda, int mul_sign,
-                              double lx, double ly, double gx0, double gy0, float2 *px, float2 *py, hipStream_t s);
+                              double lx, double ly, double gx0, double gy0, Z *px, Z *py, hipStream_t s);
+template <class Z>
 hipError_t launch_ramp_tables(int nx, int ny, double gx0, double gy0, double dRx, double dRy, double Rx_c,
-                              double Ry_c, float scale, float2 *rx, float2 *ry, hipStream_t s);
-hipError_t launch_count_aperture(int nx, int ny, float dgx, float dgy, float ux, float uy, float g2_min, float g2_max,
+                              double Ry_c, real_t<Z> scale, Z *rx, Z *ry, hipStream_t s);
+template <class R>
+hipError_t launch_count_aperture(int nx, int ny, R dgx, R dgy, R ux, R uy, R g2_min, R g2_max,
                                  unsigned int *count, hipStream_t s);
-hipError_t launch_out_accum(const OutArgs &a, hipStream_t s);
+template <class Z> hipError_t launch_out_accum(const OutArgsT<Z> &a, hipStream_t s);
 hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o,
                               double *m2, hipStream_t s);
+
 // Objective-lens transfer function on the natural reciprocal grid
 // (apply_CTF / apply_PCTF, src/cgpu_fcns.cuh:1993-2040): mode 1 coherent
 // exp(i chi(g)), mode 2 partially coherent polar(E(g), chi_10,30(g));
 // zero outside g2_min <= g^2 < g2_max.
-struct CtfParams {
+template <class R>
+struct CtfParamsT {
     int mode;
     double dgx, dgy;              // FP64 sampling for chi
-    float dgx_f, dgy_f;           // float sampling for the aperture test
-    float g2_min, g2_max;
+    R dgx_f, dgy_f;               // sampling in R for the aperture test
+    R g2_min, g2_max;
     double cc[5];                 // c_c_10, c_c_30, c_c_50 (indices 0..2); raw c_10 (3), c_30 (4) in A
     int azim;
     double az_m, az_c[11], az_phi[11];
     double ti, sth, lambda;       // ti_iehwgd, si_theta_c (after the temporal/spatial selection), lambda
 };
-hipError_t launch_ctf_table(int nx, int ny, const CtfParams &p, float2 *ctf, hipStream_t s);
+template <class R>
+hipError_t launch_ctf_table(int nx, int ny, const CtfParamsT<R> &p, cplx_t<R> *ctf, hipStream_t s);
 
 // image_coh[ithk][det][iscan] = sum over the ring of f |coh|^2 (f = bwl^2 if extra_bwl, else 1)
-hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, float dgx, float dgy,
-                            const float *det_g2_min, const float *det_g2_max, int extra_bwl, float bwl_alpha,
-                            float gl2_max, double *image_coh, hipStream_t s);
+template <class R>
+hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, R dgx, R dgy,
+                            const R *det_g2_min, const R *det_g2_max, int extra_bwl, R bwl_alpha, R gl2_max,
+                            double *image_coh, hipStream_t s);
 hipError_t launch_stream_copy(const void *a, void *b, size_t bytes, hipStream_t s);
-hipError_t launch_slice_sum(const float *V, int n_slices, long long nxy, double *out, hipStream_t s);
-hipError_t launch_shift_complex(const float2 *src, float2 *dst, int nx, int ny, hipStream_t s);
+template <class R>
+hipError_t launch_slice_sum(const R *V, int n_slices, long long nxy, double *out, hipStream_t s);
+template <class Z> hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_t s);
 
 // Launch variants of the slice passes.  Variant 0 is the default; the tuner
 // may pick another.  Variants differ in the radix schedule (different
@@ -198,6 +219,9 @@ constexpr int sched_elems(int n, int sc)
 }
 constexpr int row_tile0(int n) { return n == 64 ? 32 : n == 128 ? 16 : n == 256 ? 8 : n <= 1024 ? 4 : 2; }
 constexpr int col_tile0(int n) { return n >= 2048 ? 8 : 16; }
+// FP64 column pass: columns per workgroup that keep the double LDS tile
+// (N x C x 16 bytes) at or below 128 KiB
+constexpr int col_tile0_fp64(int n) { return n >= 2048 ? 4 : n >= 1024 ? 8 : 16; }
 
 // Row pass, v: 0 default; 1 table twiddles; 2 twice the rows; 3 alternative
 // schedule at the same thread count; 4 = 3 with table twiddles; 5 half the
@@ -242,6 +266,7 @@ constexpr PassVariant col_variant(int n, int v)
 }
 
 int col_tile_width(int nx);   // columns per column-pass workgroup (variant 0; the detector partial layout)
+int col_tile_width_fp64(int nx);   // the same for the FP64 passes
 int row_group(int ny);        // rows per row-pass workgroup (variant 0)
 
 } // namespace mtg

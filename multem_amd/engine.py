@@ -36,6 +36,7 @@ def lib():
         L.mt_hip_output_dims.argtypes = [sysp, inp, outp]
         L.mt_hip_run_multislice.argtypes = [sysp, inp, outp]
         L.mt_hip_run_multislice_f32.argtypes = [sysp, inp, outp]
+        L.mt_hip_run_multislice_f64.argtypes = [sysp, inp, outp]
         L.mt_hip_run_multislice_group.argtypes = [sysp, P(inp), C.c_int, outp]
         L.mt_hip_spec_slicing.argtypes = [sysp, inp, C.c_int, P(capi.Slice), C.c_int, P(C.c_int)]
         L.mt_hip_projected_potential.argtypes = [sysp, inp, C.c_int, C.c_int, dp]
@@ -75,16 +76,21 @@ def output_dims(params):
     return d
 
 
-def run_multislice(params):
-    """ilc_multem on the GPU; returns a capi.Output."""
+def run_multislice(params, precision=None):
+    """ilc_multem on the GPU; returns a capi.Output.  The wave path runs in
+    params.system_conf.precision (1 float, 2 double) unless `precision`
+    forces one through the _f32 / _f64 entry."""
     L = lib()
     s, keep = params.to_struct()
     sysc = params.system_conf.as_struct()
+    if precision is not None:
+        sysc.precision = int(precision)     # output sizes of the forced precision's model
     d = capi.OutputMultislice()
     _check(L.mt_hip_output_dims(C.byref(sysc), C.byref(s), C.byref(d)))
     out = capi.Output(d)
     o = out.as_struct()
-    _check(L.mt_hip_run_multislice(C.byref(sysc), C.byref(s), C.byref(o)))
+    entry = {None: L.mt_hip_run_multislice, 1: L.mt_hip_run_multislice_f32, 2: L.mt_hip_run_multislice_f64}[precision]
+    _check(entry(C.byref(sysc), C.byref(s), C.byref(o)))
     return out
 
 
