@@ -253,9 +253,7 @@ struct Setup {
     {
         ProbeParamsT<R> P{};
         const double lam = m.lambda_d;
-        // defocus referred to the first slice (Incident_Wave::operator(), src/incident_wave.cuh:72-91)
-        const R f_s = m.cond.c_10 - (m.cond.zero_defocus_plane - m.z_init());
-        P.c10 = is_zero(f_s) ? 0.0 : -kPi * (double)f_s * lam;
+        P.c10 = probe_c10(m.cond.c_10);
         P.c30 = is_zero(m.cond.c_30) ? 0.0 : -kPi * (double)m.cond.c_30 * std::pow(lam, 3) / 2.0;
         P.c50 = is_zero(m.cond.c_50) ? 0.0 : -kPi * (double)m.cond.c_50 * std::pow(lam, 5) / 3.0;
         P.dgx = 1.0 / m.lx_d; P.dgy = 1.0 / m.ly_d;
@@ -282,10 +280,33 @@ struct Setup {
         return P;
     }
 
-    Beam beam(R xb, R yb) const
+    // defocus coefficient of chi for condenser defocus c_10, referred to the
+    // first slice (Incident_Wave::operator(), src/incident_wave.cuh:72-91)
+    double probe_c10(R c_10) const
+    {
+        const R f_s = c_10 - (m.cond.zero_defocus_plane - m.z_init());
+        return is_zero(f_s) ? 0.0 : -kPi * (double)f_s * m.lambda_d;
+    }
+
+    Beam beam(R xb, R yb, double c10) const
     {
         // x = -2 pi (x_b - lx/2)  (Grid_2d::exp_factor_Rx, src/types.cuh:2599-2611)
-        return Beam{-2.0 * kPi * ((double)xb - 0.5 * m.lx_d), -2.0 * kPi * ((double)yb - 0.5 * m.ly_d)};
+        return Beam{-2.0 * kPi * ((double)xb - 0.5 * m.lx_d), -2.0 * kPi * ((double)yb - 0.5 * m.ly_d), c10};
+    }
+
+    // Defocus points of the condenser lens: the probe's defocus coefficient
+    // and the weight of each point.  Fully integrated illumination follows
+    // the defocus loop of STEM_ISTEM / CBED_CBEI (src/tem_simulation.cuh:138-178,
+    // 207-261): c_10 = c_10_0 + x_t, weight w_t; otherwise one point.
+    void cond_defocus_points(const Quad1 &qt, std::vector<double> &c10, std::vector<double> &wq) const
+    {
+        c10.clear();
+        wq.clear();
+        const double c_10_0 = (double)m.cond.c_10;
+        for (size_t i = 0; i < qt.x.size(); ++i) {
+            c10.push_back(probe_c10((R)(c_10_0 + qt.x[i])));
+            wq.push_back(qt.w[i]);
+        }
     }
 
     // pixels inside the condenser aperture, |g + u| evaluated as the probe does
@@ -302,9 +323,12 @@ struct Setup {
         return h;
     }
 
-    // objective-lens transfer function of HRTEM (MicroscopeEffects,
-    // src/microscope_effects.cuh:57-145)
-    void ctf_table(DBuf<Z> &ctf)
+    // Objective-lens transfer functions of HRTEM (MicroscopeEffects,
+    // src/microscope_effects.cuh:57-202): one table for the coherent and the
+    // partially coherent models; for the fully integrated model one coherent
+    // table per defocus point c_10 = c_10_0 + ti_iehwgd x_t of num_int_TEM,
+    // with its weight in wq.
+    void ctf_tables(DBuf<Z> &ctf, std::vector<double> &wq)
     {
         const LensModelT<R> &L = m.obj;
         const double lam = m.lambda_d;
@@ -338,8 +362,22 @@ struct Setup {
         P.ti = m.temporal_spatial_incoh == 3 ? 0.0 : (double)L.ti_iehwgd;
         P.sth = m.temporal_spatial_incoh == 2 ? 0.0 : (double)L.si_theta_c;
         P.lambda = lam;
-        ctf.alloc(nxy);
-        check(launch_ctf_table(nx, ny, P, ctf.p, s), "ctf table");
+        wq.assign(1, 1.0);
+        if (m.illumination_model == 4) {
+            Quad1 qt;
+            m.obj_temporal_quadrature(qt);
+            wq = qt.w;
+            ctf.alloc(qt.x.size() * nxy);
+            for (size_t j = 0; j < qt.x.size(); ++j) {
+                const R c10 = L.ti_iehwgd * (R)qt.x[j] + L.c_10;
+                P.cc[0] = nz(c10) ? -kPi * (double)c10 * lam : 0.0;
+                P.cc[3] = (double)c10;
+                check(launch_ctf_table(nx, ny, P, ctf.p + j * nxy, s), "ctf table");
+            }
+        } else {
+            ctf.alloc(nxy);
+            check(launch_ctf_table(nx, ny, P, ctf.p, s), "ctf table");
+        }
         check(hipStreamSynchronize(s), "sync");
     }
 
@@ -555,6 +593,8 @@ public:
     ProbeParamsT<R> P{};
     unsigned int nap = 0;
     R w = 0;
+    int n_t = 1;                     // defocus points (fully integrated illumination)
+    std::vector<double> tc10, twq;   // their probe defocus coefficients and weights
     RowArgsT<Z> r{};
     ColArgsT<Z> c{};
     PassChoice choice;
@@ -627,6 +667,15 @@ public:
         nap = S.count_aperture();
         P = S.probe_params(nap);
         w = m.phonon_weight();
+        {
+            // STEM integrates the defocus spread only (src/tem_simulation.cuh:138-178)
+            Quad1 qt;
+            Quad2 qs;
+            if (m.full_integration()) m.cond_quadratures(qt, qs);
+            else qt = Quad1{{0.0}, {1.0}};
+            S.cond_defocus_points(qt, tc10, twq);
+            n_t = (int)tc10.size();
+        }
         atoms.alloc(m.atoms_u.x.size());
         hm.resize(B);
         hb.resize(B);
@@ -651,7 +700,7 @@ public:
         ColArgsT<Z> cl = c;
         cl.flags = col_flags(true);
         cl.px = px.p; cl.py = py.p; cl.ithk = m.slices[ns_run - 1].ithk;
-        for (int j = 0; j < B; ++j) hm[j] = Member{0, -1, j, 1};   // valid entries while tuning
+        for (int j = 0; j < B; ++j) hm[j] = Member{0, -1, j, 1, 1.0};   // valid entries while tuning
         members.upload(hm.data(), B, s);
         check(hipMemsetAsync(psi.p, 0, psi.n * sizeof(Z), s), "hipMemsetAsync");
         choice = choose_variants(m.nx, m.ny, B, rt, ct, cl.ithk >= 0 ? &cl : nullptr, s);
@@ -680,7 +729,7 @@ public:
         const int bm = cand.back();
         DBuf<Z> w((size_t)bm * nxy);
         DBuf<Member> mem(bm);
-        std::vector<Member> hmv(bm, Member{0, -1, 0, 1});
+        std::vector<Member> hmv(bm, Member{0, -1, 0, 1, 1.0});
         mem.upload(hmv.data(), bm, s);
         check(hipMemsetAsync(w.p, 0, w.n * sizeof(Z), s), "hipMemsetAsync");
         std::ostringstream log;
@@ -718,7 +767,9 @@ public:
                (last ? 0 : (COL_IFFT | COL_STORE));
     }
 
-    int n_work() const { return m.n_conf_run() * nbatch; }
+    // work item k: configuration k / (n_t nbatch), defocus point (k / nbatch) % n_t,
+    // probe batch k % nbatch
+    int n_work() const { return m.n_conf_run() * n_t * nbatch; }
     int batch_size(int k) const { return std::min(B, nscan - (k % nbatch) * B); }
 
     std::string plan() const
@@ -727,13 +778,14 @@ public:
         o << "stem " << (sizeof(R) == 8 ? "fp64" : "fp32") << " nx=" << m.nx << " ny=" << m.ny << " batch=" << B << " slices=" << ns_run
           << " row_variant=" << choice.row << " col_variant=" << choice.col << " col_det_variant=" << choice.col_det
           << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
+        if (n_t > 1) o << " defocus_points=" << n_t;
         return o.str();
     }
 
     // inputs of work item k resident on the device (returns after the uploads)
     void prepare(int k)
     {
-        const int iconf = m.fp_iconf_0 + k / nbatch, ib = k % nbatch;
+        const int iconf = m.fp_iconf_0 + k / (n_t * nbatch), it = (k / nbatch) % n_t, ib = k % nbatch;
         if (iconf != cur_conf) {
             auto a = S.atoms_of(iconf);
             atoms.upload(a.data(), a.size(), s);
@@ -749,8 +801,8 @@ public:
             const int is = ib * B + j;
             const bool valid = is < nscan;
             const int iq = valid ? is : nscan - 1;
-            hm[j] = Member{0, valid ? is : -1, j, 1};
-            hb[j] = S.beam(m.sc_x[iq], m.sc_y[iq]);
+            hm[j] = Member{0, valid ? is : -1, j, 1, twq[it]};
+            hb[j] = S.beam(m.sc_x[iq], m.sc_y[iq], tc10[it]);
         }
         members.upload(hm.data(), B, s);
         beams.upload(hb.data(), B, s);
@@ -848,7 +900,7 @@ void run_stem(const ModelT<R> &m, mt_output_multislice *out)
     hipStream_t s = J.s;
     tm.plan = J.plan();
     tm.tune_log = (J.batch_log.empty() ? "" : J.batch_log + " ") + J.choice.log;
-    run_control().niter = m.n_conf_run() * J.nscan;
+    run_control().niter = J.n_work() / J.nbatch * J.nscan;
     run_control().iter = 0;
 
     GraphExec G;
@@ -930,7 +982,7 @@ void run_stem_group(const std::vector<const ModelT<R> *> &ms, const std::vector<
         tune << (i ? " | " : "") << jobs.back()->batch_log << (jobs.back()->batch_log.empty() ? "" : " ")
              << jobs.back()->choice.log;
         rounds = std::max(rounds, jobs.back()->n_work());
-        niter += ms[i]->n_conf_run() * jobs.back()->nscan;
+        niter += jobs.back()->n_work() / jobs.back()->nbatch * jobs.back()->nscan;
     }
     tm.plan = "group " + plan.str();
     tm.tune_log = tune.str();
@@ -1030,18 +1082,29 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     }();
 
     const int nconf = m.n_conf_run();
+    // waves per configuration: one, or for fully integrated CBED one per
+    // (source point, defocus point) in the loop order of CBED_CBEI
+    // (src/tem_simulation.cuh:207-261), all sharing the configuration's stack
+    Quad1 qt{{0.0}, {1.0}};
+    Quad2 qs{{0.0}, {0.0}, {1.0}};
+    if (m.is_cbed() && m.full_integration()) m.cond_quadratures(qt, qs);
+    std::vector<double> tc10, twq;
+    S.cond_defocus_points(qt, tc10, twq);
+    const int n_s = (int)qs.w.size(), n_t = (int)tc10.size(), nq = n_s * n_t;
+    const int nbeam = m.iw_type == 2 ? (int)m.iw_x.size() : 0;
+
     const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(Z);
-    const size_t per_member = stack_bytes + 2 * nxy * sizeof(Z);
+    const size_t per_conf = stack_bytes + 2 * (size_t)nq * nxy * sizeof(Z);
     const size_t fixed = (size_t)n_thk * (no * 3 + 2 * nxy) * sizeof(double);
     const size_t budget = budget_bytes();
-    if (fixed + per_member > budget) throw Error{MT_ERR_OUT_OF_MEMORY, "one configuration exceeds the memory budget"};
-    int K = (int)std::min<size_t>((size_t)nconf, (budget - fixed) / per_member);
+    if (fixed + per_conf > budget) throw Error{MT_ERR_OUT_OF_MEMORY, "one configuration exceeds the memory budget"};
+    int K = (int)std::min<size_t>((size_t)nconf, (budget - fixed) / per_conf);
     if (run_control().batch_override > 0) K = std::min(K, run_control().batch_override);
     K = std::max(K, 1);
 
-    DBuf<Z> trans((size_t)K * m.n_slice() * nxy), psi((size_t)K * nxy), scratch((size_t)K * nxy);
-    DBuf<Member> members(K);
-    DBuf<Beam> beams(std::max<size_t>(1, m.iw_x.size()));
+    DBuf<Z> trans((size_t)K * m.n_slice() * nxy), psi((size_t)K * nq * nxy), scratch((size_t)K * nq * nxy);
+    DBuf<Member> members((size_t)K * nq);
+    DBuf<Beam> beams(std::max<size_t>(1, (size_t)nq * nbeam));
     DBuf<double> m2(std::max<size_t>(1, (size_t)n_thk * no)), pc(std::max<size_t>(1, (size_t)n_thk * no * 2));
     const bool need_full = (ot == MT_OT_M2PSI_TOT_COH);
     DBuf<double> pfull(need_full ? (size_t)n_thk * nxy * 2 : 1);
@@ -1059,22 +1122,34 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     DBuf<Z> rx, ry;
     if (m.tilted()) S.ramp_tables(rx, ry);
     DBuf<Z> ctf;
-    if (m.is_hrtem()) S.ctf_table(ctf);
+    std::vector<double> cwq{1.0};   // weights of the objective-lens tables
+    if (m.is_hrtem()) S.ctf_tables(ctf, cwq);
+    const int n_ctf = (int)cwq.size();
+    // several tables of a tilted run: the ramp output must survive each
+    DBuf<Z> scratch2(m.is_hrtem() && n_ctf > 1 && m.tilted() ? (size_t)K * nxy : 0);
     ProbeParamsT<R> P{};
     if (m.iw_type == 2) {
         P = S.probe_params(S.count_aperture());
         std::vector<Beam> hb;
-        for (size_t i = 0; i < m.iw_x.size(); ++i) hb.push_back(S.beam(m.iw_x[i], m.iw_y[i]));
+        for (int i_s = 0; i_s < n_s; ++i_s)
+            for (int i_t = 0; i_t < n_t; ++i_t)
+                for (int ib = 0; ib < nbeam; ++ib) {
+                    // beam_x = iw_x + source offset, stored in R (src/tem_simulation.cuh:233-237)
+                    const R xb = (R)((double)m.iw_x[ib] + qs.x[i_s]), yb = (R)((double)m.iw_y[ib] + qs.y[i_s]);
+                    hb.push_back(S.beam(xb, yb, tc10[i_t]));
+                }
         beams.upload(hb.data(), hb.size(), s);
     }
     std::ostringstream plan;
     plan << (m.is_cbed() ? "cbed" : (m.is_hrtem() ? "hrtem" : (m.is_ewrs() ? "ewrs" : "ewfs")))
          << (sizeof(R) == 8 ? " fp64" : " fp32") << " nx=" << m.nx << " ny=" << m.ny
          << " configs_per_batch=" << K << " slices=" << ns_run;
+    if (nq > 1) plan << " waves_per_config=" << nq;
+    if (n_ctf > 1) plan << " ctf_defocus_points=" << n_ctf;
     tm.plan = plan.str();
 
     const double w = (double)m.phonon_weight();
-    run_control().niter = nconf;
+    run_control().niter = nconf * nq;
     run_control().iter = 0;
     Event t0, t1;
     record(t0, s);
@@ -1082,35 +1157,36 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     PassChoice choice;
     for (int c0 = 0; c0 < nconf; c0 += K) {
         stop_check();
-        const int nb = std::min(K, nconf - c0);
-        std::vector<Member> hm(nb);
+        const int nb = std::min(K, nconf - c0), nm = nb * nq;   // configurations, waves
+        std::vector<Member> hm(nm);
         for (int j = 0; j < nb; ++j) {
-            hm[j] = Member{j, -1, 0, m.iw_type == 2 ? (int)m.iw_x.size() : 0};
+            for (int q = 0; q < nq; ++q)
+                hm[j * nq + q] = Member{j, -1, q * nbeam, nbeam, qs.w[q / n_t] * twq[q % n_t]};
             auto a = S.atoms_of(m.fp_iconf_0 + c0 + j);
             atoms.upload(a.data(), a.size(), s);
             S.build_tf(atoms, 0, m.n_slice(), trans.p + (size_t)j * m.n_slice() * nxy, nullptr);
             check(hipStreamSynchronize(s), "sync");   // atoms buffer is reused by the next configuration
         }
-        members.upload(hm.data(), nb, s);
+        members.upload(hm.data(), nm, s);
 
         RowArgsT<Z> r{};
-        r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nb;
+        r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nm;
         r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = (long long)m.n_slice() * nxy; r.members = members.p;
         if (c0 == 0) {   // pick the slice-step launch variants on this batch's buffers
             RowArgsT<Z> rt = r;
             rt.slice = 0;
             rt.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
-            ColArgsT<Z> ct = S.col_base(psi.p, psi.p, nb);
+            ColArgsT<Z> ct = S.col_base(psi.p, psi.p, nm);
             ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
             ct.px = px.p; ct.py = py.p;
-            check(hipMemsetAsync(psi.p, 0, (size_t)nb * nxy * sizeof(Z), s), "hipMemsetAsync");
-            choice = choose_variants(m.nx, m.ny, nb, rt, ct, (const ColArgsT<Z> *)nullptr, s);
+            check(hipMemsetAsync(psi.p, 0, (size_t)nm * nxy * sizeof(Z), s), "hipMemsetAsync");
+            choice = choose_variants(m.nx, m.ny, nm, rt, ct, (const ColArgsT<Z> *)nullptr, s);
             tm.tune_log = choice.log;
             tm.plan += " row_variant=" + std::to_string(choice.row) + " col_variant=" + std::to_string(choice.col);
         }
         r.variant = choice.row;
         if (m.iw_type == 2) {
-            ColArgsT<Z> cp = S.col_base(nullptr, psi.p, nb);
+            ColArgsT<Z> cp = S.col_base(nullptr, psi.p, nm);
             cp.flags = COL_IFFT | COL_STORE;
             cp.members = members.p; cp.beams = beams.p; cp.probe = P;
             check(launch_col_pass(m.nx, cp, true, false, s), "probe pass");
@@ -1119,7 +1195,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
             r.slice = is;
             r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
             check(launch_row_pass(m.ny, r, s), "row pass");
-            ColArgsT<Z> c = S.col_base(psi.p, psi.p, nb);
+            ColArgsT<Z> c = S.col_base(psi.p, psi.p, nm);
             c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
             c.px = px.p + (size_t)is * m.nx;
             c.py = py.p + (size_t)is * m.ny;
@@ -1129,7 +1205,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
             if (ithk < 0) continue;
             // O = F_x(L)/nx * bwl * exp(i phi_back)  (get_psi, src/wave_function.cuh:96-105)
             const R zb = m.thk[ithk].z_back_prop;
-            const Z *osrc = psi.p;
+            const Z *osrc0 = psi.p;
             if (m.tilted()) {
                 // psi exp(-2 pi i g0.r) in real space first (get_psi, src/wave_function.cuh:96-105)
                 RowArgsT<Z> ro = r;
@@ -1137,46 +1213,51 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                 ro.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_RAMP | ROW_FFT | ROW_STORE;
                 ro.ramp_x = rx.p; ro.ramp_y = ry.p;
                 check(launch_row_pass(m.ny, ro, s), "output ramp pass");
-                osrc = scratch.p;
+                osrc0 = scratch.p;
             }
-            if (fs || zb != R(0) || m.bwl || m.tilted() || m.is_hrtem()) {
-                // HRTEM: the objective lens acts on O(g) before the inverse
-                // transforms (microscope_effects, src/wave_function.cuh:172-180)
-                ColArgsT<Z> co = S.col_base(osrc, scratch.p, nb);
-                co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) | COL_STORE |
-                           (m.is_hrtem() ? COL_MUL_CTF : 0);
-                co.ctf = ctf.p;
-                co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
-                co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
-                co.p_scale = (R)m.ny;
-                check(launch_col_pass(m.nx, co, false, false, s), "output col pass");
-                osrc = scratch.p;
+            Z *odst = scratch2.n ? scratch2.p : scratch.p;
+            for (int jc = 0; jc < n_ctf; ++jc) {
+                const Z *osrc = osrc0;
+                if (fs || zb != R(0) || m.bwl || m.tilted() || m.is_hrtem()) {
+                    // HRTEM: the objective lens acts on O(g) before the inverse
+                    // transforms (microscope_effects, src/wave_function.cuh:172-180)
+                    ColArgsT<Z> co = S.col_base(osrc, odst, nm);
+                    co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) |
+                               COL_STORE | (m.is_hrtem() ? COL_MUL_CTF : 0);
+                    co.ctf = ctf.p + (size_t)jc * nxy;
+                    co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
+                    co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
+                    co.p_scale = (R)m.ny;
+                    check(launch_col_pass(m.nx, co, false, false, s), "output col pass");
+                    osrc = odst;
+                }
+                if (!fs) {
+                    RowArgsT<Z> ro = r;
+                    ro.src = osrc; ro.dst = odst;
+                    ro.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+                    check(launch_row_pass(m.ny, ro, s), "output row pass");
+                    osrc = odst;
+                }
+                OutArgsT<Z> oa{};
+                oa.src = osrc; oa.member_stride = (long long)nxy; oa.n_members = nm;
+                oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
+                oa.w = w * cwq[jc];
+                oa.members = members.p;
+                oa.m2 = m2.p + (size_t)ithk * no;
+                oa.psi = pc.p + (size_t)ithk * no * 2;
+                oa.psi_full = pfull.p + (need_full ? (size_t)ithk * nxy * 2 : 0);
+                if (ot == MT_OT_PSI_COH) oa.mode = OUT_SET_PSI;
+                else if (ot == MT_OT_M2PSI_TOT_PSI_COH) oa.mode = OUT_ADD_M2 | OUT_ADD_PSI;
+                else oa.mode = OUT_ADD_M2 | (need_full ? OUT_ADD_PSI_FULL : 0);
+                check(launch_out_accum(oa, s), "output accumulate");
             }
-            if (!fs) {
-                RowArgsT<Z> ro = r;
-                ro.src = osrc; ro.dst = scratch.p;
-                ro.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
-                check(launch_row_pass(m.ny, ro, s), "output row pass");
-                osrc = scratch.p;
-            }
-            OutArgsT<Z> oa{};
-            oa.src = osrc; oa.member_stride = (long long)nxy; oa.n_members = nb;
-            oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
-            oa.w = w;
-            oa.m2 = m2.p + (size_t)ithk * no;
-            oa.psi = pc.p + (size_t)ithk * no * 2;
-            oa.psi_full = pfull.p + (need_full ? (size_t)ithk * nxy * 2 : 0);
-            if (ot == MT_OT_PSI_COH) oa.mode = OUT_SET_PSI;
-            else if (ot == MT_OT_M2PSI_TOT_PSI_COH) oa.mode = OUT_ADD_M2 | OUT_ADD_PSI;
-            else oa.mode = OUT_ADD_M2 | (need_full ? OUT_ADD_PSI_FULL : 0);
-            check(launch_out_accum(oa, s), "output accumulate");
         }
-        progress_add(nb);
+        progress_add(nm);
     }
     record(t1, s);
     wait(t1, s);
     tm.ms_total = elapsed(t0, t1);
-    tm.probe_slices = (double)nconf * ns_run;
+    tm.probe_slices = (double)nconf * nq * ns_run;
 
     if (out->m2psi_tot && (ot == MT_OT_M2PSI_TOT || ot == MT_OT_M2PSI_TOT_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
         check(hipMemcpyAsync(out->m2psi_tot, m2.p, (size_t)n_thk * no * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
@@ -1343,11 +1424,11 @@ void incident_wave(const ModelT<R> &m, int device, double *psi0)
         P.gxu = (double)m.gx0; P.gyu = (double)m.gy0;
         P.gxu_f = m.gx0; P.gyu_f = m.gy0;
         std::vector<Beam> hb;
-        for (size_t i = 0; i < m.iw_x.size(); ++i) hb.push_back(S.beam(m.iw_x[i], m.iw_y[i]));
+        for (size_t i = 0; i < m.iw_x.size(); ++i) hb.push_back(S.beam(m.iw_x[i], m.iw_y[i], P.c10));
         if (hb.empty()) throw Error{MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y"};
         DBuf<Beam> beams(hb.size());
         beams.upload(hb.data(), hb.size(), st.s);
-        Member hm{0, -1, 0, (int)hb.size()};
+        Member hm{0, -1, 0, (int)hb.size(), 1.0};
         DBuf<Member> mem(1);
         mem.upload(&hm, 1, st.s);
         ColArgsT<Z> c = S.col_base(nullptr, w.p, 1);
@@ -1382,7 +1463,7 @@ void propagate(const ModelT<R> &m, int device, const double *psi_in, double z, i
     std::vector<double> zl{z};
     S.prop_tables(zl, px, py);
     const bool zero = is_zero((R)z);
-    Member hm{0, -1, 0, 0};
+    Member hm{0, -1, 0, 0, 1.0};
     DBuf<Member> mem(1);
     mem.upload(&hm, 1, st.s);
     RowArgsT<Z> r{};

@@ -12,6 +12,7 @@
 #include <random>
 
 #include "lobato_table.inc"
+#include "quadrature.hpp"
 
 namespace mtg {
 
@@ -25,6 +26,7 @@ constexpr double kPotf = 47.877645145863056;      // c_Potf (src/types.cuh:66)
 constexpr double kDegToRad = 0.01745329251994329576924;
 constexpr double kMradToRad = 1.0e-3;
 constexpr double kMmToAngs = 1.0e7;
+constexpr double kSqrtPi = 1.772453850905516027298;   // c_Pii2 (src/types.cuh:90)
 
 [[noreturn]] void fail(mt_status c, const std::string &m) { throw Error{c, m}; }
 
@@ -218,6 +220,24 @@ void read_lens(const mt_lens &m, LensModelT<R> &L)
     L.outer_aper_ang = (R)(m.outer_aper_ang * kMradToRad);
     L.zero_defocus_type = m.zero_defocus_type;
     L.zero_defocus_plane = (R)m.zero_defocus_plane;
+    L.ti_a = (R)m.ti_a; L.ti_sigma = (R)m.ti_sigma; L.ti_beta = (R)m.ti_beta; L.ti_npts = m.ti_npts;
+    L.si_a = (R)m.si_a; L.si_sigma = (R)m.si_sigma; L.si_beta = (R)m.si_beta;
+    L.si_rad_npts = m.si_rad_npts; L.si_azm_npts = m.si_azm_npts;
+}
+
+// the spread-function clamps of Lens::set_input_data (src/types.cuh:2953-2963)
+template <class R>
+void clamp_spreads(LensModelT<R> &L)
+{
+    L.ti_a = std::max(R(0), std::min(R(1), L.ti_a));
+    L.ti_sigma = std::max(R(0), L.ti_sigma);
+    L.ti_beta = std::max(R(0), L.ti_beta);
+    L.ti_npts = std::max(1, L.ti_npts);
+    L.si_a = std::max(R(0), std::min(R(1), L.si_a));
+    L.si_sigma = std::max(R(0), L.si_sigma);
+    L.si_beta = std::max(R(0), L.si_beta);
+    L.si_rad_npts = std::max(1, L.si_rad_npts);
+    L.si_azm_npts = std::max(1, L.si_azm_npts);
 }
 
 // aperture bounds of Lens::set_input_data (src/types.cuh:2924-2976)
@@ -366,9 +386,15 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     if (!(in.potential_slicing == 1 || in.potential_slicing == 2))
         fail(MT_ERR_UNSUPPORTED, "potential_slicing 3/4 (sub-slicing)");
     if (std::fabs(in.spec_rot_theta) > 0) fail(MT_ERR_UNSUPPORTED, "specimen rotation");
-    if (in.illumination_model == 4) fail(MT_ERR_UNSUPPORTED, "illumination_model 4 (full integration)");
-    illumination_model = (is_ewfs() || is_ewrs() || is_cbed()) ? 1 : in.illumination_model;
-    temporal_spatial_incoh = in.temporal_spatial_incoh;
+    // exit waves are coherent; CBED is coherent unless fully integrated
+    // (Input_Multislice::validate_parameters, src/input_multislice.cuh:672-682)
+    illumination_model = in.illumination_model;
+    if (is_ewfs() || is_ewrs() || (is_cbed() && illumination_model != 4)) illumination_model = 1;
+    // no source spread: temporal incoherence only (src/input_multislice.cuh:746-751)
+    temporal_spatial_incoh = near_zero((R)in.cond_lens.si_sigma) ? 2 : in.temporal_spatial_incoh;
+    if (is_hrtem() && illumination_model == 4 && temporal_spatial_incoh != 2)
+        fail(MT_ERR_UNSUPPORTED, "HRTEM full integration with spatial incoherence: the reference's source "
+                                 "quadrature is empty (src/cpu_fcns.hpp:1535) and its image is zero");
     if (is_hrtem() && illumination_model == 3)
         fail(MT_ERR_UNSUPPORTED, "HRTEM with illumination_model 3 (transmission cross coefficient)");
     if (is_hrtem() && in.pn_coh_contrib)
@@ -445,6 +471,17 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     read_lens(in.cond_lens, cond);
     set_lens(cond, E_0, g_max);
     read_lens(in.obj_lens, obj);
+    // the objective lens takes its source spread from the cond_lens_si_* fields
+    // (ilc_multem.cu:246-250)
+    obj.si_a = cond.si_a; obj.si_sigma = cond.si_sigma; obj.si_beta = cond.si_beta;
+    obj.si_rad_npts = cond.si_rad_npts; obj.si_azm_npts = cond.si_azm_npts;
+    // integration points (src/input_multislice.cuh:753-776), then the clamps
+    for (LensModel *L : {&cond, &obj}) {
+        if (illumination_model != 4) L->si_rad_npts = L->si_azm_npts = L->ti_npts = 1;
+        if (temporal_spatial_incoh == 2) L->si_rad_npts = L->si_azm_npts = 1;
+        else if (temporal_spatial_incoh == 3) L->ti_npts = 1;
+        clamp_spreads(*L);
+    }
     // Lens::set_ti_sigma / set_si_sigma (src/types.cuh:2978-2989); the objective
     // lens reads si_sigma from cond_lens_si_sigma (ilc_multem.cu:247)
     obj.ti_iehwgd = (R)kSqrt2 * std::fmax(R(0.0), (R)in.obj_lens.ti_sigma);
@@ -733,6 +770,76 @@ void ModelT<R>::fill_axes(mt_output_multislice *out) const
     bool rs = is_ewrs() || is_hrtem();
     if (out->x) for (int ix = oa_ix_0; ix < oa_ix_e; ++ix) out->x[ix - oa_ix_0] = rs ? ix * dRx : (ix - nxh) * dgx;
     if (out->y) for (int iy = oa_iy_0; iy < oa_iy_e; ++iy) out->y[iy - oa_iy_0] = rs ? iy * dRy : (iy - nyh) * dgy;
+}
+
+template <class R>
+void ModelT<R>::cond_quadratures(Quad1 &qt, Quad2 &qs) const
+{
+    const LensModel &L = cond;
+    qt = Quad1{};
+    qs = Quad2{};
+    // defocus spread: Gaussian (weight ti_a, sigma ti_sigma) + exponential
+    // (ti_beta), integrated with a Gauss-Hermite rule of the pooled width
+    if (L.ti_npts > 1 && !near_zero(L.ti_sigma)) {
+        const R s2 = L.ti_sigma * L.ti_sigma, b2 = L.ti_beta * L.ti_beta;
+        const double a = (double)L.ti_a / (std::sqrt(kTwoPi) * (double)L.ti_sigma);
+        const double b = 0.5 / (double)s2;
+        const double c = near_zero(L.ti_beta) ? 0.0 : (1.0 - (double)L.ti_a) / (2.0 * (double)L.ti_beta);
+        const double d = near_zero(L.ti_beta) ? 0.0 : 1.0 / (double)L.ti_beta;
+        const R st2 = L.ti_a * s2 + R(2) * (R(1) - L.ti_a) * b2;
+        const double sigma_t = std::sqrt((double)st2);
+        const double b_q = 0.5 / (sigma_t * sigma_t);
+        const GaussRule r = gauss_gen_hermite(L.ti_npts, 0.0, 0.0, b_q);
+        for (int i = 0; i < L.ti_npts; ++i) {
+            const double x = std::fabs(r.x[i]);
+            qt.x.push_back(r.x[i]);
+            qt.w.push_back(r.w[i] * (a * std::exp(-b * x * x + b_q * x * x) + c * std::exp(-d * x + b_q * x * x)));
+        }
+    } else {
+        qt.x = {0.0};
+        qt.w = {1.0};
+    }
+    // source spread: radial generalized-Laguerre rule x uniform azimuths
+    if ((L.si_rad_npts > 1 || L.si_azm_npts > 1) && (!near_zero(L.si_sigma) || !near_zero(L.si_beta))) {
+        const R s2 = L.si_sigma * L.si_sigma, b2 = L.si_beta * L.si_beta;
+        const double a = near_zero(L.si_sigma) ? 0.0 : (double)L.si_a / (kTwoPi * (double)s2);
+        const double b = near_zero(L.si_sigma) ? 0.0 : 0.5 / (double)s2;
+        const double c = near_zero(L.si_beta) ? 0.0 : (1.0 - (double)L.si_a) / (kTwoPi * (double)b2);
+        const double d = near_zero(L.si_beta) ? 0.0 : 1.0 / (double)L.si_beta;
+        const R st2 = (R(2) * L.si_a * s2 + R(6) * (R(1) - L.si_a) * b2) / R(6);
+        const double b_q = 1.0 / std::sqrt((double)st2);
+        // a one-point rule is (0, 1) whatever the weight (src/quadrature.hpp:37-43)
+        GaussRule qr = L.si_rad_npts > 1 ? gauss_gen_laguerre(L.si_rad_npts, 1.0, 0.0, b_q) : GaussRule{{0.0}, {1.0}};
+        for (size_t i = 0; i < qr.x.size(); ++i) {
+            const double r = std::fabs(qr.x[i]);
+            qr.w[i] *= a * std::exp(-b * r * r + b_q * r) + c * std::exp(-d * r + b_q * r);
+        }
+        const double h = kTwoPi / L.si_azm_npts;
+        for (size_t ir = 0; ir < qr.x.size(); ++ir)
+            for (int ia = 0; ia < L.si_azm_npts; ++ia) {
+                const double th = ia * h;
+                qs.x.push_back(qr.x[ir] * std::cos(th));
+                qs.y.push_back(qr.x[ir] * std::sin(th));
+                qs.w.push_back(qr.w[ir] * h);
+            }
+    } else {
+        qs.x = {0.0};
+        qs.y = {0.0};
+        qs.w = {1.0};
+    }
+}
+
+template <class R>
+void ModelT<R>::obj_temporal_quadrature(Quad1 &qt) const
+{
+    // Gauss-Hermite rule in double, stored in R, weights divided by sqrt(pi)
+    // in double and stored in R
+    const GaussRule r = obj.ti_npts > 1 ? gauss_gen_hermite(obj.ti_npts, 0.0, 0.0, 1.0) : GaussRule{{0.0}, {1.0}};
+    qt = Quad1{};
+    for (size_t i = 0; i < r.x.size(); ++i) {
+        qt.x.push_back((double)(R)r.x[i]);
+        qt.w.push_back((double)(R)((double)(R)r.w[i] / kSqrtPi));
+    }
 }
 
 template class ModelT<float>;

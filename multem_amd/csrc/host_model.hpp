@@ -47,6 +47,20 @@ struct LensModelT {
     R g2_min = 0, g2_max = 0;      // aperture (in R, as the reference evaluates it)
     bool phi_required = false;
     R ti_iehwgd = 0, si_theta_c = 0;   // partial-coherence envelope (objective lens)
+    // defocus and source spread functions (after Lens::set_input_data's clamps)
+    R ti_a = 1, ti_sigma = 0, ti_beta = 0;
+    int ti_npts = 1;
+    R si_a = 1, si_sigma = 0, si_beta = 0;
+    int si_rad_npts = 1, si_azm_npts = 1;
+};
+
+// One-dimensional (defocus) and two-dimensional (source) integration rules:
+// offsets and weights
+struct Quad1 {
+    std::vector<double> x, w;
+};
+struct Quad2 {
+    std::vector<double> x, y, w;
 };
 
 template <class R>
@@ -96,6 +110,14 @@ public:
     int illumination_model = 1;     // 1 coherent, 2 partially coherent (objective-lens envelope)
     int temporal_spatial_incoh = 1; // 1 both, 2 temporal only, 3 spatial only
     bool frozen() const { return pn_model == 3; }
+    // illumination model 4: defocus (and, for CBED, source) spread integrated
+    // numerically; the coherent STEM image keeps the coherent loop
+    // (src/tem_simulation.cuh:115-137)
+    bool full_integration() const { return illumination_model == 4 && !(is_stem() && pn_coh_contrib); }
+    // cond_lens_temporal_spatial_quadratures (src/cpu_fcns.hpp:1539-1626)
+    void cond_quadratures(Quad1 &qt, Quad2 &qs) const;
+    // obj_lens_temporal_spatial_quadratures, defocus part (src/cpu_fcns.hpp:1503-1510)
+    void obj_temporal_quadrature(Quad1 &qt) const;
 
     // electron / phonon
     int pn_model = 1, pn_coh_contrib = 0, pn_single_conf = 0, pn_nconf = 1, fp_iconf_0 = 1, pn_seed = 300183;

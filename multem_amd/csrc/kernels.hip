@@ -186,7 +186,8 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
             if (P.g2_min <= g2f && g2f < P.g2_max) {
                 const double gx = igx * P.dgx + P.gxu;
                 const double g2 = gx * gx + gy * gy;
-                double aber = (P.c10 + (P.c30 + P.c50 * g2) * g2) * g2;
+                const double c35 = (P.c30 + P.c50 * g2) * g2;
+                double aber = 0.0;   // azimuthal terms; the defocus term is per beam
                 if (P.azim) {
                     // powers of |g| and multiplicities of the 11 azimuthal terms
                     constexpr int pw[11] = {2, 3, 3, 4, 4, 5, 5, 5, 6, 6, 6};
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
                 }
                 for (int b = 0; b < m.nbeam; ++b) {
                     const Beam bm = a.beams[m.beam0 + b];
-                    double chi = bm.xph * gx + bm.yph * gy + aber;
+                    double chi = bm.xph * gx + bm.yph * gy + (bm.c10 + c35) * g2 + aber;
                     chi -= kTwoPiD * rint(chi * (1.0 / kTwoPiD));
                     R s, co;
                     rsincos((R)chi, &s, &co);
@@ -411,11 +412,12 @@ __global__ void k_det_reduce(const double *part, int n_members, int n_thk, int n
     const int m = blockIdx.x;
     const int iscan = members[m].iscan;
     if (iscan < 0) return;
+    const double wm = w * members[m].wq;
     for (int k = threadIdx.x; k < n_thk * n_det; k += blockDim.x) {
         const double *p = part + ((long long)m * n_thk * n_det + k) * ntile;
         double s = 0.0;
         for (int i = 0; i < ntile; ++i) s += p[i];
-        image[(long long)k * nscan + iscan] += w * s;
+        image[(long long)k * nscan + iscan] += wm * s;
     }
 }
 
@@ -496,14 +498,15 @@ __global__ void k_out_accum(OutArgsT<Z> a)
     const long long si = (long long)sx * a.ny + sy;
     for (int m = 0; m < a.n_members; ++m) {
         const Z v = a.src[(long long)m * a.member_stride + si];
+        const double w = a.members ? a.w * a.members[m].wq : a.w;
         if (a.mode & OUT_SET_PSI) {
             a.psi[2 * o] = v.x;
             a.psi[2 * o + 1] = v.y;
         }
-        if (a.mode & OUT_ADD_M2) a.m2[o] += a.w * (double)(v.x * v.x + v.y * v.y);
+        if (a.mode & OUT_ADD_M2) a.m2[o] += w * (double)(v.x * v.x + v.y * v.y);
         if (a.mode & OUT_ADD_PSI) {
-            a.psi[2 * o] += a.w * (double)v.x;
-            a.psi[2 * o + 1] += a.w * (double)v.y;
+            a.psi[2 * o] += w * (double)v.x;
+            a.psi[2 * o + 1] += w * (double)v.y;
         }
     }
     if (a.mode & OUT_ADD_PSI_FULL) {
@@ -511,8 +514,9 @@ __global__ void k_out_accum(OutArgsT<Z> a)
         for (long long k = o; k < (long long)a.nx * a.ny; k += (long long)a.nx_o * a.ny_o) {
             for (int m = 0; m < a.n_members; ++m) {
                 const Z v = a.src[(long long)m * a.member_stride + k];
-                a.psi_full[2 * k] += a.w * (double)v.x;
-                a.psi_full[2 * k + 1] += a.w * (double)v.y;
+                const double w = a.members ? a.w * a.members[m].wq : a.w;
+                a.psi_full[2 * k] += w * (double)v.x;
+                a.psi_full[2 * k + 1] += w * (double)v.y;
             }
         }
     }

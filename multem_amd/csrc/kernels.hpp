@@ -12,22 +12,27 @@ constexpr int kMaxDet = 8;
 constexpr int kMaxTypes = 16;
 
 // One wave in a batch: its probe position(s), its configuration's
-// transmission stack and its scan index.
+// transmission stack, its scan index and its integration weight.
 struct Member {
     int tslot;      // which transmission stack (configuration slot)
     int iscan;      // STEM scan index, -1 for padding members
     int beam0;      // first beam in the beam table
     int nbeam;      // number of beams (0: plane wave)
+    double wq;      // weight of the wave's defocus / source point (1 without partial-coherence integration)
 };
 
-// Probe phase x*gx + y*gy with x = -2 pi (x_b - lx/2): per beam, FP64.
+// One beam of a convergent incident wave: the position phase x*gx + y*gy
+// with x = -2 pi (x_b - lx/2), and the defocus coefficient of chi,
+// -pi lambda f with f the defocus referred to the first slice; FP64.
 struct Beam {
     double xph, yph;
+    double c10;
 };
 
 template <class R>
 struct ProbeParamsT {
-    double c10, c30, c50;         // -pi*lambda*f, -pi*Cs3*lambda^3/2, -pi*Cs5*lambda^5/3 (FP64)
+    double c10, c30, c50;         // -pi*lambda*f, -pi*Cs3*lambda^3/2, -pi*Cs5*lambda^5/3 (FP64);
+                                  // the probe kernel takes the defocus term from each Beam
     double dgx, dgy;              // FP64 reciprocal sampling (1/lx, 1/ly)
     R dgx_f, dgy_f;               // sampling in R, for the aperture mask as the reference evaluates it
     // azimuthal aberrations (Lens::eval_m ... eval_c_52_c_54_c_56, src/types.cuh:3180-3300):
@@ -146,6 +151,7 @@ struct OutArgsT {
     int nx, ny;
     int ix_0, iy_0, nx_o, ny_o;
     double w;
+    const Member *members; // per-member weight factor wq (null: 1)
     int mode;
     double *m2;            // [nx_o*ny_o]
     double *psi;           // complex [nx_o*ny_o]

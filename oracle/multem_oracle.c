@@ -626,6 +626,8 @@ typedef struct {
     real c_c_50, c_c_52, c_c_54, c_c_56;
     real g2_min, g2_max;
     real ti_iehwgd, si_theta_c, lambda2;   /* partial-coherence envelope (objective lens) */
+    real ti_a, ti_sigma, ti_beta; int ti_npts;                 /* defocus spread function */
+    real si_a, si_sigma, si_beta; int si_rad_npts, si_azm_npts;   /* source spread function */
 } lens_t;
 
 typedef struct {
@@ -1041,7 +1043,189 @@ static void lens_read(lens_t *l, const mt_lens *m)
     l->outer_aper_ang = (real)(m->outer_aper_ang * c_mrad_2_rad);
     l->zero_defocus_type = m->zero_defocus_type;
     l->zero_defocus_plane = (real)m->zero_defocus_plane;
+    l->ti_a = (real)m->ti_a; l->ti_sigma = (real)m->ti_sigma; l->ti_beta = (real)m->ti_beta; l->ti_npts = m->ti_npts;
+    l->si_a = (real)m->si_a; l->si_sigma = (real)m->si_sigma; l->si_beta = (real)m->si_beta;
+    l->si_rad_npts = m->si_rad_npts; l->si_azm_npts = m->si_azm_npts;
 }
+
+/* spread-function clamps of Lens::set_input_data (src/types.cuh:2953-2963) */
+static void lens_clamp_spreads(lens_t *l)
+{
+    l->ti_a = l->ti_a < 0 ? 0 : (l->ti_a > 1 ? 1 : l->ti_a);
+    if (l->ti_sigma < 0) l->ti_sigma = 0;
+    if (l->ti_beta < 0) l->ti_beta = 0;
+    if (l->ti_npts < 1) l->ti_npts = 1;
+    l->si_a = l->si_a < 0 ? 0 : (l->si_a > 1 ? 1 : l->si_a);
+    if (l->si_sigma < 0) l->si_sigma = 0;
+    if (l->si_beta < 0) l->si_beta = 0;
+    if (l->si_rad_npts < 1) l->si_rad_npts = 1;
+    if (l->si_azm_npts < 1) l->si_azm_npts = 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Gauss rules (cgqf kinds 5 and 6 of src/quadrature.hpp:32-126),      */
+/* Golub-Welsch: nodes = eigenvalues of the Jacobi matrix, weights =    */
+/* Christoffel numbers; eigenvalues by cyclic Jacobi rotations on the   */
+/* dense matrix (n is small).                                           */
+/* ------------------------------------------------------------------ */
+static void gw_rule(int n, const double *diag, const double *off, double mu0, double *x, double *w)
+{
+    double *A = calloc((size_t)n * n, sizeof(double));
+    for (int i = 0; i < n; i++) {
+        A[i * n + i] = diag[i];
+        if (i + 1 < n) A[i * n + i + 1] = A[(i + 1) * n + i] = off[i];
+    }
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double offs = 0, dn = 0;
+        for (int i = 0; i < n; i++) {
+            dn += A[i * n + i] * A[i * n + i];
+            for (int j = i + 1; j < n; j++) offs += A[i * n + j] * A[i * n + j];
+        }
+        if (offs == 0 || offs <= 1e-34 * dn) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[p * n + q];
+                if (apq == 0) continue;
+                double th = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                double c = 1 / sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < n; k++) {   /* columns p, q */
+                    double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - sn * akq;
+                    A[k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {   /* rows p, q */
+                    double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - sn * aqk;
+                    A[q * n + k] = sn * apk + c * aqk;
+                }
+                A[p * n + q] = A[q * n + p] = 0;
+            }
+    }
+    /* ascending nodes (insertion sort); weights 1 / sum_k p_k(x)^2 of the
+       orthonormal polynomials (Christoffel numbers), accurate also for the
+       tiny weights of the outer nodes */
+    for (int i = 0; i < n; i++) {
+        double v = A[i * n + i];
+        int j = i;
+        while (j > 0 && x[j - 1] > v) { x[j] = x[j - 1]; j--; }
+        x[j] = v;
+    }
+    for (int i = 0; i < n; i++) {
+        double pm = 0, p = 1 / sqrt(mu0), sum = p * p;
+        for (int k = 0; k + 1 < n; k++) {
+            double pn = ((x[i] - diag[k]) * p - (k > 0 ? off[k - 1] * pm : 0)) / off[k];
+            pm = p; p = pn;
+            sum += p * p;
+        }
+        w[i] = 1 / sum;
+    }
+    free(A);
+}
+
+/* kind 6: int |x-a|^alpha exp(-b (x-a)^2) f;  kind 5: int_a^inf (x-a)^alpha exp(-b (x-a)) f.
+   A rule of one point is (0, 1) whatever the kind (src/quadrature.hpp:37-43). */
+int orc_gauss_rule(int kind, int n, double alpha, double a, double b, double *x, double *w)
+{
+    if (n <= 1) { x[0] = 0; w[0] = 1; return MT_OK; }
+    if ((kind != 5 && kind != 6) || !(b > 0)) { set_err("gauss rule: kind 5 or 6, b > 0"); return MT_ERR_INVALID_ARGUMENT; }
+    double *d = malloc(sizeof(double) * (size_t)n), *e = malloc(sizeof(double) * (size_t)n);
+    double mu0, sl;
+    if (kind == 6) {
+        for (int k = 0; k < n; k++) d[k] = 0;
+        for (int k = 1; k < n; k++) e[k - 1] = sqrt((k + ((k & 1) ? alpha : 0)) / 2.0);
+        mu0 = tgamma((alpha + 1) / 2);
+        sl = 1 / sqrt(b);
+    } else {
+        for (int k = 1; k <= n; k++) d[k - 1] = 2.0 * k - 1 + alpha;
+        for (int k = 1; k < n; k++) e[k - 1] = sqrt(k * (k + alpha));
+        mu0 = tgamma(alpha + 1);
+        sl = 1 / b;
+    }
+    gw_rule(n, d, e, mu0, x, w);
+    double pw = pow(sl, alpha + 1);
+    for (int i = 0; i < n; i++) { x[i] = a + sl * x[i]; w[i] *= pw; }
+    free(d); free(e);
+    return MT_OK;
+}
+
+typedef struct { int n; double *x, *y, *w; } lens_quad;
+static void quad_free(lens_quad *q) { free(q->x); free(q->y); free(q->w); q->x = q->y = q->w = NULL; q->n = 0; }
+static void quad_one(lens_quad *q)
+{
+    q->n = 1;
+    q->x = calloc(1, sizeof(double)); q->y = calloc(1, sizeof(double)); q->w = malloc(sizeof(double));
+    q->w[0] = 1;
+}
+
+/* cond_lens_temporal_spatial_quadratures (src/cpu_fcns.hpp:1539-1626):
+   qt defocus offsets, qs source offsets (x, y) with weights */
+static void cond_quadratures(const lens_t *L, lens_quad *qt, lens_quad *qs)
+{
+    if (L->ti_npts > 1 && !is_zero_r(L->ti_sigma)) {
+        real s2 = L->ti_sigma * L->ti_sigma, b2 = L->ti_beta * L->ti_beta;
+        double a = (double)L->ti_a / (sqrt(c_2Pi) * (double)L->ti_sigma);
+        double b = 0.5 / (double)s2;
+        double c = is_zero_r(L->ti_beta) ? 0 : (1 - (double)L->ti_a) / (2 * (double)L->ti_beta);
+        double d = is_zero_r(L->ti_beta) ? 0 : 1.0 / (double)L->ti_beta;
+        real st2 = L->ti_a * s2 + (real)2 * ((real)1 - L->ti_a) * b2;
+        double sigma_t = sqrt((double)st2);
+        double b_q = 0.5 / (sigma_t * sigma_t);
+        int n = L->ti_npts;
+        qt->n = n;
+        qt->x = malloc(sizeof(double) * n); qt->y = calloc((size_t)n, sizeof(double)); qt->w = malloc(sizeof(double) * n);
+        orc_gauss_rule(6, n, 0, 0, b_q, qt->x, qt->w);
+        for (int i = 0; i < n; i++) {
+            double r = fabs(qt->x[i]);
+            qt->w[i] *= a * exp(-b * r * r + b_q * r * r) + c * exp(-d * r + b_q * r * r);
+        }
+    } else quad_one(qt);
+    if ((L->si_rad_npts > 1 || L->si_azm_npts > 1) && (!is_zero_r(L->si_sigma) || !is_zero_r(L->si_beta))) {
+        real s2 = L->si_sigma * L->si_sigma, b2 = L->si_beta * L->si_beta;
+        double a = is_zero_r(L->si_sigma) ? 0 : (double)L->si_a / (c_2Pi * (double)s2);
+        double b = is_zero_r(L->si_sigma) ? 0 : 0.5 / (double)s2;
+        double c = is_zero_r(L->si_beta) ? 0 : (1 - (double)L->si_a) / (c_2Pi * (double)b2);
+        double d = is_zero_r(L->si_beta) ? 0 : 1.0 / (double)L->si_beta;
+        real st2 = ((real)2 * L->si_a * s2 + (real)6 * ((real)1 - L->si_a) * b2) / (real)6;
+        double b_q = 1.0 / sqrt((double)st2);
+        int nr = L->si_rad_npts, na = L->si_azm_npts;
+        double *xr = malloc(sizeof(double) * nr), *wr = malloc(sizeof(double) * nr);
+        orc_gauss_rule(5, nr, 1, 0, b_q, xr, wr);
+        for (int i = 0; i < nr; i++) {
+            double r = fabs(xr[i]);
+            wr[i] *= a * exp(-b * r * r + b_q * r) + c * exp(-d * r + b_q * r);
+        }
+        double h = c_2Pi / na;
+        qs->n = nr * na;
+        qs->x = malloc(sizeof(double) * qs->n); qs->y = malloc(sizeof(double) * qs->n); qs->w = malloc(sizeof(double) * qs->n);
+        for (int ir = 0; ir < nr; ir++)
+            for (int ia = 0; ia < na; ia++) {
+                int k = ir * na + ia;
+                qs->x[k] = xr[ir] * cos(ia * h);
+                qs->y[k] = xr[ir] * sin(ia * h);
+                qs->w[k] = wr[ir] * h;
+            }
+        free(xr); free(wr);
+    } else quad_one(qs);
+}
+
+/* obj_lens_temporal_spatial_quadratures, defocus part (src/cpu_fcns.hpp:1503-1510):
+   Gauss-Hermite in double, stored in real; weights / sqrt(pi) */
+static void obj_temporal_quadrature(const lens_t *L, lens_quad *qt)
+{
+    int n = L->ti_npts > 1 ? L->ti_npts : 1;
+    qt->n = n;
+    qt->x = malloc(sizeof(double) * n); qt->y = calloc((size_t)n, sizeof(double)); qt->w = malloc(sizeof(double) * n);
+    orc_gauss_rule(6, n, 0, 0, 1, qt->x, qt->w);
+    for (int i = 0; i < n; i++) {
+        qt->x[i] = (double)(real)qt->x[i];
+        qt->w[i] = (double)(real)((double)(real)qt->w[i] / 1.772453850905516027298);
+    }
+}
+
+/* the two rules of a lens for tests: which 0 = condenser defocus, 1 = condenser
+   source, 2 = objective defocus; returns the number of points (n_max at most) */
+int orc_lens_quadrature(const mt_input_multislice *in, int which, int n_max, double *x, double *y, double *w);
 
 /* Lens::set_input_data (src/types.cuh:2924-2976), coefficient part */
 static void lens_set(lens_t *l, real E_0, const grid_t *g)
@@ -1256,7 +1440,6 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     s->theta = (real)(in->theta * c_deg_2_rad);
     s->phi = (real)(in->phi * c_deg_2_rad);
     s->illumination_model = in->illumination_model;
-    if (s->illumination_model == 4) FAIL(MT_ERR_UNSUPPORTED, "illumination_model 4 (full integration)");
 
     lens_read(&s->cond, &in->cond_lens);
     lens_set(&s->cond, s->E_0, &s->g);
@@ -1267,7 +1450,11 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     s->obj.ti_iehwgd = (real)1.414213562373095048802 * (real)fmax(0.0, in->obj_lens.ti_sigma);
     s->obj.si_theta_c = asin((real)1.414213562373095048802 * (real)fmax(0.0, in->cond_lens.si_sigma) * s->obj.lambda);
     s->obj.lambda2 = pow(s->obj.lambda, 2);
-    s->tsi = in->temporal_spatial_incoh;
+    /* no source spread: temporal incoherence only (src/input_multislice.cuh:746-751) */
+    s->tsi = is_zero_r((real)in->cond_lens.si_sigma) ? 2 : in->temporal_spatial_incoh;
+    /* the objective lens takes its source spread from cond_lens_si_* (ilc_multem.cu:246-250) */
+    s->obj.si_a = s->cond.si_a; s->obj.si_sigma = s->cond.si_sigma; s->obj.si_beta = s->cond.si_beta;
+    s->obj.si_rad_npts = s->cond.si_rad_npts; s->obj.si_azm_npts = s->cond.si_azm_npts;
 
     if (is_scanning_t(s->simulation_type)) {
         s->sc_type = in->scanning_type; s->sc_pbc = in->scanning_periodic != 0;
@@ -1311,7 +1498,18 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
         s->theta = is_zero_r(s->theta) ? 0 : asin(n * s->lambda * dgmin);
     }
     if (is_ewfs(s) || is_ewrs(s)) { s->pn_coh_contrib = 1; s->illumination_model = 1; }
-    if (is_cbed(s)) s->illumination_model = 1;
+    if (is_cbed(s) && s->illumination_model != 4) s->illumination_model = 1;
+    if (is_hrtem(s) && s->illumination_model == 4 && s->tsi != 2)
+        FAIL(MT_ERR_UNSUPPORTED, "HRTEM full integration with spatial incoherence (empty source quadrature)");
+    /* integration points (src/input_multislice.cuh:753-776), then the clamps */
+    lens_t *lenses[2] = { &s->cond, &s->obj };
+    for (int k = 0; k < 2; k++) {
+        lens_t *L = lenses[k];
+        if (s->illumination_model != 4) L->si_rad_npts = L->si_azm_npts = L->ti_npts = 1;
+        if (s->tsi == 2) L->si_rad_npts = L->si_azm_npts = 1;
+        else if (s->tsi == 3) L->ti_npts = 1;
+        lens_clamp_spreads(L);
+    }
     s->pn_dim_z = 0;  /* not sub-slicing */
 
     /* match_thickness (src/slicing.hpp:53-71) */
@@ -1805,24 +2003,36 @@ static int run_stem(sim_t *s, mt_output_multislice *out, int nthreads)
         if (!coh) { free(img); free(trans); free(res); free(items); set_err("out of memory"); return MT_ERR_OUT_OF_MEMORY; }
     }
     int st = MT_OK;
+    /* fully integrated illumination: the defocus loop of STEM_ISTEM
+       (src/tem_simulation.cuh:138-178); the coherent image keeps one point */
+    lens_quad qt = { 0 }, qs = { 0 };
+    if (s->illumination_model == 4 && !s->pn_coh_contrib) cond_quadratures(&s->cond, &qt, &qs);
+    else { quad_one(&qt); quad_one(&qs); }
+    double c_10_0 = (double)s->cond.c_10;
     for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
         move_atoms(s, iconf);
         st = build_trans_stack(s, trans);
         if (st != MT_OK) break;
-        stem_job J = { s, trans, nscan, items, res, coh, w, 0, PTHREAD_MUTEX_INITIALIZER, 0 };
-        int nt = nthreads > 0 ? nthreads : 1;
-        pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nt);
-        for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, stem_worker, &J);
-        for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
-        free(th);
-        /* image_tot[ithk].image[iDet][iscan] += w*I, in probe order (wave_function.cuh:145-151) */
-        for (int iscan = 0; iscan < nscan; iscan++)
-            for (int ithk = 0; ithk < s->n_thk; ithk++)
-                for (int id = 0; id < s->n_det; id++) {
-                    real val = (real)res[((size_t)iscan * s->n_thk + ithk) * s->n_det + id];
-                    img[((size_t)ithk * s->n_det + id) * nscan + iscan] += w * val;
-                }
+        for (int it = 0; it < qt.n; it++) {
+            lens_set_defocus(&s->cond, (real)(c_10_0 + qt.x[it]));
+            real wt = (real)((double)w * qt.w[it]);
+            stem_job J = { s, trans, nscan, items, res, coh, w, 0, PTHREAD_MUTEX_INITIALIZER, 0 };
+            int nt = nthreads > 0 ? nthreads : 1;
+            pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nt);
+            for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, stem_worker, &J);
+            for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+            free(th);
+            /* image_tot[ithk].image[iDet][iscan] += w*I, in probe order (wave_function.cuh:145-151) */
+            for (int iscan = 0; iscan < nscan; iscan++)
+                for (int ithk = 0; ithk < s->n_thk; ithk++)
+                    for (int id = 0; id < s->n_det; id++) {
+                        real val = (real)res[((size_t)iscan * s->n_thk + ithk) * s->n_det + id];
+                        img[((size_t)ithk * s->n_det + id) * nscan + iscan] += wt * val;
+                    }
+        }
     }
+    lens_set_defocus(&s->cond, (real)c_10_0);
+    quad_free(&qt); quad_free(&qs);
     if (st == MT_OK && out->image_tot)
         for (size_t i = 0; i < nimg; i++) out->image_tot[i] = img[i];
     if (st == MT_OK && coh && out->image_coh) {
@@ -1879,39 +2089,89 @@ typedef struct {
     int status;
 } ew_conf;
 
-/* EWFS_EWRS (tem_simulation.cuh:303-327) for one configuration.  Each slice
-   is used once, so its transmission function is built just before use
-   (the reference builds a stack when memory allows; same values). */
-static int run_ew_conf(sim_t *s, cplx **po_thk)
+/* The incident waves of one configuration: one, or for fully integrated
+   CBED one per (source point, defocus point) of CBED_CBEI
+   (src/tem_simulation.cuh:207-261), with the beams shifted by the source
+   offset (stored in real) and the condenser defocus c_10_0 + x_t. */
+typedef struct {
+    int nw, nb;
+    real *bx, *by;     /* [nw][nb] */
+    real *c10;         /* [nw] */
+    double *wq;        /* [nw] weight factor */
+} wave_set;
+
+static void wave_set_build(const sim_t *s, wave_set *ws)
+{
+    lens_quad qt = { 0 }, qs = { 0 };
+    if (is_cbed(s) && s->illumination_model == 4) cond_quadratures(&s->cond, &qt, &qs);
+    else { quad_one(&qt); quad_one(&qs); }
+    ws->nw = qs.n * qt.n;
+    ws->nb = s->n_iw > 0 ? s->n_iw : 1;
+    ws->bx = calloc((size_t)ws->nw * ws->nb, sizeof(real));
+    ws->by = calloc((size_t)ws->nw * ws->nb, sizeof(real));
+    ws->c10 = malloc(sizeof(real) * (size_t)ws->nw);
+    ws->wq = malloc(sizeof(double) * (size_t)ws->nw);
+    double c_10_0 = (double)s->cond.c_10;
+    for (int i = 0; i < qs.n; i++)
+        for (int t = 0; t < qt.n; t++) {
+            int k = i * qt.n + t;
+            for (int b = 0; b < s->n_iw; b++) {
+                ws->bx[(size_t)k * ws->nb + b] = (real)((double)s->iw_x[b] + qs.x[i]);
+                ws->by[(size_t)k * ws->nb + b] = (real)((double)s->iw_y[b] + qs.y[i]);
+            }
+            ws->c10[k] = (real)(c_10_0 + qt.x[t]);
+            ws->wq[k] = qs.w[i] * qt.w[t];
+        }
+    quad_free(&qt); quad_free(&qs);
+}
+
+static void wave_set_free(wave_set *ws) { free(ws->bx); free(ws->by); free(ws->c10); free(ws->wq); }
+
+/* EWFS_EWRS (tem_simulation.cuh:303-327) for one configuration and its
+   incident waves.  Each slice is used once, so its transmission function is
+   built just before use (the reference builds a stack when memory allows;
+   same values).  po_thk[w * n_thk + ithk]. */
+static int run_ew_conf(sim_t *s, const wave_set *ws, cplx **po_thk)
 {
     const grid_t *g = &s->g;
     size_t nxy = (size_t)g->nx * g->ny;
+    int nw = ws->nw;
     fft2 F;
     fft2_init(&F, g->nx, g->ny);
     cplx *T = malloc(sizeof(cplx) * nxy);
     real *V = malloc(sizeof(real) * nxy);
-    cplx *psi = malloc(sizeof(cplx) * nxy), *fpsi = malloc(sizeof(cplx) * nxy);
+    cplx *psi = malloc(sizeof(cplx) * nxy * (size_t)nw), *fpsi = malloc(sizeof(cplx) * nxy);
     cplx *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
     int st = (T && V && psi && fpsi && work) ? MT_OK : MT_ERR_OUT_OF_MEMORY;
     if (st == MT_OK) {
-        incident_wave(s, s->iw_x, s->iw_y, s->n_iw, 0, 0, psi, &F, work, fpsi);
+        real c_10_0 = s->cond.c_10;
+        for (int k = 0; k < nw; k++) {
+            lens_set_defocus(&s->cond, ws->c10[k]);
+            incident_wave(s, ws->bx + (size_t)k * ws->nb, ws->by + (size_t)k * ws->nb, s->n_iw, 0, 0,
+                          psi + (size_t)k * nxy, &F, work, fpsi);
+        }
+        lens_set_defocus(&s->cond, c_10_0);
         int space = is_ewrs(s) ? 1 : 2;
         for (int is = 0; is < s->n_slice && st == MT_OK; is++) {
             st = slice_potential(s, is, V);
             if (st != MT_OK) break;
             slice_transmission(s, V, T, &F, work);
-            for (size_t i = 0; i < nxy; i++) {
-                cplx a = psi[i], b = T[i];
-                psi[i].re = b.re * a.re - b.im * a.im;
-                psi[i].im = b.re * a.im + b.im * a.re;
-            }
             real dzs = (s->slice[is].z_e - s->slice[is].z_0) / cos(s->theta);
-            propagator(s, 1, dzs, psi, &F, work);
             int ithk = s->slice[is].ithk;
-            if (ithk >= 0) {
-                memcpy(po_thk[ithk], psi, sizeof(cplx) * nxy);
-                tilt_ramp_out(s, po_thk[ithk]);
-                propagator(s, space, s->thk[ithk].z_back_prop, po_thk[ithk], &F, work);
+            for (int k = 0; k < nw; k++) {
+                cplx *p = psi + (size_t)k * nxy;
+                for (size_t i = 0; i < nxy; i++) {
+                    cplx a = p[i], b = T[i];
+                    p[i].re = b.re * a.re - b.im * a.im;
+                    p[i].im = b.re * a.im + b.im * a.re;
+                }
+                propagator(s, 1, dzs, p, &F, work);
+                if (ithk >= 0) {
+                    cplx *o = po_thk[(size_t)k * s->n_thk + ithk];
+                    memcpy(o, p, sizeof(cplx) * nxy);
+                    tilt_ramp_out(s, o);
+                    propagator(s, space, s->thk[ithk].z_back_prop, o, &F, work);
+                }
             }
         }
     }
@@ -1924,10 +2184,9 @@ static int run_ew_conf(sim_t *s, cplx **po_thk)
    in place: MicroscopeEffects::CTF_TEM / PCTF_LI_WPO_TEM
    (src/microscope_effects.cuh:57-145) with apply_CTF / apply_PCTF
    (src/cgpu_fcns.cuh:1993-2040), then the unnormalised inverse FFT. */
-static void microscope_effects(const sim_t *s, cplx *f, const fft2 *F, cplx *work)
+static void apply_obj_lens(const sim_t *s, const lens_t *L, int coherent, cplx *f, const fft2 *F, cplx *work)
 {
     const grid_t *g = &s->g;
-    const lens_t *L = &s->obj;
     const real c_Pi_ = (real)3.141592653589793238463;
     int phireq = lens_phi_required(L);
     real ti = L->ti_iehwgd, sth = L->si_theta_c;
@@ -1941,7 +2200,7 @@ static void microscope_effects(const sim_t *s, cplx *f, const fft2 *F, cplx *wor
             real g2 = gx * gx + gy * gy;
             cplx v = { 0, 0 };
             if ((L->g2_min <= g2) && (g2 < L->g2_max)) {
-                if (s->illumination_model == 1) {
+                if (coherent) {
                     /* exp_i_chi with x = y = 0 (apply_CTF) */
                     real g4 = g2 * g2, g6 = g4 * g2;
                     real chi = (is_zero_r(L->c_10) ? 0 : L->c_c_10 * g2) + (is_zero_r(L->c_30) ? 0 : L->c_c_30 * g4)
@@ -1983,6 +2242,33 @@ static void microscope_effects(const sim_t *s, cplx *f, const fft2 *F, cplx *wor
     fft2_exec(F, f, 1, work);
 }
 
+static void microscope_effects(const sim_t *s, cplx *f, const fft2 *F, cplx *work)
+{
+    apply_obj_lens(s, &s->obj, s->illumination_model == 1, f, F, work);
+}
+
+/* MicroscopeEffects::num_int_TEM, temporal incoherence
+   (src/microscope_effects.cuh:150-202): sum over the defocus points
+   c_10 = c_10_0 + ti_iehwgd x_j of w_j |IFFT(CTF_j f)|^2, into m2 (real part) */
+static void num_int_tem(const sim_t *s, const cplx *f, cplx *m2, cplx *tmp, const fft2 *F, cplx *work)
+{
+    const grid_t *g = &s->g;
+    size_t nxy = (size_t)g->nx * g->ny;
+    lens_quad qt = { 0 };
+    obj_temporal_quadrature(&s->obj, &qt);
+    lens_t L = s->obj;
+    for (size_t i = 0; i < nxy; i++) { m2[i].re = 0; m2[i].im = 0; }
+    for (int j = 0; j < qt.n; j++) {
+        real c10 = s->obj.ti_iehwgd * (real)qt.x[j] + s->obj.c_10;
+        lens_set_defocus(&L, c10);
+        memcpy(tmp, f, sizeof(cplx) * nxy);
+        apply_obj_lens(s, &L, 1, tmp, F, work);
+        real wj = (real)qt.w[j];
+        for (size_t i = 0; i < nxy; i++) m2[i].re += wj * (tmp[i].re * tmp[i].re + tmp[i].im * tmp[i].im);
+    }
+    quad_free(&qt);
+}
+
 /* Accumulators of the exit-wave / CBED outputs over configurations. */
 typedef struct {
     double *m2, *pc, *pcs;
@@ -1990,35 +2276,42 @@ typedef struct {
     int ot;
     double w;
     fft2 F;            /* HRTEM image formation */
-    cplx *tmp, *work;
+    cplx *tmp, *tmp2, *work;
 } ew_acc;
 
-static void ew_accumulate(const sim_t *s, ew_acc *A, cplx **po)
+/* one wave's outputs po[ithk], weight w (the configuration weight times
+   the wave's integration weight) */
+static void ew_accumulate(const sim_t *s, ew_acc *A, cplx **po, double w)
 {
     for (int k = 0; k < s->n_thk; k++) {
         if (is_hrtem(s)) {
             /* add_scale_crop_shift_m2psi_tot_from_m2psi of the image; psi_coh in k-space */
+            if (s->illumination_model == 4) {
+                num_int_tem(s, po[k], A->tmp, A->tmp2, &A->F, A->work);
+                accumulate_output(s, A->tmp, w, 0, 0, A->m2 + A->no * k, 0);
+                continue;
+            }
             memcpy(A->tmp, po[k], sizeof(cplx) * A->nxy);
             microscope_effects(s, A->tmp, &A->F, A->work);
-            accumulate_output(s, A->tmp, A->w, 1, 0, A->m2 + A->no * k, 0);
+            accumulate_output(s, A->tmp, w, 1, 0, A->m2 + A->no * k, 0);
             if (s->pn_coh_contrib)
                 for (size_t i = 0; i < A->nxy; i++) {
-                    A->pc[2 * (A->nxy * k + i)] += A->w * po[k][i].re;
-                    A->pc[2 * (A->nxy * k + i) + 1] += A->w * po[k][i].im;
+                    A->pc[2 * (A->nxy * k + i)] += w * po[k][i].re;
+                    A->pc[2 * (A->nxy * k + i) + 1] += w * po[k][i].im;
                 }
             continue;
         }
         if (A->ot == MT_OT_PSI_COH) {
             accumulate_output(s, po[k], 1.0, 0, 1, A->pcs + 2 * A->no * k, 1);
         } else if (A->ot == MT_OT_M2PSI_TOT_PSI_COH) {
-            accumulate_output(s, po[k], A->w, 1, 0, A->m2 + A->no * k, 0);
-            accumulate_output(s, po[k], A->w, 0, 0, A->pcs + 2 * A->no * k, 1);
+            accumulate_output(s, po[k], w, 1, 0, A->m2 + A->no * k, 0);
+            accumulate_output(s, po[k], w, 0, 0, A->pcs + 2 * A->no * k, 1);
         } else {
-            accumulate_output(s, po[k], A->w, 1, 0, A->m2 + A->no * k, 0);
+            accumulate_output(s, po[k], w, 1, 0, A->m2 + A->no * k, 0);
             if (s->pn_coh_contrib)
                 for (size_t i = 0; i < A->nxy; i++) {
-                    A->pc[2 * (A->nxy * k + i)] += A->w * po[k][i].re;
-                    A->pc[2 * (A->nxy * k + i) + 1] += A->w * po[k][i].im;
+                    A->pc[2 * (A->nxy * k + i)] += w * po[k][i].re;
+                    A->pc[2 * (A->nxy * k + i) + 1] += w * po[k][i].im;
                 }
         }
     }
@@ -2031,6 +2324,7 @@ static void ew_accumulate(const sim_t *s, ew_acc *A, cplx **po)
 typedef struct {
     const mt_input_multislice *in;
     ew_acc *A;
+    const wave_set *ws;
     int iconf_0, iconf_e, nt;
     int next;                  /* configuration whose turn it is to accumulate */
     int status;
@@ -2040,6 +2334,13 @@ typedef struct {
 
 typedef struct { ew_job *J; int t; } ew_arg;
 
+/* weight of wave k: w_pr_0 qs.w qt.w stored in real (psi(T_r w, ...)); the
+   configuration weight alone without integration */
+static double wave_weight(const ew_acc *A, const wave_set *ws, int k)
+{
+    return ws->nw > 1 ? (double)(real)(A->w * ws->wq[k]) : A->w;
+}
+
 static void *ew_worker(void *arg)
 {
     ew_arg *a = (ew_arg *)arg;
@@ -2047,24 +2348,25 @@ static void *ew_worker(void *arg)
     int st;
     sim_t *s = sim_create(J->in, &st);
     size_t nxy = s ? (size_t)s->g.nx * s->g.ny : 0;
-    int nthk = s ? s->n_thk : 0;
-    cplx **po = s ? malloc(sizeof(cplx *) * (size_t)(nthk > 0 ? nthk : 1)) : NULL;
-    for (int k = 0; po && k < nthk; k++) po[k] = malloc(sizeof(cplx) * nxy);
+    int nthk = s ? s->n_thk : 0, npo = nthk * J->ws->nw;
+    cplx **po = s ? malloc(sizeof(cplx *) * (size_t)(npo > 0 ? npo : 1)) : NULL;
+    for (int k = 0; po && k < npo; k++) po[k] = malloc(sizeof(cplx) * nxy);
     for (int iconf = J->iconf_0 + a->t; iconf <= J->iconf_e; iconf += J->nt) {
         int ok = 0;
         if (s && po) {
             move_atoms(s, iconf);
-            ok = run_ew_conf(s, po) == MT_OK;
+            ok = run_ew_conf(s, J->ws, po) == MT_OK;
         }
         pthread_mutex_lock(&J->mu);
         while (J->next != iconf) pthread_cond_wait(&J->cv, &J->mu);
-        if (ok && J->status == MT_OK) ew_accumulate(s, J->A, po);
+        if (ok && J->status == MT_OK)
+            for (int k = 0; k < J->ws->nw; k++) ew_accumulate(s, J->A, po + (size_t)k * nthk, wave_weight(J->A, J->ws, k));
         else if (J->status == MT_OK) J->status = s ? MT_ERR_OUT_OF_MEMORY : st;
         J->next++;
         pthread_cond_broadcast(&J->cv);
         pthread_mutex_unlock(&J->mu);
     }
-    for (int k = 0; po && k < nthk; k++) free(po[k]);
+    for (int k = 0; po && k < npo; k++) free(po[k]);
     free(po);
     if (s) sim_free(s);
     return NULL;
@@ -2090,22 +2392,28 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
     A.pcs = calloc(2 * no * (size_t)s->n_thk, sizeof(double));
     fft2_init(&A.F, g->nx, g->ny);
     A.tmp = malloc(sizeof(cplx) * nxy);
+    A.tmp2 = malloc(sizeof(cplx) * nxy);
     A.work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    wave_set ws;
+    wave_set_build(s, &ws);
     int st = MT_OK;
     int nconf = s->pn_nconf - s->fp_iconf_0 + 1;
     int nt = nthreads > 1 ? (nthreads < nconf ? nthreads : nconf) : 1;
     if (nt <= 1) {
-        cplx **po = malloc(sizeof(cplx *) * (size_t)s->n_thk);
-        for (int k = 0; k < s->n_thk; k++) po[k] = malloc(sizeof(cplx) * nxy);
+        int npo = s->n_thk * ws.nw;
+        cplx **po = malloc(sizeof(cplx *) * (size_t)npo);
+        for (int k = 0; k < npo; k++) po[k] = malloc(sizeof(cplx) * nxy);
         for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
             move_atoms(s, iconf);
-            st = run_ew_conf(s, po);
-            if (st == MT_OK) ew_accumulate(s, &A, po);
+            st = run_ew_conf(s, &ws, po);
+            for (int k = 0; st == MT_OK && k < ws.nw; k++)
+                ew_accumulate(s, &A, po + (size_t)k * s->n_thk, wave_weight(&A, &ws, k));
         }
-        for (int k = 0; k < s->n_thk; k++) free(po[k]);
+        for (int k = 0; k < npo; k++) free(po[k]);
         free(po);
     } else {
         ew_job J;
+        J.ws = &ws;
         J.in = in; J.A = &A; J.iconf_0 = s->fp_iconf_0; J.iconf_e = s->pn_nconf; J.nt = nt;
         J.next = s->fp_iconf_0; J.status = MT_OK;
         pthread_mutex_init(&J.mu, NULL);
@@ -2141,8 +2449,9 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
         if (out->psi_coh && (ot == MT_OT_PSI_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
             memcpy(out->psi_coh, A.pcs, sizeof(double) * 2 * no * (size_t)s->n_thk);
     }
-    free(A.m2); free(A.pc); free(A.pcs); free(A.tmp); free(A.work);
+    free(A.m2); free(A.pc); free(A.pcs); free(A.tmp); free(A.tmp2); free(A.work);
     fft2_free(&A.F);
+    wave_set_free(&ws);
     return st;
 }
 
@@ -2303,4 +2612,20 @@ int orc_propagate(const mt_system_conf *sys, const mt_input_multislice *in, cons
     free(p); free(work);
     sim_free(s);
     return MT_OK;
+}
+
+int orc_lens_quadrature(const mt_input_multislice *in, int which, int n_max, double *x, double *y, double *w)
+{
+    int st;
+    sim_t *s = sim_create(in, &st);
+    if (!s) return -st;
+    lens_quad qt = { 0 }, qs = { 0 };
+    if (which == 2) obj_temporal_quadrature(&s->obj, &qt);
+    else cond_quadratures(&s->cond, &qt, &qs);
+    const lens_quad *q = which == 1 ? &qs : &qt;
+    int n = q->n;
+    for (int i = 0; i < n && i < n_max; i++) { x[i] = q->x[i]; if (y) y[i] = q->y[i]; w[i] = q->w[i]; }
+    quad_free(&qt); quad_free(&qs);
+    sim_free(s);
+    return n;
 }

@@ -49,6 +49,8 @@ def lib(precision="f64"):
         L.orc_mt19937_64.argtypes = [C.c_uint64, C.c_int, P(C.c_uint64)]
         L.orc_uniform_int.argtypes = [C.c_uint64, C.c_int, P(C.c_int)]
         L.orc_normal.argtypes = [C.c_uint64, C.c_int, dp]
+        L.orc_gauss_rule.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.orc_lens_quadrature.argtypes = [inp, C.c_int, C.c_int, dp, dp, dp]
         _libs[precision] = L
     return _libs[precision]
 
@@ -148,3 +150,24 @@ def fft2(a, inverse=False, precision="f64"):
     buf = np.ascontiguousarray(a, dtype=np.complex128).view(np.float64).ravel().copy()
     _check(L, L.orc_fft2(nx, ny, capi.dptr(buf), int(bool(inverse))))
     return buf.view(np.complex128).reshape(nx, ny)
+
+
+def gauss_rule(kind, n, alpha=0.0, a=0.0, b=1.0, precision="f64"):
+    """Gauss rule of kind 5 (generalized Laguerre) or 6 (generalized Hermite)."""
+    L = lib(precision)
+    x, w = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    _check(L, L.orc_gauss_rule(int(kind), int(n), float(alpha), float(a), float(b), capi.dptr(x), capi.dptr(w)))
+    return x, w
+
+
+def lens_quadrature(params, which, precision="f64"):
+    """which 0: condenser defocus points (x, w); 1: condenser source points
+    (x, y, w); 2: objective defocus points (x, w)."""
+    L = lib(precision)
+    s, keep = params.to_struct()
+    nmax = 4096
+    x, y, w = np.zeros(nmax), np.zeros(nmax), np.zeros(nmax)
+    n = L.orc_lens_quadrature(C.byref(s), int(which), nmax, capi.dptr(x), capi.dptr(y), capi.dptr(w))
+    if n < 0:
+        raise RuntimeError(L.orc_last_error().decode())
+    return (x[:n], y[:n], w[:n]) if which == 1 else (x[:n], w[:n])
