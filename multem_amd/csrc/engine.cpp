@@ -214,6 +214,7 @@ struct Setup {
         a.slice0 = slice0; a.n_slices = n_slices;
         a.tables = tables.p; a.rmax = rmax.p; a.n_types = (int)m.tables.size();
         a.w = m.Vr_factor;
+        a.weak_phase = m.interaction_model == 3 ? 1 : 0;
         a.trans = trans; a.V = V;
         check(launch_tf_build(a, s), "tf_build");
         if (m.bwl && trans) bandwidth_limit(trans, n_slices);
@@ -763,7 +764,10 @@ public:
 
     int col_flags(bool last) const
     {
-        return COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL | COL_DET_BWL : 0) |
+        // the (weak) phase object is not propagated inside the specimen, so its
+        // slice step has no bandwidth limit (Wave_Function::psi_slice,
+        // src/wave_function.cuh:229-238); the back-propagation's still applies
+        return COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | (m.bwl ? COL_DET_BWL : 0) |
                (last ? 0 : (COL_IFFT | COL_STORE));
     }
 
@@ -1177,7 +1181,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
             rt.slice = 0;
             rt.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
             ColArgsT<Z> ct = S.col_base(psi.p, psi.p, nm);
-            ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+            ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
             ct.px = px.p; ct.py = py.p;
             check(hipMemsetAsync(psi.p, 0, (size_t)nm * nxy * sizeof(Z), s), "hipMemsetAsync");
             choice = choose_variants(m.nx, m.ny, nm, rt, ct, (const ColArgsT<Z> *)nullptr, s);
@@ -1196,7 +1200,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
             r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
             check(launch_row_pass(m.ny, r, s), "row pass");
             ColArgsT<Z> c = S.col_base(psi.p, psi.p, nm);
-            c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+            c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
             c.px = px.p + (size_t)is * m.nx;
             c.py = py.p + (size_t)is * m.ny;
             c.variant = choice.col;

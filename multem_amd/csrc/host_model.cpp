@@ -379,12 +379,20 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     sim_type = in.simulation_type;
     if (!(is_stem() || is_cbed() || is_hrtem() || is_ewfs() || is_ewrs()))
         fail(MT_ERR_UNSUPPORTED, "simulation_type must be 11 (STEM), 21 (CBED), 32 (HRTEM), 51 (EWFS) or 52 (EWRS)");
-    if (in.interaction_model != 1) fail(MT_ERR_UNSUPPORTED, "only interaction_model 1 (multislice)");
+    if (in.interaction_model < 1 || in.interaction_model > 3)
+        fail(MT_ERR_INVALID_ARGUMENT, "interaction_model must be 1 (multislice), 2 (phase object) or 3 (weak phase object)");
+    interaction_model = in.interaction_model;
     if (in.potential_type != 6) fail(MT_ERR_UNSUPPORTED, "only potential_type 6 (Lobato)");
     if (in.pn_model == 2) fail(MT_ERR_UNSUPPORTED, "absorptive potential model");
     if (in.pn_model < 1 || in.pn_model > 3) fail(MT_ERR_INVALID_ARGUMENT, "pn_model must be 1..3");
+    // dz_Sub: the reference's sub-slice spline coefficients are never computed
+    // (get_cubic_poly_coef_Vz is commented out at src/projected_potential.cuh:118,148),
+    // so its sub-sliced potential is zero; Auto has no implementation at all
+    if (in.potential_slicing == 3)
+        fail(MT_ERR_UNSUPPORTED, "potential_slicing 3 (dz_Sub): the reference leaves the sub-slice potential "
+                                 "unevaluated (zero); use 1 (planes) or 2 (dz_Proj)");
     if (!(in.potential_slicing == 1 || in.potential_slicing == 2))
-        fail(MT_ERR_UNSUPPORTED, "potential_slicing 3/4 (sub-slicing)");
+        fail(MT_ERR_UNSUPPORTED, "potential_slicing must be 1 (planes) or 2 (dz_Proj)");
     if (std::fabs(in.spec_rot_theta) > 0) fail(MT_ERR_UNSUPPORTED, "specimen rotation");
     // exit waves are coherent; CBED is coherent unless fully integrated
     // (Input_Multislice::validate_parameters, src/input_multislice.cuh:672-682)
@@ -571,7 +579,16 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     zminmax(in_atoms, zmin, zmax);
 
     // thicknesses matched to planes (Slicing::match_thickness, src/slicing.hpp:53-71)
-    const int thick_type = in.thick_type;
+    // validate_parameters for the (weak) phase object (src/input_multislice.cuh:686-728):
+    // slicing by planes, through-slices thicknesses become through-thickness,
+    // and a whole-specimen run slices by dz
+    int thick_type = in.thick_type;
+    int pot_sli = in.potential_slicing;
+    if (!multislice()) {
+        pot_sli = 1;
+        if (thick_type == 3) thick_type = 2;
+        if (thick_type == 1) pot_sli = 2;
+    }
     if (thick_type == 1) {
         thick = {zmax};
     } else {
@@ -579,7 +596,7 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
         std::vector<R> zc;
         for (int i = 0; i < na; ++i) if (in_atoms.region[i] == 0) zc.push_back(in_atoms.z[i]);
         std::sort(zc.begin(), zc.end());
-        std::vector<R> zp = (in.potential_slicing == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
+        std::vector<R> zp = (pot_sli == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
         unique_sorted(zp);
         if (thick_type == 3) zp = slice_bounds(zp, spec_dz, zmin - R(2.5), zmax + R(2.5));
         std::vector<R> req;
@@ -661,17 +678,22 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     {
         std::vector<R> zc;
         for (size_t i = 0; i < atoms_u.z.size(); ++i) if (atoms_u.region[i] == 0) zc.push_back(atoms_u.z[i]);
-        z_plane = (in.potential_slicing == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
+        z_plane = (pot_sli == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
         unique_sorted(z_plane);
     }
     R uzmin, uzmax;
     zminmax(atoms_u, uzmin, uzmax);
     const R s_z_int = (uzmax + R_int_max) - (uzmin - R_int_max);
-    if (s_z_int < R(2.0) * spec_dz || (z_plane.size() == 1 && in.potential_slicing == 1))
-        fail(MT_ERR_UNSUPPORTED, "specimen reduces to a single phase object");
+    // a thin specimen is a phase object (Spec::set_input_data, src/spec.hpp:65-77)
+    if (interaction_model == 2 || s_z_int < R(2.0) * spec_dz || (z_plane.size() == 1 && pot_sli == 1)) {
+        interaction_model = 2;
+        if (thick_type == 3) thick_type = 2;
+    }
 
     // slices and thicknesses (Slicing::calculate, src/slicing.hpp:93-100 / 253-403)
-    std::vector<R> zs = slice_bounds(z_plane, spec_dz, uzmin - R_int_max, uzmax + R_int_max);
+    // the phase object's dz is the specimen's interaction depth (src/spec.hpp:67)
+    const R dz_sl = interaction_model == 2 ? s_z_int : spec_dz;
+    std::vector<R> zs = slice_bounds(z_plane, dz_sl, uzmin - R_int_max, uzmax + R_int_max);
     const std::vector<R> &az = atoms_u.z;
     auto lower = [&](R v) { return (int)(std::lower_bound(az.begin(), az.end(), v) - az.begin()); };
     for (size_t is = 0; is + 1 < zs.size(); ++is) {
@@ -694,13 +716,31 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
         R ze = zs[ti.islice + 1];
         int iatom_e = (az.back() <= ze) ? (int)az.size() - 1 : lower(ze) - 1;
         if (iatom_e < 0) iatom_e = 0;
+        ti.iatom_e = iatom_e;
         R zdp = zero_defocus_plane(obj, az[0], az[iatom_e]);
-        if (thick_type == 3) ti.z_back_prop = R(0.0);
+        if (!multislice()) ti.z_back_prop = zdp - t;   // src/slicing.hpp:326-333
+        else if (thick_type == 3) ti.z_back_prop = R(0.0);
         else {
             ti.z_back_prop = zdp - R(0.5) * (zs[ti.islice] + zs[ti.islice + 1]);
             if (std::fabs(ti.z_back_prop) < R(1e-4)) ti.z_back_prop = R(0.0);
         }
         thk.push_back(ti);
+    }
+    if (!multislice()) {
+        // one slice per thickness, from the previous thickness (or the
+        // specimen's lower bound) to this one (get_slicing, src/slicing.hpp:341-355)
+        slices.clear();
+        const R z_int_min = uzmin - R_int_max, z_int_max = uzmax + R_int_max;
+        for (size_t k = 0; k < thk.size(); ++k) {
+            SliceInfoT<R> s;
+            s.z_0 = k == 0 ? z_int_min : thk[k - 1].z;
+            s.z_e = thk.size() == 1 ? z_int_max : thk[k].z;
+            s.iatom_0 = k == 0 ? 0 : thk[k - 1].iatom_e + 1;
+            s.iatom_e = thk[k].iatom_e;
+            s.ithk = (int)k;
+            slices.push_back(s);
+            thk[k].islice = (int)k;
+        }
     }
     for (size_t k = 0; k < thk.size(); ++k) slices[thk[k].islice].ithk = (int)k;
 }

@@ -75,6 +75,7 @@ struct ThickInfoT {
     R z;
     int islice;
     R z_back_prop;
+    int iatom_e;    // last atom below the slice boundary that the thickness falls in
 };
 
 template <class R>
@@ -110,6 +111,10 @@ public:
     int illumination_model = 1;     // 1 coherent, 2 partially coherent (objective-lens envelope)
     int temporal_spatial_incoh = 1; // 1 both, 2 temporal only, 3 spatial only
     bool frozen() const { return pn_model == 3; }
+    // 1 multislice, 2 phase object, 3 weak-phase object (after validation: a
+    // thin specimen becomes a phase object, src/spec.hpp:65-77)
+    int interaction_model = 1;
+    bool multislice() const { return interaction_model == 1; }
     // illumination model 4: defocus (and, for CBED, source) spread integrated
     // numerically; the coherent STEM image keeps the coherent loop
     // (src/tem_simulation.cuh:115-137)
@@ -166,7 +171,8 @@ public:
     int n_slice() const { return (int)slices.size(); }
     int n_conf_run() const { return pn_nconf - fp_iconf_0 + 1; }
     R phonon_weight() const;      // get_phonon_rot_weight (input_multislice.cuh:885-889)
-    R dz_slice(int is) const { return (slices[is].z_e - slices[is].z_0) / std::cos(theta); }
+    // propagation distance after slice is (none for the (weak) phase object)
+    R dz_slice(int is) const { return multislice() ? (slices[is].z_e - slices[is].z_0) / std::cos(theta) : R(0); }
     R z_init() const { return R(0.5) * (slices[0].z_e + slices[0].z_0); }
 
     // Displaced positions of configuration iconf (Spec::move_atoms, src/spec.hpp:85-127);

@@ -397,8 +397,15 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
         const long long sl = is - a.slice0;
         if (a.V) a.V[sl * nxy + (long long)px * a.ny + py] = vacc[h];
         if (a.trans) {
+            // exp(i w V), or 1 + i w V for the weak-phase object (transmission_function,
+            // src/cgpu_fcns.cuh:3071-3086)
             R s, c;
-            rsincos(a.w * vacc[h], &s, &c);
+            if (a.weak_phase) {
+                c = R(1);
+                s = a.w * vacc[h];
+            } else {
+                rsincos(a.w * vacc[h], &s, &c);
+            }
             a.trans[sl * nxy + (long long)shift_idx(px, a.nxh) * a.ny + shift_idx(py, a.nyh)] = cmake(c, s);
         }
     }

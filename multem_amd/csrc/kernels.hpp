@@ -131,6 +131,7 @@ struct TfArgsT {
     const cplx_t<R> *rmax;        // [type]: R_max, R2_max
     int n_types;
     R w;                          // sigma factor (Vr_factor)
+    int weak_phase;               // transmission 1 + i w V instead of exp(i w V)
     cplx_t<R> *trans;             // out, shifted layout: [slice][nx*ny] (relative to slice0), may be null
     R *V;                         // out, natural layout: [slice][nx*ny], may be null
 };

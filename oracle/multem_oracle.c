@@ -976,7 +976,11 @@ static void get_thick(sim_t *s, const atoms_t *ar)
         int iatom_e = fd_by_z_e(ar->z, ar->n, zs[islice + 1], 0);
         t->iatom_e = iatom_e;
         t->z_zero_def_plane = lens_zero_defocus_plane(&s->obj, ar->z[0], ar->z[iatom_e]);
-        if (s->thick_type == 3) {
+        if (s->interaction_model != 1) {
+            /* (weak) phase object: one slice per thickness (src/slicing.hpp:326-333) */
+            t->islice = ik;
+            t->z_back_prop = t->z_zero_def_plane - t->z;
+        } else if (s->thick_type == 3) {
             t->z_back_prop = 0;
         } else {
             t->z_back_prop = t->z_zero_def_plane - 0.5 * (zs[islice] + zs[islice + 1]);
@@ -989,6 +993,23 @@ static void get_thick(sim_t *s, const atoms_t *ar)
 static void get_slicing(sim_t *s, const atoms_t *a)
 {
     free(s->slice);
+    if (s->interaction_model != 1) {
+        /* (weak) phase object: slices between consecutive thicknesses
+           (src/slicing.hpp:341-355) */
+        s->n_slice = s->n_thk;
+        s->slice = calloc((size_t)s->n_slice, sizeof(slice_t));
+        for (int is = 0; is < s->n_slice; is++) {
+            slice_t *sl = &s->slice[is];
+            sl->z_0 = (is == 0) ? a->z_int_min : s->thk[is - 1].z;
+            sl->z_e = (s->n_thk == 1) ? a->z_int_max : s->thk[is].z;
+            sl->z_int_0 = sl->z_0;
+            sl->z_int_e = sl->z_e;
+            sl->iatom_0 = (is == 0) ? 0 : s->thk[is - 1].iatom_e + 1;
+            sl->iatom_e = s->thk[is].iatom_e;
+            sl->ithk = is;
+        }
+        return;
+    }
     s->n_slice = s->n_zslice - 1;
     s->slice = calloc((size_t)s->n_slice, sizeof(slice_t));
     for (int is = 0; is < s->n_slice; is++) {
@@ -1358,7 +1379,8 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     if (!(is_stem(s) || is_cbed(s) || is_hrtem(s) || is_ewfs(s) || is_ewrs(s)))
         FAIL(MT_ERR_UNSUPPORTED, "simulation_type not provided (STEM 11, CBED 21, HRTEM 32, EWFS 51, EWRS 52)");
     s->interaction_model = in->interaction_model;
-    if (s->interaction_model != 1) FAIL(MT_ERR_UNSUPPORTED, "only interaction_model 1 (multislice)");
+    if (s->interaction_model < 1 || s->interaction_model > 3)
+        FAIL(MT_ERR_INVALID_ARGUMENT, "interaction_model must be 1, 2 or 3");
     s->potential_type = in->potential_type;
     if (s->potential_type != 6) FAIL(MT_ERR_UNSUPPORTED, "only potential_type 6 (Lobato)");
     s->pn_model = in->pn_model;
@@ -1511,6 +1533,13 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
         lens_clamp_spreads(L);
     }
     s->pn_dim_z = 0;  /* not sub-slicing */
+    /* (weak) phase object: slicing by planes, through-slices becomes
+       through-thickness, whole specimen slices by dz (src/input_multislice.cuh:686-728) */
+    if (s->interaction_model != 1) {
+        s->potential_slicing = 1;
+        if (s->thick_type == 3) s->thick_type = 2;
+        if (s->thick_type == 1) s->potential_slicing = 2;
+    }
 
     /* match_thickness (src/slicing.hpp:53-71) */
     atoms_statistic(s, &s->in_atoms, 0);
@@ -1582,8 +1611,14 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
         atoms_sort_by_z(&s->atoms_u);
         atoms_statistic(s, &s->atoms_u, 1);
         s->z_plane = get_z_plane(s->potential_slicing, &s->atoms_u, &s->n_plane);
-        if ((s->atoms_u.s_z_int < 2.0 * s->g.dz) || (s->n_plane == 1 && s->potential_slicing == 1))
-            FAIL(MT_ERR_UNSUPPORTED, "specimen reduces to a single phase object");
+        /* a thin specimen is a phase object (src/spec.hpp:65-77) */
+        if (s->interaction_model == 2 || (s->atoms_u.s_z_int < 2.0 * s->g.dz)
+            || (s->n_plane == 1 && s->potential_slicing == 1)) {
+            s->interaction_model = 2;
+            s->g.dz = s->atoms_u.s_z_int;
+            s->atoms_u.dz = s->g.dz;
+            if (s->thick_type == 3) s->thick_type = 2;
+        }
         atoms_alloc(&s->atoms, s->atoms_u.n);
         for (int i = 0; i < s->atoms_u.n; i++) atoms_copy_row(&s->atoms, i, &s->atoms_u, i);
         s->atoms.l_x = s->atoms_u.l_x; s->atoms.l_y = s->atoms_u.l_y; s->atoms.l_z = s->atoms_u.l_z;
@@ -1705,10 +1740,11 @@ static void slice_transmission(sim_t *s, const real *V, cplx *T, const fft2 *F, 
     const grid_t *g = &s->g;
     size_t nxy = (size_t)g->nx * g->ny;
     real w = s->mul_sign * get_Vr_factor(s->E_0, s->theta);
+    int wpo = s->interaction_model == 3;   /* weak phase object: 1 + i theta (cgpu_fcns.cuh:3071-3086) */
     for (size_t i = 0; i < nxy; i++) {
         real theta = w * V[i];
-        T[i].re = cos(theta);
-        T[i].im = sin(theta);
+        T[i].re = wpo ? 1 : cos(theta);
+        T[i].im = wpo ? theta : sin(theta);
     }
     if (g->bwl) {
         fft2_exec(F, T, 0, work);
@@ -1886,7 +1922,7 @@ static void *stem_worker(void *arg)
                 psi[i].im = b.re * a.im + b.im * a.re;
             }
             real dzs = (s->slice[is].z_e - s->slice[is].z_0) / cos(s->theta);
-            propagator(s, 1, dzs, psi, &F, work);
+            if (s->interaction_model == 1) propagator(s, 1, dzs, psi, &F, work);   /* psi_slice */
             int ithk = s->slice[is].ithk;
             if (ithk >= 0) {
                 memcpy(po, psi, sizeof(cplx) * nxy);
@@ -2165,7 +2201,7 @@ static int run_ew_conf(sim_t *s, const wave_set *ws, cplx **po_thk)
                     p[i].re = b.re * a.re - b.im * a.im;
                     p[i].im = b.re * a.im + b.im * a.re;
                 }
-                propagator(s, 1, dzs, p, &F, work);
+                if (s->interaction_model == 1) propagator(s, 1, dzs, p, &F, work);   /* psi_slice */
                 if (ithk >= 0) {
                     cplx *o = po_thk[(size_t)k * s->n_thk + ithk];
                     memcpy(o, p, sizeof(cplx) * nxy);

@@ -235,7 +235,7 @@ def test_group_uneven_sizes_matches_solo_runs(batch):
     finally:
         engine.set_batch(0, True)
         engine.set_pass_variants(-1, -1, -1)
-    assert plan.startswith("group ") and plan.count("stem nx=") == 3
+    assert plan.startswith("group ") and plan.count("stem fp32 nx=") == 3
     for a, o in zip(solo, grp):
         assert np.array_equal(a, o.image_tot)
 
