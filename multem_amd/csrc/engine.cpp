@@ -396,6 +396,31 @@ struct Setup {
         check(hipStreamSynchronize(s), "sync");
     }
 
+    // Spectrum of the user-defined incident wave: fft2_shift then the
+    // unnormalised forward FFT (Incident_Wave::set_input_data,
+    // src/incident_wave.cuh:50-55), natural k-space order
+    void user_wave_spectrum(DBuf<Z> &f)
+    {
+        std::vector<Z> h(nxy);
+        for (size_t i = 0; i < nxy; ++i) h[i] = cz<Z>(m.iw_psi[2 * i], m.iw_psi[2 * i + 1]);
+        DBuf<Z> a(nxy);
+        a.upload(h.data(), nxy, s);
+        f.alloc(nxy);
+        check(launch_shift_complex(a.p, f.p, nx, ny, s), "shift");
+        Member hm{0, -1, 0, 0, 1.0};
+        DBuf<Member> mem(1);
+        mem.upload(&hm, 1, s);
+        RowArgsT<Z> r{};
+        r.src = f.p; r.dst = f.p; r.member_stride = (long long)nxy; r.nrows = nx; r.n_members = 1;
+        r.tw = tw_y.p; r.members = mem.p;
+        r.flags = ROW_LOAD | ROW_FFT | ROW_STORE;
+        check(launch_row_pass(ny, r, s), "row pass");
+        ColArgsT<Z> c = col_base(f.p, f.p, 1);
+        c.flags = COL_LOAD | COL_FFT | COL_STORE;
+        check(launch_col_pass(nx, c, false, false, s), "col pass");
+        check(hipStreamSynchronize(s), "sync");
+    }
+
     // propagator tables for the given distances: px [nz][nx], py [nz][ny]
     void prop_tables(const std::vector<double> &z, DBuf<Z> &px, DBuf<Z> &py)
     {
@@ -590,6 +615,7 @@ public:
     DBuf<Z> px, py;           // [n_slice + n_thk]: slice propagators, then back-propagations
     DBuf<Z> rx, ry, scratch;   // tilted illumination: output ramp and the output copy of the batch
     DBuf<double2> coh;              // pn_coh_contrib: sum over configurations of w psi(g), [scan][n_thk][nxy]
+    DBuf<Z> fpsi0;                  // iw_type 3: spectrum of the user's incident wave
     DBuf<vec4_t<R>> atoms;
     ProbeParamsT<R> P{};
     unsigned int nap = 0;
@@ -668,6 +694,7 @@ public:
         nap = S.count_aperture();
         P = S.probe_params(nap);
         w = m.phonon_weight();
+        if (m.iw_type == 3) S.user_wave_spectrum(fpsi0);
         {
             // STEM integrates the defocus spread only (src/tem_simulation.cuh:138-178)
             Quad1 qt;
@@ -817,7 +844,8 @@ public:
     void pipeline(hipStream_t st, std::vector<Event> *ev)
     {
         ColArgsT<Z> c0 = S.col_base(nullptr, psi.p, B);
-        c0.flags = COL_IFFT | COL_STORE;
+        c0.flags = COL_IFFT | COL_STORE | (m.iw_type == 3 ? COL_USER_WAVE : 0);
+        c0.fpsi0 = fpsi0.p;
         c0.members = members.p; c0.beams = beams.p; c0.probe = P;
         check(launch_col_pass(m.nx, c0, true, false, st), "probe pass");
         int k = 0;
@@ -1095,7 +1123,8 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     std::vector<double> tc10, twq;
     S.cond_defocus_points(qt, tc10, twq);
     const int n_s = (int)qs.w.size(), n_t = (int)tc10.size(), nq = n_s * n_t;
-    const int nbeam = m.iw_type == 2 ? (int)m.iw_x.size() : 0;
+    // beams: every iw_x, iw_y of a convergent wave; the first for a user wave
+    const int nbeam = m.iw_type == 2 ? (int)m.iw_x.size() : (m.iw_type == 3 ? 1 : 0);
 
     const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(Z);
     const size_t per_conf = stack_bytes + 2 * (size_t)nq * nxy * sizeof(Z);
@@ -1129,10 +1158,13 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     std::vector<double> cwq{1.0};   // weights of the objective-lens tables
     if (m.is_hrtem()) S.ctf_tables(ctf, cwq);
     const int n_ctf = (int)cwq.size();
-    // several tables of a tilted run: the ramp output must survive each
-    DBuf<Z> scratch2(m.is_hrtem() && n_ctf > 1 && m.tilted() ? (size_t)K * nxy : 0);
+    // a tilted HRTEM run that reads the ramp output more than once (several
+    // lens tables, or the coherent pass) keeps it apart from the output passes
+    DBuf<Z> scratch2(m.is_hrtem() && m.tilted() && (n_ctf > 1 || need_full) ? (size_t)K * nxy : 0);
     ProbeParamsT<R> P{};
-    if (m.iw_type == 2) {
+    DBuf<Z> fpsi0;
+    if (m.iw_type == 3) S.user_wave_spectrum(fpsi0);
+    if (m.iw_type >= 2) {
         P = S.probe_params(S.count_aperture());
         std::vector<Beam> hb;
         for (int i_s = 0; i_s < n_s; ++i_s)
@@ -1189,9 +1221,10 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
             tm.plan += " row_variant=" + std::to_string(choice.row) + " col_variant=" + std::to_string(choice.col);
         }
         r.variant = choice.row;
-        if (m.iw_type == 2) {
+        if (m.iw_type >= 2) {
             ColArgsT<Z> cp = S.col_base(nullptr, psi.p, nm);
-            cp.flags = COL_IFFT | COL_STORE;
+            cp.flags = COL_IFFT | COL_STORE | (m.iw_type == 3 ? COL_USER_WAVE : 0);
+            cp.fpsi0 = fpsi0.p;
             cp.members = members.p; cp.beams = beams.p; cp.probe = P;
             check(launch_col_pass(m.nx, cp, true, false, s), "probe pass");
         }
@@ -1252,8 +1285,26 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                 oa.psi_full = pfull.p + (need_full ? (size_t)ithk * nxy * 2 : 0);
                 if (ot == MT_OT_PSI_COH) oa.mode = OUT_SET_PSI;
                 else if (ot == MT_OT_M2PSI_TOT_PSI_COH) oa.mode = OUT_ADD_M2 | OUT_ADD_PSI;
-                else oa.mode = OUT_ADD_M2 | (need_full ? OUT_ADD_PSI_FULL : 0);
+                else oa.mode = OUT_ADD_M2 | (need_full && !m.is_hrtem() ? OUT_ADD_PSI_FULL : 0);
                 check(launch_out_accum(oa, s), "output accumulate");
+            }
+            if (need_full && m.is_hrtem()) {
+                // psi_coh += w O(g), the back-propagated wave before the objective
+                // lens (add_scale_psi_coh, src/wave_function.cuh:166-172)
+                ColArgsT<Z> co = S.col_base(osrc0, odst, nm);
+                co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE;
+                co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
+                co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
+                co.p_scale = (R)m.ny;
+                check(launch_col_pass(m.nx, co, false, false, s), "coherent output col pass");
+                OutArgsT<Z> oa{};
+                oa.src = odst; oa.member_stride = (long long)nxy; oa.n_members = nm;
+                oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
+                oa.w = w;
+                oa.members = members.p;
+                oa.psi_full = pfull.p + (size_t)ithk * nxy * 2;
+                oa.mode = OUT_ADD_PSI_FULL;
+                check(launch_out_accum(oa, s), "coherent output accumulate");
             }
         }
         progress_add(nm);
@@ -1267,7 +1318,45 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
         check(hipMemcpyAsync(out->m2psi_tot, m2.p, (size_t)n_thk * no * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
     if (out->psi_coh && (ot == MT_OT_PSI_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
         check(hipMemcpyAsync(out->psi_coh, pc.p, (size_t)n_thk * no * 2 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-    if (need_full && out->m2psi_coh) {
+    if (need_full && out->m2psi_coh && m.is_hrtem()) {
+        // set_m2psi_coh for HRTEM (src/wave_function.cuh:206-215): the objective
+        // lens applied to psi_coh, |IFFT|^2 (summed over the defocus points of a
+        // fully integrated lens), cropped and shifted
+        DBuf<double> m2c(no);
+        std::vector<double> zero(no, 0.0), hp(nxy * 2);
+        m2c.upload(zero.data(), no, s);
+        std::vector<Z> hz(nxy);
+        DBuf<Z> pz(nxy), wz(nxy);
+        for (int k = 0; k < n_thk; ++k) {
+            check(hipMemcpyAsync(hp.data(), pfull.p + (size_t)k * nxy * 2, nxy * 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "sync");
+            for (size_t i = 0; i < nxy; ++i) hz[i] = cz<Z>(hp[2 * i], hp[2 * i + 1]);
+            pz.upload(hz.data(), nxy, s);
+            for (int jc = 0; jc < n_ctf; ++jc) {
+                ColArgsT<Z> cc = S.col_base(pz.p, wz.p, 1);
+                cc.flags = COL_LOAD | COL_MUL_CTF | COL_IFFT | COL_STORE;
+                cc.ctf = ctf.p + (size_t)jc * nxy;
+                check(launch_col_pass(m.nx, cc, false, false, s), "coherent image col pass");
+                RowArgsT<Z> rr{};
+                rr.src = wz.p; rr.dst = wz.p; rr.member_stride = (long long)nxy; rr.nrows = m.nx; rr.n_members = 1;
+                rr.tw = S.tw_y.p; rr.members = members.p;
+                rr.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+                check(launch_row_pass(m.ny, rr, s), "coherent image row pass");
+                OutArgsT<Z> oa{};
+                oa.src = wz.p; oa.member_stride = (long long)nxy; oa.n_members = 1;
+                oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
+                oa.w = cwq[jc];
+                oa.m2 = m2c.p;
+                oa.mode = OUT_ADD_M2;
+                check(launch_out_accum(oa, s), "coherent image accumulate");
+            }
+            check(hipMemcpyAsync(out->m2psi_coh + (size_t)k * no, m2c.p, no * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "D2H");
+            check(hipStreamSynchronize(s), "sync");
+            m2c.upload(zero.data(), no, s);
+        }
+    } else if (need_full && out->m2psi_coh) {
         DBuf<double> tmp(no);
         for (int k = 0; k < n_thk; ++k) {
             check(launch_m2_from_psi(pfull.p + (size_t)k * nxy * 2, m.nx, m.ny, m.oa_ix_0, m.oa_iy_0, nxo, nyo, tmp.p, s),
@@ -1430,13 +1519,17 @@ void incident_wave(const ModelT<R> &m, int device, double *psi0)
         std::vector<Beam> hb;
         for (size_t i = 0; i < m.iw_x.size(); ++i) hb.push_back(S.beam(m.iw_x[i], m.iw_y[i], P.c10));
         if (hb.empty()) throw Error{MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y"};
+        if (m.iw_type == 3) hb.resize(1);   // a user wave is placed at the first position
         DBuf<Beam> beams(hb.size());
         beams.upload(hb.data(), hb.size(), st.s);
         Member hm{0, -1, 0, (int)hb.size(), 1.0};
         DBuf<Member> mem(1);
         mem.upload(&hm, 1, st.s);
+        DBuf<Z> fpsi0;
+        if (m.iw_type == 3) S.user_wave_spectrum(fpsi0);
         ColArgsT<Z> c = S.col_base(nullptr, w.p, 1);
-        c.flags = COL_IFFT | COL_STORE;
+        c.flags = COL_IFFT | COL_STORE | (m.iw_type == 3 ? COL_USER_WAVE : 0);
+        c.fpsi0 = fpsi0.p;
         c.members = mem.p; c.beams = beams.p; c.probe = P;
         check(launch_col_pass(m.nx, c, true, false, st.s), "probe pass");
         RowArgsT<Z> r{};

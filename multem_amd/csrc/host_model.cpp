@@ -405,8 +405,6 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
                                  "quadrature is empty (src/cpu_fcns.hpp:1535) and its image is zero");
     if (is_hrtem() && illumination_model == 3)
         fail(MT_ERR_UNSUPPORTED, "HRTEM with illumination_model 3 (transmission cross coefficient)");
-    if (is_hrtem() && in.pn_coh_contrib)
-        fail(MT_ERR_UNSUPPORTED, "HRTEM coherent image (pn_coh_contrib)");
 
     pn_model = in.pn_model;
     pn_coh_contrib = in.pn_coh_contrib;
@@ -443,10 +441,16 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
 
     iw_type = in.iw_type;
     if (iw_type == 4) iw_type = (is_stem() || is_cbed()) ? 2 : 1;
-    if (iw_type == 3) fail(MT_ERR_UNSUPPORTED, "user-defined incident wave");
-    if (iw_type != 1 && iw_type != 2) fail(MT_ERR_INVALID_ARGUMENT, "iw_type must be 1..4");
+    if (iw_type < 1 || iw_type > 3) fail(MT_ERR_INVALID_ARGUMENT, "iw_type must be 1..4");
     for (int i = 0; i < in.n_iw_xy; ++i) { iw_x.push_back((R)in.iw_x[i]); iw_y.push_back((R)in.iw_y[i]); }
     if (!is_stem() && iw_type == 2 && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y");
+    if (iw_type == 3) {
+        // user-defined incident wave, ny x nx complex, placed at (iw_x[0], iw_y[0])
+        // (Incident_Wave::operator(), src/incident_wave.cuh:92-110)
+        if (!in.iw_psi) fail(MT_ERR_INVALID_ARGUMENT, "iw_type 3 needs iw_psi (ny x nx complex)");
+        if (!is_stem() && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "iw_type 3 needs iw_x, iw_y");
+        iw_psi.assign(in.iw_psi, in.iw_psi + 2 * (size_t)nx * ny);
+    }
 
     E_0 = (R)in.E_0;
     if (!(E_0 > 0)) fail(MT_ERR_INVALID_ARGUMENT, "E_0 must be positive");

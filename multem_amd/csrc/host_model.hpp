@@ -133,8 +133,9 @@ public:
     bool tilted() const { return gx0 != R(0) || gy0 != R(0); }
     double lambda_d = 0;            // FP64 wavelength for phase arguments
     R Vr_factor = 0;            // mul_sign * gamma*lambda/(c_Potf*cos(theta)), evaluated in R
-    int iw_type = 1;
+    int iw_type = 1;                // 1 plane, 2 convergent, 3 user defined
     std::vector<R> iw_x, iw_y;
+    std::vector<double> iw_psi;     // iw_type 3: interleaved complex, natural layout ix*ny + iy
 
     // grid
     int nx = 0, ny = 0, nxh = 0, nyh = 0;

@@ -172,6 +172,25 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
     TW tw;
     tw.load(t, a.tw);
     if constexpr (PROBE) {
+      if (flags & COL_USER_WAVE) {
+        // user-defined incident wave moved to the first beam's position:
+        // fpsi0(g) exp(i (x gx + y gy)) / (nx ny) (exp_g_factor_2d,
+        // src/cgpu_fcns.cuh:1920-1927), the phase in FP64 reduced mod 2 pi
+        const Member m = a.members[member];
+        const ProbeParamsT<R> &P = a.probe;
+        const Beam bm = a.beams[m.beam0];
+        const double gy = igy * P.dgy;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ix = t + T * e;
+            const double gx = signed_freq(ix, a.nx) * P.dgx;
+            double ph = bm.xph * gx + bm.yph * gy;
+            ph -= kTwoPiD * rint(ph * (1.0 / kTwoPiD));
+            R s, co;
+            rsincos((R)ph, &s, &co);
+            v[e] = cscale(cmul(cmake(co, s), a.fpsi0[(long long)ix * a.ncols + col]), a.inv_nxy);
+        }
+      } else {
         // Convergent probe in Fourier space (exp_i_chi, src/cgpu_fcns.cuh:1949-1979):
         // aperture mask in R, FP64 phase reduced mod 2 pi, sincos in R.
         const Member m = a.members[member];
@@ -212,6 +231,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
             }
             v[e] = acc;
         }
+      }
     }
     if (flags & COL_FFT) fft_run<S, false>(v, t, tw, lds, idx, sync);
     if (flags & (COL_MUL_P | COL_BWL)) {

@@ -84,6 +84,7 @@ enum ColFlags : int {
     COL_DET_BWL = 64,   // detector sums take one more bandwidth-limit factor (the back-propagation's)
     COL_COH = 128,      // detector passes: coh[iscan][ithk] += coh_w * psi(g) (coherent STEM image)
     COL_MUL_CTF = 256,  // multiply by ctf[ix*ny + iy] after the propagator (objective-lens image formation)
+    COL_USER_WAVE = 512,  // probe passes: the wave is fpsi0(g) exp(i (x_b gx + y_b gy)) / (nx ny), first beam
 };
 
 template <class Z>
@@ -109,6 +110,7 @@ struct ColArgsT {
     double *det_part;             // [member][n_thk][n_det][ntile]
     int n_thk, ithk;
     const Z *ctf;            // COL_MUL_CTF table [nx*ny], natural order
+    const Z *fpsi0;          // COL_USER_WAVE: FFT of the user's incident wave [nx*ny], natural order
     // coherent STEM accumulators [scan][n_thk][nx*ny], FP64
     double2 *coh;
     double coh_w;

@@ -648,6 +648,7 @@ typedef struct {
     int n_thick; real *thick;
     grid_t g;
     int iw_type; int n_iw; real *iw_x, *iw_y;
+    cplx *iw_psi;          /* iw_type 3: user wave, natural layout ix*ny + iy */
     real E_0, lambda, theta, phi;
     int mul_sign;
     lens_t cond, obj;
@@ -1362,7 +1363,7 @@ static void match_vectors(const real *vr, int nvr, real **vs_io, int *nvs_io, re
 static void sim_free(sim_t *s)
 {
     if (!s) return;
-    free(s->thick); free(s->iw_x); free(s->iw_y); free(s->sc_x); free(s->sc_y);
+    free(s->thick); free(s->iw_x); free(s->iw_y); free(s->iw_psi); free(s->sc_x); free(s->sc_y);
     free(s->det_g_inner); free(s->det_g_outer);
     atoms_free(&s->in_atoms); atoms_free(&s->atoms_u); atoms_free(&s->atoms);
     free(s->tab); free(s->z_plane); free(s->z_slice); free(s->slice); free(s->thk);
@@ -1452,7 +1453,12 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     /* incident wave type (validate_incident_wave_type, input_multislice.cuh:439-449) */
     s->iw_type = in->iw_type;
     if (s->iw_type == 4) s->iw_type = (is_scanning_t(s->simulation_type) || is_cbed(s)) ? 2 : 1;
-    if (s->iw_type == 3) FAIL(MT_ERR_UNSUPPORTED, "user-defined incident wave");
+    if (s->iw_type == 3) {
+        if (!in->iw_psi) FAIL(MT_ERR_INVALID_ARGUMENT, "iw_type 3 needs iw_psi");
+        size_t n = (size_t)in->nx * in->ny;
+        s->iw_psi = malloc(sizeof(cplx) * n);
+        for (size_t i = 0; i < n; i++) { s->iw_psi[i].re = (real)in->iw_psi[2 * i]; s->iw_psi[i].im = (real)in->iw_psi[2 * i + 1]; }
+    }
     s->n_iw = in->n_iw_xy;
     s->iw_x = malloc(sizeof(real) * (size_t)(s->n_iw > 0 ? s->n_iw : 1));
     s->iw_y = malloc(sizeof(real) * (size_t)(s->n_iw > 0 ? s->n_iw : 1));
@@ -1825,6 +1831,29 @@ static void incident_wave(sim_t *s, const real *xb, const real *yb, int nb, real
     size_t nxy = (size_t)g->nx * g->ny;
     if (s->iw_type == 1) {
         for (size_t i = 0; i < nxy; i++) { psi[i].re = 1; psi[i].im = 0; }
+        return;
+    }
+    if (s->iw_type == 3) {
+        /* user wave: fft2_shift + FFT (Incident_Wave::set_input_data, incident_wave.cuh:50-55),
+           then exp_g_factor_2d at the first beam, / nxy, and the inverse FFT
+           (incident_wave.cuh:92-110; cgpu_fcns.cuh:1920-1927), in real */
+        for (int ix = 0; ix < g->nx; ix++)
+            for (int iy = 0; iy < g->ny; iy++)
+                fpsi[(size_t)iRx_shift(g, ix) * g->ny + iRy_shift(g, iy)] = s->iw_psi[(size_t)ix * g->ny + iy];
+        fft2_exec(F, fpsi, 0, work);
+        const real c2pi = 6.283185307179586476925;
+        real x = -c2pi * (xb[0] - 0.5 * g->lx), y = -c2pi * (yb[0] - 0.5 * g->ly);
+        real inxy = (real)1 / (real)nxy;
+        for (int ix = 0; ix < g->nx; ix++)
+            for (int iy = 0; iy < g->ny; iy++) {
+                int igx = (ix < g->nxh) ? ix : ix - g->nx;
+                int igy = (iy < g->nyh) ? iy : iy - g->ny;
+                real ph = x * (igx * g->dgx) + y * (igy * g->dgy);
+                cplx e = { cos(ph), sin(ph) }, f = fpsi[(size_t)ix * g->ny + iy];
+                psi[(size_t)ix * g->ny + iy].re = (e.re * f.re - e.im * f.im) * inxy;
+                psi[(size_t)ix * g->ny + iy].im = (e.re * f.im + e.im * f.re) * inxy;
+            }
+        fft2_exec(F, psi, 1, work);
         return;
     }
     real z_init = (s->n_slice > 0) ? (real)(0.5 * (s->slice[0].z_0 + s->slice[0].z_e)) : 0;
@@ -2325,11 +2354,11 @@ static void ew_accumulate(const sim_t *s, ew_acc *A, cplx **po, double w)
             if (s->illumination_model == 4) {
                 num_int_tem(s, po[k], A->tmp, A->tmp2, &A->F, A->work);
                 accumulate_output(s, A->tmp, w, 0, 0, A->m2 + A->no * k, 0);
-                continue;
+            } else {
+                memcpy(A->tmp, po[k], sizeof(cplx) * A->nxy);
+                microscope_effects(s, A->tmp, &A->F, A->work);
+                accumulate_output(s, A->tmp, w, 1, 0, A->m2 + A->no * k, 0);
             }
-            memcpy(A->tmp, po[k], sizeof(cplx) * A->nxy);
-            microscope_effects(s, A->tmp, &A->F, A->work);
-            accumulate_output(s, A->tmp, w, 1, 0, A->m2 + A->no * k, 0);
             if (s->pn_coh_contrib)
                 for (size_t i = 0; i < A->nxy; i++) {
                     A->pc[2 * (A->nxy * k + i)] += w * po[k][i].re;
@@ -2475,6 +2504,11 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
             cplx *tmp = malloc(sizeof(cplx) * nxy);
             for (int k = 0; k < s->n_thk; k++) {
                 for (size_t i = 0; i < nxy; i++) { tmp[i].re = (real)A.pc[2 * (nxy * k + i)]; tmp[i].im = (real)A.pc[2 * (nxy * k + i) + 1]; }
+                if (is_hrtem(s) && s->illumination_model == 4) {
+                    num_int_tem(s, tmp, A.tmp, A.tmp2, &A.F, A.work);
+                    accumulate_output(s, A.tmp, 1.0, 0, 1, out->m2psi_coh + no * k, 0);
+                    continue;
+                }
                 if (is_hrtem(s)) microscope_effects(s, tmp, &A.F, A.work);
                 accumulate_output(s, tmp, 1.0, 1, 1, out->m2psi_coh + no * k, 0);
             }

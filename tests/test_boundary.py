@@ -82,7 +82,9 @@ def test_scan_grid_square_region():
     ("nx", 300, 2),                   # not a power of two
     ("nx", 0, 1),
     ("E_0", 0.0, 1),
-    ("iw_type", 3, 2),                # user-defined wave
+    ("iw_type", 3, 1),                # user-defined wave without iw_psi
+    ("interaction_model", 4, 1),
+    ("potential_slicing", 3, 2),      # dz_Sub: the reference leaves it unevaluated
     ("theta", 95.0, 1),               # tilt beyond 90 degrees
 ])
 def test_errors_are_reported(field, value, status):

@@ -350,3 +350,50 @@ def test_hrtem(illum):
     e32 = rel_l2(out.m2psi_tot, oracle.run(p, "f32").m2psi_tot)
     print("HRTEM illumination %d: rel L2 vs FP64 %.3e, vs FP32 restatement %.3e" % (illum, e64, e32))
     assert e64 < 1e-4 and e32 < 1e-4
+
+
+@pytest.mark.parametrize("illum,theta", [(1, 0.0), (2, 0.0), (4, 0.0), (1, 0.5)])
+def test_hrtem_coherent_image(illum, theta):
+    """HRTEM with pn_coh_contrib: m2psi_coh = |IFFT(lens psi_coh)|^2 of the
+    configuration-averaged wave (set_m2psi_coh, src/wave_function.cuh:206-215)."""
+    p = configs.si_ewrs_256(nc=4)
+    p.simulation_type = 32
+    p.pn_model, p.pn_nconf, p.pn_coh_contrib = 3, 2, 1
+    p.illumination_model = illum
+    p.temporal_spatial_incoh = 2
+    p.theta, p.phi = theta, 20.0
+    p.obj_lens_c_10, p.obj_lens_c_30, p.obj_lens_outer_aper_ang = -55.0, 0.8, 25.0
+    p.obj_lens_ti_sigma, p.obj_lens_ti_npts = 30.0, 5
+    out, ref = engine.run_multislice(p), oracle.run(p, "f64")
+    e_tot, e_coh = rel_l2(out.m2psi_tot, ref.m2psi_tot), rel_l2(out.m2psi_coh, ref.m2psi_coh)
+    print("HRTEM illum %d theta %.1f: m2psi_tot %.2e, m2psi_coh %.2e vs FP64" % (illum, theta, e_tot, e_coh))
+    assert e_tot < 1e-4 and e_coh < 1e-4
+
+
+def _user_probe(p):
+    """A convergent probe at the box centre, as a user-defined incident wave (ny x nx)."""
+    q = p.copy()
+    q.simulation_type, q.iw_type = 21, 2
+    q.iw_x, q.iw_y = np.array([0.5 * q.spec_lx]), np.array([0.5 * q.spec_ly])
+    q.cond_lens_c_10 = 25.0
+    return oracle.incident_wave(q, precision="f64").T
+
+
+def test_user_defined_incident_wave():
+    """iw_type 3: the user's wave moved to iw_x[0], iw_y[0] (EW, CBED) or to
+    each scan position (STEM), against the FP64 oracle."""
+    p = small_stem(ns=3, nconf=1)
+    p.iw_type = 3
+    p.iw_psi = _user_probe(p)
+    e = rel_l2(engine.run_multislice(p).image_tot, oracle.run(p, "f64").image_tot)
+    print("STEM with a user incident wave: rel L2 vs FP64 %.2e" % e)
+    assert e < 1e-4
+    q = configs.pt_cbed_1024(nconf=2, n=256, na=4, nb=6, nc=8, thick=[10.0, 20.0])
+    q.iw_type = 3
+    q.iw_psi = _user_probe(q)
+    out, ref = engine.run_multislice(q), oracle.run(q, "f64")
+    for k in range(out.n_thk):
+        e = rel_l2(out.m2psi_tot[k], ref.m2psi_tot[k])
+        print("CBED with a user incident wave, thickness %d: rel L2 vs FP64 %.2e" % (k, e))
+        assert e < 1e-4
+    assert rel_l2(engine.incident_wave(q), oracle.incident_wave(q, precision="f64")) < 1e-5

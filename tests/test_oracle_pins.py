@@ -330,3 +330,20 @@ def test_hrtem_partial_coherence_without_spread_equals_coherent():
     # a finite focal spread damps the high frequencies: less image contrast
     c = oracle.run(_hrtem_case(illumination_model=2, **dict(kw, obj_lens_ti_sigma=40.0)), "f64").m2psi_tot[0]
     assert np.std(c) < np.std(b)
+
+
+@pytest.mark.parametrize("illum", [1, 2, 4])
+def test_hrtem_coherent_image_of_one_configuration(illum):
+    """One configuration: the coherent image |IFFT(lens psi_coh)|^2 is the
+    total image (set_m2psi_coh, src/wave_function.cuh:206-215)."""
+    p = configs.si_ewrs_256(nc=3)
+    p.simulation_type = 32
+    p.pn_model = 1
+    p.pn_coh_contrib = 1
+    p.illumination_model = illum
+    p.temporal_spatial_incoh = 2
+    p.obj_lens_c_10, p.obj_lens_c_30, p.obj_lens_outer_aper_ang = -40.0, 0.8, 25.0
+    p.obj_lens_ti_sigma, p.obj_lens_ti_npts = 20.0, 5
+    r = oracle.run(p, "f64")
+    assert r.m2psi_coh is not None
+    assert np.array_equal(r.m2psi_coh, r.m2psi_tot)
