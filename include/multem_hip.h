@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MT_ABI_VERSION 1
+#define MT_ABI_VERSION 2
 
 typedef enum {
     MT_OK = 0,
@@ -93,7 +93,7 @@ typedef struct {
     int spec_atoms_cols;
     double spec_lx, spec_ly, spec_lz, spec_dz;
 
-    double spec_rot_theta;     /* degrees; only 0 is accepted */
+    double spec_rot_theta;     /* degrees; axis and centre at the end of the struct */
 
     int thick_type;            /* 1 whole specimen, 2 through thickness, 3 through slices */
     const double *thick;
@@ -130,6 +130,14 @@ typedef struct {
     const mt_detector_cir *detector_cir;
 
     int output_area_ix_0, output_area_iy_0, output_area_ix_e, output_area_iy_e;  /* 1-based as in MATLAB */
+
+    /* specimen rotation (ilc_multem.cu:98-102; applied by validate_parameters,
+       src/input_multislice.cuh:707-722) by spec_rot_theta (degrees, above)
+       about the axis spec_rot_u0 through the centre point: the atoms'
+       geometric centre (spec_rot_center_type 1) or spec_rot_center_p (2) */
+    double spec_rot_u0[3];
+    int spec_rot_center_type;
+    double spec_rot_center_p[3];
 } mt_input_multislice;
 
 /* Output kinds (eTEM_Output_Type, types.cuh:209-214) */

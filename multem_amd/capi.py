@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-MT_ABI_VERSION = 1
+MT_ABI_VERSION = 2
 
 MT_OK = 0
 STATUS_NAMES = {
@@ -70,6 +70,7 @@ class InputMultislice(C.Structure):
         ("detector_type", C.c_int), ("n_detector", C.c_int), ("detector_cir", C.POINTER(DetectorCir)),
         ("output_area_ix_0", C.c_int), ("output_area_iy_0", C.c_int),
         ("output_area_ix_e", C.c_int), ("output_area_iy_e", C.c_int),
+        ("spec_rot_u0", C.c_double * 3), ("spec_rot_center_type", C.c_int), ("spec_rot_center_p", C.c_double * 3),
     ]
 
 

@@ -393,7 +393,6 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
                                  "unevaluated (zero); use 1 (planes) or 2 (dz_Proj)");
     if (!(in.potential_slicing == 1 || in.potential_slicing == 2))
         fail(MT_ERR_UNSUPPORTED, "potential_slicing must be 1 (planes) or 2 (dz_Proj)");
-    if (std::fabs(in.spec_rot_theta) > 0) fail(MT_ERR_UNSUPPORTED, "specimen rotation");
     // exit waves are coherent; CBED is coherent unless fully integrated
     // (Input_Multislice::validate_parameters, src/input_multislice.cuh:672-682)
     illumination_model = in.illumination_model;
@@ -591,8 +590,41 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     if (!multislice()) {
         pot_sli = 1;
         if (thick_type == 3) thick_type = 2;
-        if (thick_type == 1) pot_sli = 2;
     }
+    // specimen rotation (src/input_multislice.cuh:707-722): the whole specimen,
+    // rotated by theta about the unit axis u0 through the geometric centre of
+    // the atoms or a given point (rotate_atoms, src/atomic_data_mt.hpp:61-74,
+    // get_rotation_matrix, src/cgpu_fcns.cuh:185-204)
+    const R rot_theta = (R)(in.spec_rot_theta * kDegToRad);
+    if (!near_zero(rot_theta)) {
+        thick_type = 1;
+        R u[3] = {(R)in.spec_rot_u0[0], (R)in.spec_rot_u0[1], (R)in.spec_rot_u0[2]};
+        const R un = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        if (!(un > R(0))) fail(MT_ERR_INVALID_ARGUMENT, "spec_rot_u0 must be a nonzero vector");
+        for (R &c : u) c /= un;
+        R p0[3] = {(R)in.spec_rot_center_p[0], (R)in.spec_rot_center_p[1], (R)in.spec_rot_center_p[2]};
+        if (in.spec_rot_center_type == 1) {
+            double sx = 0, sy = 0, sz = 0;
+            for (size_t i = 0; i < in_atoms.x.size(); ++i) { sx += in_atoms.x[i]; sy += in_atoms.y[i]; sz += in_atoms.z[i]; }
+            const double n = (double)in_atoms.x.size();
+            p0[0] = (R)(sx / n); p0[1] = (R)(sy / n); p0[2] = (R)(sz / n);
+        }
+        R a = R(1) - std::cos(rot_theta), b = std::sin(rot_theta);
+        if (near_zero(a)) a = 0;
+        if (near_zero(b)) b = 0;
+        // column-major 3x3: r'_i = sum_j M[i + 3 j] (r_j - p0_j) + p0_i
+        const R M[9] = {R(1) + a * (u[0] * u[0] - R(1)), u[1] * u[0] * a + u[2] * b, u[2] * u[0] * a - u[1] * b,
+                        u[0] * u[1] * a - u[2] * b, R(1) + a * (u[1] * u[1] - R(1)), u[2] * u[1] * a + u[0] * b,
+                        u[0] * u[2] * a + u[1] * b, u[1] * u[2] * a - u[0] * b, R(1) + a * (u[2] * u[2] - R(1))};
+        for (size_t i = 0; i < in_atoms.x.size(); ++i) {
+            const R x = in_atoms.x[i] - p0[0], y = in_atoms.y[i] - p0[1], z = in_atoms.z[i] - p0[2];
+            in_atoms.x[i] = M[0] * x + M[3] * y + M[6] * z + p0[0];
+            in_atoms.y[i] = M[1] * x + M[4] * y + M[7] * z + p0[1];
+            in_atoms.z[i] = M[2] * x + M[5] * y + M[8] * z + p0[2];
+        }
+    }
+    // the phase object's whole specimen is sliced by dz (src/input_multislice.cuh:724-728)
+    if (!multislice() && thick_type == 1) pot_sli = 2;
     if (thick_type == 1) {
         thick = {zmax};
     } else {

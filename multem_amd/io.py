@@ -84,8 +84,7 @@ def _params_from(inp, sc=None):
             sc = v
             continue
         if k.startswith("spec_cryst_") or k.startswith("eels_") or k.startswith("eftem_") \
-                or k.startswith("ped_") or k.startswith("hci_") or k in ("spec_rot_u0", "spec_rot_center_type",
-                                                                         "spec_rot_center_p"):
+                or k.startswith("ped_") or k.startswith("hci_"):
             continue     # read by the reference for other simulation types / builders; no effect here
         _set_field(p, k, v)
     if sc is not None:

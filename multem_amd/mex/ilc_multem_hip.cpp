@@ -100,6 +100,14 @@ void read_input(const mxArray *s, mt_input_multislice *in, InputArrays &A)
     in->spec_lx = num(s, "spec_lx"); in->spec_ly = num(s, "spec_ly");
     in->spec_lz = num(s, "spec_lz"); in->spec_dz = num(s, "spec_dz");
     in->spec_rot_theta = num(s, "spec_rot_theta");
+    {
+        const std::vector<double> u0 = vec(s, "spec_rot_u0"), p0 = vec(s, "spec_rot_center_p");
+        for (int k = 0; k < 3; ++k) {
+            in->spec_rot_u0[k] = k < (int)u0.size() ? u0[k] : 0.0;
+            in->spec_rot_center_p[k] = k < (int)p0.size() ? p0[k] : 0.0;
+        }
+        in->spec_rot_center_type = (int)num(s, "spec_rot_center_type");
+    }
 
     in->thick_type = (int)num(s, "thick_type");
     A.thick = vec(s, "thick");

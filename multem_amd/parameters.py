@@ -62,6 +62,9 @@ class Parameters:
         self.spec_ly = 10.0
         self.spec_lz = 10.0
         self.spec_rot_theta = 0.0
+        self.spec_rot_u0 = np.array([0.0, 0.0, 1.0])
+        self.spec_rot_center_type = 1
+        self.spec_rot_center_p = np.array([0.0, 0.0, 0.0])
         self.thick_type = 1
         self.thick = np.zeros(1)
         self.potential_slicing = 1
@@ -120,8 +123,12 @@ class Parameters:
                   "thick_type", "potential_slicing", "nx", "ny", "bwl", "iw_type",
                   "illumination_model", "temporal_spatial_incoh",
                   "scanning_type", "scanning_periodic", "scanning_square_pxs", "scanning_ns", "detector_type",
-                  "output_area_ix_0", "output_area_iy_0", "output_area_ix_e", "output_area_iy_e"):
+                  "output_area_ix_0", "output_area_iy_0", "output_area_ix_e", "output_area_iy_e",
+                  "spec_rot_center_type"):
             setattr(s, f, int(getattr(self, f)))
+        for f in ("spec_rot_u0", "spec_rot_center_p"):
+            v = np.asarray(getattr(self, f), dtype=np.float64).ravel()
+            getattr(s, f)[:] = [float(v[i]) if i < v.size else 0.0 for i in range(3)]
         for f in ("spec_lx", "spec_ly", "spec_lz", "spec_dz", "spec_rot_theta", "E_0", "theta", "phi",
                   "scanning_x0", "scanning_y0", "scanning_xe", "scanning_ye"):
             setattr(s, f, float(getattr(self, f)))

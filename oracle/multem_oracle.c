@@ -1423,7 +1423,6 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
         }
         s->in_atoms.l_x = lx; s->in_atoms.l_y = ly; s->in_atoms.l_z = lz; s->in_atoms.dz = dz;
     }
-    if (fabs(in->spec_rot_theta) > 0) FAIL(MT_ERR_UNSUPPORTED, "specimen rotation");
     s->thick_type = in->thick_type;
     if (s->thick_type != 1) {
         s->n_thick = in->n_thick;
@@ -1544,8 +1543,39 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     if (s->interaction_model != 1) {
         s->potential_slicing = 1;
         if (s->thick_type == 3) s->thick_type = 2;
-        if (s->thick_type == 1) s->potential_slicing = 2;
     }
+    /* specimen rotation (input_multislice.cuh:707-722; rotate_atoms,
+       atomic_data_mt.hpp:61-74; get_rotation_matrix, cgpu_fcns.cuh:185-204) */
+    {
+        real th = (real)(in->spec_rot_theta * c_deg_2_rad);
+        if (!is_zero_r(th)) {
+            s->thick_type = 1;
+            atoms_t *a = &s->in_atoms;
+            real u[3] = { (real)in->spec_rot_u0[0], (real)in->spec_rot_u0[1], (real)in->spec_rot_u0[2] };
+            real un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            if (!(un > 0)) FAIL(MT_ERR_INVALID_ARGUMENT, "spec_rot_u0 must be a nonzero vector");
+            for (int k = 0; k < 3; k++) u[k] /= un;
+            real p0[3] = { (real)in->spec_rot_center_p[0], (real)in->spec_rot_center_p[1], (real)in->spec_rot_center_p[2] };
+            if (in->spec_rot_center_type == 1) {
+                double sx = 0, sy = 0, sz = 0;
+                for (int i = 0; i < a->n; i++) { sx += a->x[i]; sy += a->y[i]; sz += a->z[i]; }
+                p0[0] = (real)(sx / a->n); p0[1] = (real)(sy / a->n); p0[2] = (real)(sz / a->n);
+            }
+            real al = 1 - cos(th), be = sin(th);
+            if (is_zero_r(al)) al = 0;
+            if (is_zero_r(be)) be = 0;
+            real Rm[9] = { 1 + al * (u[0] * u[0] - 1), u[1] * u[0] * al + u[2] * be, u[2] * u[0] * al - u[1] * be,
+                           u[0] * u[1] * al - u[2] * be, 1 + al * (u[1] * u[1] - 1), u[2] * u[1] * al + u[0] * be,
+                           u[0] * u[2] * al + u[1] * be, u[1] * u[2] * al - u[0] * be, 1 + al * (u[2] * u[2] - 1) };
+            for (int i = 0; i < a->n; i++) {
+                real x = a->x[i] - p0[0], y = a->y[i] - p0[1], z = a->z[i] - p0[2];
+                a->x[i] = Rm[0] * x + Rm[3] * y + Rm[6] * z + p0[0];
+                a->y[i] = Rm[1] * x + Rm[4] * y + Rm[7] * z + p0[1];
+                a->z[i] = Rm[2] * x + Rm[5] * y + Rm[8] * z + p0[2];
+            }
+        }
+    }
+    if (s->interaction_model != 1 && s->thick_type == 1) s->potential_slicing = 2;
 
     /* match_thickness (src/slicing.hpp:53-71) */
     atoms_statistic(s, &s->in_atoms, 0);

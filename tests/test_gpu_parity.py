@@ -397,3 +397,15 @@ def test_user_defined_incident_wave():
         print("CBED with a user incident wave, thickness %d: rel L2 vs FP64 %.2e" % (k, e))
         assert e < 1e-4
     assert rel_l2(engine.incident_wave(q), oracle.incident_wave(q, precision="f64")) < 1e-5
+
+
+def test_rotated_specimen():
+    """spec_rot_theta about a tilted axis: the engine slices the same rotated
+    specimen as the oracle (pinned against scipy in tests/test_rotation.py)."""
+    p = configs.si_ewrs_256(nc=4)
+    p.spec_rot_theta, p.spec_rot_u0 = 12.0, np.array([1.0, 0.3, 0.0])
+    out = engine.run_multislice(p)
+    e32 = rel_l2(out.psi_coh, oracle.run(p, "f32").psi_coh)
+    e64 = rel_l2(out.psi_coh, oracle.run(p, "f64").psi_coh)
+    print("rotated Si exit wave: rel L2 vs FP32 restatement %.2e, vs FP64 %.2e" % (e32, e64))
+    assert e32 < 1e-5 and e64 < 1e-4
