@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--cpu-ns", type=int, default=16)
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-size group measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM counters")
     return ap.parse_args()
@@ -100,6 +101,30 @@ def group_measurement():
                         "256 probes per batch" % "/".join(str(n) for n in nsl),
             "probe_slices": work, "value": work / t_group, "unit": "probe·slices/s",
             "sequential_value": work / t_seq, "plan": plan}
+
+
+def fp64_measurement():
+    """The FP64 wave path (system_conf.precision = 2) on the same Au[001]
+    512^2 STEM workload, 32x32 probes x 2 configurations: probe.slices/s and
+    the column pass's algorithmic HBM rate (32 N^2 B per wave)."""
+    from multem_amd import configs, engine
+    p = configs.au_stem_512(ns=32, nconf=2, n=512)
+    p.system_conf.precision = 2
+    nsl = len(engine.spec_slicing(p))
+    engine.run_multislice(p)                   # warm-up (tuning, graphs)
+    sync()
+    t0 = time.perf_counter()
+    engine.run_multislice(p)
+    sync()
+    wall = time.perf_counter() - t0
+    t = engine.last_timing()
+    plan = engine.last_plan()
+    B = int([kv.split("=")[1] for kv in plan.split() if kv.startswith("batch=")][0])
+    col_ms = t["ms_col"] / max(t["n_col"], 1)
+    return {"workload": "au_stem_512 FP64: 512^2, %d slices, 32x32 probes x 2 configs" % nsl,
+            "value": 32 * 32 * 2 * nsl / wall, "unit": "probe·slices/s", "dtype": "f64", "plan": plan,
+            "col_pass_ms": col_ms,
+            "col_pass_alg_GBs": 32.0 * 512 * 512 * B / (col_ms * 1e-3) / 1e9 if col_ms > 0 else None}
 
 
 SURROGATE_WEIGHTS = os.path.join(ROOT, "multem_amd", "data", "surrogate_6x32.npz")
@@ -247,6 +272,8 @@ def main():
     }
     if not args.no_group:
         out["uneven_group"] = group_measurement()
+    if not args.no_fp64:
+        out["fp64"] = fp64_measurement()
     if not args.no_surrogate and os.path.exists(SURROGATE_WEIGHTS):
         out["surrogate"] = surrogate_measurement()
     if not args.no_cpu_baseline:

@@ -622,6 +622,7 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
             in_atoms.y[i] = M[1] * x + M[4] * y + M[7] * z + p0[1];
             in_atoms.z[i] = M[2] * x + M[5] * y + M[8] * z + p0[2];
         }
+        zminmax(in_atoms, zmin, zmax);   // get_statistic after the rotation
     }
     // the phase object's whole specimen is sliced by dz (src/input_multislice.cuh:724-728)
     if (!multislice() && thick_type == 1) pot_sli = 2;

@@ -404,6 +404,8 @@ def test_rotated_specimen():
     specimen as the oracle (pinned against scipy in tests/test_rotation.py)."""
     p = configs.si_ewrs_256(nc=4)
     p.spec_rot_theta, p.spec_rot_u0 = 12.0, np.array([1.0, 0.3, 0.0])
+    p.potential_slicing = 2          # a rotated crystal has no atomic planes along z
+    assert len(oracle.spec_slicing(p)) > 10
     out = engine.run_multislice(p)
     e32 = rel_l2(out.psi_coh, oracle.run(p, "f32").psi_coh)
     e64 = rel_l2(out.psi_coh, oracle.run(p, "f64").psi_coh)

@@ -52,3 +52,17 @@ def test_full_turn_is_the_identity():
     q = p.copy()
     q.spec_rot_theta, q.spec_rot_u0 = 360.0, np.array([0.2, 0.4, 1.0])
     assert np.array_equal(oracle.run(p, "f64").psi_coh, oracle.run(q, "f64").psi_coh)
+
+
+@pytest.mark.parametrize("pot_sli", [1, 2])
+def test_engine_slicing_of_a_rotated_specimen(pot_sli):
+    """The C ABI's host model slices the rotated specimen as the oracle does
+    (slice bounds, atom ranges, thickness slice); no device work."""
+    from multem_amd import engine
+    p = configs.si_ewrs_256(nc=4)
+    p.spec_rot_theta, p.spec_rot_u0 = 12.0, np.array([1.0, 0.3, 0.0])
+    p.potential_slicing = pot_sli
+    a, b = engine.spec_slicing(p), oracle.spec_slicing(p, precision="f32")
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert np.allclose(x[:4], y[:4], rtol=0, atol=1e-4) and tuple(x[4:]) == tuple(y[4:]), (x, y)
