@@ -125,7 +125,7 @@ typedef struct {
     int scanning_ns;
     double scanning_x0, scanning_y0, scanning_xe, scanning_ye;
 
-    int detector_type;         /* 1 circular */
+    int detector_type;         /* 1 circular, 3 matrix (2, radial, integrates to zero in the reference: refused) */
     int n_detector;
     const mt_detector_cir *detector_cir;
 
@@ -138,6 +138,11 @@ typedef struct {
     double spec_rot_u0[3];
     int spec_rot_center_type;
     double spec_rot_center_p[3];
+
+    /* matrix detectors (detector_type 3, detector.matrix(i).fR;
+       ilc_multem.cu:317-333): n_detector sensitivity maps, each ny x nx in
+       the natural (unshifted) Fourier layout, contiguous */
+    const double *detector_fR;
 } mt_input_multislice;
 
 /* Output kinds (eTEM_Output_Type, types.cuh:209-214) */

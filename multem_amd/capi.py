@@ -71,6 +71,7 @@ class InputMultislice(C.Structure):
         ("output_area_ix_0", C.c_int), ("output_area_iy_0", C.c_int),
         ("output_area_ix_e", C.c_int), ("output_area_iy_e", C.c_int),
         ("spec_rot_u0", C.c_double * 3), ("spec_rot_center_type", C.c_int), ("spec_rot_center_p", C.c_double * 3),
+        ("detector_fR", _dp),
     ]
 
 

@@ -616,6 +616,7 @@ public:
     DBuf<Z> rx, ry, scratch;   // tilted illumination: output ramp and the output copy of the batch
     DBuf<double2> coh;              // pn_coh_contrib: sum over configurations of w psi(g), [scan][n_thk][nxy]
     DBuf<Z> fpsi0;                  // iw_type 3: spectrum of the user's incident wave
+    DBuf<R> det_fR;                 // matrix detectors
     DBuf<vec4_t<R>> atoms;
     ProbeParamsT<R> P{};
     unsigned int nap = 0;
@@ -715,6 +716,11 @@ public:
         c = S.col_base(psi.p, psi.p, B);
         c.n_det = n_det;
         for (int d = 0; d < n_det; ++d) { c.det_g2_min[d] = m.det_g2_min[d]; c.det_g2_max[d] = m.det_g2_max[d]; }
+        if (!m.det_fR.empty()) {
+            det_fR.alloc(m.det_fR.size());
+            det_fR.upload(m.det_fR.data(), m.det_fR.size(), s);
+        }
+        c.det_fR = det_fR.p;
         c.det_part = part.p; c.n_thk = n_thk;
         c.members = members.p;
         c.coh = coh.p; c.coh_w = (double)w;
@@ -901,7 +907,8 @@ public:
         if (out->image_coh && m.pn_coh_contrib) {
             DBuf<double> ic((size_t)n_thk * n_det * nscan);
             check(launch_coh_image(coh.p, nscan, n_thk, n_det, m.nx, m.ny, m.dgx, m.dgy, m.det_g2_min.data(),
-                                   m.det_g2_max.data(), (m.bwl && !m.tilted()) ? 1 : 0, m.bwl_alpha, m.gl2_max, ic.p,
+                                   m.det_g2_max.data(), (const R *)det_fR.p, (m.bwl && !m.tilted()) ? 1 : 0,
+                                   m.bwl_alpha, m.gl2_max, ic.p,
                                    s),
                   "coherent image");
             check(hipMemcpyAsync(out->image_coh, ic.p, ic.n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");

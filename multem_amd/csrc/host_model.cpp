@@ -537,13 +537,28 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
                 }
         }
         sc_x0 = x0; sc_y0 = y0;
-        if (in.detector_type != 1) fail(MT_ERR_UNSUPPORTED, "only circular detectors");
-        if (in.n_detector <= 0 || !in.detector_cir) fail(MT_ERR_INVALID_ARGUMENT, "STEM needs at least one detector");
-        for (int i = 0; i < in.n_detector; ++i) {
-            R gi = std::sin((R)(in.detector_cir[i].inner_ang * kMradToRad)) / lambda;
-            R go = std::sin((R)(in.detector_cir[i].outer_ang * kMradToRad)) / lambda;
-            det_g2_min.push_back((R)std::pow(gi, 2));
-            det_g2_max.push_back((R)std::pow(go, 2));
+        // detectors (ilc_multem.cu:272-337): circular rings or matrix
+        // sensitivity maps; the reference's radial detector integrates to zero
+        // (integrated_intensity_over_det, src/wave_function.cuh:120-124)
+        if (in.detector_type == 2)
+            fail(MT_ERR_UNSUPPORTED, "radial detectors: the reference's radial detector integrates to zero");
+        if (in.detector_type != 1 && in.detector_type != 3) fail(MT_ERR_INVALID_ARGUMENT, "detector_type must be 1..3");
+        if (in.n_detector <= 0) fail(MT_ERR_INVALID_ARGUMENT, "STEM needs at least one detector");
+        if (in.detector_type == 1) {
+            if (!in.detector_cir) fail(MT_ERR_INVALID_ARGUMENT, "circular detectors need detector_cir");
+            for (int i = 0; i < in.n_detector; ++i) {
+                R gi = std::sin((R)(in.detector_cir[i].inner_ang * kMradToRad)) / lambda;
+                R go = std::sin((R)(in.detector_cir[i].outer_ang * kMradToRad)) / lambda;
+                det_g2_min.push_back((R)std::pow(gi, 2));
+                det_g2_max.push_back((R)std::pow(go, 2));
+            }
+        } else {
+            if (!in.detector_fR) fail(MT_ERR_INVALID_ARGUMENT, "matrix detectors need detector_fR");
+            const size_t n = (size_t)in.n_detector * nx * ny;
+            det_fR.resize(n);
+            for (size_t i = 0; i < n; ++i) det_fR[i] = (R)in.detector_fR[i];
+            det_g2_min.assign(in.n_detector, R(0));
+            det_g2_max.assign(in.n_detector, R(0));
         }
     }
 

@@ -151,8 +151,10 @@ public:
     R sc_x0 = 0, sc_y0 = 0, sc_dRx = 0, sc_dRy = 0;
     std::vector<R> sc_x, sc_y;
 
-    // detectors: g^2 bounds (pow(g, 2) in R)
+    // detectors: g^2 bounds (pow(g, 2) in R), or matrix sensitivity maps
+    // [n_det][nx*ny] in natural k-space order (detector_type 3)
     std::vector<R> det_g2_min, det_g2_max;
+    std::vector<R> det_fR;
 
     // output area (0-based, half open)
     int oa_ix_0 = 0, oa_ix_e = 0, oa_iy_0 = 0, oa_iy_e = 0;

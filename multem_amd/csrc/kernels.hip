@@ -281,8 +281,15 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
                 const R f = bwl_factor(g2f, a.bwl_alpha, a.gl2_max);
                 m2 *= f * f;
             }
-            for (int d = 0; d < a.n_det; ++d)
-                if (a.det_g2_min[d] <= g2f && g2f < a.det_g2_max[d]) acc[d] += (double)m2;
+            if (a.det_fR) {
+                // matrix detectors: sum fR |psi|^2 (sum_square_over_Det with fR,
+                // src/wave_function.cuh:125-129)
+                const long long px = (long long)ix * a.ncols + col;
+                for (int d = 0; d < a.n_det; ++d) acc[d] += (double)(a.det_fR[(long long)d * a.nx * a.ny + px] * m2);
+            } else {
+                for (int d = 0; d < a.n_det; ++d)
+                    if (a.det_g2_min[d] <= g2f && g2f < a.det_g2_max[d]) acc[d] += (double)m2;
+            }
         }
         const int lane = tid & 63, wave = tid >> 6;
         for (int d = 0; d < a.n_det; ++d) {
@@ -602,8 +609,8 @@ struct CohDet {
 
 template <class R>
 __global__ __launch_bounds__(256) void k_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny,
-                                                   R dgx, R dgy, CohDet<R> det, int extra_bwl, R bwl_alpha, R gl2_max,
-                                                   double *image_coh)
+                                                   R dgx, R dgy, CohDet<R> det, const R *fR, int extra_bwl,
+                                                   R bwl_alpha, R gl2_max, double *image_coh)
 {
     __shared__ double red[kMaxDet][256];
     const int iscan = blockIdx.x % nscan, ithk = blockIdx.x / nscan;
@@ -619,8 +626,12 @@ __global__ __launch_bounds__(256) void k_coh_image(const double2 *coh, int nscan
             const double f = (double)bwl_factor(g2, bwl_alpha, gl2_max);
             m2 *= f * f;
         }
-        for (int d = 0; d < n_det; ++d)
-            if (det.g2_min[d] <= g2 && g2 < det.g2_max[d]) acc[d] += m2;
+        if (fR) {
+            for (int d = 0; d < n_det; ++d) acc[d] += (double)fR[(long long)d * nx * ny + i] * m2;
+        } else {
+            for (int d = 0; d < n_det; ++d)
+                if (det.g2_min[d] <= g2 && g2 < det.g2_max[d]) acc[d] += m2;
+        }
     }
     for (int d = 0; d < n_det; ++d) red[d][threadIdx.x] = acc[d];
     __syncthreads();
@@ -880,14 +891,14 @@ hipError_t launch_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, 
 
 template <class R>
 hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, R dgx, R dgy,
-                            const R *det_g2_min, const R *det_g2_max, int extra_bwl, R bwl_alpha, R gl2_max,
-                            double *image_coh, hipStream_t s)
+                            const R *det_g2_min, const R *det_g2_max, const R *det_fR, int extra_bwl, R bwl_alpha,
+                            R gl2_max, double *image_coh, hipStream_t s)
 {
     if (n_det > kMaxDet || n_det < 0) return hipErrorInvalidValue;
     CohDet<R> d{};
     for (int k = 0; k < n_det; ++k) { d.g2_min[k] = det_g2_min[k]; d.g2_max[k] = det_g2_max[k]; }
     hipLaunchKernelGGL((k_coh_image<R>), dim3((unsigned)(nscan * n_thk)), dim3(256), 0, s, coh, nscan, n_thk, n_det,
-                       nx, ny, dgx, dgy, d, extra_bwl, bwl_alpha, gl2_max, image_coh);
+                       nx, ny, dgx, dgy, d, det_fR, extra_bwl, bwl_alpha, gl2_max, image_coh);
     return hipGetLastError();
 }
 
@@ -934,8 +945,8 @@ hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_
                                               hipStream_t);                                                            \
     template hipError_t launch_count_aperture<R>(int, int, R, R, R, R, R, R, unsigned int *, hipStream_t);            \
     template hipError_t launch_out_accum<Z>(const OutArgsT<Z> &, hipStream_t);                                         \
-    template hipError_t launch_coh_image<R>(const double2 *, int, int, int, int, int, R, R, const R *, const R *, int, \
-                                            R, R, double *, hipStream_t);                                              \
+    template hipError_t launch_coh_image<R>(const double2 *, int, int, int, int, int, R, R, const R *, const R *,      \
+                                            const R *, int, R, R, double *, hipStream_t);                              \
     template hipError_t launch_ctf_table<R>(int, int, const CtfParamsT<R> &, Z *, hipStream_t);                        \
     template hipError_t launch_slice_sum<R>(const R *, int, long long, double *, hipStream_t);                         \
     template hipError_t launch_shift_complex<Z>(const Z *, Z *, int, int, hipStream_t);

@@ -107,6 +107,7 @@ struct ColArgsT {
     // detector epilogue
     int n_det;
     R det_g2_min[kMaxDet], det_g2_max[kMaxDet];
+    const R *det_fR;              // matrix detectors [n_det][nx*ny] natural k-space order (null: rings)
     double *det_part;             // [member][n_thk][n_det][ntile]
     int n_thk, ithk;
     const Z *ctf;            // COL_MUL_CTF table [nx*ny], natural order
@@ -199,11 +200,12 @@ struct CtfParamsT {
 template <class R>
 hipError_t launch_ctf_table(int nx, int ny, const CtfParamsT<R> &p, cplx_t<R> *ctf, hipStream_t s);
 
-// image_coh[ithk][det][iscan] = sum over the ring of f |coh|^2 (f = bwl^2 if extra_bwl, else 1)
+// image_coh[ithk][det][iscan] = sum over the ring (or weighted by the matrix
+// detector det_fR) of f |coh|^2 (f = bwl^2 if extra_bwl, else 1)
 template <class R>
 hipError_t launch_coh_image(const double2 *coh, int nscan, int n_thk, int n_det, int nx, int ny, R dgx, R dgy,
-                            const R *det_g2_min, const R *det_g2_max, int extra_bwl, R bwl_alpha, R gl2_max,
-                            double *image_coh, hipStream_t s);
+                            const R *det_g2_min, const R *det_g2_max, const R *det_fR, int extra_bwl, R bwl_alpha,
+                            R gl2_max, double *image_coh, hipStream_t s);
 hipError_t launch_stream_copy(const void *a, void *b, size_t bytes, hipStream_t s);
 template <class R>
 hipError_t launch_slice_sum(const R *V, int n_slices, long long nxy, double *out, hipStream_t s);

@@ -53,11 +53,21 @@ def _set_field(p, name, v):
         setattr(p, name, float(np.asarray(v).ravel()[0]))
 
 
+def _elements(v):
+    if v is None:
+        return []
+    if isinstance(v, (list, tuple)):
+        return list(v)
+    return list(np.atleast_1d(np.asarray(v, dtype=object)).ravel())
+
+
 def _set_detector(p, det):
     if isinstance(det, dict):
-        typ, cir = det.get("type", 1), det.get("cir", [])
+        typ, cir, mat = det.get("type", 1), det.get("cir", []), det.get("matrix", [])
     else:                                   # mat_struct
-        typ, cir = getattr(det, "type", 1), getattr(det, "cir", [])
+        typ, cir, mat = getattr(det, "type", 1), getattr(det, "cir", []), getattr(det, "matrix", [])
+    p.detector_fR = [np.asarray(m["fR"] if isinstance(m, dict) else m.fR, dtype=np.float64)
+                     for m in _elements(mat)]
     p.detector_type = int(np.asarray(typ).ravel()[0])
     if not isinstance(cir, (list, tuple, np.ndarray)):
         cir = [cir]
@@ -114,7 +124,7 @@ def load_input(path, name=None):
 def _input_dict(p):
     d = {}
     for k, v in p.__dict__.items():
-        if k in ("system_conf", "detector_cir", "detector_type"):
+        if k in ("system_conf", "detector_cir", "detector_type", "detector_fR"):
             continue
         if v is None:
             continue
@@ -126,7 +136,8 @@ def _input_dict(p):
         else:
             d[k] = v
     d["detector"] = {"type": int(p.detector_type),
-                     "cir": [{"inner_ang": a, "outer_ang": b} for a, b in p.detector_cir]}
+                     "cir": [{"inner_ang": a, "outer_ang": b} for a, b in p.detector_cir],
+                     "matrix": [{"fR": np.asarray(f, dtype=np.float64).tolist()} for f in p.detector_fR]}
     return d
 
 
@@ -143,7 +154,10 @@ def save_input(p, path):
     cir = _struct_array(len(p.detector_cir), ["inner_ang", "outer_ang"])
     for i, (a, b) in enumerate(p.detector_cir):
         cir[0, i]["inner_ang"], cir[0, i]["outer_ang"] = a, b
-    d["detector"] = {"type": int(p.detector_type), "cir": cir}
+    mat = _struct_array(len(p.detector_fR), ["fR"])
+    for i, f in enumerate(p.detector_fR):
+        mat[0, i]["fR"] = np.asarray(f, dtype=np.float64)
+    d["detector"] = {"type": int(p.detector_type), "cir": cir, "matrix": mat}
     sio.savemat(path, {"input_multislice": d, "system_conf": dict(p.system_conf.__dict__)}, oned_as="row")
 
 

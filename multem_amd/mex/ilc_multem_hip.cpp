@@ -72,6 +72,7 @@ void read_lens(const mxArray *s, const char *pfx, const char *si_pfx, mt_lens *L
 struct InputArrays {
     std::vector<double> atoms, thick, iw_psi, iw_x, iw_y;
     std::vector<mt_detector_cir> det;
+    std::vector<double> det_fR;
 };
 
 void read_input(const mxArray *s, mt_input_multislice *in, InputArrays &A)
@@ -161,9 +162,23 @@ void read_input(const mxArray *s, mt_input_multislice *in, InputArrays &A)
             const mwSize n = mxGetN(cir);
             for (mwSize i = 0; i < n; ++i) A.det.push_back(mt_detector_cir{num(cir, "inner_ang", i), num(cir, "outer_ang", i)});
         }
+        // detector.matrix(i).fR, ny x nx each (ilc_multem.cu:317-333)
+        const mxArray *mat = mxGetField(det, 0, "matrix");
+        if (mat && in->detector_type == 3) {
+            const mwSize n = mxGetN(mat);
+            for (mwSize i = 0; i < n; ++i) {
+                const mxArray *f = mxGetField(mat, i, "fR");
+                if (!f || mxGetNumberOfElements(f) != (size_t)in->nx * in->ny)
+                    mexErrMsgIdAndTxt("multem_hip:args", "detector.matrix(%d).fR must be ny x nx", (int)i + 1);
+                const double *p = mxGetPr(f);
+                A.det_fR.insert(A.det_fR.end(), p, p + mxGetNumberOfElements(f));
+            }
+            in->n_detector = (int)n;
+        }
     }
-    in->n_detector = (int)A.det.size();
+    if (in->detector_type != 3) in->n_detector = (int)A.det.size();
     in->detector_cir = A.det.empty() ? nullptr : A.det.data();
+    in->detector_fR = A.det_fR.empty() ? nullptr : A.det_fR.data();
 
     in->output_area_ix_0 = (int)num(s, "output_area_ix_0");
     in->output_area_iy_0 = (int)num(s, "output_area_iy_0");

@@ -87,6 +87,7 @@ class Parameters:
         self.obj_lens_outer_aper_ang = 24.0
         self.detector_type = 1
         self.detector_cir = [(40.0, 160.0)]   # (inner_ang, outer_ang) in mrad
+        self.detector_fR = []                 # detector_type 3: ny x nx sensitivity maps (natural Fourier layout)
         self.scanning_type = 2
         self.scanning_periodic = 0
         self.scanning_square_pxs = 1
@@ -162,6 +163,11 @@ class Parameters:
         keep.append(dets)
         s.detector_cir = C.cast(dets, C.POINTER(capi.DetectorCir))
         s.n_detector = len(self.detector_cir)
+        if int(self.detector_type) == 3:
+            fr = arr(np.concatenate([np.asarray(f, dtype=np.float64).T.ravel() for f in self.detector_fR])
+                     if self.detector_fR else np.zeros(1))
+            s.detector_fR = capi.dptr(fr)
+            s.n_detector = len(self.detector_fR)
         return s, keep
 
     # ------------------------------------------------------------------

@@ -658,6 +658,7 @@ typedef struct {
     real *sc_x, *sc_y; int sc_n;
     /* detector */
     int n_det; real *det_g_inner, *det_g_outer;
+    real *det_fR;          /* matrix detectors [n_det][nx*ny], natural k-space order, or NULL */
     /* output area (0-based half-open) */
     int oa_ix_0, oa_ix_e, oa_iy_0, oa_iy_e;
     real Vrl; int nR;
@@ -1364,7 +1365,7 @@ static void sim_free(sim_t *s)
 {
     if (!s) return;
     free(s->thick); free(s->iw_x); free(s->iw_y); free(s->iw_psi); free(s->sc_x); free(s->sc_y);
-    free(s->det_g_inner); free(s->det_g_outer);
+    free(s->det_g_inner); free(s->det_g_outer); free(s->det_fR);
     atoms_free(&s->in_atoms); atoms_free(&s->atoms_u); atoms_free(&s->atoms);
     free(s->tab); free(s->z_plane); free(s->z_slice); free(s->slice); free(s->thk);
     free(s);
@@ -1492,11 +1493,18 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     }
     if (is_stem(s)) {
         real lambda = get_lambda(s->E_0);
-        if (in->detector_type != 1) FAIL(MT_ERR_UNSUPPORTED, "only circular detectors");
+        if (in->detector_type == 2) FAIL(MT_ERR_UNSUPPORTED, "radial detectors integrate to zero in the reference");
+        if (in->detector_type != 1 && in->detector_type != 3) FAIL(MT_ERR_INVALID_ARGUMENT, "detector_type must be 1..3");
         s->n_det = in->n_detector;
-        s->det_g_inner = malloc(sizeof(real) * (size_t)(s->n_det > 0 ? s->n_det : 1));
-        s->det_g_outer = malloc(sizeof(real) * (size_t)(s->n_det > 0 ? s->n_det : 1));
-        for (int i = 0; i < s->n_det; i++) {
+        s->det_g_inner = calloc((size_t)(s->n_det > 0 ? s->n_det : 1), sizeof(real));
+        s->det_g_outer = calloc((size_t)(s->n_det > 0 ? s->n_det : 1), sizeof(real));
+        if (in->detector_type == 3) {
+            if (!in->detector_fR) FAIL(MT_ERR_INVALID_ARGUMENT, "matrix detectors need detector_fR");
+            size_t n = (size_t)s->n_det * in->nx * in->ny;
+            s->det_fR = malloc(sizeof(real) * (n > 0 ? n : 1));
+            for (size_t i = 0; i < n; i++) s->det_fR[i] = (real)in->detector_fR[i];
+        }
+        for (int i = 0; in->detector_type == 1 && i < s->n_det; i++) {
             real ia = (real)(in->detector_cir[i].inner_ang * c_mrad_2_rad);
             real oa = (real)(in->detector_cir[i].outer_ang * c_mrad_2_rad);
             s->det_g_inner[i] = sin(ia) / lambda;
@@ -1941,6 +1949,31 @@ static void incident_wave(sim_t *s, const real *xb, const real *yb, int nb, real
 /* ------------------------------------------------------------------ */
 /* Engine run                                                          */
 /* ------------------------------------------------------------------ */
+/* integrated_intensity_over_det (wave_function.cuh:107-133) without the
+   weight: sum of |psi(g)|^2 over the ring g2_in <= g^2 < g2_out
+   (sum_square_over_Det, cgpu_fcns.cuh:1794-1805), or weighted by the matrix
+   detector's sensitivity map */
+static real det_sum(const sim_t *s, int id, const cplx *po)
+{
+    const grid_t *g = &s->g;
+    real sum = 0;
+    if (s->det_fR) {
+        const real *f = s->det_fR + (size_t)id * g->nx * g->ny;
+        for (size_t i = 0; i < (size_t)g->nx * g->ny; i++) sum += f[i] * (po[i].re * po[i].re + po[i].im * po[i].im);
+        return sum;
+    }
+    real g2_min = (real)pow(s->det_g_inner[id], 2), g2_max = (real)pow(s->det_g_outer[id], 2);
+    for (int ix = 0; ix < g->nx; ix++)
+        for (int iy = 0; iy < g->ny; iy++) {
+            real g2 = g2_shift(g, ix, iy, 0, 0);
+            if ((g2_min <= g2) && (g2 < g2_max)) {
+                cplx v = po[(size_t)ix * g->ny + iy];
+                sum += v.re * v.re + v.im * v.im;
+            }
+        }
+    return sum;
+}
+
 typedef struct {
     sim_t *s;
     cplx *trans;            /* n_slice x nxy, one configuration */
@@ -1993,19 +2026,8 @@ static void *stem_worker(void *arg)
                     cplx *c = J->coh + ((size_t)iscan * s->n_thk + ithk) * nxy;
                     for (size_t i = 0; i < nxy; i++) { c[i].re += J->w * po[i].re; c[i].im += J->w * po[i].im; }
                 }
-                for (int id = 0; id < s->n_det; id++) {
-                    real g2_min = (real)pow(s->det_g_inner[id], 2), g2_max = (real)pow(s->det_g_outer[id], 2);
-                    real sum = 0;
-                    for (int ix = 0; ix < g->nx; ix++)
-                        for (int iy = 0; iy < g->ny; iy++) {
-                            real g2 = g2_shift(g, ix, iy, 0, 0);
-                            if ((g2_min <= g2) && (g2 < g2_max)) {
-                                cplx v = po[(size_t)ix * g->ny + iy];
-                                sum += v.re * v.re + v.im * v.im;
-                            }
-                        }
-                    J->result[((size_t)it * s->n_thk + ithk) * s->n_det + id] = (double)sum;
-                }
+                for (int id = 0; id < s->n_det; id++)
+                    J->result[((size_t)it * s->n_thk + ithk) * s->n_det + id] = (double)det_sum(s, id, po);
             }
         }
     }
@@ -2136,19 +2158,8 @@ static int run_stem(sim_t *s, mt_output_multislice *out, int nthreads)
         for (int iscan = 0; iscan < nscan; iscan++)
             for (int ithk = 0; ithk < s->n_thk; ithk++) {
                 const cplx *c = coh + ((size_t)iscan * s->n_thk + ithk) * nxy;
-                for (int id = 0; id < s->n_det; id++) {
-                    real g2_min = (real)pow(s->det_g_inner[id], 2), g2_max = (real)pow(s->det_g_outer[id], 2);
-                    real sum = 0;
-                    for (int ix = 0; ix < g->nx; ix++)
-                        for (int iy = 0; iy < g->ny; iy++) {
-                            real g2 = g2_shift(g, ix, iy, 0, 0);
-                            if ((g2_min <= g2) && (g2 < g2_max)) {
-                                cplx v = c[(size_t)ix * g->ny + iy];
-                                sum += v.re * v.re + v.im * v.im;
-                            }
-                        }
-                    out->image_coh[((size_t)ithk * s->n_det + id) * nscan + iscan] = (double)sum;
-                }
+                for (int id = 0; id < s->n_det; id++)
+                    out->image_coh[((size_t)ithk * s->n_det + id) * nscan + iscan] = (double)det_sum(s, id, c);
             }
     }
     free(coh); free(img); free(trans); free(res); free(items);

@@ -411,3 +411,20 @@ def test_rotated_specimen():
     e64 = rel_l2(out.psi_coh, oracle.run(p, "f64").psi_coh)
     print("rotated Si exit wave: rel L2 vs FP32 restatement %.2e, vs FP64 %.2e" % (e32, e64))
     assert e32 < 1e-5 and e64 < 1e-4
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_matrix_detectors(precision):
+    """detector_type 3: sensitivity maps weight |psi(g)|^2 in the detector
+    pass and in the coherent image, against the oracle."""
+    p = small_stem(ns=3, nconf=2)
+    p.system_conf.precision = precision
+    p.pn_coh_contrib = 1
+    p.detector_type = 3
+    rng = np.random.default_rng(11)
+    p.detector_fR = [rng.random((p.ny, p.nx)), (rng.random((p.ny, p.nx)) > 0.7).astype(float)]
+    out, ref = engine.run_multislice(p), oracle.run(p, "f64")
+    e_tot, e_coh = rel_l2(out.image_tot, ref.image_tot), rel_l2(out.image_coh, ref.image_coh)
+    print("matrix detectors (precision %d): image_tot %.2e, image_coh %.2e vs FP64" % (precision, e_tot, e_coh))
+    tol = 1e-4 if precision == 1 else 1e-9
+    assert e_tot < tol and e_coh < tol

@@ -89,10 +89,13 @@ def _cases():
     h = configs.si_ewrs_256(nc=3)
     h.simulation_type = 32
     h.system_conf.precision = 2
+    m = configs.au_stem_512(ns=4, nconf=1, n=64, ncells=2, nc=3)
+    m.detector_type = 3
+    m.detector_fR = [np.random.default_rng(i).random((m.ny, m.nx)) for i in range(2)]
     w = configs.si_ewrs_256(nc=2)
     w.iw_type = 3
     w.iw_psi = np.exp(1j * np.linspace(0, 1, w.nx * w.ny)).reshape(w.ny, w.nx)
-    return [p, q, h, w]
+    return [p, q, h, w, m]
 
 
 def _same(a, b):
@@ -106,6 +109,8 @@ def _same(a, b):
             assert v is None and u is None, k
         elif isinstance(v, np.ndarray) or isinstance(u, np.ndarray):
             assert np.array_equal(np.asarray(v), np.asarray(u)), k
+        elif k == "detector_fR":
+            assert len(v) == len(u) and all(np.array_equal(x, y) for x, y in zip(v, u)), k
         elif k == "detector_cir":
             assert [tuple(map(float, x)) for x in v] == [tuple(map(float, x)) for x in u], k
         else:
@@ -126,8 +131,8 @@ def test_matlab_named_struct_reads_to_the_same_input(tmp_path):
     'input_multislice')` would hold it) gives the engine the same input:
     same output sizes from the C ABI (no device work)."""
     p = configs.au_stem_512(ns=4, nconf=2, n=256, ncells=4, nc=3)
-    fields = {k: v for k, v in p.__dict__.items() if k not in ("system_conf", "detector_cir", "detector_type")
-              and v is not None}
+    fields = {k: v for k, v in p.__dict__.items() if k not in ("system_conf", "detector_cir", "detector_type",
+                                                               "detector_fR") and v is not None}
     cir = np.empty((1, 1), dtype=object)
     cir[0, 0] = {"inner_ang": p.detector_cir[0][0], "outer_ang": p.detector_cir[0][1]}
     fields["detector"] = {"type": 1, "cir": cir}

@@ -59,7 +59,7 @@ def to_mx(L, p):
         L.mh_set(sc, 0, k.encode(), _mat(L, float(v)))
     s = L.mh_struct(1)
     for k, v in p.__dict__.items():
-        if k in ("system_conf", "detector_cir", "detector_type") or v is None:
+        if k in ("system_conf", "detector_cir", "detector_type", "detector_fR") or v is None:
             continue
         L.mh_set(s, 0, k.encode(), _mat(L, v if isinstance(v, np.ndarray) else float(v)))
     det = L.mh_struct(1)
@@ -69,6 +69,11 @@ def to_mx(L, p):
         L.mh_set(cir, i, b"inner_ang", _mat(L, float(a)))
         L.mh_set(cir, i, b"outer_ang", _mat(L, float(b)))
     L.mh_set(det, 0, b"cir", cir)
+    if p.detector_fR:
+        mat = L.mh_struct(len(p.detector_fR))
+        for i, f in enumerate(p.detector_fR):
+            L.mh_set(mat, i, b"fR", _mat(L, np.asarray(f, dtype=np.float64)))
+        L.mh_set(det, 0, b"matrix", mat)
     L.mh_set(s, 0, b"detector", det)
     return sc, s
 
@@ -160,4 +165,14 @@ def test_gateway_cbed_hrtem_ewrs():
     _compare(L, p)
     p = configs.si_ewrs_256(nc=3)
     p.system_conf.precision = 2                 # the double path through the same gateway
+    _compare(L, p)
+
+
+@pytest.mark.gpu
+def test_gateway_matrix_detectors():
+    L = _lib()
+    p = configs.au_stem_512(ns=3, nconf=1, n=256, ncells=4, nc=3)
+    p.detector_type = 3
+    rng = np.random.default_rng(5)
+    p.detector_fR = [rng.random((p.ny, p.nx)) for _ in range(3)]
     _compare(L, p)
