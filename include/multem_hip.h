@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MT_ABI_VERSION 2
+#define MT_ABI_VERSION 3
 
 typedef enum {
     MT_OK = 0,
@@ -75,7 +75,8 @@ typedef struct {
 
 /* The ilc_multem input struct (MATLAB multem_input.parameters, parameters.m:6-239). */
 typedef struct {
-    int simulation_type;       /* 11 STEM, 21 CBED, 51 EWFS, 52 EWRS, ... (types.cuh:193-205) */
+    int simulation_type;       /* 11 STEM, 12 ISTEM, 21 CBED, 22 CBEI, 31 ED, 32 HRTEM, 41 PED, 42 HCTEM,
+                                  51 EWFS, 52 EWRS (types.cuh:193-205) */
     int interaction_model;     /* 1 multislice, 2 phase object, 3 weak phase object */
     int potential_type;        /* 6 Lobato (the only parameterisation provided) */
     int operation_mode;
@@ -143,6 +144,14 @@ typedef struct {
        ilc_multem.cu:317-333): n_detector sensitivity maps, each ny x nx in
        the natural (unshifted) Fourier layout, contiguous */
     const double *detector_fR;
+
+    /* precession electron diffraction (simulation_type 41) and hollow-cone
+       illumination TEM (42): cone angle in degrees and number of azimuths
+       (ped_theta, ped_nrot, hci_theta, hci_nrot; ilc_multem.cu:338-347) */
+    double ped_theta;
+    int ped_nrot;
+    double hci_theta;
+    int hci_nrot;
 } mt_input_multislice;
 
 /* Output kinds (eTEM_Output_Type, types.cuh:209-214) */

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-MT_ABI_VERSION = 2
+MT_ABI_VERSION = 3
 
 MT_OK = 0
 STATUS_NAMES = {
@@ -72,6 +72,7 @@ class InputMultislice(C.Structure):
         ("output_area_ix_e", C.c_int), ("output_area_iy_e", C.c_int),
         ("spec_rot_u0", C.c_double * 3), ("spec_rot_center_type", C.c_int), ("spec_rot_center_p", C.c_double * 3),
         ("detector_fR", _dp),
+        ("ped_theta", C.c_double), ("ped_nrot", C.c_int), ("hci_theta", C.c_double), ("hci_nrot", C.c_int),
     ]
 
 

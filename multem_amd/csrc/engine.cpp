@@ -384,15 +384,17 @@ struct Setup {
 
     // tilt ramp of the outputs, exp(-2 pi i g0.(r - r_c)), with 1/ny folded
     // into rx (it follows an unnormalised inverse transform along y)
-    void ramp_tables(DBuf<Z> &rx, DBuf<Z> &ry)
+    // (one pair per tilt azimuth: rx [nrot][nx], ry [nrot][ny])
+    void ramp_tables(DBuf<Z> &rx, DBuf<Z> &ry, int nrot = 1)
     {
-        rx.alloc(nx);
-        ry.alloc(ny);
+        rx.alloc((size_t)nrot * nx);
+        ry.alloc((size_t)nrot * ny);
         // Grid_2d::Rx_c() is an int (src/types.cuh:2284-2287)
         const double Rx_c = (double)(int)(R(0.5) * m.lx), Ry_c = (double)(int)(R(0.5) * m.ly);
-        check(launch_ramp_tables(nx, ny, (double)m.gx0, (double)m.gy0, m.lx_d / nx, m.ly_d / ny, Rx_c, Ry_c,
-                                 R(1) / (R)ny, rx.p, ry.p, s),
-              "ramp tables");
+        for (int ir = 0; ir < nrot; ++ir)
+            check(launch_ramp_tables(nx, ny, (double)m.gx0_rot(ir), (double)m.gy0_rot(ir), m.lx_d / nx, m.ly_d / ny,
+                                     Rx_c, Ry_c, R(1) / (R)ny, rx.p + (size_t)ir * nx, ry.p + (size_t)ir * ny, s),
+                  "ramp tables");
         check(hipStreamSynchronize(s), "sync");
     }
 
@@ -421,16 +423,19 @@ struct Setup {
         check(hipStreamSynchronize(s), "sync");
     }
 
-    // propagator tables for the given distances: px [nz][nx], py [nz][ny]
-    void prop_tables(const std::vector<double> &z, DBuf<Z> &px, DBuf<Z> &py)
+    // propagator tables for the given distances: px [nz][nx], py [nz][ny],
+    // one set per tilt azimuth ([nrot][nz][nx], [nrot][nz][ny])
+    void prop_tables(const std::vector<double> &z, DBuf<Z> &px, DBuf<Z> &py, int nrot = 1)
     {
         DBuf<double> dz(z.size());
         dz.upload(z.data(), z.size(), s);
-        px.alloc(z.size() * nx);
-        py.alloc(z.size() * ny);
-        check(launch_prop_tables(nx, ny, dz.p, (int)z.size(), m.lambda_d, m.mul_sign, m.lx_d, m.ly_d, (double)m.gx0,
-                                 (double)m.gy0, px.p, py.p, s),
-              "prop tables");
+        px.alloc((size_t)nrot * z.size() * nx);
+        py.alloc((size_t)nrot * z.size() * ny);
+        for (int ir = 0; ir < nrot; ++ir)
+            check(launch_prop_tables(nx, ny, dz.p, (int)z.size(), m.lambda_d, m.mul_sign, m.lx_d, m.ly_d,
+                                     (double)m.gx0_rot(ir), (double)m.gy0_rot(ir), px.p + (size_t)ir * z.size() * nx,
+                                     py.p + (size_t)ir * z.size() * ny, s),
+                  "prop tables");
         check(hipStreamSynchronize(s), "sync");
     }
 };
@@ -1096,7 +1101,18 @@ void run_stem_group(const std::vector<const ModelT<R> *> &ms, const std::vector<
 }
 
 // ---------------------------------------------------------------------------
-// Exit wave / CBED: the members of a batch are phonon configurations.
+// Exit wave, diffraction and image simulations (EWFS, EWRS, CBED, CBEI, ED,
+// HRTEM, ISTEM, PED, HCTEM): the members of a launch are the waves of one or
+// more phonon configurations.
+//
+// One wave in the reference's loop order: its beams, integration weight and
+// tilt azimuth.
+struct EwWave {
+    int beam0, nbeam;
+    double wq;
+    int irot;
+};
+
 template <class R>
 void run_ew(const ModelT<R> &m, mt_output_multislice *out)
 {
@@ -1113,7 +1129,9 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     const int ot = dims.output_type;
     const int nxo = m.oa_ix_e - m.oa_ix_0, nyo = m.oa_iy_e - m.oa_iy_0;
     const size_t no = (size_t)nxo * nyo;
-    const bool fs = !m.is_ewrs() && !m.is_hrtem();     // Fourier-space output (EWFS, CBED)
+    const bool fs = m.is_grid_fs();     // Fourier-space output (EWFS, CBED, ED, PED)
+    const bool img = m.is_image();      // objective-lens image (HRTEM, ISTEM, CBEI, HCTEM)
+    const bool ramp = m.out_ramp();
     const int ns_run = [&] {
         int last = 0;
         for (const auto &t : m.thk) last = std::max(last, t.islice);
@@ -1121,30 +1139,72 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     }();
 
     const int nconf = m.n_conf_run();
-    // waves per configuration: one, or for fully integrated CBED one per
-    // (source point, defocus point) in the loop order of CBED_CBEI
-    // (src/tem_simulation.cuh:207-261), all sharing the configuration's stack
+    // Waves of one configuration, all sharing its transmission stack, in the
+    // loop order of the reference:
+    //  - CBED / CBEI fully integrated: (source point, defocus point), beams
+    //    shifted by the source offset (CBED_CBEI, src/tem_simulation.cuh:207-261);
+    //  - ISTEM: (defocus point, scan position) (STEM_ISTEM, :100-200; the
+    //    defocus spread only when fully integrated);
+    //  - PED / HCTEM: the tilt azimuths (PED_HCTEM, :274-301);
+    //  - otherwise one wave.
     Quad1 qt{{0.0}, {1.0}};
     Quad2 qs{{0.0}, {0.0}, {1.0}};
-    if (m.is_cbed() && m.full_integration()) m.cond_quadratures(qt, qs);
+    if (m.full_integration() && (m.is_cbed_cbei() || m.is_istem())) m.cond_quadratures(qt, qs);
+    if (m.is_istem()) qs = Quad2{{0.0}, {0.0}, {1.0}};
     std::vector<double> tc10, twq;
     S.cond_defocus_points(qt, tc10, twq);
-    const int n_s = (int)qs.w.size(), n_t = (int)tc10.size(), nq = n_s * n_t;
-    // beams: every iw_x, iw_y of a convergent wave; the first for a user wave
-    const int nbeam = m.iw_type == 2 ? (int)m.iw_x.size() : (m.iw_type == 3 ? 1 : 0);
+    const int n_s = (int)qs.w.size(), n_t = (int)tc10.size();
+    const int n_pos = m.is_istem() ? (int)m.sc_x.size() : 1;
+    const int n_rot = m.nrot;
+    const int nq = n_s * n_t * n_pos * n_rot;
+    // beams of a wave: every iw_x, iw_y of a convergent wave (the scan
+    // position in ISTEM); the first for a user wave
+    const int nbeam = m.iw_type == 2 ? (m.is_istem() ? 1 : (int)m.iw_x.size()) : (m.iw_type == 3 ? 1 : 0);
+    std::vector<EwWave> wv(nq);
+    std::vector<Beam> hb;
+    for (int i_s = 0; i_s < n_s; ++i_s)
+        for (int i_t = 0; i_t < n_t; ++i_t)
+            for (int ip = 0; ip < n_pos; ++ip)
+                for (int ir = 0; ir < n_rot; ++ir) {
+                    const int q = ((i_s * n_t + i_t) * n_pos + ip) * n_rot + ir;
+                    wv[q] = EwWave{q * nbeam, nbeam, qs.w[i_s] * twq[i_t], ir};
+                    for (int ib = 0; ib < nbeam; ++ib) {
+                        if (m.is_istem()) {
+                            hb.push_back(S.beam(m.sc_x[ip], m.sc_y[ip], tc10[i_t]));
+                        } else {
+                            // beam_x = iw_x + source offset, stored in R (src/tem_simulation.cuh:233-237)
+                            const R xb = (R)((double)m.iw_x[ib] + qs.x[i_s]);
+                            const R yb = (R)((double)m.iw_y[ib] + qs.y[i_s]);
+                            hb.push_back(S.beam(xb, yb, tc10[i_t]));
+                        }
+                    }
+                }
 
+    // Configurations per group (their stacks resident) and waves per launch:
+    // all waves of the group while they fit the wave cap, else one
+    // configuration at a time in chunks of waves.
     const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(Z);
-    const size_t per_conf = stack_bytes + 2 * (size_t)nq * nxy * sizeof(Z);
+    const size_t wave_bytes = (size_t)(img && ramp ? 3 : 2) * nxy * sizeof(Z);
     const size_t fixed = (size_t)n_thk * (no * 3 + 2 * nxy) * sizeof(double);
     const size_t budget = budget_bytes();
-    if (fixed + per_conf > budget) throw Error{MT_ERR_OUT_OF_MEMORY, "one configuration exceeds the memory budget"};
-    int K = (int)std::min<size_t>((size_t)nconf, (budget - fixed) / per_conf);
-    if (run_control().batch_override > 0) K = std::min(K, run_control().batch_override);
-    K = std::max(K, 1);
+    if (fixed + stack_bytes + wave_bytes > budget)
+        throw Error{MT_ERR_OUT_OF_MEMORY, "one configuration exceeds the memory budget"};
+    const int wave_cap = run_control().batch_override > 0 ? run_control().batch_override : 256;
+    int K, W;
+    if (nq <= wave_cap) {
+        K = (int)std::min<size_t>((size_t)nconf, (budget - fixed) / (stack_bytes + (size_t)nq * wave_bytes));
+        K = std::max(1, std::min(K, wave_cap / nq));
+        W = K * nq;
+    } else {
+        K = 1;
+        W = (int)std::min<size_t>((size_t)wave_cap, (budget - fixed - stack_bytes) / wave_bytes);
+        W = std::max(W, 1);
+    }
 
-    DBuf<Z> trans((size_t)K * m.n_slice() * nxy), psi((size_t)K * nq * nxy), scratch((size_t)K * nq * nxy);
-    DBuf<Member> members((size_t)K * nq);
-    DBuf<Beam> beams(std::max<size_t>(1, (size_t)nq * nbeam));
+    DBuf<Z> trans((size_t)K * m.n_slice() * nxy), psi((size_t)W * nxy), scratch((size_t)W * nxy);
+    DBuf<Member> members((size_t)W);
+    DBuf<Beam> beams(std::max<size_t>(1, hb.size()));
+    if (!hb.empty()) beams.upload(hb.data(), hb.size(), s);
     DBuf<double> m2(std::max<size_t>(1, (size_t)n_thk * no)), pc(std::max<size_t>(1, (size_t)n_thk * no * 2));
     const bool need_full = (ot == MT_OT_M2PSI_TOT_COH);
     DBuf<double> pfull(need_full ? (size_t)n_thk * nxy * 2 : 1);
@@ -1157,39 +1217,45 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     std::vector<double> zl;
     for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
     for (const auto &t : m.thk) zl.push_back((double)t.z_back_prop);
+    // propagator (and output ramp) tables, one set per tilt azimuth, picked
+    // per wave by Member::irot
     DBuf<Z> px, py;
-    S.prop_tables(zl, px, py);
+    S.prop_tables(zl, px, py, n_rot);
+    const long long rot_px = n_rot > 1 ? (long long)zl.size() * m.nx : 0;
+    const long long rot_py = n_rot > 1 ? (long long)zl.size() * m.ny : 0;
     DBuf<Z> rx, ry;
-    if (m.tilted()) S.ramp_tables(rx, ry);
+    if (ramp) S.ramp_tables(rx, ry, n_rot);
     DBuf<Z> ctf;
     std::vector<double> cwq{1.0};   // weights of the objective-lens tables
-    if (m.is_hrtem()) S.ctf_tables(ctf, cwq);
+    if (img) S.ctf_tables(ctf, cwq);
     const int n_ctf = (int)cwq.size();
-    // a tilted HRTEM run that reads the ramp output more than once (several
-    // lens tables, or the coherent pass) keeps it apart from the output passes
-    DBuf<Z> scratch2(m.is_hrtem() && m.tilted() && (n_ctf > 1 || need_full) ? (size_t)K * nxy : 0);
+    // an image run with the output ramp that reads the ramp output more than
+    // once (several lens tables, or the coherent pass) keeps it apart from
+    // the output passes
+    DBuf<Z> scratch2(img && ramp && (n_ctf > 1 || need_full) ? (size_t)W * nxy : 0);
     ProbeParamsT<R> P{};
     DBuf<Z> fpsi0;
     if (m.iw_type == 3) S.user_wave_spectrum(fpsi0);
-    if (m.iw_type >= 2) {
-        P = S.probe_params(S.count_aperture());
-        std::vector<Beam> hb;
-        for (int i_s = 0; i_s < n_s; ++i_s)
-            for (int i_t = 0; i_t < n_t; ++i_t)
-                for (int ib = 0; ib < nbeam; ++ib) {
-                    // beam_x = iw_x + source offset, stored in R (src/tem_simulation.cuh:233-237)
-                    const R xb = (R)((double)m.iw_x[ib] + qs.x[i_s]), yb = (R)((double)m.iw_y[ib] + qs.y[i_s]);
-                    hb.push_back(S.beam(xb, yb, tc10[i_t]));
-                }
-        beams.upload(hb.data(), hb.size(), s);
-    }
+    if (m.iw_type >= 2) P = S.probe_params(S.count_aperture());
+    const char *kind = m.is_cbed() ? "cbed" : m.is_cbei() ? "cbei" : m.is_ed() ? "ed" : m.is_hrtem() ? "hrtem"
+                     : m.is_istem() ? "istem" : m.is_ped() ? "ped" : m.is_hctem() ? "hctem"
+                     : m.is_ewrs() ? "ewrs" : "ewfs";
     std::ostringstream plan;
-    plan << (m.is_cbed() ? "cbed" : (m.is_hrtem() ? "hrtem" : (m.is_ewrs() ? "ewrs" : "ewfs")))
-         << (sizeof(R) == 8 ? " fp64" : " fp32") << " nx=" << m.nx << " ny=" << m.ny
-         << " configs_per_batch=" << K << " slices=" << ns_run;
+    plan << kind << (sizeof(R) == 8 ? " fp64" : " fp32") << " nx=" << m.nx << " ny=" << m.ny
+         << " configs_per_batch=" << K << " waves_per_launch=" << W << " slices=" << ns_run;
     if (nq > 1) plan << " waves_per_config=" << nq;
     if (n_ctf > 1) plan << " ctf_defocus_points=" << n_ctf;
     tm.plan = plan.str();
+
+    // column-pass arguments with the wave's tilt-azimuth tables
+    auto with_rot = [&](ColArgsT<Z> c, int zi) {
+        c.px = px.p + (size_t)zi * m.nx;
+        c.py = py.p + (size_t)zi * m.ny;
+        c.rot_x = rot_px;
+        c.rot_y = rot_py;
+        c.members = members.p;
+        return c;
+    };
 
     const double w = (double)m.phonon_weight();
     run_control().niter = nconf * nq;
@@ -1198,123 +1264,127 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     record(t0, s);
     DBuf<vec4_t<R>> atoms(m.atoms_u.x.size());
     PassChoice choice;
+    bool tuned = false;
     for (int c0 = 0; c0 < nconf; c0 += K) {
         stop_check();
-        const int nb = std::min(K, nconf - c0), nm = nb * nq;   // configurations, waves
-        std::vector<Member> hm(nm);
+        const int nb = std::min(K, nconf - c0);   // configurations of this group
         for (int j = 0; j < nb; ++j) {
-            for (int q = 0; q < nq; ++q)
-                hm[j * nq + q] = Member{j, -1, q * nbeam, nbeam, qs.w[q / n_t] * twq[q % n_t]};
             auto a = S.atoms_of(m.fp_iconf_0 + c0 + j);
             atoms.upload(a.data(), a.size(), s);
             S.build_tf(atoms, 0, m.n_slice(), trans.p + (size_t)j * m.n_slice() * nxy, nullptr);
             check(hipStreamSynchronize(s), "sync");   // atoms buffer is reused by the next configuration
         }
-        members.upload(hm.data(), nm, s);
+        for (int w0 = 0; w0 < nb * nq; w0 += W) {
+            stop_check();
+            const int nm = std::min(W, nb * nq - w0);   // waves of this launch
+            std::vector<Member> hm(nm);
+            for (int i = 0; i < nm; ++i) {
+                const int g = w0 + i, j = g / nq;
+                const EwWave &e = wv[g % nq];
+                hm[i] = Member{j, -1, e.beam0, e.nbeam, e.wq, e.irot};
+            }
+            members.upload(hm.data(), nm, s);
 
-        RowArgsT<Z> r{};
-        r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nm;
-        r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = (long long)m.n_slice() * nxy; r.members = members.p;
-        if (c0 == 0) {   // pick the slice-step launch variants on this batch's buffers
-            RowArgsT<Z> rt = r;
-            rt.slice = 0;
-            rt.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
-            ColArgsT<Z> ct = S.col_base(psi.p, psi.p, nm);
-            ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
-            ct.px = px.p; ct.py = py.p;
-            check(hipMemsetAsync(psi.p, 0, (size_t)nm * nxy * sizeof(Z), s), "hipMemsetAsync");
-            choice = choose_variants(m.nx, m.ny, nm, rt, ct, (const ColArgsT<Z> *)nullptr, s);
-            tm.tune_log = choice.log;
-            tm.plan += " row_variant=" + std::to_string(choice.row) + " col_variant=" + std::to_string(choice.col);
-        }
-        r.variant = choice.row;
-        if (m.iw_type >= 2) {
-            ColArgsT<Z> cp = S.col_base(nullptr, psi.p, nm);
-            cp.flags = COL_IFFT | COL_STORE | (m.iw_type == 3 ? COL_USER_WAVE : 0);
-            cp.fpsi0 = fpsi0.p;
-            cp.members = members.p; cp.beams = beams.p; cp.probe = P;
-            check(launch_col_pass(m.nx, cp, true, false, s), "probe pass");
-        }
-        for (int is = 0; is < ns_run; ++is) {
-            r.slice = is;
-            r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
-            check(launch_row_pass(m.ny, r, s), "row pass");
-            ColArgsT<Z> c = S.col_base(psi.p, psi.p, nm);
-            c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
-            c.px = px.p + (size_t)is * m.nx;
-            c.py = py.p + (size_t)is * m.ny;
-            c.variant = choice.col;
-            check(launch_col_pass(m.nx, c, false, false, s), "col pass");
-            const int ithk = m.slices[is].ithk;
-            if (ithk < 0) continue;
-            // O = F_x(L)/nx * bwl * exp(i phi_back)  (get_psi, src/wave_function.cuh:96-105)
-            const R zb = m.thk[ithk].z_back_prop;
-            const Z *osrc0 = psi.p;
-            if (m.tilted()) {
-                // psi exp(-2 pi i g0.r) in real space first (get_psi, src/wave_function.cuh:96-105)
-                RowArgsT<Z> ro = r;
-                ro.src = psi.p; ro.dst = scratch.p;
-                ro.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_RAMP | ROW_FFT | ROW_STORE;
-                ro.ramp_x = rx.p; ro.ramp_y = ry.p;
-                check(launch_row_pass(m.ny, ro, s), "output ramp pass");
-                osrc0 = scratch.p;
+            RowArgsT<Z> r{};
+            r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nm;
+            r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = (long long)m.n_slice() * nxy; r.members = members.p;
+            if (!tuned) {   // pick the slice-step launch variants on this launch's buffers
+                RowArgsT<Z> rt = r;
+                rt.slice = 0;
+                rt.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
+                ColArgsT<Z> ct = with_rot(S.col_base(psi.p, psi.p, nm), 0);
+                ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+                check(hipMemsetAsync(psi.p, 0, (size_t)nm * nxy * sizeof(Z), s), "hipMemsetAsync");
+                choice = choose_variants(m.nx, m.ny, nm, rt, ct, (const ColArgsT<Z> *)nullptr, s);
+                tm.tune_log = choice.log;
+                tm.plan += " row_variant=" + std::to_string(choice.row) + " col_variant=" + std::to_string(choice.col);
+                tuned = true;
             }
-            Z *odst = scratch2.n ? scratch2.p : scratch.p;
-            for (int jc = 0; jc < n_ctf; ++jc) {
-                const Z *osrc = osrc0;
-                if (fs || zb != R(0) || m.bwl || m.tilted() || m.is_hrtem()) {
-                    // HRTEM: the objective lens acts on O(g) before the inverse
-                    // transforms (microscope_effects, src/wave_function.cuh:172-180)
-                    ColArgsT<Z> co = S.col_base(osrc, odst, nm);
-                    co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) |
-                               COL_STORE | (m.is_hrtem() ? COL_MUL_CTF : 0);
-                    co.ctf = ctf.p + (size_t)jc * nxy;
-                    co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
-                    co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
-                    co.p_scale = (R)m.ny;
-                    check(launch_col_pass(m.nx, co, false, false, s), "output col pass");
-                    osrc = odst;
-                }
-                if (!fs) {
+            r.variant = choice.row;
+            if (m.iw_type >= 2) {
+                ColArgsT<Z> cp = S.col_base(nullptr, psi.p, nm);
+                cp.flags = COL_IFFT | COL_STORE | (m.iw_type == 3 ? COL_USER_WAVE : 0);
+                cp.fpsi0 = fpsi0.p;
+                cp.members = members.p; cp.beams = beams.p; cp.probe = P;
+                check(launch_col_pass(m.nx, cp, true, false, s), "probe pass");
+            }
+            for (int is = 0; is < ns_run; ++is) {
+                r.slice = is;
+                r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
+                check(launch_row_pass(m.ny, r, s), "row pass");
+                ColArgsT<Z> c = with_rot(S.col_base(psi.p, psi.p, nm), is);
+                c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+                c.variant = choice.col;
+                check(launch_col_pass(m.nx, c, false, false, s), "col pass");
+                const int ithk = m.slices[is].ithk;
+                if (ithk < 0) continue;
+                // O = F_x(L)/nx * bwl * exp(i phi_back)  (get_psi, src/wave_function.cuh:96-105)
+                const R zb = m.thk[ithk].z_back_prop;
+                const Z *osrc0 = psi.p;
+                if (ramp) {
+                    // psi exp(-2 pi i g0.r) in real space first (get_psi, src/wave_function.cuh:96-105)
                     RowArgsT<Z> ro = r;
-                    ro.src = osrc; ro.dst = odst;
-                    ro.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
-                    check(launch_row_pass(m.ny, ro, s), "output row pass");
-                    osrc = odst;
+                    ro.src = psi.p; ro.dst = scratch.p;
+                    ro.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_RAMP | ROW_FFT | ROW_STORE;
+                    ro.ramp_x = rx.p; ro.ramp_y = ry.p;
+                    ro.rot_x = n_rot > 1 ? m.nx : 0;
+                    ro.rot_y = n_rot > 1 ? m.ny : 0;
+                    check(launch_row_pass(m.ny, ro, s), "output ramp pass");
+                    osrc0 = scratch.p;
                 }
-                OutArgsT<Z> oa{};
-                oa.src = osrc; oa.member_stride = (long long)nxy; oa.n_members = nm;
-                oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
-                oa.w = w * cwq[jc];
-                oa.members = members.p;
-                oa.m2 = m2.p + (size_t)ithk * no;
-                oa.psi = pc.p + (size_t)ithk * no * 2;
-                oa.psi_full = pfull.p + (need_full ? (size_t)ithk * nxy * 2 : 0);
-                if (ot == MT_OT_PSI_COH) oa.mode = OUT_SET_PSI;
-                else if (ot == MT_OT_M2PSI_TOT_PSI_COH) oa.mode = OUT_ADD_M2 | OUT_ADD_PSI;
-                else oa.mode = OUT_ADD_M2 | (need_full && !m.is_hrtem() ? OUT_ADD_PSI_FULL : 0);
-                check(launch_out_accum(oa, s), "output accumulate");
+                Z *odst = scratch2.n ? scratch2.p : scratch.p;
+                for (int jc = 0; jc < n_ctf; ++jc) {
+                    const Z *osrc = osrc0;
+                    if (fs || zb != R(0) || m.bwl || m.tilted() || img) {
+                        // images: the objective lens acts on O(g) before the inverse
+                        // transforms (microscope_effects, src/wave_function.cuh:172-180)
+                        ColArgsT<Z> co = with_rot(S.col_base(osrc, odst, nm), m.n_slice() + ithk);
+                        co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) |
+                                   COL_STORE | (img ? COL_MUL_CTF : 0);
+                        co.ctf = ctf.p + (size_t)jc * nxy;
+                        co.p_scale = (R)m.ny;
+                        check(launch_col_pass(m.nx, co, false, false, s), "output col pass");
+                        osrc = odst;
+                    }
+                    if (!fs) {
+                        RowArgsT<Z> ro = r;
+                        ro.src = osrc; ro.dst = odst;
+                        ro.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+                        check(launch_row_pass(m.ny, ro, s), "output row pass");
+                        osrc = odst;
+                    }
+                    OutArgsT<Z> oa{};
+                    oa.src = osrc; oa.member_stride = (long long)nxy; oa.n_members = nm;
+                    oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
+                    oa.w = w * cwq[jc];
+                    oa.members = members.p;
+                    oa.m2 = m2.p + (size_t)ithk * no;
+                    oa.psi = pc.p + (size_t)ithk * no * 2;
+                    oa.psi_full = pfull.p + (need_full ? (size_t)ithk * nxy * 2 : 0);
+                    if (ot == MT_OT_PSI_COH) oa.mode = OUT_SET_PSI;
+                    else if (ot == MT_OT_M2PSI_TOT_PSI_COH) oa.mode = OUT_ADD_M2 | OUT_ADD_PSI;
+                    else oa.mode = OUT_ADD_M2 | (need_full && !img ? OUT_ADD_PSI_FULL : 0);
+                    check(launch_out_accum(oa, s), "output accumulate");
+                }
+                if (need_full && img) {
+                    // psi_coh += w O(g), the back-propagated wave before the objective
+                    // lens (add_scale_psi_coh, src/wave_function.cuh:166-172)
+                    ColArgsT<Z> co = with_rot(S.col_base(osrc0, odst, nm), m.n_slice() + ithk);
+                    co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE;
+                    co.p_scale = (R)m.ny;
+                    check(launch_col_pass(m.nx, co, false, false, s), "coherent output col pass");
+                    OutArgsT<Z> oa{};
+                    oa.src = odst; oa.member_stride = (long long)nxy; oa.n_members = nm;
+                    oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
+                    oa.w = w;
+                    oa.members = members.p;
+                    oa.psi_full = pfull.p + (size_t)ithk * nxy * 2;
+                    oa.mode = OUT_ADD_PSI_FULL;
+                    check(launch_out_accum(oa, s), "coherent output accumulate");
+                }
             }
-            if (need_full && m.is_hrtem()) {
-                // psi_coh += w O(g), the back-propagated wave before the objective
-                // lens (add_scale_psi_coh, src/wave_function.cuh:166-172)
-                ColArgsT<Z> co = S.col_base(osrc0, odst, nm);
-                co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE;
-                co.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
-                co.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
-                co.p_scale = (R)m.ny;
-                check(launch_col_pass(m.nx, co, false, false, s), "coherent output col pass");
-                OutArgsT<Z> oa{};
-                oa.src = odst; oa.member_stride = (long long)nxy; oa.n_members = nm;
-                oa.nx = m.nx; oa.ny = m.ny; oa.ix_0 = m.oa_ix_0; oa.iy_0 = m.oa_iy_0; oa.nx_o = nxo; oa.ny_o = nyo;
-                oa.w = w;
-                oa.members = members.p;
-                oa.psi_full = pfull.p + (size_t)ithk * nxy * 2;
-                oa.mode = OUT_ADD_PSI_FULL;
-                check(launch_out_accum(oa, s), "coherent output accumulate");
-            }
+            progress_add(nm);
         }
-        progress_add(nm);
     }
     record(t1, s);
     wait(t1, s);
@@ -1325,7 +1395,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
         check(hipMemcpyAsync(out->m2psi_tot, m2.p, (size_t)n_thk * no * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
     if (out->psi_coh && (ot == MT_OT_PSI_COH || ot == MT_OT_M2PSI_TOT_PSI_COH))
         check(hipMemcpyAsync(out->psi_coh, pc.p, (size_t)n_thk * no * 2 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-    if (need_full && out->m2psi_coh && m.is_hrtem()) {
+    if (need_full && out->m2psi_coh && img) {
         // set_m2psi_coh for HRTEM (src/wave_function.cuh:206-215): the objective
         // lens applied to psi_coh, |IFFT|^2 (summed over the defocus points of a
         // fully integrated lens), cropped and shifted

@@ -377,8 +377,10 @@ template <class R>
 ModelT<R>::ModelT(const mt_input_multislice &in)
 {
     sim_type = in.simulation_type;
-    if (!(is_stem() || is_cbed() || is_hrtem() || is_ewfs() || is_ewrs()))
-        fail(MT_ERR_UNSUPPORTED, "simulation_type must be 11 (STEM), 21 (CBED), 32 (HRTEM), 51 (EWFS) or 52 (EWRS)");
+    if (!(is_stem() || is_istem() || is_cbed() || is_cbei() || is_ed() || is_hrtem() || is_ped() || is_hctem() ||
+          is_ewfs() || is_ewrs()))
+        fail(MT_ERR_UNSUPPORTED, "simulation_type must be 11 (STEM), 12 (ISTEM), 21 (CBED), 22 (CBEI), 31 (ED), "
+                                 "32 (HRTEM), 41 (PED), 42 (HCTEM), 51 (EWFS) or 52 (EWRS)");
     if (in.interaction_model < 1 || in.interaction_model > 3)
         fail(MT_ERR_INVALID_ARGUMENT, "interaction_model must be 1 (multislice), 2 (phase object) or 3 (weak phase object)");
     interaction_model = in.interaction_model;
@@ -396,14 +398,14 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     // exit waves are coherent; CBED is coherent unless fully integrated
     // (Input_Multislice::validate_parameters, src/input_multislice.cuh:672-682)
     illumination_model = in.illumination_model;
-    if (is_ewfs() || is_ewrs() || (is_cbed() && illumination_model != 4)) illumination_model = 1;
+    if (is_ewfs() || is_ewrs() || (is_cbed_cbei() && illumination_model != 4)) illumination_model = 1;
     // no source spread: temporal incoherence only (src/input_multislice.cuh:746-751)
     temporal_spatial_incoh = near_zero((R)in.cond_lens.si_sigma) ? 2 : in.temporal_spatial_incoh;
-    if (is_hrtem() && illumination_model == 4 && temporal_spatial_incoh != 2)
-        fail(MT_ERR_UNSUPPORTED, "HRTEM full integration with spatial incoherence: the reference's source "
-                                 "quadrature is empty (src/cpu_fcns.hpp:1535) and its image is zero");
-    if (is_hrtem() && illumination_model == 3)
-        fail(MT_ERR_UNSUPPORTED, "HRTEM with illumination_model 3 (transmission cross coefficient)");
+    if (is_image() && illumination_model == 4 && temporal_spatial_incoh != 2)
+        fail(MT_ERR_UNSUPPORTED, "objective-lens full integration with spatial incoherence: the reference's "
+                                 "source quadrature is empty (src/cpu_fcns.hpp:1535) and its image is zero");
+    if (is_image() && illumination_model == 3)
+        fail(MT_ERR_UNSUPPORTED, "objective-lens image with illumination_model 3 (transmission cross coefficient)");
 
     pn_model = in.pn_model;
     pn_coh_contrib = in.pn_coh_contrib;
@@ -439,15 +441,15 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     bwl_alpha = (R)(std::log(1.0 / 1e-2 - 1.0) / ((gl_max + R(0.25)) * (gl_max + R(0.25)) - gl2_max));
 
     iw_type = in.iw_type;
-    if (iw_type == 4) iw_type = (is_stem() || is_cbed()) ? 2 : 1;
+    if (iw_type == 4) iw_type = (is_scanning() || is_cbed_cbei()) ? 2 : 1;
     if (iw_type < 1 || iw_type > 3) fail(MT_ERR_INVALID_ARGUMENT, "iw_type must be 1..4");
     for (int i = 0; i < in.n_iw_xy; ++i) { iw_x.push_back((R)in.iw_x[i]); iw_y.push_back((R)in.iw_y[i]); }
-    if (!is_stem() && iw_type == 2 && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y");
+    if (!is_scanning() && iw_type == 2 && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "convergent wave needs iw_x, iw_y");
     if (iw_type == 3) {
         // user-defined incident wave, ny x nx complex, placed at (iw_x[0], iw_y[0])
         // (Incident_Wave::operator(), src/incident_wave.cuh:92-110)
         if (!in.iw_psi) fail(MT_ERR_INVALID_ARGUMENT, "iw_type 3 needs iw_psi (ny x nx complex)");
-        if (!is_stem() && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "iw_type 3 needs iw_x, iw_y");
+        if (!is_scanning() && iw_x.empty()) fail(MT_ERR_INVALID_ARGUMENT, "iw_type 3 needs iw_x, iw_y");
         iw_psi.assign(in.iw_psi, in.iw_psi + 2 * (size_t)nx * ny);
     }
 
@@ -455,6 +457,14 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     if (!(E_0 > 0)) fail(MT_ERR_INVALID_ARGUMENT, "E_0 must be positive");
     theta = (R)(in.theta * kDegToRad);
     phi = (R)(in.phi * kDegToRad);
+    // precession (PED) and hollow-cone (HCTEM) illumination: the cone angle
+    // and the number of azimuths (ilc_multem.cu:338-347); phi follows the
+    // rotation (validate_parameters, src/input_multislice.cuh:655-660)
+    if (is_ped_hctem()) {
+        theta = (R)((is_ped() ? in.ped_theta : in.hci_theta) * kDegToRad);
+        nrot = std::max(1, is_ped() ? in.ped_nrot : in.hci_nrot);
+        phi = R(0);
+    }
     if (std::fabs(theta) >= R(0.5) * (R)kPi) fail(MT_ERR_INVALID_ARGUMENT, "theta must be below 90 degrees");
     lambda = lambda_of<R>(E_0);
     // Input_Multislice::set_incident_angle (src/input_multislice.cuh:891-895):
@@ -500,7 +510,7 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     set_lens(obj, E_0, g_max);
 
     // scanning (Scanning::set_grid, src/types.cuh:4294-4389)
-    if (is_stem()) {
+    if (is_scanning()) {
         int ns = in.scanning_ns;
         bool pbc = in.scanning_periodic != 0, spxs = in.scanning_square_pxs != 0;
         if (ns <= 0) fail(MT_ERR_INVALID_ARGUMENT, "scanning_ns must be positive");
@@ -537,6 +547,8 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
                 }
         }
         sc_x0 = x0; sc_y0 = y0;
+    }
+    if (is_stem()) {
         // detectors (ilc_multem.cu:272-337): circular rings or matrix
         // sensitivity maps; the reference's radial detector integrates to zero
         // (integrated_intensity_over_det, src/wave_function.cuh:120-124)
@@ -688,7 +700,7 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     cond.zero_defocus_plane = zero_defocus_plane(cond, zmin, zmax);
     cond.zero_defocus_type = 4;
 
-    if (is_stem()) { oa_ix_0 = 0; oa_ix_e = nx; oa_iy_0 = 0; oa_iy_e = ny; }
+    if (is_stem()) { oa_ix_0 = 0; oa_ix_e = nx; oa_iy_0 = 0; oa_iy_e = ny; }   // validate_output_area
     else {
         oa_ix_0 = in.output_area_ix_0 - 1; oa_ix_e = in.output_area_ix_e - 1;
         oa_iy_0 = in.output_area_iy_0 - 1; oa_iy_e = in.output_area_iy_e - 1;
@@ -798,10 +810,22 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
 }
 
 template <class R>
+R ModelT<R>::gx0_rot(int irot) const
+{
+    return is_ped_hctem() ? std::sin(theta) * std::cos(rot_phi(irot)) / lambda : gx0;
+}
+
+template <class R>
+R ModelT<R>::gy0_rot(int irot) const
+{
+    return is_ped_hctem() ? std::sin(theta) * std::sin(rot_phi(irot)) / lambda : gy0;
+}
+
+template <class R>
 R ModelT<R>::phonon_weight() const
 {
     int nconf = (!frozen() || pn_single_conf) ? 1 : pn_nconf;
-    return R(1.0) / R(nconf);
+    return R(1.0) / R(nconf * nrot);
 }
 
 // Randn_3d<T, e_host> (src/cgpu_rand.cuh:303-402): a 64-bit Mersenne Twister
@@ -844,9 +868,10 @@ void ModelT<R>::output_dims(mt_output_multislice *out) const
         return;
     }
     out->nx = oa_ix_e - oa_ix_0; out->ny = oa_iy_e - oa_iy_0; out->n_det = 0;
-    out->dx = (is_ewrs() || is_hrtem()) ? dRx : dgx;
-    out->dy = (is_ewrs() || is_hrtem()) ? dRy : dgy;
-    if (is_cbed() || is_hrtem()) out->output_type = pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
+    out->dx = is_grid_fs() ? dgx : dRx;
+    out->dy = is_grid_fs() ? dgy : dRy;
+    // Output_Multislice::set_output_type (src/output_multislice.hpp:1249-1280)
+    if (!(is_ewfs() || is_ewrs())) out->output_type = pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
     else out->output_type = (!frozen() || pn_single_conf) ? MT_OT_PSI_COH : MT_OT_M2PSI_TOT_PSI_COH;
 }
 
@@ -859,7 +884,7 @@ void ModelT<R>::fill_axes(mt_output_multislice *out) const
         if (out->y) for (int i = 0; i < sc_ny; ++i) out->y[i] = sc_y0 + i * sc_dRy;
         return;
     }
-    bool rs = is_ewrs() || is_hrtem();
+    bool rs = !is_grid_fs();
     if (out->x) for (int ix = oa_ix_0; ix < oa_ix_e; ++ix) out->x[ix - oa_ix_0] = rs ? ix * dRx : (ix - nxh) * dgx;
     if (out->y) for (int iy = oa_iy_0; iy < oa_iy_e; ++iy) out->y[iy - oa_iy_0] = rs ? iy * dRy : (iy - nyh) * dgy;
 }

@@ -108,7 +108,24 @@ public:
     bool is_ewfs() const { return sim_type == 51; }
     bool is_ewrs() const { return sim_type == 52; }
     bool is_hrtem() const { return sim_type == 32; }
-    int illumination_model = 1;     // 1 coherent, 2 partially coherent (objective-lens envelope)
+    bool is_istem() const { return sim_type == 12; }
+    bool is_cbei() const { return sim_type == 22; }
+    bool is_ed() const { return sim_type == 31; }
+    bool is_ped() const { return sim_type == 41; }
+    bool is_hctem() const { return sim_type == 42; }
+    bool is_scanning() const { return is_stem() || is_istem(); }
+    bool is_cbed_cbei() const { return is_cbed() || is_cbei(); }
+    bool is_ped_hctem() const { return is_ped() || is_hctem(); }
+    // objective-lens image of the exit wave (is_ISTEM_CBEI_HRTEM_HCTEM_EFTEM,
+    // src/input_multislice.cuh:402-405)
+    bool is_image() const { return is_istem() || is_cbei() || is_hrtem() || is_hctem(); }
+    // outputs on the reciprocal grid (is_grid_FS, src/input_multislice.cuh:366-372)
+    bool is_grid_fs() const { return is_cbed() || is_ed() || is_ped() || is_ewfs(); }
+    // precession / hollow-cone rotations of the tilt azimuth (set_phi,
+    // src/input_multislice.cuh:903-906)
+    int nrot = 1;
+    R rot_phi(int irot) const { return irot == 0 ? R(0) : (R(6.283185307179586476925) * (R)irot) / (R)nrot; }
+    int illumination_model = 1;    // 1 coherent, 2 partially coherent (objective-lens envelope)
     int temporal_spatial_incoh = 1; // 1 both, 2 temporal only, 3 spatial only
     bool frozen() const { return pn_model == 3; }
     // 1 multislice, 2 phase object, 3 weak-phase object (after validation: a
@@ -131,6 +148,13 @@ public:
     R E_0 = 300, lambda = 0, theta = 0, phi = 0;
     R gx0 = 0, gy0 = 0;         // tilt of the incident beam in reciprocal space (1/A)
     bool tilted() const { return gx0 != R(0) || gy0 != R(0); }
+    // tilt of azimuth irot for precession / hollow cone (gx_0, gy_0 after
+    // set_phi, src/input_multislice.cuh:903-927), else the fixed tilt
+    R gx0_rot(int irot) const;
+    R gy0_rot(int irot) const;
+    // outputs multiplied by exp(-2 pi i g0.r) before the back-propagation;
+    // PED keeps the pattern shifted (dp_Shift, src/wave_function.cuh:77-90)
+    bool out_ramp() const { return tilted() && !is_ped(); }
     double lambda_d = 0;            // FP64 wavelength for phase arguments
     R Vr_factor = 0;            // mul_sign * gamma*lambda/(c_Potf*cos(theta)), evaluated in R
     int iw_type = 1;                // 1 plane, 2 convergent, 3 user defined

@@ -126,9 +126,15 @@ __global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgsT<Z> a)
         for (int e = 0; e < E; ++e) v[e] = cmul(tv[e], v[e]);
     }
     if (flags & ROW_MUL_RAMP) {
-        const Z rxv = a.ramp_x[row];
+        const Z *rx = a.ramp_x, *ry = a.ramp_y;
+        if (a.rot_x) {
+            const int ir = a.members[member].irot;
+            rx += ir * a.rot_x;
+            ry += ir * a.rot_y;
+        }
+        const Z rxv = rx[row];
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = cmul(cmul(rxv, a.ramp_y[t + T * e]), v[e]);
+        for (int e = 0; e < E; ++e) v[e] = cmul(cmul(rxv, ry[t + T * e]), v[e]);
     }
     if (flags & ROW_FFT) fft_run<S, false>(v, t, tw, lp, idx, sync);
     if (flags & ROW_STORE) {
@@ -236,10 +242,16 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
     if (flags & COL_FFT) fft_run<S, false>(v, t, tw, lds, idx, sync);
     if (flags & (COL_MUL_P | COL_BWL)) {
         // the propagator factors are tiny tables, cache-resident across the batch
-        const Z pyv = (flags & COL_MUL_P) ? cscale(a.py[col], a.p_scale) : cmake(a.inv_nxy, R(0));
+        const Z *px = a.px, *py = a.py;
+        if (a.rot_x) {
+            const int ir = a.members[member].irot;
+            px += ir * a.rot_x;
+            py += ir * a.rot_y;
+        }
+        const Z pyv = (flags & COL_MUL_P) ? cscale(py[col], a.p_scale) : cmake(a.inv_nxy, R(0));
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            Z p = (flags & COL_MUL_P) ? cmul(a.px[t + T * e], pyv) : pyv;
+            Z p = (flags & COL_MUL_P) ? cmul(px[t + T * e], pyv) : pyv;
             if (flags & COL_BWL) {
                 const R f = bwl_factor(g2_grid(signed_freq(t + T * e, a.nx), igy, a.dgx_f, a.dgy_f),
                                            a.bwl_alpha, a.gl2_max);

@@ -19,6 +19,7 @@ struct Member {
     int beam0;      // first beam in the beam table
     int nbeam;      // number of beams (0: plane wave)
     double wq;      // weight of the wave's defocus / source point (1 without partial-coherence integration)
+    int irot;       // tilt-azimuth table of the wave (precession / hollow cone); 0 otherwise
 };
 
 // One beam of a convergent incident wave: the position phase x*gx + y*gy
@@ -71,6 +72,7 @@ struct RowArgsT {
     const Member *members;
     int variant;                  // launch variant of the full slice step (kRowSlice flags)
     const Z *ramp_x, *ramp_y;   // ROW_MUL_RAMP tables [nrows], [row length]
+    long long rot_x, rot_y;       // strides of per-azimuth ramp tables (members[].irot); 0: one table
 };
 using RowArgs = RowArgsT<float2>;
 
@@ -99,6 +101,7 @@ struct ColArgsT {
     const Z *tw;             // twiddles W_nx
     const Z *px;             // [nx]
     const Z *py;             // [ny]
+    long long rot_x, rot_y;       // strides of per-azimuth propagator tables (members[].irot); 0: one table
     R dgx_f, dgy_f;               // sampling in R for masks
     int nx, ny;
     R bwl_alpha, gl2_max;

@@ -93,8 +93,7 @@ def _params_from(inp, sc=None):
         if k == "system_conf":
             sc = v
             continue
-        if k.startswith("spec_cryst_") or k.startswith("eels_") or k.startswith("eftem_") \
-                or k.startswith("ped_") or k.startswith("hci_"):
+        if k.startswith("spec_cryst_") or k.startswith("eels_") or k.startswith("eftem_"):
             continue     # read by the reference for other simulation types / builders; no effect here
         _set_field(p, k, v)
     if sc is not None:

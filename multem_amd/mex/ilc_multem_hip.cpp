@@ -141,6 +141,11 @@ void read_input(const mxArray *s, mt_input_multislice *in, InputArrays &A)
     in->E_0 = num(s, "E_0");
     in->theta = num(s, "theta");
     in->phi = num(s, "phi");
+    // precession / hollow-cone fields (ilc_multem.cu:338-347)
+    in->ped_theta = num(s, "ped_theta");
+    in->ped_nrot = (int)num(s, "ped_nrot");
+    in->hci_theta = num(s, "hci_theta");
+    in->hci_nrot = (int)num(s, "hci_nrot");
     in->illumination_model = (int)num(s, "illumination_model");
     in->temporal_spatial_incoh = (int)num(s, "temporal_spatial_incoh");
     read_lens(s, "cond_lens", "cond_lens", &in->cond_lens);

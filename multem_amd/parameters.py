@@ -79,6 +79,10 @@ class Parameters:
         self.E_0 = 300.0
         self.theta = 0.0
         self.phi = 0.0
+        self.ped_nrot = 360                   # precession (simulation_type 41): azimuths, cone angle (deg)
+        self.ped_theta = 3.0
+        self.hci_nrot = 360                   # hollow-cone illumination (42)
+        self.hci_theta = 3.0
         self.illumination_model = 3
         self.temporal_spatial_incoh = 1
         for k, v in _LENS_DEFAULTS.items():
@@ -125,12 +129,13 @@ class Parameters:
                   "illumination_model", "temporal_spatial_incoh",
                   "scanning_type", "scanning_periodic", "scanning_square_pxs", "scanning_ns", "detector_type",
                   "output_area_ix_0", "output_area_iy_0", "output_area_ix_e", "output_area_iy_e",
-                  "spec_rot_center_type"):
+                  "spec_rot_center_type", "ped_nrot", "hci_nrot"):
             setattr(s, f, int(getattr(self, f)))
         for f in ("spec_rot_u0", "spec_rot_center_p"):
             v = np.asarray(getattr(self, f), dtype=np.float64).ravel()
             getattr(s, f)[:] = [float(v[i]) if i < v.size else 0.0 for i in range(3)]
         for f in ("spec_lx", "spec_ly", "spec_lz", "spec_dz", "spec_rot_theta", "E_0", "theta", "phi",
+                  "ped_theta", "hci_theta",
                   "scanning_x0", "scanning_y0", "scanning_xe", "scanning_ye"):
             setattr(s, f, float(getattr(self, f)))
         atoms = np.asarray(self.spec_atoms, dtype=np.float64)

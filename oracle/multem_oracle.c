@@ -650,6 +650,7 @@ typedef struct {
     int iw_type; int n_iw; real *iw_x, *iw_y;
     cplx *iw_psi;          /* iw_type 3: user wave, natural layout ix*ny + iy */
     real E_0, lambda, theta, phi;
+    int nrot;              /* tilt azimuths of PED / HCTEM (1 otherwise) */
     int mul_sign;
     lens_t cond, obj;
     /* scanning */
@@ -1044,6 +1045,17 @@ static int is_cbed(const sim_t *s) { return s->simulation_type == 21; }
 static int is_ewfs(const sim_t *s) { return s->simulation_type == 51; }
 static int is_ewrs(const sim_t *s) { return s->simulation_type == 52; }
 static int is_hrtem(const sim_t *s) { return s->simulation_type == 32; }
+static int is_istem(const sim_t *s) { return s->simulation_type == 12; }
+static int is_cbei(const sim_t *s) { return s->simulation_type == 22; }
+static int is_ed(const sim_t *s) { return s->simulation_type == 31; }
+static int is_ped(const sim_t *s) { return s->simulation_type == 41; }
+static int is_hctem(const sim_t *s) { return s->simulation_type == 42; }
+static int is_cbed_cbei(const sim_t *s) { return is_cbed(s) || is_cbei(s); }
+static int is_ped_hctem(const sim_t *s) { return is_ped(s) || is_hctem(s); }
+/* is_ISTEM_CBEI_HRTEM_HCTEM_EFTEM (input_multislice.cuh:402-405) */
+static int is_image(const sim_t *s) { return is_istem(s) || is_cbei(s) || is_hrtem(s) || is_hctem(s); }
+/* is_grid_FS (input_multislice.cuh:366-372) */
+static int is_grid_fs(const sim_t *s) { return is_cbed(s) || is_ed(s) || is_ped(s) || is_ewfs(s); }
 static int is_frozen(const sim_t *s) { return s->pn_model == 3; }
 static int is_scanning_t(int st) { return st == 11 || st == 12 || st == 61; }
 
@@ -1378,8 +1390,9 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     *status = MT_OK;
 #define FAIL(code, msg) do { set_err(msg); *status = code; sim_free(s); return NULL; } while (0)
     s->simulation_type = in->simulation_type;
-    if (!(is_stem(s) || is_cbed(s) || is_hrtem(s) || is_ewfs(s) || is_ewrs(s)))
-        FAIL(MT_ERR_UNSUPPORTED, "simulation_type not provided (STEM 11, CBED 21, HRTEM 32, EWFS 51, EWRS 52)");
+    if (!(is_stem(s) || is_istem(s) || is_cbed(s) || is_cbei(s) || is_ed(s) || is_hrtem(s) || is_ped(s)
+          || is_hctem(s) || is_ewfs(s) || is_ewrs(s)))
+        FAIL(MT_ERR_UNSUPPORTED, "simulation_type not provided");
     s->interaction_model = in->interaction_model;
     if (s->interaction_model < 1 || s->interaction_model > 3)
         FAIL(MT_ERR_INVALID_ARGUMENT, "interaction_model must be 1, 2 or 3");
@@ -1452,7 +1465,7 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
 
     /* incident wave type (validate_incident_wave_type, input_multislice.cuh:439-449) */
     s->iw_type = in->iw_type;
-    if (s->iw_type == 4) s->iw_type = (is_scanning_t(s->simulation_type) || is_cbed(s)) ? 2 : 1;
+    if (s->iw_type == 4) s->iw_type = (is_scanning_t(s->simulation_type) || is_cbed_cbei(s)) ? 2 : 1;
     if (s->iw_type == 3) {
         if (!in->iw_psi) FAIL(MT_ERR_INVALID_ARGUMENT, "iw_type 3 needs iw_psi");
         size_t n = (size_t)in->nx * in->ny;
@@ -1467,6 +1480,15 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
     s->E_0 = (real)in->E_0;
     s->theta = (real)(in->theta * c_deg_2_rad);
     s->phi = (real)(in->phi * c_deg_2_rad);
+    s->nrot = 1;
+    /* PED / HCTEM: cone angle and azimuths (ilc_multem.cu:338-347; nrot = max(1, nrot),
+       input_multislice.cuh:655-660); phi is set per azimuth (set_phi) */
+    if (is_ped_hctem(s)) {
+        s->theta = (real)((is_ped(s) ? in->ped_theta : in->hci_theta) * c_deg_2_rad);
+        s->nrot = is_ped(s) ? in->ped_nrot : in->hci_nrot;
+        if (s->nrot < 1) s->nrot = 1;
+        s->phi = 0;
+    }
     s->illumination_model = in->illumination_model;
 
     lens_read(&s->cond, &in->cond_lens);
@@ -1533,9 +1555,11 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
         s->theta = is_zero_r(s->theta) ? 0 : asin(n * s->lambda * dgmin);
     }
     if (is_ewfs(s) || is_ewrs(s)) { s->pn_coh_contrib = 1; s->illumination_model = 1; }
-    if (is_cbed(s) && s->illumination_model != 4) s->illumination_model = 1;
-    if (is_hrtem(s) && s->illumination_model == 4 && s->tsi != 2)
-        FAIL(MT_ERR_UNSUPPORTED, "HRTEM full integration with spatial incoherence (empty source quadrature)");
+    if (is_cbed_cbei(s) && s->illumination_model != 4) s->illumination_model = 1;
+    if (is_image(s) && s->illumination_model == 4 && s->tsi != 2)
+        FAIL(MT_ERR_UNSUPPORTED, "objective-lens full integration with spatial incoherence (empty source quadrature)");
+    if (is_image(s) && s->illumination_model == 3)
+        FAIL(MT_ERR_UNSUPPORTED, "objective-lens image with illumination_model 3 (empty in the reference)");
     /* integration points (src/input_multislice.cuh:753-776), then the clamps */
     lens_t *lenses[2] = { &s->cond, &s->obj };
     for (int k = 0; k < 2; k++) {
@@ -1822,7 +1846,7 @@ static void tilt_ramp_out(const sim_t *s, cplx *psi)
 {
     const grid_t *g = &s->g;
     real gxu = sin(s->theta) * cos(s->phi) / s->lambda, gyu = sin(s->theta) * sin(s->phi) / s->lambda;
-    if (is_zero_r(gxu) && is_zero_r(gyu)) return;
+    if (is_ped(s) || (is_zero_r(gxu) && is_zero_r(gyu))) return;   /* dp_Shift for PED */
     real gx = (real)(-6.283185307179586476925) * gxu, gy = (real)(-6.283185307179586476925) * gyu;
     int Rx_c = (int)((real)0.5 * g->lx), Ry_c = (int)((real)0.5 * g->ly);
     for (int ix = 0; ix < g->nx; ix++) {
@@ -2063,10 +2087,11 @@ static void output_dims_from_sim(const sim_t *s, mt_output_multislice *out)
         out->dx = s->sc_dRx; out->dy = s->sc_dRy;
     } else {
         out->nx = s->oa_ix_e - s->oa_ix_0; out->ny = s->oa_iy_e - s->oa_iy_0; out->n_det = 0;
-        int rs = is_ewrs(s) || is_hrtem(s);
+        int rs = !is_grid_fs(s);
         out->dx = rs ? s->g.dRx : s->g.dgx;
         out->dy = rs ? s->g.dRy : s->g.dgy;
-        if (is_cbed(s) || is_hrtem(s)) out->output_type = s->pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
+        /* set_output_type (output_multislice.hpp:1249-1280) */
+        if (!(is_ewfs(s) || is_ewrs(s))) out->output_type = s->pn_coh_contrib ? MT_OT_M2PSI_TOT_COH : MT_OT_M2PSI_TOT;
         else {
             int sc = !is_frozen(s) || s->pn_single_conf;
             out->output_type = sc ? MT_OT_PSI_COH : MT_OT_M2PSI_TOT_PSI_COH;
@@ -2092,7 +2117,7 @@ static void fill_axes(const sim_t *s, mt_output_multislice *out)
         if (out->x) for (int i = 0; i < s->sc_nx; i++) out->x[i] = s->sc_x0 + i * s->sc_dRx;
         if (out->y) for (int i = 0; i < s->sc_ny; i++) out->y[i] = s->sc_y0 + i * s->sc_dRy;
     } else {
-        int rs = is_ewrs(s) || is_hrtem(s);
+        int rs = !is_grid_fs(s);
         if (out->x) for (int ix = s->oa_ix_0; ix < s->oa_ix_e; ix++)
             out->x[ix - s->oa_ix_0] = rs ? ix * s->g.dRx : (ix - s->g.nxh) * s->g.dgx;
         if (out->y) for (int iy = s->oa_iy_0; iy < s->oa_iy_e; iy++)
@@ -2195,43 +2220,60 @@ typedef struct {
     int status;
 } ew_conf;
 
-/* The incident waves of one configuration: one, or for fully integrated
-   CBED one per (source point, defocus point) of CBED_CBEI
-   (src/tem_simulation.cuh:207-261), with the beams shifted by the source
-   offset (stored in real) and the condenser defocus c_10_0 + x_t. */
+/* The incident waves of one configuration, in the reference's loop order:
+   one; for fully integrated CBED / CBEI one per (source point, defocus
+   point) of CBED_CBEI (src/tem_simulation.cuh:207-261), with the beams
+   shifted by the source offset (stored in real) and the condenser defocus
+   c_10_0 + x_t; for ISTEM one per (defocus point, scan position) of
+   STEM_ISTEM (:100-200); for PED / HCTEM one per tilt azimuth of PED_HCTEM
+   (:274-301). */
 typedef struct {
     int nw, nb;
     real *bx, *by;     /* [nw][nb] */
     real *c10;         /* [nw] */
     double *wq;        /* [nw] weight factor */
+    real *phi;         /* [nw] tilt azimuth */
 } wave_set;
 
 static void wave_set_build(const sim_t *s, wave_set *ws)
 {
     lens_quad qt = { 0 }, qs = { 0 };
-    if (is_cbed(s) && s->illumination_model == 4) cond_quadratures(&s->cond, &qt, &qs);
+    if ((is_cbed_cbei(s) || is_istem(s)) && s->illumination_model == 4) cond_quadratures(&s->cond, &qt, &qs);
     else { quad_one(&qt); quad_one(&qs); }
-    ws->nw = qs.n * qt.n;
-    ws->nb = s->n_iw > 0 ? s->n_iw : 1;
+    if (is_istem(s)) { quad_free(&qs); quad_one(&qs); }   /* STEM_ISTEM integrates the defocus only */
+    int npos = is_istem(s) ? s->sc_n : 1, nrot = s->nrot;
+    ws->nw = qs.n * qt.n * npos * nrot;
+    ws->nb = is_istem(s) ? 1 : (s->n_iw > 0 ? s->n_iw : 1);
     ws->bx = calloc((size_t)ws->nw * ws->nb, sizeof(real));
     ws->by = calloc((size_t)ws->nw * ws->nb, sizeof(real));
     ws->c10 = malloc(sizeof(real) * (size_t)ws->nw);
     ws->wq = malloc(sizeof(double) * (size_t)ws->nw);
+    ws->phi = malloc(sizeof(real) * (size_t)ws->nw);
     double c_10_0 = (double)s->cond.c_10;
     for (int i = 0; i < qs.n; i++)
-        for (int t = 0; t < qt.n; t++) {
-            int k = i * qt.n + t;
-            for (int b = 0; b < s->n_iw; b++) {
-                ws->bx[(size_t)k * ws->nb + b] = (real)((double)s->iw_x[b] + qs.x[i]);
-                ws->by[(size_t)k * ws->nb + b] = (real)((double)s->iw_y[b] + qs.y[i]);
-            }
-            ws->c10[k] = (real)(c_10_0 + qt.x[t]);
-            ws->wq[k] = qs.w[i] * qt.w[t];
-        }
+        for (int t = 0; t < qt.n; t++)
+            for (int p = 0; p < npos; p++)
+                for (int r = 0; r < nrot; r++) {
+                    int k = ((i * qt.n + t) * npos + p) * nrot + r;
+                    if (is_istem(s)) {
+                        ws->bx[k] = s->sc_x[p];
+                        ws->by[k] = s->sc_y[p];
+                    } else {
+                        for (int b = 0; b < s->n_iw; b++) {
+                            ws->bx[(size_t)k * ws->nb + b] = (real)((double)s->iw_x[b] + qs.x[i]);
+                            ws->by[(size_t)k * ws->nb + b] = (real)((double)s->iw_y[b] + qs.y[i]);
+                        }
+                    }
+                    ws->c10[k] = (real)(c_10_0 + qt.x[t]);
+                    ws->wq[k] = qs.w[i] * qt.w[t];
+                    /* set_phi (input_multislice.cuh:903-906) */
+                    ws->phi[k] = !is_ped_hctem(s) ? s->phi
+                                 : (r == 0 ? (real)0 : ((real)c_2Pi * (real)r) / (real)nrot);
+                }
     quad_free(&qt); quad_free(&qs);
 }
 
-static void wave_set_free(wave_set *ws) { free(ws->bx); free(ws->by); free(ws->c10); free(ws->wq); }
+static void wave_set_free(wave_set *ws) { free(ws->bx); free(ws->by); free(ws->c10); free(ws->wq); free(ws->phi); }
 
 /* EWFS_EWRS (tem_simulation.cuh:303-327) for one configuration and its
    incident waves.  Each slice is used once, so its transmission function is
@@ -2253,7 +2295,7 @@ static int run_ew_conf(sim_t *s, const wave_set *ws, cplx **po_thk)
         real c_10_0 = s->cond.c_10;
         for (int k = 0; k < nw; k++) {
             lens_set_defocus(&s->cond, ws->c10[k]);
-            incident_wave(s, ws->bx + (size_t)k * ws->nb, ws->by + (size_t)k * ws->nb, s->n_iw, 0, 0,
+            incident_wave(s, ws->bx + (size_t)k * ws->nb, ws->by + (size_t)k * ws->nb, ws->nb, 0, 0,
                           psi + (size_t)k * nxy, &F, work, fpsi);
         }
         lens_set_defocus(&s->cond, c_10_0);
@@ -2264,8 +2306,10 @@ static int run_ew_conf(sim_t *s, const wave_set *ws, cplx **po_thk)
             slice_transmission(s, V, T, &F, work);
             real dzs = (s->slice[is].z_e - s->slice[is].z_0) / cos(s->theta);
             int ithk = s->slice[is].ithk;
+            real phi_0 = s->phi;
             for (int k = 0; k < nw; k++) {
                 cplx *p = psi + (size_t)k * nxy;
+                s->phi = ws->phi[k];   /* this wave's tilt azimuth: propagators and output ramp */
                 for (size_t i = 0; i < nxy; i++) {
                     cplx a = p[i], b = T[i];
                     p[i].re = b.re * a.re - b.im * a.im;
@@ -2279,6 +2323,7 @@ static int run_ew_conf(sim_t *s, const wave_set *ws, cplx **po_thk)
                     propagator(s, space, s->thk[ithk].z_back_prop, o, &F, work);
                 }
             }
+            s->phi = phi_0;
         }
     }
     free(T); free(V); free(psi); free(fpsi); free(work);
@@ -2390,7 +2435,7 @@ typedef struct {
 static void ew_accumulate(const sim_t *s, ew_acc *A, cplx **po, double w)
 {
     for (int k = 0; k < s->n_thk; k++) {
-        if (is_hrtem(s)) {
+        if (is_image(s)) {
             /* add_scale_crop_shift_m2psi_tot_from_m2psi of the image; psi_coh in k-space */
             if (s->illumination_model == 4) {
                 num_int_tem(s, po[k], A->tmp, A->tmp2, &A->F, A->work);
@@ -2490,7 +2535,7 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
     output_dims_from_sim(s, &dims);
     ew_acc A;
     A.ot = dims.output_type;
-    A.w = (double)(real)(1.0 / (real)nconf_w);
+    A.w = (double)(real)(1.0 / (real)(nconf_w * s->nrot));   /* get_phonon_rot_weight */
     A.no = no;
     A.nxy = nxy;
     A.m2 = calloc(no * (size_t)s->n_thk, sizeof(double));
@@ -2545,12 +2590,12 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
             cplx *tmp = malloc(sizeof(cplx) * nxy);
             for (int k = 0; k < s->n_thk; k++) {
                 for (size_t i = 0; i < nxy; i++) { tmp[i].re = (real)A.pc[2 * (nxy * k + i)]; tmp[i].im = (real)A.pc[2 * (nxy * k + i) + 1]; }
-                if (is_hrtem(s) && s->illumination_model == 4) {
+                if (is_image(s) && s->illumination_model == 4) {
                     num_int_tem(s, tmp, A.tmp, A.tmp2, &A.F, A.work);
                     accumulate_output(s, A.tmp, 1.0, 0, 1, out->m2psi_coh + no * k, 0);
                     continue;
                 }
-                if (is_hrtem(s)) microscope_effects(s, tmp, &A.F, A.work);
+                if (is_image(s)) microscope_effects(s, tmp, &A.F, A.work);
                 accumulate_output(s, tmp, 1.0, 1, 1, out->m2psi_coh + no * k, 0);
             }
             free(tmp);

@@ -77,7 +77,8 @@ def test_scan_grid_square_region():
 
 
 @pytest.mark.parametrize("field,value,status", [
-    ("simulation_type", 12, 2),       # ISTEM: not provided
+    ("simulation_type", 61, 2),       # EELS: not provided
+    ("simulation_type", 7, 2),        # not a simulation type
     ("potential_type", 4, 2),         # Kirkland tables: not provided
     ("nx", 300, 2),                   # not a power of two
     ("nx", 0, 1),

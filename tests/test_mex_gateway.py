@@ -176,3 +176,18 @@ def test_gateway_matrix_detectors():
     rng = np.random.default_rng(5)
     p.detector_fR = [rng.random((p.ny, p.nx)) for _ in range(3)]
     _compare(L, p)
+
+
+@pytest.mark.gpu
+def test_gateway_precession_and_hollow_cone():
+    """ped_theta / ped_nrot and hci_theta / hci_nrot reach the engine through
+    the gateway (ilc_multem.cu:338-347)."""
+    L = _lib()
+    p = configs.si_ewrs_256(nc=3, n=128)
+    p.simulation_type = 41
+    p.ped_theta, p.ped_nrot = 1.0, 5
+    _compare(L, p)
+    p.simulation_type = 42
+    p.hci_theta, p.hci_nrot = 0.8, 3
+    p.obj_lens_c_10, p.obj_lens_outer_aper_ang = -40.0, 25.0
+    _compare(L, p)
