@@ -309,6 +309,34 @@ def test_stem_coherent_image(theta):
     assert e_tot < 1e-4 and e_coh < 1e-4
 
 
+def test_stem_coherent_image_in_scan_blocks():
+    """A memory budget too small for every scan point's coherent sums runs
+    the scan in blocks; with the launch variants and batch fixed the images
+    are bit-identical to the unblocked run."""
+    p = small_stem(ns=4, nconf=2)
+    p.pn_coh_contrib = 1
+    d = engine.output_dims(p)
+    nscan = d.nx * d.ny
+    nxy = p.nx * p.ny
+    ns = len(oracle.spec_slicing(p))
+    per_scan = d.n_thk * nxy * 16
+    budget = ns * nxy * 8 + nscan * per_scan // 2 + 4 * nxy * 8 * 2
+    engine.set_pass_variants(0, 0, 0)
+    engine.set_batch(4, True)
+    try:
+        a = engine.run_multislice(p)
+        engine.set_memory_budget(budget)
+        b = engine.run_multislice(p)
+        plan = engine.last_plan()
+    finally:
+        engine.set_memory_budget(0)
+        engine.set_batch(0, True)
+        engine.set_pass_variants(-1, -1, -1)
+    assert "scan_blocks=" in plan and "scan_blocks=1 " not in plan, plan
+    assert np.array_equal(a.image_tot, b.image_tot)
+    assert np.array_equal(a.image_coh, b.image_coh)
+
+
 def test_projected_potential_sum():
     p = configs.si_ewrs_256(nc=3)
     ns = len(oracle.spec_slicing(p))
