@@ -775,18 +775,36 @@ public:
         log << "batch[";
         double best = -1;
         std::vector<double> per(cand.size());
-        for (size_t i = 0; i < cand.size(); ++i) {
-            RowArgsT<Z> rr{};
-            rr.src = w.p; rr.dst = w.p; rr.member_stride = (long long)nxy; rr.nrows = m.nx; rr.n_members = cand[i];
+        auto args = [&](int b, RowArgsT<Z> &rr, ColArgsT<Z> &cc) {
+            rr = RowArgsT<Z>{};
+            rr.src = w.p; rr.dst = w.p; rr.member_stride = (long long)nxy; rr.nrows = m.nx; rr.n_members = b;
             rr.tw = S.tw_y.p; rr.trans = trans.p; rr.members = mem.p; rr.slice = 0;
             rr.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
-            ColArgsT<Z> cc = S.col_base(w.p, w.p, cand[i]);
+            cc = S.col_base(w.p, w.p, b);
             cc.flags = col_flags(false);
             cc.px = px.p; cc.py = py.p; cc.ithk = -1;
-            const double t = time_variant([&] {
+        };
+        // the candidates are compared with the launch variants the tuner picks
+        // for the smallest one, so that the comparison sees tuned passes
+        PassChoice pc0;
+        {
+            RowArgsT<Z> rr;
+            ColArgsT<Z> cc;
+            args(cand[0], rr, cc);
+            pc0 = choose_variants(m.nx, m.ny, cand[0], rr, cc, (const ColArgsT<Z> *)nullptr, s);
+        }
+        for (size_t i = 0; i < cand.size(); ++i) {
+            RowArgsT<Z> rr;
+            ColArgsT<Z> cc;
+            args(cand[i], rr, cc);
+            rr.variant = pc0.row;
+            cc.variant = pc0.col;
+            auto step = [&] {
                 hipError_t e = launch_row_pass(m.ny, rr, s);
                 return e != hipSuccess ? e : launch_col_pass(m.nx, cc, false, false, s);
-            }, s);
+            };
+            const double t1 = time_variant(step, s), t2 = time_variant(step, s);
+            const double t = (t1 > 0 && t2 > 0) ? std::min(t1, t2) : std::max(t1, t2);
             per[i] = t / cand[i];
             if (per[i] > 0 && (best < 0 || per[i] < best)) best = per[i];
             log << (i ? "," : "") << cand[i] << ":" << (int)std::lround(t * 1000.0) << "us";
@@ -936,16 +954,18 @@ struct GraphExec {
 };
 
 template <class R>
-void run_stem(const ModelT<R> &m, mt_output_multislice *out)
+void run_stem_one(const ModelT<R> &m, mt_output_multislice *out, size_t budget, bool reset_progress)
 {
     Timing &tm = last_timing();
     tm = Timing{};
-    StemJob<R> J(m, budget_bytes(), run_control().batch_override);
+    StemJob<R> J(m, budget, run_control().batch_override);
     hipStream_t s = J.s;
     tm.plan = J.plan();
     tm.tune_log = (J.batch_log.empty() ? "" : J.batch_log + " ") + J.choice.log;
-    run_control().niter = J.n_work() / J.nbatch * J.nscan;
-    run_control().iter = 0;
+    if (reset_progress) {
+        run_control().niter = J.n_work() / J.nbatch * J.nscan;
+        run_control().iter = 0;
+    }
 
     GraphExec G;
     if (run_control().use_graph) {
@@ -991,6 +1011,55 @@ void run_stem(const ModelT<R> &m, mt_output_multislice *out)
     tm.ms_tf = J.ms_tf;
     tm.probe_slices = ps;
     J.finish(out);
+}
+
+// STEM.  The coherent image keeps FP64 sums of every scan point's wave
+// resident; when they would take more than half of the memory left beside
+// the transmission stack, the scan runs in blocks of points, each with all
+// configurations (scan points are independent: Multislice::STEM_ISTEM,
+// src/tem_simulation.cuh:115-133, loops them outermost).
+template <class R>
+void run_stem(const ModelT<R> &m, mt_output_multislice *out)
+{
+    const size_t budget = budget_bytes();
+    const size_t nxy = (size_t)m.nx * m.ny, nscan = m.sc_x.size(), n_thk = m.thk.size();
+    const size_t stack = (size_t)m.n_slice() * nxy * sizeof(cplx_t<R>), wave = 2 * nxy * sizeof(cplx_t<R>);
+    const size_t per_scan = n_thk * nxy * sizeof(double2);
+    if (!m.pn_coh_contrib || stack + wave >= budget || nscan * per_scan <= (budget - stack) / 2) {
+        run_stem_one(m, out, budget, true);
+        return;
+    }
+    size_t nblk = ((budget - stack) / 2) / per_scan;
+    if (nblk < 1) nblk = (budget - stack - wave) / per_scan;
+    if (nblk < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "coherent accumulators of one scan point exceed the memory budget"};
+    const int n_det = (int)m.det_g2_min.size();
+    run_control().niter = (int)(nscan * (size_t)m.n_conf_run());   // the coherent loop: every configuration per point
+    run_control().iter = 0;
+    Timing total;
+    int nb = 0;
+    for (size_t s0 = 0; s0 < nscan; s0 += nblk, ++nb) {
+        const size_t n = std::min(nblk, nscan - s0);
+        ModelT<R> mb = m;
+        mb.sc_x.assign(m.sc_x.begin() + s0, m.sc_x.begin() + s0 + n);
+        mb.sc_y.assign(m.sc_y.begin() + s0, m.sc_y.begin() + s0 + n);
+        std::vector<double> it(out->image_tot ? n_thk * n_det * n : 0), ic(out->image_coh ? n_thk * n_det * n : 0);
+        mt_output_multislice ob = *out;
+        ob.image_tot = out->image_tot ? it.data() : nullptr;
+        ob.image_coh = out->image_coh ? ic.data() : nullptr;
+        run_stem_one(mb, &ob, budget, false);
+        // images [n_thk][n_det][scan]: the block's points are a contiguous run of the scan
+        for (size_t k = 0; k < n_thk * n_det; ++k) {
+            if (out->image_tot) std::copy(it.begin() + k * n, it.begin() + (k + 1) * n, out->image_tot + k * nscan + s0);
+            if (out->image_coh) std::copy(ic.begin() + k * n, ic.begin() + (k + 1) * n, out->image_coh + k * nscan + s0);
+        }
+        const Timing &t = last_timing();
+        total.ms_total += t.ms_total; total.ms_slice += t.ms_slice; total.probe_slices += t.probe_slices;
+        total.ms_row += t.ms_row; total.ms_col += t.ms_col; total.n_row += t.n_row; total.n_col += t.n_col;
+        total.ms_tf += t.ms_tf;
+        if (s0 == 0) { total.plan = t.plan; total.tune_log = t.tune_log; }
+    }
+    total.plan += " scan_blocks=" + std::to_string(nb) + " scan_points_per_block=" + std::to_string(nblk);
+    last_timing() = total;
 }
 
 // Several STEM specimens, possibly of different grid sizes, advance
@@ -1223,6 +1292,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     S.prop_tables(zl, px, py, n_rot);
     const long long rot_px = n_rot > 1 ? (long long)zl.size() * m.nx : 0;
     const long long rot_py = n_rot > 1 ? (long long)zl.size() * m.ny : 0;
+    const int col_rot = n_rot > 1 ? COL_ROT : 0;   // keeps the specialised slice steps for one azimuth
     DBuf<Z> rx, ry;
     if (ramp) S.ramp_tables(rx, ry, n_rot);
     DBuf<Z> ctf;
@@ -1293,7 +1363,8 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                 rt.slice = 0;
                 rt.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
                 ColArgsT<Z> ct = with_rot(S.col_base(psi.p, psi.p, nm), 0);
-                ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+                ct.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT |
+                           COL_STORE | col_rot;
                 check(hipMemsetAsync(psi.p, 0, (size_t)nm * nxy * sizeof(Z), s), "hipMemsetAsync");
                 choice = choose_variants(m.nx, m.ny, nm, rt, ct, (const ColArgsT<Z> *)nullptr, s);
                 tm.tune_log = choice.log;
@@ -1313,7 +1384,8 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                 r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
                 check(launch_row_pass(m.ny, r, s), "row pass");
                 ColArgsT<Z> c = with_rot(S.col_base(psi.p, psi.p, nm), is);
-                c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT | COL_STORE;
+                c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl && m.multislice() ? COL_BWL : 0) | COL_IFFT |
+                          COL_STORE | col_rot;
                 c.variant = choice.col;
                 check(launch_col_pass(m.nx, c, false, false, s), "col pass");
                 const int ithk = m.slices[is].ithk;
@@ -1340,7 +1412,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                         // transforms (microscope_effects, src/wave_function.cuh:172-180)
                         ColArgsT<Z> co = with_rot(S.col_base(osrc, odst, nm), m.n_slice() + ithk);
                         co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (fs ? 0 : COL_IFFT) |
-                                   COL_STORE | (img ? COL_MUL_CTF : 0);
+                                   COL_STORE | (img ? COL_MUL_CTF : 0) | col_rot;
                         co.ctf = ctf.p + (size_t)jc * nxy;
                         co.p_scale = (R)m.ny;
                         check(launch_col_pass(m.nx, co, false, false, s), "output col pass");
@@ -1370,7 +1442,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                     // psi_coh += w O(g), the back-propagated wave before the objective
                     // lens (add_scale_psi_coh, src/wave_function.cuh:166-172)
                     ColArgsT<Z> co = with_rot(S.col_base(osrc0, odst, nm), m.n_slice() + ithk);
-                    co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE;
+                    co.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE | col_rot;
                     co.p_scale = (R)m.ny;
                     check(launch_col_pass(m.nx, co, false, false, s), "coherent output col pass");
                     OutArgsT<Z> oa{};

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
     if (flags & (COL_MUL_P | COL_BWL)) {
         // the propagator factors are tiny tables, cache-resident across the batch
         const Z *px = a.px, *py = a.py;
-        if (a.rot_x) {
+        if (flags & COL_ROT) {   // never set in the specialised slice steps
             const int ir = a.members[member].irot;
             px += ir * a.rot_x;
             py += ir * a.rot_y;

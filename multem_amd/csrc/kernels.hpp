@@ -87,6 +87,7 @@ enum ColFlags : int {
     COL_COH = 128,      // detector passes: coh[iscan][ithk] += coh_w * psi(g) (coherent STEM image)
     COL_MUL_CTF = 256,  // multiply by ctf[ix*ny + iy] after the propagator (objective-lens image formation)
     COL_USER_WAVE = 512,  // probe passes: the wave is fpsi0(g) exp(i (x_b gx + y_b gy)) / (nx ny), first beam
+    COL_ROT = 1024,     // COL_MUL_P tables per wave: px + members[].irot * rot_x, py + ... * rot_y
 };
 
 template <class Z>
@@ -101,7 +102,7 @@ struct ColArgsT {
     const Z *tw;             // twiddles W_nx
     const Z *px;             // [nx]
     const Z *py;             // [ny]
-    long long rot_x, rot_y;       // strides of per-azimuth propagator tables (members[].irot); 0: one table
+    long long rot_x, rot_y;       // COL_ROT: strides of the per-azimuth propagator tables (members[].irot)
     R dgx_f, dgy_f;               // sampling in R for masks
     int nx, ny;
     R bwl_alpha, gl2_max;
