@@ -567,7 +567,7 @@ PassChoice choose_variants(int nx, int ny, int B, RowArgsT<Z> r, ColArgsT<Z> c, 
     const RunControl &rc = run_control();
     const bool forced = rc.force_row >= 0 || rc.force_col >= 0 || rc.force_col_det >= 0;
     const bool tune = rc.autotune != 0;
-    const TuneKey key{nx, ny, B, c_last ? 1 : 0, (int)sizeof(real_t<Z>)};
+    const TuneKey key{nx, ny, B, c_last ? ((c_last->flags & COL_DET_FR) ? 2 : 1) : 0, (int)sizeof(real_t<Z>)};
     if (tune && rc.autotune == 1 && !forced) {
         auto it = tune_cache().find(key);
         if (it != tune_cache().end()) {
@@ -737,7 +737,7 @@ public:
         ct.flags = col_flags(false);
         ct.px = px.p; ct.py = py.p; ct.ithk = -1;
         ColArgsT<Z> cl = c;
-        cl.flags = col_flags(true);
+        cl.flags = col_flags(true) | (det_fR.p ? COL_DET_FR : 0);
         cl.px = px.p; cl.py = py.p; cl.ithk = m.slices[ns_run - 1].ithk;
         for (int j = 0; j < B; ++j) hm[j] = Member{0, -1, j, 1, 1.0};   // valid entries while tuning
         members.upload(hm.data(), B, s);
@@ -901,7 +901,8 @@ public:
                 check(launch_row_pass(m.ny, ro, st), "output ramp pass");
                 ColArgsT<Z> cd = c;
                 cd.src = scratch.p;
-                cd.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (m.pn_coh_contrib ? COL_COH : 0);
+                cd.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | (m.pn_coh_contrib ? COL_COH : 0) |
+                           (det_fR.p ? COL_DET_FR : 0);
                 cd.px = px.p + (size_t)(m.n_slice() + ithk) * m.nx;
                 cd.py = py.p + (size_t)(m.n_slice() + ithk) * m.ny;
                 cd.ithk = ithk;
@@ -910,7 +911,8 @@ public:
                 cd.p_scale = (R)m.ny;
                 check(launch_col_pass(m.nx, cd, false, true, st), "output detector pass");
             } else {
-                c.flags = col_flags(is == ns_run - 1) | ((ithk >= 0 && m.pn_coh_contrib) ? COL_COH : 0);
+                c.flags = col_flags(is == ns_run - 1) | ((ithk >= 0 && m.pn_coh_contrib) ? COL_COH : 0) |
+                          ((ithk >= 0 && det_fR.p) ? COL_DET_FR : 0);
                 c.ithk = ithk;
                 c.variant = ithk >= 0 ? choice.col_det : choice.col;
                 check(launch_col_pass(m.nx, c, false, ithk >= 0, st), "col pass");

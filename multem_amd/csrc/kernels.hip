@@ -293,7 +293,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
                 const R f = bwl_factor(g2f, a.bwl_alpha, a.gl2_max);
                 m2 *= f * f;
             }
-            if (a.det_fR) {
+            if (flags & COL_DET_FR) {   // never set in the specialised detector passes
                 // matrix detectors: sum fR |psi|^2 (sum_square_over_Det with fR,
                 // src/wave_function.cuh:125-129)
                 const long long px = (long long)ix * a.ncols + col;

@@ -88,6 +88,7 @@ enum ColFlags : int {
     COL_MUL_CTF = 256,  // multiply by ctf[ix*ny + iy] after the propagator (objective-lens image formation)
     COL_USER_WAVE = 512,  // probe passes: the wave is fpsi0(g) exp(i (x_b gx + y_b gy)) / (nx ny), first beam
     COL_ROT = 1024,     // COL_MUL_P tables per wave: px + members[].irot * rot_x, py + ... * rot_y
+    COL_DET_FR = 2048,  // detector passes: matrix detectors det_fR instead of rings
 };
 
 template <class Z>
