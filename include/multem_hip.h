@@ -141,8 +141,10 @@ typedef struct {
     double spec_rot_center_p[3];
 
     /* matrix detectors (detector_type 3, detector.matrix(i).fR;
-       ilc_multem.cu:317-333): n_detector sensitivity maps, each ny x nx in
-       the natural (unshifted) Fourier layout, contiguous */
+       ilc_multem.cu:317-333): n_detector sensitivity maps, each ny x nx,
+       contiguous, centred (g = 0 at column nx/2, row ny/2, as the
+       reference's examples build them); the engine applies fft2_shift as
+       Wave_Function::set_input_data does (src/wave_function.cuh:53-61) */
     const double *detector_fR;
 
     /* precession electron diffraction (simulation_type 41) and hollow-cone

@@ -478,11 +478,15 @@ void stop_check()
 //
 // Tuning launches overwrite the wave buffer and the detector partials; the
 // caller runs it before the first batch, which rewrites both.
+// The key holds each pass's flags: a run whose passes go to the generic
+// kernels (bandwidth limit, per-azimuth tables, matrix detectors) must not
+// share choices with one that runs the specialised kernels.
 struct TuneKey {
-    int nx, ny, B, det_last, real_bytes;
+    int nx, ny, B, row_flags, col_flags, det_flags, real_bytes;
     bool operator<(const TuneKey &o) const
     {
-        return std::tie(nx, ny, B, det_last, real_bytes) < std::tie(o.nx, o.ny, o.B, o.det_last, o.real_bytes);
+        return std::tie(nx, ny, B, row_flags, col_flags, det_flags, real_bytes) <
+               std::tie(o.nx, o.ny, o.B, o.row_flags, o.col_flags, o.det_flags, o.real_bytes);
     }
 };
 
@@ -523,10 +527,21 @@ double time_variant(F &&launch, hipStream_t s)
 }
 
 // forced >= 0: use that variant (checked to be valid for the shape);
-// tune == false: variant 0 unless forced
+// tune == false: variant 0 unless forced; variants == false: the pass runs
+// the generic kernel, which has one variant (0)
 template <class F>
-int pick_variant(const char *name, int forced, bool tune, F &&launch, hipStream_t s, std::ostringstream &log)
+int pick_variant(const char *name, int forced, bool tune, bool variants, F &&launch, hipStream_t s,
+                 std::ostringstream &log)
 {
+    if (!variants) {
+        if (forced > 0) {
+            throw Error{MT_ERR_INVALID_ARGUMENT, std::string("launch variant ") + std::to_string(forced) + " of the " +
+                                                     name + " pass is not valid for this run (generic kernel)"};
+        }
+        check(launch(0), "variant launch");
+        log << name << "=v0(generic) ";
+        return 0;
+    }
     if (forced >= 0 || !tune) {
         const int v = std::max(forced, 0);
         hipError_t e = v < kPassVariants ? launch(v) : hipErrorInvalidValue;
@@ -567,7 +582,7 @@ PassChoice choose_variants(int nx, int ny, int B, RowArgsT<Z> r, ColArgsT<Z> c, 
     const RunControl &rc = run_control();
     const bool forced = rc.force_row >= 0 || rc.force_col >= 0 || rc.force_col_det >= 0;
     const bool tune = rc.autotune != 0;
-    const TuneKey key{nx, ny, B, c_last ? ((c_last->flags & COL_DET_FR) ? 2 : 1) : 0, (int)sizeof(real_t<Z>)};
+    const TuneKey key{nx, ny, B, r.flags, c.flags, c_last ? c_last->flags : -1, (int)sizeof(real_t<Z>)};
     if (tune && rc.autotune == 1 && !forced) {
         auto it = tune_cache().find(key);
         if (it != tune_cache().end()) {
@@ -579,13 +594,13 @@ PassChoice choose_variants(int nx, int ny, int B, RowArgsT<Z> r, ColArgsT<Z> c, 
     PassChoice ch;
     std::ostringstream log;
     log << "nx=" << nx << " ny=" << ny << " batch=" << B << ": ";
-    ch.row = pick_variant("row", rc.force_row, tune, [&](int v) { r.variant = v; return launch_row_pass(ny, r, s); },
-                          s, log);
-    ch.col = pick_variant("col", rc.force_col, tune,
+    ch.row = pick_variant("row", rc.force_row, tune, row_variants_apply(r.flags),
+                          [&](int v) { r.variant = v; return launch_row_pass(ny, r, s); }, s, log);
+    ch.col = pick_variant("col", rc.force_col, tune, col_variants_apply(c.flags, false),
                           [&](int v) { c.variant = v; return launch_col_pass(nx, c, false, false, s); }, s, log);
     if (c_last) {
         ColArgsT<Z> cl = *c_last;
-        ch.col_det = pick_variant("col_det", rc.force_col_det, tune,
+        ch.col_det = pick_variant("col_det", rc.force_col_det, tune, col_variants_apply(cl.flags, true),
                                   [&](int v) { cl.variant = v; return launch_col_pass(nx, cl, false, true, s); }, s,
                                   log);
     }

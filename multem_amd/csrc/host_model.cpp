@@ -567,8 +567,19 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
         } else {
             if (!in.detector_fR) fail(MT_ERR_INVALID_ARGUMENT, "matrix detectors need detector_fR");
             const size_t n = (size_t)in.n_detector * nx * ny;
+            // the maps come centred (g = 0 at nx/2, ny/2, as the reference's
+            // examples build them); fft2_shift takes them to the natural
+            // Fourier order of the detector sums (Wave_Function::set_input_data,
+            // src/wave_function.cuh:53-61)
             det_fR.resize(n);
-            for (size_t i = 0; i < n; ++i) det_fR[i] = (R)in.detector_fR[i];
+            const int nxh = nx / 2, nyh = ny / 2;
+            for (int d = 0; d < in.n_detector; ++d)
+                for (int ix = 0; ix < nx; ++ix)
+                    for (int iy = 0; iy < ny; ++iy) {
+                        const int sx = ix < nxh ? ix + nxh : ix - nxh, sy = iy < nyh ? iy + nyh : iy - nyh;
+                        det_fR[((size_t)d * nx + ix) * ny + iy] =
+                            (R)in.detector_fR[((size_t)d * nx + sx) * ny + sy];
+                    }
             det_g2_min.assign(in.n_detector, R(0));
             det_g2_max.assign(in.n_detector, R(0));
         }

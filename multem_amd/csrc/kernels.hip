@@ -708,6 +708,13 @@ constexpr int kColProbe = COL_IFFT | COL_STORE;
 
 template <class Z> constexpr bool kFp64 = std::is_same<Z, double2>::value;
 
+} // namespace
+
+bool row_variants_apply(int flags) { return flags == kRowSlice; }
+bool col_variants_apply(int flags, bool det) { return det ? flags == kColLast : flags == kColSlice; }
+
+namespace {
+
 // variant V of the row pass at length N with compile-time flags F.  The FP64
 // passes have one variant: the default schedule with table twiddles (the
 // double registers do not leave room for register twiddles).

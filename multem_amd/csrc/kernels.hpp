@@ -281,6 +281,13 @@ constexpr PassVariant col_variant(int n, int v)
     }
 }
 
+// Whether the launch variants apply to a pass with these flags: only the
+// specialised slice-step kernels (full row step, full column step, last
+// column step with detector sums) have variants; any other flag set runs the
+// generic kernel whatever the variant.
+bool row_variants_apply(int flags);
+bool col_variants_apply(int flags, bool det);
+
 int col_tile_width(int nx);   // columns per column-pass workgroup (variant 0; the detector partial layout)
 int col_tile_width_fp64(int nx);   // the same for the FP64 passes
 int row_group(int ny);        // rows per row-pass workgroup (variant 0)

@@ -91,7 +91,7 @@ class Parameters:
         self.obj_lens_outer_aper_ang = 24.0
         self.detector_type = 1
         self.detector_cir = [(40.0, 160.0)]   # (inner_ang, outer_ang) in mrad
-        self.detector_fR = []                 # detector_type 3: ny x nx sensitivity maps (natural Fourier layout)
+        self.detector_fR = []                 # detector_type 3: ny x nx sensitivity maps, centred (g = 0 at nx/2, ny/2)
         self.scanning_type = 2
         self.scanning_periodic = 0
         self.scanning_square_pxs = 1

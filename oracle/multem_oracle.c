@@ -1524,7 +1524,16 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
             if (!in->detector_fR) FAIL(MT_ERR_INVALID_ARGUMENT, "matrix detectors need detector_fR");
             size_t n = (size_t)s->n_det * in->nx * in->ny;
             s->det_fR = malloc(sizeof(real) * (n > 0 ? n : 1));
-            for (size_t i = 0; i < n; i++) s->det_fR[i] = (real)in->detector_fR[i];
+            /* centred maps -> natural Fourier order: fft2_shift of each map
+               (Wave_Function::set_input_data, src/wave_function.cuh:53-61) */
+            int nxh = in->nx / 2, nyh = in->ny / 2;
+            for (int d = 0; d < s->n_det; d++)
+                for (int ix = 0; ix < in->nx; ix++)
+                    for (int iy = 0; iy < in->ny; iy++) {
+                        int sx = ix < nxh ? ix + nxh : ix - nxh, sy = iy < nyh ? iy + nyh : iy - nyh;
+                        s->det_fR[((size_t)d * in->nx + ix) * in->ny + iy] =
+                            (real)in->detector_fR[((size_t)d * in->nx + sx) * in->ny + sy];
+                    }
         }
         for (int i = 0; in->detector_type == 1 && i < s->n_det; i++) {
             real ia = (real)(in->detector_cir[i].inner_ang * c_mrad_2_rad);

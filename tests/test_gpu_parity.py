@@ -441,18 +441,57 @@ def test_rotated_specimen():
     assert e32 < 1e-5 and e64 < 1e-4
 
 
-@pytest.mark.parametrize("precision", [1, 2])
-def test_matrix_detectors(precision):
+@pytest.mark.parametrize("precision,theta", [(1, 0.0), (2, 0.0), (1, 0.5), (2, 0.5)])
+def test_matrix_detectors(precision, theta):
     """detector_type 3: sensitivity maps weight |psi(g)|^2 in the detector
-    pass and in the coherent image, against the oracle."""
+    pass and in the coherent image, against the oracle; with tilt the
+    output ramp comes first."""
     p = small_stem(ns=3, nconf=2)
     p.system_conf.precision = precision
     p.pn_coh_contrib = 1
+    p.theta = theta
     p.detector_type = 3
     rng = np.random.default_rng(11)
     p.detector_fR = [rng.random((p.ny, p.nx)), (rng.random((p.ny, p.nx)) > 0.7).astype(float)]
     out, ref = engine.run_multislice(p), oracle.run(p, "f64")
     e_tot, e_coh = rel_l2(out.image_tot, ref.image_tot), rel_l2(out.image_coh, ref.image_coh)
-    print("matrix detectors (precision %d): image_tot %.2e, image_coh %.2e vs FP64" % (precision, e_tot, e_coh))
+    print("matrix detectors (precision %d, theta %.1f): image_tot %.2e, image_coh %.2e vs FP64"
+          % (precision, theta, e_tot, e_coh))
     tol = 1e-4 if precision == 1 else 1e-9
     assert e_tot < tol and e_coh < tol
+
+
+def test_matrix_detector_ring_map_equals_ring():
+    """The reference example's centred ring map gives the ring detector's
+    image on the GPU (example_MULTEM_STEM_matrix_detector.m:115-130)."""
+    from tests.test_matrix_detector import ring_map
+    p = small_stem(ns=3, nconf=1)
+    p.detector_cir = [(20.0, 60.0), (80.0, 200.0)]
+    ring = oracle.run(p, "f64").image_tot          # the rings, FP64 g^2 as the map's
+    q = p.copy()
+    q.detector_type = 3
+    q.detector_fR = [ring_map(q, a, b) for a, b in p.detector_cir]
+    e = rel_l2(engine.run_multislice(q).image_tot, ring)
+    print("centred ring maps (GPU) vs rings (FP64 oracle): rel L2 %.2e" % e)
+    assert e < 1e-4
+
+
+def test_tuner_cache_separates_generic_and_specialised_passes():
+    """A bandwidth-limited STEM run (generic column kernels, one launch
+    variant) and then an unlimited one at the same grid and batch in the
+    same process: the second must tune its own specialised kernels, not
+    reuse the first's choice, and both match the FP64 oracle."""
+    p = small_stem(ns=3, nconf=1)
+    engine.set_batch(9, True)
+    try:
+        for bwl in (1, 0, 1):
+            q = p.copy()
+            q.bwl = bwl
+            out = engine.run_multislice(q)
+            e = rel_l2(out.image_tot, oracle.run(q, "f64").image_tot)
+            print("bwl=%d: %s | %s; rel L2 vs FP64 %.2e" % (bwl, engine.last_plan(), engine.tune_log(), e))
+            assert e < 1e-4
+            if bwl:
+                assert "col_det=v0(generic)" in engine.tune_log() or "cached" in engine.tune_log()
+    finally:
+        engine.set_batch(0, True)

@@ -2,10 +2,13 @@
 src/wave_function.cuh:125-129) in the oracle.
 
 Pinned by the ring detector: a sensitivity map that is 1 on the pixels a
-ring keeps (g2_in <= g^2 < g2_out on the natural Fourier grid) and 0
-elsewhere gives the ring's image; the map is linear (scaling it scales the
-image).  The radial detector, which the reference integrates to zero, is
-refused.
+ring keeps (g2_in <= g^2 < g2_out) and 0 elsewhere gives the ring's image;
+the map is linear (scaling it scales the image).  Maps are centred, as the
+reference's example builds them (example_MULTEM_STEM_matrix_detector.m:115-130:
+meshgrid((-nxh:nxh-1) dgx, (-nyh:nyh-1) dgy)), and the engine takes them to
+the natural Fourier order with fft2_shift (src/wave_function.cuh:53-61): a
+map in the natural order is a different detector.  The radial detector,
+which the reference integrates to zero, is refused.
 """
 import numpy as np
 import pytest
@@ -16,14 +19,14 @@ from tests.conftest import rel_l2
 
 
 def ring_map(p, inner, outer):
-    """The ring's pixel mask, evaluated as the FP64 oracle evaluates it."""
+    """The ring's pixel mask as the reference's example builds it: centred,
+    gx = (-nxh:nxh-1) dgx, gy = (-nyh:nyh-1) dgy, MATLAB ny x nx; g^2
+    evaluated as the FP64 oracle evaluates it."""
     lam = oracle.lib("f64").orc_lambda(float(p.E_0))
     lx, ly = float(p.spec_lx), float(p.spec_ly)
     dgx, dgy = 1.0 / lx, 1.0 / ly
-    ix = np.arange(p.nx)
-    iy = np.arange(p.ny)
-    gx = np.where(ix < p.nx // 2, ix, ix - p.nx) * dgx
-    gy = np.where(iy < p.ny // 2, iy, iy - p.ny) * dgy
+    gx = (np.arange(p.nx) - p.nx // 2) * dgx
+    gy = (np.arange(p.ny) - p.ny // 2) * dgy
     g2 = gx[:, None] * gx[:, None] + gy[None, :] * gy[None, :]
     g2i = (np.sin(inner * 1e-3) / lam) ** 2
     g2o = (np.sin(outer * 1e-3) / lam) ** 2
@@ -63,3 +66,15 @@ def test_radial_detector_refused(stem):
     q.detector_type = 2
     with pytest.raises(RuntimeError):
         oracle.run(q, "f64")
+
+
+def test_natural_order_map_is_another_detector(stem):
+    """The quadrant swap matters: the ring's map in the natural Fourier
+    order (ring at the grid corners after the engine's fft2_shift) is not
+    the ring."""
+    ref = oracle.run(stem, "f64").image_tot
+    q = stem.copy()
+    q.detector_type = 3
+    q.detector_fR = [np.fft.ifftshift(ring_map(q, a, b)) for a, b in stem.detector_cir]
+    got = oracle.run(q, "f64").image_tot
+    assert rel_l2(got, ref) > 0.1

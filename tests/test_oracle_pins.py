@@ -347,3 +347,218 @@ def test_hrtem_coherent_image_of_one_configuration(illum):
     r = oracle.run(p, "f64")
     assert r.m2psi_coh is not None
     assert np.array_equal(r.m2psi_coh, r.m2psi_tot)
+
+
+# ---------------------------------------------------------------------------
+# Closed forms from physical constants (CODATA, scipy.constants), independent
+# of the oracle's own constants: wavelength, interaction constant, Fresnel
+# propagator, aberration function, detector normalisation, partially coherent
+# envelope.  Each check fails under a flipped sign or a wrong scale.
+
+def _lambda_codata(E_kev):
+    """Relativistic electron wavelength (A): h c / sqrt(E (2 m c^2 + E))."""
+    from scipy import constants as K
+    E = E_kev * 1e3 * K.e
+    mc2 = K.m_e * K.c ** 2
+    return K.h * K.c / np.sqrt(E * (2.0 * mc2 + E)) * 1e10
+
+
+def _sigma_codata(E_kev):
+    """Interaction constant (rad / (V A)): 2 pi / (lambda V) (m c^2 + e V) / (2 m c^2 + e V)."""
+    from scipy import constants as K
+    V = E_kev * 1e3
+    mc2 = K.m_e * K.c ** 2 / K.e
+    return 2.0 * np.pi / (_lambda_codata(E_kev) * V) * (mc2 + V) / (2.0 * mc2 + V)
+
+
+def test_interaction_constant_closed_form():
+    # Vr_factor = gamma lambda / c_Potf / cos(theta) (src/cgpu_fcns.cuh:137-144) is the
+    # interaction constant sigma of the projected potential in V A
+    L = oracle.lib("f64")
+    for E, published in [(300.0, 6.5265e-4), (200.0, 7.2884e-4), (100.0, 9.2441e-4)]:
+        s = _sigma_codata(E)
+        assert abs(s / published - 1) < 1e-4            # the textbook values
+        assert abs(L.orc_vr_factor(E, 0.0) / s - 1) < 2e-6
+        th = np.radians(0.7)
+        assert abs(L.orc_vr_factor(E, th) * np.cos(th) / s - 1) < 2e-6
+        assert abs(L.orc_lambda(E) / _lambda_codata(E) - 1) < 2e-6
+
+
+def _vacuum(p):
+    p.spec_atoms = p.spec_atoms.copy()
+    p.spec_atoms[:, 5] = 0.0                               # occupancy 0: no potential
+    return p
+
+
+@pytest.mark.parametrize("reverse", [0, 1])
+@pytest.mark.parametrize("tilt", [None, (2, 0.0), (3, 90.0)])
+def test_propagator_plane_wave_closed_form(reverse, tilt):
+    """A plane wave of frequency g through the Fresnel propagator of
+    distance z picks up exp(-i s pi lambda z |g - g0|^2), s = +1 (-1 for
+    reverse multislice), with |psi(g)| kept (propagate,
+    src/cgpu_fcns.cuh:1835-1845; w = -mul_sign pi lambda z,
+    src/input_multislice.cuh:908-912); the reciprocal output carries the
+    1/(nx ny) of the unnormalised FFT (src/propagator.cuh:69-86)."""
+    p = configs.si_ewrs_256(nc=2, n=128)
+    p.reverse_multislice = reverse
+    nx, ny = p.nx, p.ny
+    lam = _lambda_codata(p.E_0)
+    g0 = np.zeros(2)
+    if tilt:
+        # sin(theta) on the tilt grid n lambda dg: g0 = n dg (cos phi, sin phi)
+        n, phi = tilt
+        p.theta = np.degrees(np.arcsin(n * lam / p.spec_lx))
+        p.phi = phi
+        g0 = n / p.spec_lx * np.array([np.cos(np.radians(phi)), np.sin(np.radians(phi))])
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    sgn = -1.0 if reverse else 1.0
+    for kx, ky in [(3, 5), (-7, 2), (40, -33), (0, 0)]:
+        psi = np.exp(2j * np.pi * (kx * ix / nx + ky * iy / ny))
+        g2 = (kx / p.spec_lx - g0[0]) ** 2 + (ky / p.spec_ly - g0[1]) ** 2
+        for z in (2.0, -13.5, 120.0):
+            out = oracle.propagate(p, psi, z)
+            phase = -sgn * np.pi * lam * z * g2
+            # the tolerance follows the phase: the reference's constants give
+            # lambda to ~1e-7 of CODATA's
+            assert np.allclose(np.abs(out), 1.0, rtol=0, atol=1e-12)
+            d = np.angle(out / (np.exp(1j * phase) * psi))
+            assert np.max(np.abs(d)) < 2e-6 * abs(phase) + 1e-12, (kx, ky, z)
+        f = oracle.propagate(p, psi, 7.0, space=2)     # centred reciprocal layout
+        k = ((kx + nx // 2) % nx, (ky + ny // 2) % ny)
+        assert abs(abs(f[k]) - 1.0) < 1e-12
+        f[k] = 0
+        assert np.max(np.abs(f)) < 1e-12
+
+
+def _centre_probe(n=128, **lens):
+    p = configs.au_stem_512(ns=2, nconf=1, n=n, ncells=4, nc=2)
+    p.simulation_type = 21
+    p.iw_x = np.array([0.5 * p.spec_lx])
+    p.iw_y = np.array([0.5 * p.spec_ly])
+    p.cond_lens_outer_aper_ang = 30.0
+    for k, v in lens.items():
+        setattr(p, "cond_lens_" + k, v)
+    return p
+
+
+def _probe_spectrum(p):
+    """fPsi(g) (natural k order) of the oracle's probe: the probe is
+    returned in natural real space, the shifted layout puts the box centre
+    at index 0, and the inverse FFT is unnormalised."""
+    psi = oracle.incident_wave(p)
+    return np.fft.fft2(np.fft.fftshift(psi)) / psi.size
+
+
+def _k_grid(p):
+    gx = np.fft.fftfreq(p.nx, d=p.spec_lx / p.nx)
+    gy = np.fft.fftfreq(p.ny, d=p.spec_ly / p.ny)
+    GX, GY = np.meshgrid(gx, gy, indexing="ij")
+    return GX, GY
+
+
+def test_probe_aberration_phases_closed_form():
+    """Phase of the probe spectrum, Lens::set_input_data / eval_* and
+    exp_i_chi (src/types.cuh:2924-2976,3180-3300; src/cgpu_fcns.cuh:1949-1979):
+    chi = -pi lambda c_10 g^2 - pi Cs lambda^3 g^4 / 2 - pi lambda c_12 g^2
+    sin(2 (phi_g - phi_12)), Cs in mm (x 1e7 A), angles in degrees.  Ratios
+    of two probes leave only the changed term."""
+    lam = _lambda_codata(300.0)
+    a = _centre_probe(c_10=0.0, c_30=0.0)
+    b = _centre_probe(c_10=85.0, c_30=0.0)
+    c = _centre_probe(c_10=85.0, c_30=1.3)
+    d = _centre_probe(c_10=85.0, c_30=1.3, c_12=40.0, phi_12=25.0)
+    Fa, Fb, Fc, Fd = (_probe_spectrum(q) for q in (a, b, c, d))
+    GX, GY = _k_grid(a)
+    g2 = GX ** 2 + GY ** 2
+    inside = np.abs(Fa) > 0.5 * np.abs(Fa).max()
+    assert inside.sum() > 100
+    # equal moduli: the aperture and the normalisation do not depend on chi
+    assert np.allclose(np.abs(Fb)[inside], np.abs(Fa)[inside], rtol=1e-12)
+    def check(num, den, phase):
+        # the tolerance follows the phase: the reference's constants give
+        # lambda to ~1e-7 of CODATA's (lambda^3 for Cs)
+        d = np.angle(num[inside] / den[inside] * np.exp(-1j * phase))
+        assert np.max(np.abs(d)) < 1e-6 * np.max(np.abs(phase)) + 1e-9
+
+    g2i = g2[inside]
+    check(Fb, Fa, -np.pi * lam * 85.0 * g2i)
+    check(Fc, Fb, -np.pi * 1.3e7 * lam ** 3 * g2i ** 2 / 2)
+    phig = np.arctan2(GY, GX)[inside]
+    check(Fd, Fc, -np.pi * lam * 40.0 * g2i * np.sin(2 * (phig - np.radians(25.0))))
+    # the aperture is sin(alpha)/lambda, alpha in mrad
+    gmax = np.sin(30e-3) / lam
+    assert np.all(g2[inside] < gmax ** 2)
+    assert np.all(np.abs(Fa)[g2 > gmax ** 2 * 1.0001] < 1e-15)     # FFT round-off only
+
+
+def test_stem_detector_normalisation_in_vacuum():
+    """Through vacuum |psi(g)|^2 is the probe's normalised spectrum: a ring
+    covering the whole grid and a ring just outside the aperture give 1, a
+    ring outside the aperture gives 0 (integrated_intensity_over_det,
+    src/wave_function.cuh:107-133; probe normalised to sum |fPsi|^2 = 1,
+    src/gpu_fcns.cuh:1612-1625); with atoms the full ring still gives 1
+    (unitary slices without a bandwidth limit)."""
+    p = configs.au_stem_512(ns=3, nconf=1, n=128, ncells=2, nc=3)
+    p.detector_cir = [(0.0, 1000.0), (0.0, 22.0), (25.0, 200.0)]
+    q = _vacuum(p.copy())
+    img = oracle.run(q, "f64").image_tot          # [n_thk][n_det][nscan]
+    assert np.allclose(img[0, 0], 1.0, rtol=0, atol=1e-12)
+    assert np.allclose(img[0, 1], 1.0, rtol=0, atol=1e-12)
+    assert np.all(np.abs(img[0, 2]) < 1e-20)
+    img = oracle.run(p, "f64").image_tot
+    assert np.allclose(img[0, 0], 1.0, rtol=0, atol=1e-12)
+    assert np.all(img[0, 2] > 1e-4)                  # the specimen scatters into the ADF ring
+
+
+def _two_beam(illum, **lens):
+    """HRTEM of the user wave 1 + eps exp(2 pi i g.r) through vacuum: the
+    image's Fourier coefficient at g is eps E(g) exp(i chi(g)) times the
+    vacuum propagation phase (the 0 beam is unchanged)."""
+    p = _vacuum(configs.si_ewrs_256(nc=2, n=128))
+    p.simulation_type = 32
+    p.iw_type = 3
+    p.iw_x, p.iw_y = np.array([0.5 * p.spec_lx]), np.array([0.5 * p.spec_ly])
+    p.illumination_model = illum
+    p.temporal_spatial_incoh = 1
+    p.obj_lens_outer_aper_ang = 60.0
+    for k, v in lens.items():
+        setattr(p, k, v)
+    ix, iy = np.meshgrid(np.arange(p.nx), np.arange(p.ny), indexing="ij")
+    out = {}
+    for kx, ky in [(5, 0), (9, 4), (-6, 11), (17, -13)]:
+        psi = 1 + 0.05 * np.exp(2j * np.pi * (kx * ix / p.nx + ky * iy / p.ny))
+        p.iw_psi = psi.T.copy()
+        img = oracle.run(p, "f64").m2psi_tot[0]
+        F = np.fft.fft2(img) / img.size
+        out[(kx, ky)] = F[kx % p.nx, ky % p.ny]
+    return p, out
+
+
+def test_objective_lens_ctf_and_pctf_envelope_closed_form():
+    """Coherent CTF exp(i chi) and the partially coherent envelope of
+    apply_PCTF (src/cgpu_fcns.cuh:2010-2040; src/microscope_effects.cuh:57-145):
+    E(g) = exp(-(spa + temp) / u) / sqrt(u), u = 1 + (pi th_c D)^2 g^2,
+    temp = (pi D lambda g^2)^2 / 4, spa = (pi th_c (Cs lambda^2 g^2 + df))^2 g^2,
+    D = sqrt2 ti_sigma, th_c = asin(sqrt2 si_sigma lambda)
+    (Lens::set_ti_sigma / set_si_sigma, src/types.cuh:2978-2989)."""
+    lam = _lambda_codata(300.0)
+    df, cs_mm, ti, si = -60.0, 1.2, 40.0, 0.01
+    lens = dict(obj_lens_c_10=df, obj_lens_c_30=cs_mm, obj_lens_ti_sigma=ti, cond_lens_si_sigma=si)
+    p, coh = _two_beam(1, **lens)
+    _, pcoh = _two_beam(2, **lens)
+    _, coh2 = _two_beam(1, **dict(lens, obj_lens_c_10=df + 30.0))
+    D, thc, cs = np.sqrt(2) * ti, np.arcsin(np.sqrt(2) * si * lam), cs_mm * 1e7
+    for (kx, ky), a in coh.items():
+        g2 = (kx / p.spec_lx) ** 2 + (ky / p.spec_ly) ** 2
+        u = 1 + (np.pi * thc * D) ** 2 * g2
+        temp = (np.pi * D * lam * g2) ** 2 / 4
+        spa = (np.pi * thc * (cs * lam ** 2 * g2 + df)) ** 2 * g2
+        E = np.exp(-(spa + temp) / u) / np.sqrt(u)
+        r = pcoh[(kx, ky)] / a
+        # compared in the exponent: the reference's constants differ from
+        # CODATA's by ~3e-7, which moves ln E by a few 1e-6 of itself
+        assert abs(r.imag) < 1e-9 * E + 1e-14 and r.real > 0     # same chi, real envelope
+        assert abs(np.log(r.real) - np.log(E)) < 5e-6 * abs(np.log(E)) + 1e-9, ((kx, ky), r, E)
+        assert abs(abs(a) - 0.05) < 1e-12           # coherent: |exp(i chi)| = 1
+        # a 30 A defocus change multiplies the coefficient by exp(-i pi lambda 30 g^2)
+        assert abs(coh2[(kx, ky)] / a - np.exp(-1j * np.pi * lam * 30.0 * g2)) < 1e-6
