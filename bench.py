@@ -53,32 +53,50 @@ def sync():
     engine.synchronize()
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args):
+    """The FP32 C oracle, one thread per core (the transmission build over
+    slices, then the probes over threads), on the cores this process may
+    use up to the host's CPU share of one GPU (16 cores per GPU on these
+    hosts, whatever nproc shows); nproc and the CPU model are recorded."""
     from multem_amd import configs
     from oracle import oracle
     p = configs.au_stem_512(ns=args.cpu_ns, nconf=1, n=args.n)
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    cores = min(cores, 16)
+    cores = oracle.default_threads()
     nsl = len(oracle.spec_slicing(p))
     t0 = time.perf_counter()
     oracle.run(p, "f32", nthreads=cores)
     dt = time.perf_counter() - t0
     ps = args.cpu_ns * args.cpu_ns * nsl
     return {"value": ps / dt, "unit": "probe·slices/s", "cores": cores, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
             "sample": "Au[001] 512^2, %dx%d probes x 1 configuration x %d slices (%d probe.slices), "
-                      "FP32 C oracle, %.1f s" % (args.cpu_ns, args.cpu_ns, nsl, ps, dt)}
+                      "FP32 C oracle (own radix-2 FFT, FFTW absent), %.1f s" % (args.cpu_ns, args.cpu_ns, nsl, ps, dt)}
 
 
 def group_measurement():
     """SrTiO3 [001] at 256^2 / 512^2 / 1024^2 (21 / 51 / 101 slices), 16x16
     probes and 4 phonon configurations each, 256 probes per batch: one group
     run (one hipGraph launch per round) against the three runs one after the
-    other."""
+    other, and each grid's own rate and column-pass roofline fraction from
+    its solo run (HIP events around the passes of each configuration's
+    first batch)."""
     from multem_amd import configs, engine
     ps = configs.sto_group(ns=16, nconf=4)
     nsl = [len(engine.spec_slicing(p)) for p in ps]
-    work = sum(16 * 16 * 4 * n for n in nsl)
+    work_i = [16 * 16 * 4 * n for n in nsl]
+    work = sum(work_i)
     engine.set_batch(256, True)
+    per_grid = []
     try:
         engine.run_multislice_group(ps)          # warm-up (tuning, graphs)
         sync()
@@ -90,17 +108,34 @@ def group_measurement():
         for p in ps:
             engine.run_multislice(p)
         sync()
-        t0 = time.perf_counter()
-        for p in ps:
+        t_seq = 0.0
+        for p, w in zip(ps, work_i):
+            t0 = time.perf_counter()
             engine.run_multislice(p)
-        sync()
-        t_seq = time.perf_counter() - t0
+            sync()
+            dt = time.perf_counter() - t0
+            t_seq += dt
+            t = engine.last_timing()
+            pl = engine.last_plan()
+            B = int([kv.split("=")[1] for kv in pl.split() if kv.startswith("batch=")][0])
+            N2 = p.nx * p.ny
+            col_ms = t["ms_col"] / max(t["n_col"], 1)
+            row_ms = t["ms_row"] / max(t["n_row"], 1)
+            col_gbs = 16.0 * N2 * B / (col_ms * 1e-3) / 1e9 if col_ms > 0 else None
+            per_grid.append({"grid": "%d^2" % p.nx, "slices": nsl[len(per_grid)], "probe_slices": w,
+                             "value": w / dt, "unit": "probe·slices/s", "batch": B,
+                             "row_pass_ms": row_ms, "col_pass_ms": col_ms,
+                             "col_pass_alg_GBs": col_gbs,
+                             "col_pass_roofline_frac": col_gbs / HBM_PEAK_GBS if col_gbs else None,
+                             "slice_loop_alg_GBs": 40.0 * N2 * t["probe_slices"] / (t["ms_slice"] * 1e-3) / 1e9
+                             if t["ms_slice"] > 0 else None,
+                             "plan": pl})
     finally:
         engine.set_batch(0, True)
     return {"workload": "sto_group: SrTiO3[001] 256^2/512^2/1024^2, %s slices, 16x16 probes x 4 configs each, "
                         "256 probes per batch" % "/".join(str(n) for n in nsl),
             "probe_slices": work, "value": work / t_group, "unit": "probe·slices/s",
-            "sequential_value": work / t_seq, "plan": plan}
+            "sequential_value": work / t_seq, "plan": plan, "per_grid": per_grid}
 
 
 def fp64_measurement():

@@ -284,10 +284,16 @@ mt_status mt_hip_atom_table(int Z, double dR_min, double *R2, double *c0, double
 void mt_hip_request_stop(void);
 void mt_hip_progress(int *iter, int *niter);
 
-/* Caps the device memory the engine plans for (bytes; 0 = no cap).  The
-   counterpart of Memory_Slice's free-memory probe
-   (transmission_function.cuh:161-182): a smaller budget makes the engine
-   keep fewer transmission slices resident and rebuild the rest. */
+/* Caps the device memory the engine plans for (bytes; 0 = 85 % of the free
+   device memory).  The counterpart of Memory_Slice's free-memory probe
+   (transmission_function.cuh:146-215): when one configuration's
+   transmission stack does not fit beside two waves, the engine stores the
+   potentials of as many slices as fit and rebuilds each slice's
+   transmission -- from its potential, or from the atoms beyond those --
+   just before it is used; results are bit-identical to the resident stack.
+   Below one slice and one wave a run fails with MT_ERR_OUT_OF_MEMORY.  The
+   last plan (mt_hip_last_plan) names the mode ("slices=rebuilt
+   stored_potentials=k/n"). */
 void mt_hip_set_memory_budget(size_t bytes);
 
 /* Engine choices made for the last run (batch width, FFT schedule, tiles),

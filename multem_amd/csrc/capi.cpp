@@ -98,7 +98,10 @@ static mt_status run_multislice_as(const mt_system_conf *sys, const mt_input_mul
                                    mt_output_multislice *out, int prec)
 {
     if (!out) return fail(MT_ERR_INVALID_ARGUMENT, "null output");
+    // progress restarts from (0, 0): a poll never sees the previous run's count
     mtg::run_control().stop = false;
+    mtg::run_control().iter = 0;
+    mtg::run_control().niter = 0;
     return guarded(sys, in, [&](const auto &m) { mtg::run_multislice(m, sys->gpu_device, out); }, prec);
 }
 

@@ -203,8 +203,9 @@ struct Setup {
     }
 
     // Transmission stack of one configuration into trans (n_slice * nxy),
-    // optionally the potential V (natural layout) of the same slices.
-    void build_tf(const DBuf<V4> &atoms, int slice0, int n_slices, Z *trans, R *V)
+    // optionally the potential V (natural layout) of the same slices; on
+    // stream st (default: the setup stream).
+    void build_tf(const DBuf<V4> &atoms, int slice0, int n_slices, Z *trans, R *V, hipStream_t st = nullptr)
     {
         TfArgsT<R> a{};
         a.nx = nx; a.ny = ny; a.nxh = m.nxh; a.nyh = m.nyh;
@@ -216,24 +217,35 @@ struct Setup {
         a.w = m.Vr_factor;
         a.weak_phase = m.interaction_model == 3 ? 1 : 0;
         a.trans = trans; a.V = V;
-        check(launch_tf_build(a, s), "tf_build");
-        if (m.bwl && trans) bandwidth_limit(trans, n_slices);
+        check(launch_tf_build(a, st ? st : s), "tf_build");
+        if (m.bwl && trans) bandwidth_limit(trans, n_slices, st);
+    }
+
+    // Transmission of n_slices stored potentials V into trans (+ bandwidth
+    // limit): the same values build_tf writes for those slices.
+    void tf_from_pot(const R *V, int n_slices, Z *trans, hipStream_t st = nullptr)
+    {
+        check(launch_trans_from_pot(V, n_slices, nx, ny, m.Vr_factor, m.interaction_model == 3 ? 1 : 0, trans,
+                                    st ? st : s),
+              "trans_from_pot");
+        if (m.bwl) bandwidth_limit(trans, n_slices, st);
     }
 
     // FFT -> x bwl/(nx ny) -> IFFT on n waves in place (Transmission_Function::trans,
     // src/transmission_function.cuh:72-82)
-    void bandwidth_limit(Z *w, int n)
+    void bandwidth_limit(Z *w, int n, hipStream_t st = nullptr)
     {
+        if (!st) st = s;
         RowArgsT<Z> r{};
         r.src = w; r.dst = w; r.member_stride = (long long)nxy; r.nrows = nx; r.n_members = n;
         r.tw = tw_y.p;
         r.flags = ROW_LOAD | ROW_FFT | ROW_STORE;
-        check(launch_row_pass(ny, r, s), "row pass");
+        check(launch_row_pass(ny, r, st), "row pass");
         ColArgsT<Z> c = col_base(w, w, n);
         c.flags = COL_LOAD | COL_FFT | COL_BWL | COL_IFFT | COL_STORE;
-        check(launch_col_pass(nx, c, false, false, s), "col pass");
+        check(launch_col_pass(nx, c, false, false, st), "col pass");
         r.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
-        check(launch_row_pass(ny, r, s), "row pass");
+        check(launch_row_pass(ny, r, st), "row pass");
     }
 
     ColArgsT<Z> col_base(const Z *src, Z *dst, int n_members) const
@@ -440,6 +452,91 @@ struct Setup {
     }
 };
 
+// Slice storage of one configuration (Transmission_Function::Memory_Slice,
+// src/transmission_function.cuh:146-215, used at :84-103 and :118-134).
+// When the transmission stack of every slice fits the memory given to it,
+// the stack is resident and the row pass reads slice s from it.  Otherwise,
+// as in the reference, the potentials of as many slices as fit are stored
+// (half the bytes of a transmission), and each slice's transmission is
+// rebuilt just before the row pass that multiplies by it, into one slot:
+// from its stored potential, or, beyond those, from the atoms.  Either way
+// the values are those of the resident stack, bit for bit.
+template <class R>
+struct SliceStore {
+    using Z = cplx_t<R>;
+    using V4 = vec4_t<R>;
+    Setup<R> &S;
+    int n_slice = 0;
+    bool resident = true;
+    int n_pot = 0;        // rebuilt mode: slices whose potential is stored
+    int K = 1;            // resident mode: configurations held at once
+    DBuf<Z> trans;        // resident: [K][n_slice][nxy]; rebuilt: the slot [nxy]
+    DBuf<R> V;            // rebuilt: [n_pot][nxy], natural layout
+
+    explicit SliceStore(Setup<R> &s) : S(s), n_slice(s.m.n_slice()) {}
+
+    size_t stack_bytes() const { return (size_t)n_slice * S.nxy * sizeof(Z); }
+    size_t slot_bytes() const { return S.nxy * sizeof(Z); }
+    size_t pot_bytes() const { return S.nxy * sizeof(R); }
+
+    // Plan for `avail` bytes and up to k_max configurations: the stacks of
+    // as many configurations as fit (at least one), else the rebuilt mode
+    // with as many stored potentials as fit beside the slot.  Returns the
+    // bytes taken; throws if not even the slot fits.
+    size_t plan(size_t avail, int k_max)
+    {
+        if (stack_bytes() <= avail) {
+            resident = true;
+            K = (int)std::max<size_t>(1, std::min<size_t>((size_t)k_max, avail / stack_bytes()));
+            trans.alloc((size_t)K * n_slice * S.nxy);
+            return (size_t)K * stack_bytes();
+        }
+        if (slot_bytes() > avail) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for one transmission slice"};
+        resident = false;
+        K = 1;
+        n_pot = (int)std::min<size_t>((size_t)n_slice, (avail - slot_bytes()) / pot_bytes());
+        trans.alloc(S.nxy);
+        check(hipMemsetAsync(trans.p, 0, slot_bytes(), S.s), "hipMemsetAsync");
+        if (n_pot > 0) V.alloc((size_t)n_pot * S.nxy);
+        return slot_bytes() + (size_t)n_pot * pot_bytes();
+    }
+
+    // what a configuration needs before its waves run: its transmission stack
+    // (into configuration slot j), or its stored potentials
+    void build(const DBuf<V4> &atoms, int j, hipStream_t st = nullptr)
+    {
+        if (resident) S.build_tf(atoms, 0, n_slice, trans.p + (size_t)j * n_slice * S.nxy, nullptr, st);
+        else if (n_pot > 0) S.build_tf(atoms, 0, n_pot, nullptr, V.p, st);
+    }
+
+    // Row-pass operands for slice is: r.trans, r.tslot_stride and r.slice
+    template <class RA>
+    void bind(int is, RA &r) const
+    {
+        r.trans = trans.p;
+        r.tslot_stride = resident ? (long long)n_slice * S.nxy : 0;
+        r.slice = resident ? is : 0;
+    }
+
+    // bind(), after enqueueing (rebuilt mode) the slice's transmission into
+    // the slot on st, ordered before the row pass on the same stream
+    template <class RA>
+    void for_slice(const DBuf<V4> &atoms, int is, RA &r, hipStream_t st)
+    {
+        if (!resident) {
+            if (is < n_pot) S.tf_from_pot(V.p + (size_t)is * S.nxy, 1, trans.p, st);
+            else S.build_tf(atoms, is, 1, trans.p, nullptr, st);
+        }
+        bind(is, r);
+    }
+
+    std::string describe() const
+    {
+        if (resident) return "slices=resident";
+        return "slices=rebuilt stored_potentials=" + std::to_string(n_pot) + "/" + std::to_string(n_slice);
+    }
+};
+
 size_t free_device_memory()
 {
     size_t fr = 0, tot = 0;
@@ -626,9 +723,10 @@ public:
     Stream own;
     hipStream_t s;            // uploads and transmission builds
     Setup<R> S;
+    SliceStore<R> store;      // transmission stack, or stored potentials rebuilt per slice
     int ns_run, nscan, n_thk, n_det, B, ntile, nbatch;
     size_t nxy;
-    DBuf<Z> trans, psi;
+    DBuf<Z> psi;
     DBuf<Member> members;
     DBuf<Beam> beams;
     DBuf<double> part, image;
@@ -654,7 +752,7 @@ public:
 
     // budget: bytes this job may use for its transmission stack and waves
     StemJob(const ModelT<R> &mm, size_t budget, int batch_override)
-        : m(mm), s(own.s), S(mm, own.s), nxy((size_t)mm.nx * mm.ny)
+        : m(mm), s(own.s), S(mm, own.s), store(S), nxy((size_t)mm.nx * mm.ny)
     {
         ns_run = 0;
         for (const auto &t : m.thk) ns_run = std::max(ns_run, t.islice + 1);   // deeper slices add nothing
@@ -663,19 +761,18 @@ public:
         n_det = (int)m.det_g2_min.size();
         if (n_det > kMaxDet) throw Error{MT_ERR_UNSUPPORTED, "more than 8 detectors"};
 
-        // memory plan: one transmission stack + B waves (+ the coherent
-        // accumulators of every scan point, all resident)
+        // memory plan: the coherent accumulators of every scan point (all
+        // resident), the slice store -- one transmission stack, or stored
+        // potentials when the stack does not fit beside two waves -- and B waves
         const size_t coh_bytes = m.pn_coh_contrib ? (size_t)nscan * n_thk * nxy * sizeof(double2) : 0;
-        const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(Z) + coh_bytes;
-        if (stack_bytes + 2 * nxy * sizeof(Z) > budget)
-            throw Error{MT_ERR_OUT_OF_MEMORY, m.pn_coh_contrib ? "transmission stack and coherent accumulators "
-                                                                  "exceed the memory budget"
-                                                                : "transmission stack exceeds the memory budget"};
+        const size_t min_waves = 2 * nxy * sizeof(Z);
+        if (coh_bytes + min_waves > budget)
+            throw Error{MT_ERR_OUT_OF_MEMORY, "coherent accumulators exceed the memory budget"};
+        const size_t stack_bytes = coh_bytes + store.plan(budget - coh_bytes - min_waves, 1);
         const size_t per_wave = (m.tilted() ? 2 : 1) * nxy * sizeof(Z);
         const int B_cap = (int)std::min<size_t>((size_t)std::max(1, nscan), (budget - stack_bytes) / per_wave);
         if (B_cap < 1) throw Error{MT_ERR_OUT_OF_MEMORY, "no room for a wave batch"};
         ntile = m.ny / col_tile_w<R>(m.nx);
-        trans.alloc(m.n_slice() * nxy);
         std::vector<double> zl;
         for (int i = 0; i < m.n_slice(); ++i) zl.push_back((double)m.dz_slice(i));
         for (const auto &t : m.thk) zl.push_back((double)t.z_back_prop);
@@ -731,7 +828,7 @@ public:
 
         // slice-step arguments shared by the tuner and the pipeline
         r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = B;
-        r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = 0; r.members = members.p;
+        r.tw = S.tw_y.p; r.trans = store.trans.p; r.tslot_stride = 0; r.members = members.p;
         r.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
         c = S.col_base(psi.p, psi.p, B);
         c.n_det = n_det;
@@ -793,7 +890,7 @@ public:
         auto args = [&](int b, RowArgsT<Z> &rr, ColArgsT<Z> &cc) {
             rr = RowArgsT<Z>{};
             rr.src = w.p; rr.dst = w.p; rr.member_stride = (long long)nxy; rr.nrows = m.nx; rr.n_members = b;
-            rr.tw = S.tw_y.p; rr.trans = trans.p; rr.members = mem.p; rr.slice = 0;
+            rr.tw = S.tw_y.p; rr.trans = store.trans.p; rr.members = mem.p; rr.slice = 0;
             rr.flags = ROW_LOAD | ROW_IFFT | ROW_MUL_T | ROW_FFT | ROW_STORE;
             cc = S.col_base(w.p, w.p, b);
             cc.flags = col_flags(false);
@@ -854,6 +951,7 @@ public:
           << " row_variant=" << choice.row << " col_variant=" << choice.col << " col_det_variant=" << choice.col_det
           << " graph=" << (run_control().use_graph ? 1 : 0) << " aperture_px=" << nap;
         if (n_t > 1) o << " defocus_points=" << n_t;
+        if (!store.resident) o << " " << store.describe();
         return o.str();
     }
 
@@ -866,7 +964,7 @@ public:
             atoms.upload(a.data(), a.size(), s);
             Event t0, t1;
             record(t0, s);
-            S.build_tf(atoms, 0, m.n_slice(), trans.p, nullptr);
+            store.build(atoms, 0);
             record(t1, s);
             wait(t1, s);
             ms_tf += elapsed(t0, t1);
@@ -894,7 +992,7 @@ public:
         check(launch_col_pass(m.nx, c0, true, false, st), "probe pass");
         int k = 0;
         for (int is = 0; is < ns_run; ++is) {
-            r.slice = is;
+            store.for_slice(atoms, is, r, st);
             if (ev) record((*ev)[k++], st);
             check(launch_row_pass(m.ny, r, st), "row pass");
             if (ev) record((*ev)[k++], st);
@@ -1267,27 +1365,34 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                 }
 
     // Configurations per group (their stacks resident) and waves per launch:
-    // all waves of the group while they fit the wave cap, else one
-    // configuration at a time in chunks of waves.
-    const size_t stack_bytes = (size_t)m.n_slice() * nxy * sizeof(Z);
+    // all waves of the group while they fit the wave cap and the memory, else
+    // one configuration at a time in chunks of waves; when even one stack and
+    // one wave do not fit, the configuration's stored potentials are rebuilt
+    // slice by slice (SliceStore).
+    SliceStore<R> store(S);
+    const size_t stack_bytes = store.stack_bytes();
     const size_t wave_bytes = (size_t)(img && ramp ? 3 : 2) * nxy * sizeof(Z);
     const size_t fixed = (size_t)n_thk * (no * 3 + 2 * nxy) * sizeof(double);
     const size_t budget = budget_bytes();
-    if (fixed + stack_bytes + wave_bytes > budget)
-        throw Error{MT_ERR_OUT_OF_MEMORY, "one configuration exceeds the memory budget"};
+    if (fixed + store.slot_bytes() + wave_bytes > budget)
+        throw Error{MT_ERR_OUT_OF_MEMORY, "one wave and one transmission slice exceed the memory budget"};
+    const size_t avail = budget - fixed;
     const int wave_cap = run_control().batch_override > 0 ? run_control().batch_override : 256;
-    int K, W;
-    if (nq <= wave_cap) {
-        K = (int)std::min<size_t>((size_t)nconf, (budget - fixed) / (stack_bytes + (size_t)nq * wave_bytes));
-        K = std::max(1, std::min(K, wave_cap / nq));
+    int K = 0, W;
+    if (nq <= wave_cap)
+        K = (int)std::min<size_t>({(size_t)nconf, avail / (stack_bytes + (size_t)nq * wave_bytes),
+                                   (size_t)std::max(1, wave_cap / nq)});
+    if (K >= 1) {
+        store.plan((size_t)K * stack_bytes, K);
         W = K * nq;
     } else {
         K = 1;
-        W = (int)std::min<size_t>((size_t)wave_cap, (budget - fixed - stack_bytes) / wave_bytes);
+        const size_t taken = store.plan(avail - wave_bytes, 1);
+        W = (int)std::min<size_t>((size_t)std::min(nq, wave_cap), (avail - taken) / wave_bytes);
         W = std::max(W, 1);
     }
 
-    DBuf<Z> trans((size_t)K * m.n_slice() * nxy), psi((size_t)W * nxy), scratch((size_t)W * nxy);
+    DBuf<Z> psi((size_t)W * nxy), scratch((size_t)W * nxy);
     DBuf<Member> members((size_t)W);
     DBuf<Beam> beams(std::max<size_t>(1, hb.size()));
     if (!hb.empty()) beams.upload(hb.data(), hb.size(), s);
@@ -1332,6 +1437,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
          << " configs_per_batch=" << K << " waves_per_launch=" << W << " slices=" << ns_run;
     if (nq > 1) plan << " waves_per_config=" << nq;
     if (n_ctf > 1) plan << " ctf_defocus_points=" << n_ctf;
+    if (!store.resident) plan << " " << store.describe();
     tm.plan = plan.str();
 
     // column-pass arguments with the wave's tilt-azimuth tables
@@ -1358,8 +1464,10 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
         for (int j = 0; j < nb; ++j) {
             auto a = S.atoms_of(m.fp_iconf_0 + c0 + j);
             atoms.upload(a.data(), a.size(), s);
-            S.build_tf(atoms, 0, m.n_slice(), trans.p + (size_t)j * m.n_slice() * nxy, nullptr);
-            check(hipStreamSynchronize(s), "sync");   // atoms buffer is reused by the next configuration
+            store.build(atoms, j, s);
+            // the atoms buffer is reused by the next configuration (in the rebuilt
+            // mode only after this one's waves: K = 1)
+            check(hipStreamSynchronize(s), "sync");
         }
         for (int w0 = 0; w0 < nb * nq; w0 += W) {
             stop_check();
@@ -1374,7 +1482,8 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
 
             RowArgsT<Z> r{};
             r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = nm;
-            r.tw = S.tw_y.p; r.trans = trans.p; r.tslot_stride = (long long)m.n_slice() * nxy; r.members = members.p;
+            r.tw = S.tw_y.p; r.members = members.p;
+            store.bind(0, r);   // the tuner's launches read slice 0's operands
             if (!tuned) {   // pick the slice-step launch variants on this launch's buffers
                 RowArgsT<Z> rt = r;
                 rt.slice = 0;
@@ -1397,7 +1506,7 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
                 check(launch_col_pass(m.nx, cp, true, false, s), "probe pass");
             }
             for (int is = 0; is < ns_run; ++is) {
-                r.slice = is;
+                store.for_slice(atoms, is, r, s);
                 r.flags = ((is == 0 && m.iw_type == 1) ? 0 : (ROW_LOAD | ROW_IFFT)) | ROW_MUL_T | ROW_FFT | ROW_STORE;
                 check(launch_row_pass(m.ny, r, s), "row pass");
                 ColArgsT<Z> c = with_rot(S.col_base(psi.p, psi.p, nm), is);

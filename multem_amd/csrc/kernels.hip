@@ -450,6 +450,31 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
     }
 }
 
+// Transmission of stored potentials: T = exp(i w V) (or 1 + i w V), V in the
+// natural layout [slice][nx*ny], T written fft-shifted -- the same operations
+// as the end of k_tf_build, so T is bit-identical to the one built from the
+// atoms (Transmission_Function::trans with a stored potential,
+// src/transmission_function.cuh:72-90).
+template <class R>
+__global__ __launch_bounds__(256) void k_trans_from_pot(const R *V, int nx, int ny, int nxh, int nyh, R w,
+                                                        int weak_phase, long long n, cplx_t<R> *trans)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long nxy = (long long)nx * ny;
+    const long long sl = i / nxy, p = i - sl * nxy;
+    const int px = (int)(p / ny), py = (int)(p - (long long)px * ny);
+    const R v = V[i];
+    R sn, c;
+    if (weak_phase) {
+        c = R(1);
+        sn = w * v;
+    } else {
+        rsincos(w * v, &sn, &c);
+    }
+    trans[sl * nxy + (long long)shift_idx(px, nxh) * ny + shift_idx(py, nyh)] = cmake(c, sn);
+}
+
 // ---------------------------------------------------------------------------
 // Detector partials -> image (FP64), in fixed order.
 __global__ void k_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
@@ -854,6 +879,17 @@ hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s)
     return hipGetLastError();
 }
 
+template <class R>
+hipError_t launch_trans_from_pot(const R *V, int n_slices, int nx, int ny, R w, int weak_phase, cplx_t<R> *trans,
+                                 hipStream_t s)
+{
+    if (n_slices <= 0 || (nx & 1) || (ny & 1)) return hipErrorInvalidValue;
+    const long long n = (long long)n_slices * nx * ny;
+    hipLaunchKernelGGL((k_trans_from_pot<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, V, nx, ny, nx / 2,
+                       ny / 2, w, weak_phase, n, trans);
+    return hipGetLastError();
+}
+
 hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
                              const Member *members, double w, double *image, int nscan, hipStream_t s)
 {
@@ -958,6 +994,7 @@ hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_
     template hipError_t launch_row_pass<Z>(int, const RowArgsT<Z> &, hipStream_t);                                     \
     template hipError_t launch_col_pass<Z>(int, const ColArgsT<Z> &, bool, bool, hipStream_t);                         \
     template hipError_t launch_tf_build<R>(const TfArgsT<R> &, hipStream_t);                                           \
+    template hipError_t launch_trans_from_pot<R>(const R *, int, int, int, R, int, Z *, hipStream_t);                  \
     template hipError_t launch_prop_tables<Z>(int, int, const double *, int, double, int, double, double, double,      \
                                               double, Z *, Z *, hipStream_t);                                          \
     template hipError_t launch_ramp_tables<Z>(int, int, double, double, double, double, double, double, R, Z *, Z *,   \

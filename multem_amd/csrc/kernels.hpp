@@ -172,6 +172,11 @@ using OutArgs = OutArgsT<float2>;
 template <class Z> hipError_t launch_row_pass(int n, const RowArgsT<Z> &a, hipStream_t s);
 template <class Z> hipError_t launch_col_pass(int n, const ColArgsT<Z> &a, bool probe, bool det, hipStream_t s);
 template <class R> hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s);
+// T = exp(i w V) (1 + i w V if weak_phase) of n_slices stored potentials V
+// (natural layout), written fft-shifted; bit-identical to k_tf_build's T
+template <class R>
+hipError_t launch_trans_from_pot(const R *V, int n_slices, int nx, int ny, R w, int weak_phase, cplx_t<R> *trans,
+                                 hipStream_t s);
 hipError_t launch_det_reduce(const double *part, int n_members, int n_thk, int n_det, int ntile,
                              const Member *members, double w, double *image, int nscan, hipStream_t s);
 template <class Z>

@@ -2069,21 +2069,61 @@ static void *stem_worker(void *arg)
     return NULL;
 }
 
-static int build_trans_stack(sim_t *s, cplx *trans)
+/* The transmission stack of one configuration, slices shared out over
+   nthreads threads (slice k goes to thread k % nthreads).  Every slice is
+   computed exactly as alone: the threads only change which core runs it.
+   The atom tables are built first, in the calling thread, because get_tab
+   fills them on first use. */
+typedef struct {
+    sim_t *s;
+    cplx *trans;
+    int t, nt, st;
+    char err[512];   /* the worker's error message (g_err is per thread) */
+} trans_job;
+
+static void *trans_worker(void *arg)
 {
+    trans_job *J = arg;
+    sim_t *s = J->s;
     const grid_t *g = &s->g;
     size_t nxy = (size_t)g->nx * g->ny;
     fft2 F;
     fft2_init(&F, g->nx, g->ny);
     real *V = malloc(sizeof(real) * nxy);
     cplx *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
-    int st = MT_OK;
-    for (int is = 0; is < s->n_slice && st == MT_OK; is++) {
-        st = slice_potential(s, is, V);
-        if (st == MT_OK) slice_transmission(s, V, trans + (size_t)is * nxy, &F, work);
+    J->st = (V && work) ? MT_OK : MT_ERR_OUT_OF_MEMORY;
+    for (int is = J->t; is < s->n_slice && J->st == MT_OK; is += J->nt) {
+        J->st = slice_potential(s, is, V);
+        if (J->st == MT_OK) slice_transmission(s, V, J->trans + (size_t)is * nxy, &F, work);
     }
+    if (J->st != MT_OK) snprintf(J->err, sizeof J->err, "%s", J->st == MT_ERR_OUT_OF_MEMORY ? "out of memory" : g_err);
     free(V); free(work);
     fft2_free(&F);
+    return NULL;
+}
+
+static int build_trans_stack(sim_t *s, cplx *trans, int nthreads)
+{
+    for (int is = 0; is < s->n_slice; is++)
+        for (int ia = s->slice[is].iatom_0; ia <= s->slice[is].iatom_e; ia++) (void)get_tab(s, s->atoms.Z[ia] % 1000);
+    int nt = nthreads > 0 ? nthreads : 1;
+    if (nt > s->n_slice) nt = s->n_slice > 0 ? s->n_slice : 1;
+    pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nt);
+    trans_job *jobs = malloc(sizeof(trans_job) * (size_t)nt);
+    if (!th || !jobs) { free(th); free(jobs); set_err("out of memory"); return MT_ERR_OUT_OF_MEMORY; }
+    for (int t = 0; t < nt; t++) {
+        jobs[t] = (trans_job){ s, trans, t, nt, MT_OK, "" };
+        pthread_create(&th[t], NULL, trans_worker, &jobs[t]);
+    }
+    int st = MT_OK;
+    for (int t = 0; t < nt; t++) {
+        pthread_join(th[t], NULL);
+        if (jobs[t].st != MT_OK && st == MT_OK) {
+            st = jobs[t].st;
+            set_err(jobs[t].err);
+        }
+    }
+    free(th); free(jobs);
     return st;
 }
 
@@ -2162,7 +2202,7 @@ static int run_stem(sim_t *s, mt_output_multislice *out, int nthreads)
     double c_10_0 = (double)s->cond.c_10;
     for (int iconf = s->fp_iconf_0; iconf <= s->pn_nconf && st == MT_OK; iconf++) {
         move_atoms(s, iconf);
-        st = build_trans_stack(s, trans);
+        st = build_trans_stack(s, trans, nthreads);
         if (st != MT_OK) break;
         for (int it = 0; it < qt.n; it++) {
             lens_set_defocus(&s->cond, (real)(c_10_0 + qt.x[it]));

@@ -64,6 +64,13 @@ def _check(L, st):
         raise OracleError("%s: %s" % (capi.STATUS_NAMES.get(st, st), L.orc_last_error().decode()))
 
 
+def default_threads():
+    """Host threads for the oracle: the cores this process may use, at most 16
+    (the CPU share of one GPU's host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
 def run(params, precision="f64", nthreads=None):
     L = lib(precision)
     s, keep = params.to_struct()
@@ -72,7 +79,7 @@ def run(params, precision="f64", nthreads=None):
     _check(L, L.orc_output_dims(C.byref(sysc), C.byref(s), C.byref(d)))
     out = capi.Output(d)
     o = out.as_struct()
-    nt = nthreads if nthreads else (os.cpu_count() or 1)
+    nt = nthreads if nthreads else default_threads()
     _check(L, L.orc_run(C.byref(sysc), C.byref(s), C.byref(o), int(nt)))
     return out
 
