@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--n", type=int, default=512, help="grid size")
     ap.add_argument("--batch", type=int, default=0, help="waves per batch (0: planner)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--variants", default="", help="row,col,col_det launch variants to force (default: tuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ns", type=int, default=16)
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-size group measurement")
@@ -232,6 +233,8 @@ def main():
     from multem_amd import configs, engine
     p = configs.au_stem_512(ns=args.ns, nconf=args.nconf, n=args.n)
     engine.set_batch(args.batch, not args.no_graph)
+    if args.variants:
+        engine.set_pass_variants(*[int(v) for v in args.variants.split(",")])
     nsl = len(engine.spec_slicing(p))
 
     for _ in range(args.warmup):

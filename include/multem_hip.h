@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MT_ABI_VERSION 3
+#define MT_ABI_VERSION 4
 
 typedef enum {
     MT_OK = 0,
@@ -336,7 +336,7 @@ mt_status mt_hip_stream_copy_bandwidth(int device, size_t bytes, int reps, doubl
    epilogue) and the choice kept for the process (default); 2: timed on
    every run.  The choice is the lowest-numbered variant within 2 % of the
    fastest. */
-#define MT_PASS_VARIANTS 8
+#define MT_PASS_VARIANTS 11
 void mt_hip_set_autotune(int mode);
 
 /* Forces the variants of the row pass, the column pass and the last-slice

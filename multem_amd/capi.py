@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-MT_ABI_VERSION = 3
+MT_ABI_VERSION = 4
 
 MT_OK = 0
 STATUS_NAMES = {

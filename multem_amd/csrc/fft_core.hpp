@@ -95,20 +95,32 @@ __device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3)
     a3 = csub(d02, d13);
 }
 
-// W8^k for k = 1, 3 (forward) or their conjugates (inverse)
-template <bool INV, class C>
-__device__ __forceinline__ C w8_1(C a)
+// fused a + h*b with one rounding, written out so that every kernel instance
+// rounds alike whatever the compiler's contraction choices
+__device__ __forceinline__ float fmah(float b, float h, float a) { return __builtin_fmaf(b, h, a); }
+__device__ __forceinline__ double fmah(double b, double h, double a) { return __builtin_fma(b, h, a); }
+
+// e +/- W8^k d for k = 1 (K3 = false) or 3 (K3 = true), forward (W8 =
+// exp(-i pi/4)) or inverse (conjugate): W8^k d = h (sx, sy) with the sums
+// below, so each output component is one fused multiply-add.
+template <bool INV, bool K3, class C>
+__device__ __forceinline__ void w8_butterfly(C e, C d, C &plus, C &minus)
 {
-    // (1 -/+ i)/sqrt2 * a
-    const real_t<C> h = sqrt_half<C>();
-    return INV ? cmake((a.x - a.y) * h, (a.x + a.y) * h) : cmake((a.x + a.y) * h, (a.y - a.x) * h);
-}
-template <bool INV, class C>
-__device__ __forceinline__ C w8_3(C a)
-{
-    // (-1 -/+ i)/sqrt2 * a
-    const real_t<C> h = sqrt_half<C>();
-    return INV ? cmake((-a.x - a.y) * h, (a.x - a.y) * h) : cmake((a.y - a.x) * h, (-a.x - a.y) * h);
+#pragma clang fp contract(off)
+    using R = real_t<C>;
+    const R h = sqrt_half<C>();
+    R sx, sy;
+    if constexpr (!K3) {
+        // forward (1 - i)/sqrt2, inverse (1 + i)/sqrt2
+        sx = INV ? d.x - d.y : d.x + d.y;
+        sy = INV ? d.x + d.y : d.y - d.x;
+    } else {
+        // forward (-1 - i)/sqrt2, inverse (-1 + i)/sqrt2
+        sx = INV ? -d.x - d.y : d.y - d.x;
+        sy = INV ? d.x - d.y : -d.x - d.y;
+    }
+    plus = cmake(fmah(sx, h, e.x), fmah(sy, h, e.y));
+    minus = cmake(fmah(-sx, h, e.x), fmah(-sy, h, e.y));
 }
 
 // 8-point DFT of v[0], v[S], ..., v[7S], result in natural order
@@ -119,13 +131,11 @@ __device__ __forceinline__ void dft8(C (&v)[E], int o)
     C d0 = v[o + S], d1 = v[o + 3 * S], d2 = v[o + 5 * S], d3 = v[o + 7 * S];
     dft4<INV>(e0, e1, e2, e3);
     dft4<INV>(d0, d1, d2, d3);
-    d1 = w8_1<INV>(d1);
     d2 = mul_mi<INV>(d2);
-    d3 = w8_3<INV>(d3);
     v[o] = cadd(e0, d0);         v[o + 4 * S] = csub(e0, d0);
-    v[o + S] = cadd(e1, d1);     v[o + 5 * S] = csub(e1, d1);
+    w8_butterfly<INV, false>(e1, d1, v[o + S], v[o + 5 * S]);
     v[o + 2 * S] = cadd(e2, d2); v[o + 6 * S] = csub(e2, d2);
-    v[o + 3 * S] = cadd(e3, d3); v[o + 7 * S] = csub(e3, d3);
+    w8_butterfly<INV, true>(e3, d3, v[o + 3 * S], v[o + 7 * S]);
 }
 
 // 16-point DFT as 4 x 4 with internal twiddles W16^(q*kp)

@@ -13,6 +13,8 @@
 
 #include <type_traits>
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <utility>
 
 namespace mtg {
@@ -319,6 +321,119 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
     }
     if (flags & COL_IFFT) fft_run<S, true>(v, t, tw, lds, idx, sync);
     if (flags & COL_STORE) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * a.ncols] = v[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent slice steps: one workgroup per resident slot loops over the
+// tiles w = blockIdx.x, blockIdx.x + gridDim.x, ...  and issues the loads of
+// its next tile before transforming the current one, so HBM traffic of one
+// tile overlaps the FFT work of the previous one.  The arithmetic per tile is
+// that of k_row_pass / k_col_pass with the full slice-step flags, operation
+// for operation, so the waves are bit-identical to the variant with the same
+// schedule.  gridDim.x is a multiple of 8 (or the tile count), so tile w
+// runs on XCD w % 8 as in the one-tile-per-workgroup launch.
+template <class S, int RW, int TWT, class Z>
+__global__ __launch_bounds__(RW *S::T) void k_row_slice_pp(RowArgsT<Z> a)
+{
+    constexpr int N = S::N, E = S::E, T = S::T;
+    using TW = std::conditional_t<TWT != 0, TwTable<S, Z>, TwReg<S, Z>>;
+    constexpr int PADN = N + N / 8;
+    __shared__ Z lds[RW * PADN];
+    const int tid = threadIdx.x;
+    const int t = tid % T, r = tid / T;
+    const int G = a.nrows / RW, n_items = G * a.n_members;
+    Z *lp = lds + r * PADN;
+    auto idx = [](int i) { return i + (i >> 3); };
+    auto sync = [] { __syncthreads(); };
+    Z nv[E], ntv[E];
+    long long noff = 0;
+    // loads of tile `item` into nv (wave) and ntv (transmission rows)
+    auto prefetch = [&](int item) {
+        int rg, member;
+        map_rows(item, G, a.n_members, rg, member);
+        const int row = rg * RW + r;
+        noff = (long long)member * a.member_stride + (long long)row * N;
+        const Z *T_s = a.trans + (long long)a.members[member].tslot * a.tslot_stride +
+                       (long long)a.slice * a.nrows * N + (long long)row * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) nv[e] = a.src[noff + t + T * e];
+#pragma unroll
+        for (int e = 0; e < E; ++e) ntv[e] = T_s[t + T * e];
+    };
+
+    TW tw;
+    tw.load(t, a.tw);
+    int item = blockIdx.x;
+    if (item < n_items) prefetch(item);
+    // everything loaded so far has arrived: inside the loop the only loads in
+    // flight are the next tile's, and waits for them stay at the loop top
+    __builtin_amdgcn_s_waitcnt(0);
+    for (; item < n_items; item += gridDim.x) {
+        Z v[E], tv[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) { v[e] = nv[e]; tv[e] = ntv[e]; }
+        const long long off = noff;
+        if (item + (int)gridDim.x < n_items) prefetch(item + gridDim.x);
+        // keep the next tile's loads ahead of this tile's transforms
+        __builtin_amdgcn_sched_barrier(0);
+        fft_run<S, true>(v, t, tw, lp, idx, sync);
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cmul(tv[e], v[e]);
+        fft_run<S, false>(v, t, tw, lp, idx, sync);
+#pragma unroll
+        for (int e = 0; e < E; ++e) a.dst[off + t + T * e] = v[e];
+    }
+}
+
+template <class S, int C, int TWT, class Z>
+__global__ __launch_bounds__(C *S::T) void k_col_slice_pp(ColArgsT<Z> a)
+{
+    constexpr int N = S::N, E = S::E, T = S::T;
+    using TW = std::conditional_t<TWT != 0, TwTable<S, Z>, TwReg<S, Z>>;
+    __shared__ Z lds[N * C];
+    const int tid = threadIdx.x;
+    const int c = tid % C, t = tid / C;
+    const int ntile = a.ncols / C, n_items = ntile * a.n_members;
+    auto idx = [c](int i) { return i * C + c; };
+    auto sync = [] { __syncthreads(); };
+    Z nv[E], npy;
+    long long nbase = 0;
+    // loads of tile `item` into nv (wave columns) and npy (propagator factor of the column)
+    auto prefetch = [&](int item) {
+        const int col = (item % ntile) * C + c;
+        nbase = (long long)(item / ntile) * a.member_stride + col;
+#pragma unroll
+        for (int e = 0; e < E; ++e) nv[e] = a.src[nbase + (long long)(t + T * e) * a.ncols];
+        npy = a.py[col];
+    };
+
+    // the x factors of the propagator are the same for every tile of this thread
+    Z pxv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) pxv[e] = a.px[t + T * e];
+    TW tw;
+    tw.load(t, a.tw);
+    int item = blockIdx.x;
+    if (item < n_items) prefetch(item);
+    // everything loaded so far has arrived: inside the loop the only loads in
+    // flight are the next tile's, and waits for them stay at the loop top
+    __builtin_amdgcn_s_waitcnt(0);
+    for (; item < n_items; item += gridDim.x) {
+        Z v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = nv[e];
+        const Z pyv = cscale(npy, a.p_scale);
+        const long long base = nbase;
+        if (item + (int)gridDim.x < n_items) prefetch(item + gridDim.x);
+        // keep the next tile's loads ahead of this tile's transforms
+        __builtin_amdgcn_sched_barrier(0);
+        fft_run<S, false>(v, t, tw, lds, idx, sync);
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cmul(cmul(pxv[e], pyv), v[e]);
+        fft_run<S, true>(v, t, tw, lds, idx, sync);
 #pragma unroll
         for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * a.ncols] = v[e];
     }
@@ -733,6 +848,37 @@ constexpr int kColProbe = COL_IFFT | COL_STORE;
 
 template <class Z> constexpr bool kFp64 = std::is_same<Z, double2>::value;
 
+// Workgroups of a persistent launch: the resident slots of the current
+// device (occupancy of `kernel` at `threads` per workgroup times the CU
+// count), rounded down to a multiple of 8 so that a workgroup stays on one
+// XCD's tiles, and at most the tile count.  Cached per kernel and device.
+int persistent_grid(const void *kernel, int threads, int tiles)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, int> slots;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return tiles;
+    int n = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = slots.find({kernel, dev});
+        if (it != slots.end()) {
+            n = it->second;
+        } else {
+            int per_cu = 0, cus = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+                (void)hipGetLastError();
+                return tiles;
+            }
+            n = std::max(per_cu, 1) * std::max(cus, 1);
+            n = n >= 8 ? n / 8 * 8 : n;
+            slots[{kernel, dev}] = n;
+        }
+    }
+    return std::min(n, tiles);
+}
+
 } // namespace
 
 bool row_variants_apply(int flags) { return flags == kRowSlice; }
@@ -752,7 +898,17 @@ hipError_t row_go(const RowArgsT<Z> &a, hipStream_t s)
     if (a.nrows % pv.tile) return hipErrorInvalidValue;
     const int grid = (a.nrows / pv.tile) * a.n_members;
     constexpr int twt = kFp64<Z> ? 1 : pv.tw_table;
-    hipLaunchKernelGGL((k_row_pass<S, pv.tile, F, twt, Z>), dim3(grid), dim3(pv.tile * S::T), 0, s, a);
+    if constexpr (pv.persist != 0) {
+        if constexpr (F != kRowSlice) {
+            return hipErrorInvalidValue;
+        } else {
+            auto k = k_row_slice_pp<S, pv.tile, twt, Z>;
+            hipLaunchKernelGGL(k, dim3(persistent_grid((const void *)k, pv.tile * S::T, grid)),
+                               dim3(pv.tile * S::T), 0, s, a);
+        }
+    } else {
+        hipLaunchKernelGGL((k_row_pass<S, pv.tile, F, twt, Z>), dim3(grid), dim3(pv.tile * S::T), 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -795,13 +951,21 @@ hipError_t col_go(const ColArgsT<Z> &a, hipStream_t s)
         constexpr PassVariant pv = col_variant(N, V);
         using S = Sched<N, pv.sched>;
         static_assert(S::E == sched_elems(N, pv.sched) && pv.tile * S::T <= 1024, "column variant");
-        if constexpr (DET && pv.tile != col_tile0(N)) {
+        if constexpr (DET && (pv.tile != col_tile0(N) || pv.persist)) {
             return hipErrorInvalidValue;   // the detector partials assume the default width
+        } else if constexpr (pv.persist != 0 && (F != kColSlice || PROBE)) {
+            return hipErrorInvalidValue;   // persistent workgroups run the full slice step only
         } else {
             if (a.ncols % pv.tile) return hipErrorInvalidValue;
             const int grid = (a.ncols / pv.tile) * a.n_members;
-            hipLaunchKernelGGL((k_col_pass<S, pv.tile, F, PROBE, DET, pv.tw_table, Z>), dim3(grid),
-                               dim3(pv.tile * S::T), 0, s, a);
+            if constexpr (pv.persist != 0) {
+                auto k = k_col_slice_pp<S, pv.tile, pv.tw_table, Z>;
+                hipLaunchKernelGGL(k, dim3(persistent_grid((const void *)k, pv.tile * S::T, grid)),
+                                   dim3(pv.tile * S::T), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((k_col_pass<S, pv.tile, F, PROBE, DET, pv.tw_table, Z>), dim3(grid),
+                                   dim3(pv.tile * S::T), 0, s, a);
+            }
             return hipGetLastError();
         }
     }

@@ -230,8 +230,10 @@ struct PassVariant {
     int sched;      // Sched<N, sched>
     int tile;       // rows (row pass) or columns (column pass) per workgroup
     int tw_table;   // 1: twiddles read from the table at each use
+    int persist;    // 1: one workgroup per resident slot, looping over tiles and
+                    //    loading the next tile while transforming the current one
 };
-constexpr int kPassVariants = 8;
+constexpr int kPassVariants = 11;
 
 // elements per thread of schedule sc at length n (Sched<n, sc>::E)
 constexpr int sched_elems(int n, int sc)
@@ -247,42 +249,51 @@ constexpr int col_tile0_fp64(int n) { return n >= 2048 ? 4 : n >= 1024 ? 8 : 16;
 // Row pass, v: 0 default; 1 table twiddles; 2 twice the rows; 3 alternative
 // schedule at the same thread count; 4 = 3 with table twiddles; 5 half the
 // rows; 6 = 2 with table twiddles; 7 alternative schedule, twice the rows
-// (capped at 1024 threads).
+// (capped at 1024 threads); 8, 9: variants 0 and 3 as persistent
+// workgroups that prefetch the next tile; 10: variant 2 persistent.
 constexpr PassVariant row_variant(int n, int v)
 {
     const int r = row_tile0(n), alt = r * sched_elems(n, 1) / sched_elems(n, 0);
     const int t1 = n / sched_elems(n, 1);
     switch (v) {
-    case 7: return PassVariant{1, 2 * alt * t1 > 1024 ? 1024 / t1 : 2 * alt, 0};
-    case 1: return PassVariant{0, r, 1};
-    case 2: return PassVariant{0, 2 * r, 0};
-    case 3: return PassVariant{1, alt, 0};
-    case 4: return PassVariant{1, alt, 1};
-    case 5: return PassVariant{0, r > 1 ? r / 2 : 1, 0};
-    case 6: return PassVariant{0, 2 * r, 1};
-    default: return PassVariant{0, r, 0};
+    case 10: return PassVariant{0, 2 * r, 0, 1};
+    case 9: return PassVariant{1, alt, 0, 1};
+    case 8: return PassVariant{0, r, 0, 1};
+    case 7: return PassVariant{1, 2 * alt * t1 > 1024 ? 1024 / t1 : 2 * alt, 0, 0};
+    case 1: return PassVariant{0, r, 1, 0};
+    case 2: return PassVariant{0, 2 * r, 0, 0};
+    case 3: return PassVariant{1, alt, 0, 0};
+    case 4: return PassVariant{1, alt, 1, 0};
+    case 5: return PassVariant{0, r > 1 ? r / 2 : 1, 0, 0};
+    case 6: return PassVariant{0, 2 * r, 1, 0};
+    default: return PassVariant{0, r, 0, 0};
     }
 }
 // Column pass, v: 0 default; 1 table twiddles; 2 half the columns;
 // 3 alternative schedule (columns capped at 1024 threads per workgroup);
 // 4 = 3 with table twiddles; 5 a quarter of the columns; 6 = 2 with table
 // twiddles; 7 alternative schedule, twice the columns (capped at 1024
-// threads: 256-byte row segments at 512^2).  Detector epilogues use only
-// variants with the default width, which fixes the layout of the detector
-// partials.
+// threads: 256-byte row segments at 512^2); 8, 9: variants 0 and 3 as
+// persistent workgroups that prefetch the next tile; 10: variant 2
+// persistent.  Detector epilogues
+// use only non-persistent variants with the default width, which fixes the
+// layout of the detector partials.
 constexpr PassVariant col_variant(int n, int v)
 {
     const int c = col_tile0(n), t1 = n / sched_elems(n, 1);
     const int c1 = c * t1 > 1024 ? 1024 / t1 : c;
     switch (v) {
-    case 7: return PassVariant{1, 2 * c * t1 > 1024 ? 1024 / t1 : 2 * c, 0};
-    case 1: return PassVariant{0, c, 1};
-    case 2: return PassVariant{0, c / 2, 0};
-    case 3: return PassVariant{1, c1, 0};
-    case 4: return PassVariant{1, c1, 1};
-    case 5: return PassVariant{0, c / 4, 0};
-    case 6: return PassVariant{0, c / 2, 1};
-    default: return PassVariant{0, c, 0};
+    case 10: return PassVariant{0, c / 2, 0, 1};
+    case 9: return PassVariant{1, c1, 0, 1};
+    case 8: return PassVariant{0, c, 0, 1};
+    case 7: return PassVariant{1, 2 * c * t1 > 1024 ? 1024 / t1 : 2 * c, 0, 0};
+    case 1: return PassVariant{0, c, 1, 0};
+    case 2: return PassVariant{0, c / 2, 0, 0};
+    case 3: return PassVariant{1, c1, 0, 0};
+    case 4: return PassVariant{1, c1, 1, 0};
+    case 5: return PassVariant{0, c / 4, 0, 0};
+    case 6: return PassVariant{0, c / 2, 1, 0};
+    default: return PassVariant{0, c, 0, 0};
     }
 }
 

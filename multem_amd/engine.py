@@ -183,7 +183,7 @@ def set_batch(batch=0, use_graph=True):
     lib().mt_hip_set_batch(int(batch), int(bool(use_graph)))
 
 
-PASS_VARIANTS = 8
+PASS_VARIANTS = 11
 
 
 def stream_copy_bandwidth(nbytes=1 << 30, reps=10, device=0):

@@ -51,6 +51,29 @@ def test_slicing_matches_oracle(maker):
         assert abs(x[0] - y[0]) < 1e-6 and abs(x[1] - y[1]) < 1e-6
 
 
+def test_fp32_slicing_quirk_srtio3():
+    """SrTiO3 [001], 24x24x50 cells: atomic planes at z = 78.1 A and 156.2 A
+    (40 and 80 times a/2) sit on edges of the plane finder's 0.1 A histogram
+    (Identify_Planes, src/types.cuh:1166-1372).  The reference's FP32 path
+    bins z in float and places the slice boundaries around those planes
+    differently from its FP64 path; the engine's FP32 slicing is the FP32
+    oracle's, and differs from the FP64 slicing at exactly slices 40-42 and
+    80-82.  The 6x6x10-cell specimen slices alike in both precisions."""
+    p = configs.sto_stem(n=1024, ncells=24, nc=50, ns=2)
+    s_eng = engine.spec_slicing(p)
+    s32, s64 = oracle.spec_slicing(p, precision="f32"), oracle.spec_slicing(p, precision="f64")
+    assert len(s_eng) == len(s32) == len(s64) == 101
+    for a, b in zip(s_eng, s32):
+        assert a[4:] == b[4:] and abs(a[0] - b[0]) < 1e-4 and abs(a[1] - b[1]) < 1e-4
+    differ = [k for k, (a, b) in enumerate(zip(s32, s64)) if any(abs(x - y) > 1e-4 for x, y in zip(a, b))]
+    assert differ == [40, 41, 42, 80, 81, 82]
+    # the FP32 slice 40 is 0.39 A thinner and slice 42 as much thicker
+    assert abs((s32[40][1] - s32[40][0]) - (s64[40][1] - s64[40][0]) + 0.39) < 0.01
+    q = configs.sto_stem(n=256, ncells=6, nc=10, ns=2)
+    for a, b in zip(oracle.spec_slicing(q, precision="f32"), oracle.spec_slicing(q, precision="f64")):
+        assert a[4:] == b[4:] and abs(a[0] - b[0]) < 1e-4
+
+
 @pytest.mark.parametrize("maker", [
     configs.si_ewrs_256,
     lambda: configs.au_stem_512(ns=8, nconf=2),

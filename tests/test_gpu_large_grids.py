@@ -9,7 +9,16 @@ with the default radix schedule are bit-identical to variant 0.
 
 Workloads:
   SrTiO3 [001] 1024^2, 24x24 cells, 50 cells thick (101 slices): the 1024^2
-      member of the uneven-size group, alone and inside a group;
+      member of the uneven-size group, alone and inside a group.  Its atomic
+      planes at z = 78.1 A and 156.2 A (40 and 80 times a/2) sit on edges of
+      the plane finder's 0.1 A histogram, and the reference's FP32 path,
+      which bins z in float (Identify_Planes, src/types.cuh:1166-1372, with
+      Input_Multislice<float>), places the slice boundaries around them
+      differently from its FP64 path: slices 40-42 and 80-82 change
+      thickness, and the FP32 image differs from the FP64 one by 6e-3.  The
+      engine follows the FP32 path, so this member is held to the FP32
+      oracle (same slicing); the slicing difference is asserted by the CPU
+      test test_boundary.py::test_fp32_slicing_quirk_srtio3;
   Au [001] 2048^2, 16x16 cells, 2 cells thick (5 slices);
   Au [001] 512^2, 8x8x20 cells (41 slices), 11x11 scan, 2 configurations,
       batch 120 and launch variants 3/3/3 -- the specimen, grid, batch and
@@ -28,7 +37,7 @@ from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SAME_SCHEDULE = (0, 1, 2, 5, 6)
+SAME_SCHEDULE = (0, 1, 2, 5, 6, 8, 10)
 
 
 def _forced(p, row, col, col_det, batch=0):
@@ -48,7 +57,7 @@ def _sweep(p, ref, label):
     base, _ = _forced(p, 0, 0, 0)
     base = base.image_tot.copy()
     e0 = rel_l2(base, ref)
-    print("%s variant 0: rel L2 vs FP64 %.3e" % (label, e0))
+    print("%s variant 0: rel L2 vs oracle %.3e" % (label, e0))
     assert e0 < 1e-4
     ran = [0]
     for v in range(1, engine.PASS_VARIANTS):
@@ -59,7 +68,7 @@ def _sweep(p, ref, label):
             out, plan = _forced(p, v, v, 0)
         img = out.image_tot.copy()
         e64 = rel_l2(img, ref)
-        print("%s %s: rel L2 vs FP64 %.3e" % (label, plan.split(" batch=")[1], e64))
+        print("%s %s: rel L2 vs oracle %.3e" % (label, plan.split(" batch=")[1], e64))
         assert e64 < 1e-4, v
         if v in SAME_SCHEDULE:
             assert np.array_equal(img, base), v
@@ -73,15 +82,17 @@ def _sto1024(ns=2):
 
 @pytest.fixture(scope="module")
 def sto1024_ref():
+    """The FP32 oracle's image (the reference's FP32 slicing) of the SrTiO3
+    1024^2 member."""
     p = _sto1024()
-    return p, oracle.run(p, "f64").image_tot
+    return p, oracle.run(p, "f32").image_tot
 
 
 def test_sto_1024_tuned_and_variant_sweep(sto1024_ref):
     p, ref = sto1024_ref
     out = engine.run_multislice(p)               # the tuner's choice
     e = rel_l2(out.image_tot, ref)
-    print("SrTiO3 1024^2 (101 slices), tuned %s: rel L2 vs FP64 %.3e" % (engine.last_plan(), e))
+    print("SrTiO3 1024^2 (101 slices), tuned %s: rel L2 vs FP32 oracle %.3e" % (engine.last_plan(), e))
     assert e < 1e-4
     _sweep(p, ref, "SrTiO3 1024^2")
 
@@ -104,7 +115,7 @@ def test_sto_1024_inside_group(sto1024_ref):
         assert plan.count("stem fp32 nx=") == 3 and "nx=1024" in plan
         assert np.array_equal(grp[2].image_tot, solo)
         e = rel_l2(grp[2].image_tot, ref)
-        print("group (row/col variant %d) 1024^2 member: rel L2 vs FP64 %.3e" % (v, e))
+        print("group (row/col variant %d) 1024^2 member: rel L2 vs FP32 oracle %.3e" % (v, e))
         assert e < 1e-4
     e256 = rel_l2(grp[0].image_tot, oracle.run(ps[0], "f64").image_tot)
     print("group 256^2 member: rel L2 vs FP64 %.3e" % e256)
