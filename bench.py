@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--ns", type=int, default=64, help="scan points per side")
     ap.add_argument("--nconf", type=int, default=16)
     ap.add_argument("--n", type=int, default=512, help="grid size")
+    ap.add_argument("--sto", action="store_true",
+                    help="SrTiO3 [001] workload of the uneven-size group at grid --n (256: 6x6x10 cells, "
+                         "512: 12x12x25, 1024: 24x24x50) instead of Au[001] (for profiling)")
     ap.add_argument("--batch", type=int, default=0, help="waves per batch (0: planner)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--variants", default="", help="row,col,col_det launch variants to force (default: tuner)")
@@ -208,11 +211,11 @@ def copy_calibration():
                       "stream_kernel_read_plus_write_GBs": engine.stream_copy_bandwidth(n, reps)}))
 
 
-def traffic_from_profiles(kernel, n, batch):
+def traffic_from_profiles(kernel, workload, n, batch):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration), if a summary has been recorded:
     the recorded bytes per wave times this run's batch width."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s_au_stem_%d.txt" % (kernel, n))
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s_%d.txt" % (kernel, workload, n))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -231,7 +234,11 @@ def main():
               "path shards)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     from multem_amd import configs, engine
-    p = configs.au_stem_512(ns=args.ns, nconf=args.nconf, n=args.n)
+    if args.sto:
+        cells = {256: (6, 10), 512: (12, 25), 1024: (24, 50)}[args.n]
+        p = configs.sto_stem(n=args.n, ncells=cells[0], nc=cells[1], ns=args.ns, nconf=args.nconf)
+    else:
+        p = configs.au_stem_512(ns=args.ns, nconf=args.nconf, n=args.n)
     engine.set_batch(args.batch, not args.no_graph)
     if args.variants:
         engine.set_pass_variants(*[int(v) for v in args.variants.split(",")])
@@ -286,8 +293,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic (crystal built in-process, seeded frozen phonons)",
         "config": {
-            "workload": "au_stem_512: Au[001] a=4.078 8x8x20 cells, %d^2 grid, %d slices, 300 kV 21.4 mrad probe, "
-                        "%dx%d scan, %d phonon configs, ADF 80-200 mrad" % (args.n, nsl, args.ns, args.ns, args.nconf),
+            "workload": ("sto_stem: SrTiO3[001] %d^2 grid, %d slices, 300 kV 25 mrad probe, %dx%d scan, %d phonon "
+                         "configs, ADF 80-200 mrad" % (args.n, nsl, args.ns, args.ns, args.nconf)) if args.sto else
+                        ("au_stem_512: Au[001] a=4.078 8x8x20 cells, %d^2 grid, %d slices, 300 kV 21.4 mrad probe, "
+                         "%dx%d scan, %d phonon configs, ADF 80-200 mrad" % (args.n, nsl, args.ns, args.ns, args.nconf)),
             "probe_slices_per_step": ps_step,
             "batch": B,
             "plan": plan,
@@ -300,14 +309,19 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": traffic_from_profiles(kern, args.n, B),
+            "traffic": traffic_from_profiles(kern, "sto_stem" if args.sto else "au_stem", args.n, B),
             "bytes_alg_per_launch": kb,
             "mean_launch_ms": kms,
             "row_pass_ms": row_ms,
             "col_pass_ms": col_ms,
             "slice_loop_alg_GBs": slice_loop_gbs,
+            # the achievable rate on this GPU: a streaming copy kernel (16-byte
+            # nontemporal loads/stores, 1 GiB, read + write)
+            "copy_ceiling_GBs": engine.stream_copy_bandwidth(1 << 30, 10),
         },
     }
+    out["roofline"]["frac_of_copy_ceiling"] = (achieved / out["roofline"]["copy_ceiling_GBs"]
+                                               if achieved else None)
     if not args.no_group:
         out["uneven_group"] = group_measurement()
     if not args.no_fp64:

@@ -100,7 +100,7 @@ typedef struct {
     const double *thick;
     int n_thick;
 
-    int potential_slicing;     /* 1 planes, 2 dz_Proj, 3 dz_Sub, 4 auto */
+    int potential_slicing;     /* 1 planes, 2 dz_Proj, 3 dz_Sub (4 auto: refused) */
 
     int nx, ny;
     int bwl;
@@ -235,9 +235,14 @@ mt_status mt_hip_run_multislice_group(const mt_system_conf *sys, const mt_input_
 
 /* ---- per-piece entries (same input struct) --------------------------- */
 
-/* Slices of the (undisplaced) specimen.  Replaces ilc_spec_slicing
+/* Slices of the specimen.  Replaces ilc_spec_slicing
    (mex_files_multem/ilc_spec_slicing.cpp:98-135).  *n_slice receives the
-   count; slices (capacity max_slice, may be NULL) receives the slices. */
+   count; slices (capacity max_slice, may be NULL) receives the slices.  The
+   boundaries are the undisplaced specimen's; with potential_slicing = 3 and
+   z displacements (pn_dim 111, 101, 011, 001) the atom ranges are those of
+   configuration iconf (>= 1) into its z-sorted atoms, else iconf is
+   ignored.  z_int_0 .. z_int_e is the z range whose atoms contribute to the
+   slice (wider than z_0 .. z_e with potential_slicing = 3). */
 mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multislice *in,
                               int iconf, mt_slice *slices, int max_slice, int *n_slice);
 
@@ -322,9 +327,10 @@ void mt_hip_set_batch(int batch, int use_graph);
    launch counts; ms_tf is the transmission-function build time. */
 void mt_hip_kernel_timing(double *ms_row, int *n_row, double *ms_col, int *n_col, double *ms_tf);
 
-/* Read + write bandwidth (GB/s) of a plain streaming copy kernel (16-byte
-   accesses) over `bytes` bytes, `reps` launches: the achievable-bandwidth
-   reference for the slice passes on this GPU. */
+/* Read + write bandwidth (GB/s) of a streaming copy kernel (one 16-byte
+   element per thread, nontemporal loads and stores) over `bytes` bytes
+   (a multiple of 16), `reps` launches: the achievable-bandwidth reference
+   for the slice passes on this GPU. */
 mt_status mt_hip_stream_copy_bandwidth(int device, size_t bytes, int reps, double *gbs);
 
 /* Autotuner of the slice step.  Each pass has MT_PASS_VARIANTS launch

@@ -147,15 +147,24 @@ mt_status mt_hip_spec_slicing(const mt_system_conf *sys, const mt_input_multisli
                               int max_slice, int *n_slice)
 {
     if (!n_slice) return fail(MT_ERR_INVALID_ARGUMENT, "null n_slice");
-    (void)iconf;   // plane slicing without z displacement: identical for every configuration
     if (!sys || !in) return fail(MT_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lock(g_engine_mutex);
     auto fill = [&](const auto &m) {
-        *n_slice = m.n_slice();
-        for (int i = 0; slices && i < m.n_slice() && i < max_slice; ++i) {
-            const auto &s = m.slices[i];
+        // the boundaries are the same for every configuration; with z
+        // displacements (sub-slicing) the atom ranges are configuration iconf's,
+        // into its z-sorted atoms
+        using Rm = typename std::decay_t<decltype(m)>::real;
+        auto sl = m.slices;
+        if (m.subslicing && iconf >= 1) {
+            std::vector<int> order;
+            std::vector<Rm> x, y, z;
+            m.config_atoms(iconf, order, x, y, z, sl);
+        }
+        *n_slice = (int)sl.size();
+        for (int i = 0; slices && i < (int)sl.size() && i < max_slice; ++i) {
+            const auto &s = sl[i];
             slices[i].z_0 = s.z_0; slices[i].z_e = s.z_e;
-            slices[i].z_int_0 = s.z_0; slices[i].z_int_e = s.z_e;
+            slices[i].z_int_0 = s.z_int_0; slices[i].z_int_e = s.z_int_e;
             slices[i].iatom_0 = s.iatom_0; slices[i].iatom_e = s.iatom_e;
             slices[i].ithk = s.ithk;
         }

@@ -143,6 +143,32 @@ std::vector<Z> twiddles(int n)
     return w;
 }
 
+// One configuration's atoms on the device: x, y, occ, type in the
+// configuration's z order, and with sub-slicing the configuration's atom
+// range per slice and the spline coefficients of every (slice, atom) pair.
+// Captured graphs hold these buffers' addresses: once `fixed` (Setup::reserve,
+// sized for every configuration of the run) they are never reallocated.
+template <class R>
+struct DevAtoms {
+    DBuf<vec4_t<R>> a;
+    DBuf<R> z;
+    DBuf<int2> ranges, pairs;
+    DBuf<int> pair_base;
+    DBuf<R> coef;
+    bool fixed = false;
+    size_t size() const { return a.n; }
+};
+
+template <class T>
+void upload_grow(DBuf<T> &b, const std::vector<T> &h, hipStream_t s, bool fixed = false)
+{
+    if (b.n < h.size()) {
+        if (fixed) throw Error{MT_ERR_DEVICE, "reserved atom buffers too small for a configuration"};
+        b.alloc(h.size());
+    }
+    if (!h.empty()) b.upload(h.data(), h.size(), s);
+}
+
 // Device-resident data shared by all entries of one model; R is the real
 // type of the wave path (float or double).
 template <class R>
@@ -158,6 +184,8 @@ struct Setup {
     DBuf<Z> rmax;
     DBuf<int2> ranges;
     std::vector<int2> h_ranges;
+    DBuf<R> sub_slice_z;                             // sub-slicing: slice bounds
+    DBuf<double> sub_par, sub_qx, sub_qw;            // sub-slicing: element terms, rule
 
     Setup(const ModelT<R> &mm, hipStream_t st) : m(mm), s(st), nx(mm.nx), ny(mm.ny), nxy((size_t)mm.nx * mm.ny)
     {
@@ -183,6 +211,107 @@ struct Setup {
         rmax.alloc(nt); rmax.upload(rm.data(), nt, s);
         for (const auto &sl : m.slices) h_ranges.push_back(make_int2(sl.iatom_0, sl.iatom_e));
         ranges.alloc(h_ranges.size()); ranges.upload(h_ranges.data(), h_ranges.size(), s);
+        if (m.subslicing) {
+            std::vector<R> sz;
+            std::vector<double> par((size_t)nt * 12), qx, qw;
+            for (const auto &sl : m.slices) { sz.push_back(sl.z_0); sz.push_back(sl.z_e); }
+            for (int i = 0; i < nt; ++i) {
+                // the element's terms and taper in double, as its projected
+                // table is built (get_atom_coef, src/atomic_data.hpp:1610-1665)
+                const AtomTableT<R> &t = m.tables[i];
+                for (int j = 0; j < 5; ++j) {
+                    par[(size_t)i * 12 + j] = t.vr_a[j];
+                    par[(size_t)i * 12 + 5 + j] = t.vr_k[j];
+                }
+                par[(size_t)i * 12 + 10] = t.R_tap * t.R_tap;
+                par[(size_t)i * 12 + 11] = t.tap_cf;
+            }
+            m.subslice_rule(qx, qw);
+            upload_grow(sub_slice_z, sz, s);
+            upload_grow(sub_par, par, s);
+            upload_grow(sub_qx, qx, s);
+            upload_grow(sub_qw, qw, s);
+        }
+    }
+
+    // Allocates d for every configuration of the run, before any graph is
+    // captured on it: the atoms, and with sub-slicing the ranges and the
+    // pairs of the configuration that has the most.
+    void reserve(DevAtoms<R> &d) const
+    {
+        const size_t na = m.atoms_u.x.size(), ns = (size_t)m.n_slice();
+        d.a.alloc(na);
+        if (m.subslicing) {
+            size_t np = 0;
+            auto count = [&](const std::vector<SliceInfoT<R>> &sl) {
+                size_t c = 0;
+                for (const auto &x : sl) c += (size_t)std::max(0, x.iatom_e - x.iatom_0 + 1);
+                np = std::max(np, c);
+            };
+            if (m.frozen() && m.pn_z) {
+                std::vector<int> order;
+                std::vector<R> x, y, z;
+                std::vector<SliceInfoT<R>> sl;
+                for (int ic = m.fp_iconf_0; ic <= m.pn_nconf; ++ic) {
+                    m.config_atoms(ic, order, x, y, z, sl);
+                    count(sl);
+                }
+            } else {
+                count(m.slices);
+            }
+            d.z.alloc(na);
+            d.ranges.alloc(ns);
+            d.pair_base.alloc(ns);
+            d.pairs.alloc(std::max<size_t>(np, 1));
+            d.coef.alloc(std::max<size_t>(np, 1) * 512);
+        }
+        d.fixed = true;
+    }
+
+    // Configuration iconf's atoms onto the device (with sub-slicing: in the
+    // configuration's z order, with its atom ranges and the coefficients of
+    // every (slice, atom) pair, computed on stream st).
+    void load(int iconf, DevAtoms<R> &d, hipStream_t st = nullptr)
+    {
+        if (!st) st = s;
+        if (!m.subslicing) {
+            upload_grow(d.a, atoms_of(iconf), st, d.fixed);
+            return;
+        }
+        std::vector<int> order;
+        std::vector<R> x, y, z;
+        std::vector<SliceInfoT<R>> sl;
+        m.config_atoms(iconf, order, x, y, z, sl);
+        std::vector<V4> a(order.size());
+        for (size_t i = 0; i < order.size(); ++i) {
+            const int ty = m.type_of_atom[order[i]];
+            R tyr;
+            if constexpr (sizeof(R) == 4) std::memcpy(&tyr, &ty, sizeof tyr);
+            else tyr = (R)ty;
+            a[i].x = x[i]; a[i].y = y[i]; a[i].z = (R)m.atoms_u.occ[order[i]]; a[i].w = tyr;
+        }
+        std::vector<int2> rg, pairs;
+        std::vector<int> base;
+        for (size_t is = 0; is < sl.size(); ++is) {
+            rg.push_back(make_int2(sl[is].iatom_0, sl[is].iatom_e));
+            base.push_back((int)pairs.size());
+            for (int ia = sl[is].iatom_0; ia <= sl[is].iatom_e; ++ia) pairs.push_back(make_int2((int)is, ia));
+        }
+        upload_grow(d.a, a, st, d.fixed);
+        upload_grow(d.z, z, st, d.fixed);
+        upload_grow(d.ranges, rg, st, d.fixed);
+        upload_grow(d.pair_base, base, st, d.fixed);
+        upload_grow(d.pairs, pairs, st, d.fixed);
+        if (d.coef.n < pairs.size() * 512) {
+            if (d.fixed) throw Error{MT_ERR_DEVICE, "reserved sub-slice coefficients too small for a configuration"};
+            d.coef.alloc(pairs.size() * 512);
+        }
+        SubArgsT<R> sa{};
+        sa.n_pairs = (int)pairs.size();
+        sa.pairs = d.pairs.p; sa.atoms = d.a.p; sa.z = d.z.p;
+        sa.slice_z = sub_slice_z.p; sa.tables = tables.p; sa.par = sub_par.p;
+        sa.qx = sub_qx.p; sa.qw = sub_qw.p; sa.coef = d.coef.p;
+        check(launch_subslice_coef(sa, st), "sub-slice coefficients");
     }
 
     // atoms (x, y, occ, type) of configuration iconf: the type is the bit
@@ -205,13 +334,15 @@ struct Setup {
     // Transmission stack of one configuration into trans (n_slice * nxy),
     // optionally the potential V (natural layout) of the same slices; on
     // stream st (default: the setup stream).
-    void build_tf(const DBuf<V4> &atoms, int slice0, int n_slices, Z *trans, R *V, hipStream_t st = nullptr)
+    void build_tf(const DevAtoms<R> &atoms, int slice0, int n_slices, Z *trans, R *V, hipStream_t st = nullptr)
     {
         TfArgsT<R> a{};
         a.nx = nx; a.ny = ny; a.nxh = m.nxh; a.nyh = m.nyh;
         a.dRx = m.dRx; a.dRy = m.dRy;
-        a.atoms = atoms.p;
-        a.slice_range = ranges.p;
+        a.atoms = atoms.a.p;
+        a.slice_range = m.subslicing ? atoms.ranges.p : ranges.p;
+        a.sub_coef = m.subslicing ? atoms.coef.p : nullptr;
+        a.pair_base = m.subslicing ? atoms.pair_base.p : nullptr;
         a.slice0 = slice0; a.n_slices = n_slices;
         a.tables = tables.p; a.rmax = rmax.p; a.n_types = (int)m.tables.size();
         a.w = m.Vr_factor;
@@ -503,7 +634,7 @@ struct SliceStore {
 
     // what a configuration needs before its waves run: its transmission stack
     // (into configuration slot j), or its stored potentials
-    void build(const DBuf<V4> &atoms, int j, hipStream_t st = nullptr)
+    void build(const DevAtoms<R> &atoms, int j, hipStream_t st = nullptr)
     {
         if (resident) S.build_tf(atoms, 0, n_slice, trans.p + (size_t)j * n_slice * S.nxy, nullptr, st);
         else if (n_pot > 0) S.build_tf(atoms, 0, n_pot, nullptr, V.p, st);
@@ -521,7 +652,7 @@ struct SliceStore {
     // bind(), after enqueueing (rebuilt mode) the slice's transmission into
     // the slot on st, ordered before the row pass on the same stream
     template <class RA>
-    void for_slice(const DBuf<V4> &atoms, int is, RA &r, hipStream_t st)
+    void for_slice(const DevAtoms<R> &atoms, int is, RA &r, hipStream_t st)
     {
         if (!resident) {
             if (is < n_pot) S.tf_from_pot(V.p + (size_t)is * S.nxy, 1, trans.p, st);
@@ -735,7 +866,7 @@ public:
     DBuf<double2> coh;              // pn_coh_contrib: sum over configurations of w psi(g), [scan][n_thk][nxy]
     DBuf<Z> fpsi0;                  // iw_type 3: spectrum of the user's incident wave
     DBuf<R> det_fR;                 // matrix detectors
-    DBuf<vec4_t<R>> atoms;
+    DevAtoms<R> atoms;
     ProbeParamsT<R> P{};
     unsigned int nap = 0;
     R w = 0;
@@ -822,7 +953,9 @@ public:
             S.cond_defocus_points(qt, tc10, twq);
             n_t = (int)tc10.size();
         }
-        atoms.alloc(m.atoms_u.x.size());
+        // the captured pipeline reads the atom buffers (rebuilt slices): their
+        // addresses are fixed before any capture
+        S.reserve(atoms);
         hm.resize(B);
         hb.resize(B);
 
@@ -960,8 +1093,7 @@ public:
     {
         const int iconf = m.fp_iconf_0 + k / (n_t * nbatch), it = (k / nbatch) % n_t, ib = k % nbatch;
         if (iconf != cur_conf) {
-            auto a = S.atoms_of(iconf);
-            atoms.upload(a.data(), a.size(), s);
+            S.load(iconf, atoms, s);
             Event t0, t1;
             record(t0, s);
             store.build(atoms, 0);
@@ -1455,15 +1587,15 @@ void run_ew(const ModelT<R> &m, mt_output_multislice *out)
     run_control().iter = 0;
     Event t0, t1;
     record(t0, s);
-    DBuf<vec4_t<R>> atoms(m.atoms_u.x.size());
+    DevAtoms<R> atoms;
+    S.reserve(atoms);
     PassChoice choice;
     bool tuned = false;
     for (int c0 = 0; c0 < nconf; c0 += K) {
         stop_check();
         const int nb = std::min(K, nconf - c0);   // configurations of this group
         for (int j = 0; j < nb; ++j) {
-            auto a = S.atoms_of(m.fp_iconf_0 + c0 + j);
-            atoms.upload(a.data(), a.size(), s);
+            S.load(m.fp_iconf_0 + c0 + j, atoms, s);
             store.build(atoms, j, s);
             // the atoms buffer is reused by the next configuration (in the rebuilt
             // mode only after this one's waves: K = 1)
@@ -1693,9 +1825,8 @@ void projected_potential(const ModelT<R> &m, int device, int iconf, int islice, 
     if (islice < 0 || islice >= m.n_slice()) throw Error{MT_ERR_INVALID_ARGUMENT, "islice out of range"};
     Stream st;
     Setup<R> S(m, st.s);
-    auto a = S.atoms_of(iconf);
-    DBuf<vec4_t<R>> atoms(a.size());
-    atoms.upload(a.data(), a.size(), st.s);
+    DevAtoms<R> atoms;
+    S.load(iconf, atoms, st.s);
     DBuf<R> dv(S.nxy);
     S.build_tf(atoms, islice, 1, nullptr, dv.p);
     std::vector<R> h(S.nxy);
@@ -1732,9 +1863,8 @@ void projected_potential_sum(const ModelT<R> &m, int device, int iconf, double *
     tm = Timing{};
     Stream st;
     Setup<R> S(m, st.s);
-    auto a = S.atoms_of(iconf);
-    DBuf<vec4_t<R>> atoms(a.size());
-    atoms.upload(a.data(), a.size(), st.s);
+    DevAtoms<R> atoms;
+    S.load(iconf, atoms, st.s);
     DBuf<R> dv((size_t)m.n_slice() * S.nxy);
     DBuf<double> sum(S.nxy);
     Event t0, t1;
@@ -1766,9 +1896,8 @@ void transmission_function(const ModelT<R> &m, int device, int iconf, int islice
     if (islice < 0 || islice >= m.n_slice()) throw Error{MT_ERR_INVALID_ARGUMENT, "islice out of range"};
     Stream st;
     Setup<R> S(m, st.s);
-    auto a = S.atoms_of(iconf);
-    DBuf<vec4_t<R>> atoms(a.size());
-    atoms.upload(a.data(), a.size(), st.s);
+    DevAtoms<R> atoms;
+    S.load(iconf, atoms, st.s);
     DBuf<cplx_t<R>> t(S.nxy);
     S.build_tf(atoms, islice, 1, t.p, nullptr);
     download_unshifted(S, t.p, trans);

@@ -185,6 +185,7 @@ AtomTableT<T> build_atom_table(int Z, double dR_min, double Vrl, int nR)
     }
     tb.R_max_f = (T)R_max;
     tb.R2_max_f = tb.R_max_f * tb.R_max_f;
+    for (int j = 0; j < 5; ++j) { tb.vr_a[j] = vr.a[j]; tb.vr_k[j] = vr.k[j]; }
     return tb;
 }
 
@@ -361,13 +362,31 @@ R spacing(const std::vector<R> &p, int i, R dz)
 }
 
 template <class R>
-std::vector<R> slice_bounds(const std::vector<R> &p, R dz, R z_int_min, R z_int_max)
+std::vector<R> slice_bounds(const std::vector<R> &p, R dz, R z_int_min, R z_int_max, bool sub = false)
 {
-    if (p.size() == 1) return {z_int_min, z_int_max};
+    if (p.size() == 1 && !sub) return {z_int_min, z_int_max};
     std::vector<R> zs(p.size() + 1);
     zs[0] = p[0] - R(0.5) * spacing(p, 0, dz);
     for (size_t i = 1; i + 1 < zs.size(); ++i) zs[i] = R(0.5) * (p[i] + p[i - 1]);
     zs.back() = zs[zs.size() - 2] + spacing(p, (int)p.size(), dz);
+    if (sub) {
+        // dz_Sub: slices of the neighbouring spacing out to the interaction
+        // range on both sides (Slicing::get_z_slice, src/slicing.hpp:220-247)
+        R dz_b = spacing(p, 1, dz);
+        if (z_int_min < zs.front() - dz_b) {
+            std::vector<R> top = even_planes(z_int_min, zs.front() - dz_b, spacing(p, 2, dz));
+            zs.insert(zs.begin(), top.begin(), top.end());
+        } else {
+            zs.front() = z_int_min;
+        }
+        dz_b = spacing(p, (int)p.size() - 1, dz);
+        if (zs.back() + dz_b < z_int_max) {
+            std::vector<R> bottom = even_planes(zs.back() + dz_b, z_int_max, spacing(p, (int)p.size() - 2, dz));
+            zs.insert(zs.end(), bottom.begin(), bottom.end());
+        } else {
+            zs.back() = z_int_max;
+        }
+    }
     return zs;
 }
 
@@ -387,14 +406,9 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     if (in.potential_type != 6) fail(MT_ERR_UNSUPPORTED, "only potential_type 6 (Lobato)");
     if (in.pn_model == 2) fail(MT_ERR_UNSUPPORTED, "absorptive potential model");
     if (in.pn_model < 1 || in.pn_model > 3) fail(MT_ERR_INVALID_ARGUMENT, "pn_model must be 1..3");
-    // dz_Sub: the reference's sub-slice spline coefficients are never computed
-    // (get_cubic_poly_coef_Vz is commented out at src/projected_potential.cuh:118,148),
-    // so its sub-sliced potential is zero; Auto has no implementation at all
-    if (in.potential_slicing == 3)
-        fail(MT_ERR_UNSUPPORTED, "potential_slicing 3 (dz_Sub): the reference leaves the sub-slice potential "
-                                 "unevaluated (zero); use 1 (planes) or 2 (dz_Proj)");
-    if (!(in.potential_slicing == 1 || in.potential_slicing == 2))
-        fail(MT_ERR_UNSUPPORTED, "potential_slicing must be 1 (planes) or 2 (dz_Proj)");
+    // Auto (4) has no implementation in the reference (only the enum, src/types.cuh:237)
+    if (!(in.potential_slicing >= 1 && in.potential_slicing <= 3))
+        fail(MT_ERR_UNSUPPORTED, "potential_slicing must be 1 (planes), 2 (dz_Proj) or 3 (dz_Sub)");
     // exit waves are coherent; CBED is coherent unless fully integrated
     // (Input_Multislice::validate_parameters, src/input_multislice.cuh:672-682)
     illumination_model = in.illumination_model;
@@ -610,7 +624,9 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     pn_nconf = frozen() ? std::max(1, pn_nconf) : 1;
     fp_iconf_0 = frozen() ? (pn_single_conf ? pn_nconf : 1) : 1;
     if (is_ewfs() || is_ewrs()) pn_coh_contrib = 1;
-    pn_z = false;
+    // z displacements only with sub-slicing (src/input_multislice.cuh:697-705)
+    subslicing = multislice() && in.potential_slicing == 3;
+    if (!subslicing) pn_z = false;
 
     auto zminmax = [](const SpecimenT<R> &s, R &lo, R &hi) {
         auto mm = std::minmax_element(s.z.begin(), s.z.end());
@@ -662,6 +678,8 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
         }
         zminmax(in_atoms, zmin, zmax);   // get_statistic after the rotation
     }
+    // sub-slicing keeps only the whole-specimen exit (src/input_multislice.cuh:697-700)
+    if (subslicing && thick_type == 2) thick_type = 1;
     // the phase object's whole specimen is sliced by dz (src/input_multislice.cuh:724-728)
     if (!multislice() && thick_type == 1) pot_sli = 2;
     if (thick_type == 1) {
@@ -673,7 +691,7 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
         std::sort(zc.begin(), zc.end());
         std::vector<R> zp = (pot_sli == 1) ? find_planes(zc) : even_planes(zc.front(), zc.back(), spec_dz);
         unique_sorted(zp);
-        if (thick_type == 3) zp = slice_bounds(zp, spec_dz, zmin - R(2.5), zmax + R(2.5));
+        if (thick_type == 3) zp = slice_bounds(zp, spec_dz, zmin - R(2.5), zmax + R(2.5), subslicing);
         std::vector<R> req;
         for (int i = 0; i < in.n_thick; ++i) req.push_back((R)in.thick[i]);
         // match_vectors (src/cpu_fcns.hpp:3613-3654): each request snaps to the
@@ -763,27 +781,34 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
     if (interaction_model == 2 || s_z_int < R(2.0) * spec_dz || (z_plane.size() == 1 && pot_sli == 1)) {
         interaction_model = 2;
         if (thick_type == 3) thick_type = 2;
+        subslicing = false;
+        pn_z = false;
     }
 
     // slices and thicknesses (Slicing::calculate, src/slicing.hpp:93-100 / 253-403)
     // the phase object's dz is the specimen's interaction depth (src/spec.hpp:67)
     const R dz_sl = interaction_model == 2 ? s_z_int : spec_dz;
-    std::vector<R> zs = slice_bounds(z_plane, dz_sl, uzmin - R_int_max, uzmax + R_int_max);
+    std::vector<R> zs = slice_bounds(z_plane, dz_sl, uzmin - R_int_max, uzmax + R_int_max, subslicing);
     const std::vector<R> &az = atoms_u.z;
     auto lower = [&](R v) { return (int)(std::lower_bound(az.begin(), az.end(), v) - az.begin()); };
     for (size_t is = 0; is + 1 < zs.size(); ++is) {
         SliceInfoT<R> s{zs[is], zs[is + 1], 1, 0, -1};
-        R z0 = s.z_0, ze = s.z_e;
-        if (!(z0 > ze || az.back() < z0 || ze < az.front())) {
-            int i0 = (z0 <= az.front()) ? 0 : lower(z0);
-            int ie = (az.back() <= ze) ? (int)az.size() - 1 : lower(ze) - 1;
-            if (!(i0 > ie || az[ie] < z0 || ze < az[i0])) { s.iatom_0 = i0; s.iatom_e = ie; }
+        s.z_int_0 = s.z_0;
+        s.z_int_e = s.z_e;
+        if (subslicing) {
+            // atoms within R_int_max of the slice centre (get_slicing, src/slicing.hpp:376-386)
+            const R z_m = R(0.5) * (s.z_0 + s.z_e);
+            s.z_int_0 = std::fmin(z_m - R_int_max, s.z_0);
+            s.z_int_e = std::fmax(z_m + R_int_max, s.z_e);
         }
+        atoms_in(az, s.z_int_0, s.z_int_e, subslicing, s.iatom_0, s.iatom_e);
         slices.push_back(s);
     }
     for (R t : thick) {
         ThickInfoT<R> ti{t, 0, R(0.0)};
-        if (thick_type == 3) {
+        if (subslicing && thick_type == 1) {
+            ti.islice = (int)zs.size() - 2;   // whole specimen: the last slice (src/slicing.hpp:297-298)
+        } else if (thick_type == 3) {
             for (size_t i = 0; i + 1 < zs.size(); ++i) if (std::fabs(zs[i + 1] - t) < R(1e-4)) { ti.islice = (int)i; break; }
         } else {
             for (size_t i = 0; i + 1 < zs.size(); ++i) if (zs[i] < t && t <= zs[i + 1]) { ti.islice = (int)i; break; }
@@ -813,6 +838,8 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
             s.iatom_0 = k == 0 ? 0 : thk[k - 1].iatom_e + 1;
             s.iatom_e = thk[k].iatom_e;
             s.ithk = (int)k;
+            s.z_int_0 = s.z_0;
+            s.z_int_e = s.z_e;
             slices.push_back(s);
             thk[k].islice = (int)k;
         }
@@ -865,6 +892,75 @@ void ModelT<R>::displaced(int iconf, std::vector<R> &x, std::vector<R> &y) const
         R dy = pn_y ? s * ny_(gy) : R(0.0);
         x[i] = atoms_u.x[i] + dx;
         y[i] = atoms_u.y[i] + dy;
+    }
+}
+
+template <class R>
+void ModelT<R>::atoms_in(const std::vector<R> &z, R z_0, R z_e, bool inc, int &i0, int &ie)
+{
+    const R eps = sizeof(R) == 4 ? R(1e-4) : R(1e-8);   // Epsilon<T>::rel
+    if (inc) { z_0 -= eps; z_e += eps; }
+    i0 = 1; ie = 0;
+    if (z.empty() || z_0 > z_e || z.back() < z_0 || z_e < z.front()) return;
+    auto lower = [&](R v) { return (int)(std::lower_bound(z.begin(), z.end(), v) - z.begin()); };
+    const int a = (z_0 <= z.front()) ? 0 : lower(z_0);
+    const int b = (z.back() <= z_e) ? (int)z.size() - 1 : lower(z_e) - 1;
+    if (!(a > b || z[b] < z_0 || z_e < z[a])) { i0 = a; ie = b; }
+}
+
+template <class R>
+void ModelT<R>::displaced(int iconf, std::vector<R> &x, std::vector<R> &y, std::vector<R> &z) const
+{
+    displaced(iconf, x, y);
+    z = atoms_u.z;
+    if (!frozen() || !pn_z) return;
+    // the third seed of the configuration drives z (Randn_3d, src/cgpu_rand.cuh:322-345)
+    std::mt19937_64 gen_u((uint64_t)(int64_t)pn_seed);
+    std::uniform_int_distribution<int> rand_u(1, INT_MAX);
+    unsigned int sz = 0;
+    for (int ic = 0; ic < std::max(1, iconf); ++ic) {
+        (void)rand_u(gen_u);
+        (void)rand_u(gen_u);
+        sz = (unsigned int)rand_u(gen_u);
+    }
+    std::mt19937_64 gz(sz);
+    std::normal_distribution<R> nz_(R(0), R(1));
+    for (size_t i = 0; i < z.size(); ++i) z[i] = atoms_u.z[i] + (R)atoms_u.sigma[i] * nz_(gz);
+}
+
+template <class R>
+void ModelT<R>::config_atoms(int iconf, std::vector<int> &order, std::vector<R> &x, std::vector<R> &y,
+                             std::vector<R> &z, std::vector<SliceInfo> &sl) const
+{
+    std::vector<R> dx, dy, dz;
+    displaced(iconf, dx, dy, dz);
+    order.resize(dz.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    // z displacements re-sort the atoms (Atom_Data::sort_by_z, stable)
+    if (frozen() && pn_z) std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return dz[a] < dz[b]; });
+    x.resize(order.size()); y.resize(order.size()); z.resize(order.size());
+    for (size_t i = 0; i < order.size(); ++i) { x[i] = dx[order[i]]; y[i] = dy[order[i]]; z[i] = dz[order[i]]; }
+    sl = slices;
+    if (subslicing && frozen() && pn_z)
+        for (SliceInfo &s : sl) atoms_in(z, s.z_int_0, s.z_int_e, true, s.iatom_0, s.iatom_e);
+}
+
+template <class R>
+void ModelT<R>::subslice_rule(std::vector<double> &x, std::vector<double> &w) const
+{
+    // nw_TanhSinh on [-1 + eps, 1 - eps], eps = Epsilon<float>::eps, limits in R
+    const int n = 128;
+    const double kHalfPi = 1.570796326794896619231;
+    const float eps = 10.0f * 1.19209290e-07f;
+    const double x_min = (double)(R)(-1.0 + eps), x_max = (double)(R)(1.0 - eps);
+    auto aprox = [&](double v) { return std::asinh(std::atanh(v) / kHalfPi); };
+    const double t_min = aprox(x_min), t_max = aprox(x_max);
+    const double h = (t_max - t_min) / (n - 1);
+    x.resize(n); w.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const double ti = t_min + i * h, tti = kHalfPi * std::sinh(ti);
+        x[i] = std::tanh(tti);
+        w[i] = h * kHalfPi * std::cosh(ti) / std::pow(std::cosh(tti), 2);
     }
 }
 

@@ -27,6 +27,10 @@ struct AtomTableT {
     double R_min = 0, R_max = 0, R_tap = 0, tap_cf = 0;   // as built (double)
     R R_max_f = 0, R2_max_f = 0;
     std::vector<R> R2, c0, c1, c2, c3;                     // 128 each
+    // 3D potential Vr(r) = sum_i a_i exp(-k_i r) (2/(k_i r) + 1) (Lobato terms,
+    // set_Lobato_neutral_coef, src/atomic_data.hpp:1440-1485): integrated over
+    // each sub-slice's z range for potential_slicing = 3
+    double vr_a[5] = {0, 0, 0, 0, 0}, vr_k[5] = {1, 1, 1, 1, 1};
 };
 
 template <class R>
@@ -68,6 +72,9 @@ struct SliceInfoT {
     R z_0, z_e;
     int iatom_0, iatom_e;   // inclusive range into the z-sorted atoms; empty if iatom_0 > iatom_e
     int ithk;
+    // z range whose atoms contribute to the slice: [z_0, z_e], widened by the
+    // largest cut-off radius around the slice centre for potential_slicing = 3
+    R z_int_0 = 0, z_int_e = 0;
 };
 
 template <class R>
@@ -205,14 +212,37 @@ public:
     // Displaced positions of configuration iconf (Spec::move_atoms, src/spec.hpp:85-127);
     // returns x, y (z unchanged for plane slicing).
     void displaced(int iconf, std::vector<R> &x, std::vector<R> &y) const;
+    // x, y and z of configuration iconf (z displaced only when sub-slicing with
+    // pn_dim's z digit set), in atoms_u order
+    void displaced(int iconf, std::vector<R> &x, std::vector<R> &y, std::vector<R> &z) const;
+
+    // potential_slicing = 3 (dz_Sub) on a multislice run: each slice's
+    // potential is the z integral over the slice of every atom within the
+    // largest cut-off radius of it (set_atom_Vp with linear_Vz / cubic_poly_coef,
+    // src/projected_potential.cuh:226-284, src/gpu_fcns.cuh:175-285)
+    bool subslicing = false;
+    // Configuration iconf's atoms for the sub-sliced potential: `order` lists
+    // atoms_u indices sorted by the displaced z (stable), x/y/z are the
+    // displaced positions in that order, and `sl` the slices with the atom
+    // ranges of that order (Spec::move_atoms re-sorts and re-slices,
+    // src/spec.hpp:119-126).  Slice boundaries are the undisplaced specimen's.
+    void config_atoms(int iconf, std::vector<int> &order, std::vector<R> &x, std::vector<R> &y, std::vector<R> &z,
+                      std::vector<SliceInfo> &sl) const;
+    // tanh-sinh rule on [-1, 1] with 128 nodes (Quadrature type 0, src/quadrature.hpp:134-151):
+    // limits in R, nodes and weights in double
+    void subslice_rule(std::vector<double> &x, std::vector<double> &w) const;
 
     // output dims
     void output_dims(mt_output_multislice *out) const;
     void fill_axes(mt_output_multislice *out) const;
 
+    R R_int_max = R(2.5);          // largest cut-off radius of the elements present
+
 private:
-    R R_int_max = R(2.5);
     void slice_specimen();
+    // fd_by_z over [z_0, z_e] of sorted z (src/slicing.hpp:428-452); Inc_Borders
+    // widens the range by the precision's relative epsilon
+    static void atoms_in(const std::vector<R> &z, R z_0, R z_e, bool inc, int &i0, int &ie);
 };
 
 using Model = ModelT<float>;      // FP32 path

@@ -510,6 +510,10 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
             const A4 at = a.atoms[list[li]];
             const int ty_ = atom_type(at);
             const R *tb = tab + ty_ * 5 * 128;
+            // the knots are the element's; the coefficients c0..c3 the
+            // element's, or those of this atom's sub-slice
+            const R *cf = a.sub_coef ? a.sub_coef + (size_t)(a.pair_base[is] + (list[li] - rng.x)) * 512
+                                     : tb + 128;
             const R R_max = rmx[ty_].x, R2_max = rmx[ty_].y;
             const int ix0 = (int)floor((at.x - R_max) / a.dRx), ixe = (int)ceil((at.x + R_max) / a.dRx);
             const int iy0 = (int)floor((at.y - R_max) / a.dRy), iye = (int)ceil((at.y + R_max) / a.dRy);
@@ -533,8 +537,8 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
                                 if (R2 < tb[im]) ie = im; else i0 = im;
                             }
                             const R dx = R2 - tb[i0];
-                            const R p = ((tb[4 * 128 + i0] * dx + tb[3 * 128 + i0]) * dx + tb[2 * 128 + i0]) * dx
-                                            + tb[1 * 128 + i0];
+                            const R p = ((cf[3 * 128 + i0] * dx + cf[2 * 128 + i0]) * dx + cf[1 * 128 + i0]) * dx
+                                            + cf[i0];
                             vacc[h] = vacc[h] + at.z * p;
                         }
                     }
@@ -563,6 +567,91 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
             a.trans[sl * nxy + (long long)shift_idx(px, a.nxh) * a.ny + shift_idx(py, a.nyh)] = cmake(c, s);
         }
     }
+}
+
+// Lobato 3D potential and its radial derivative over r, times f
+// (Vr_dVrir_Lobato_neutral_0_12 with Vr_dVrir's factor, src/cgpu_fcns.cuh:1460-1475,
+// 3026-3064): Vr = sum a_i e^(-k_i r) (2/(k_i r) + 1),
+// dVrir = -(Vr + sum a_i e^(-k_i r) (k_i r + 1)) / r^2.
+__device__ __forceinline__ void vr_dvrir(double z, double R2, const double *P, double f, double &Vr, double &dVrir)
+{
+    const double r = sqrt(z * z + R2);
+    double e[5], kr[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        kr[i] = P[5 + i] * r;
+        e[i] = P[i] * exp(-kr[i]);
+    }
+    Vr = e[0] * (2.0 / kr[0] + 1.0) + e[1] * (2.0 / kr[1] + 1.0) + e[2] * (2.0 / kr[2] + 1.0) +
+         e[3] * (2.0 / kr[3] + 1.0) + e[4] * (2.0 / kr[4] + 1.0);
+    dVrir = -(Vr + e[0] * (kr[0] + 1.0) + e[1] * (kr[1] + 1.0) + e[2] * (kr[2] + 1.0) + e[3] * (kr[3] + 1.0) +
+              e[4] * (kr[4] + 1.0)) / (r * r);
+    Vr *= f;
+    dVrir *= f;
+}
+
+// One workgroup per (slice, atom) pair, one thread per knot R2_k of the
+// atom's element (linear_Vz + cubic_poly_coef, src/gpu_fcns.cuh:175-285):
+//   V(R2_k)  = integral over z in [z_0, z_e] - z_atom of Vr(sqrt(z^2 + R2_k)),
+//   dV(R2_k) = d V / d R^2,
+// on the 128-node rule mapped to the range (split at the atom when the
+// slice contains it), nodes summed in order; then the cosine taper beyond
+// R2_tap and the cubic Hermite coefficients in R^2 between knots.  All in
+// FP64, the coefficients cast to R at the end (the reference's FP32 path
+// evaluates these in float, and its Hermite coefficients, differences of
+// neighbouring float values, then lose three to four digits).
+template <class R>
+__global__ __launch_bounds__(128) void k_subslice_coef(SubArgsT<R> a)
+{
+    __shared__ double c0s[128], c1s[128];
+    const int p = blockIdx.x, k = threadIdx.x;
+    const int2 pr = a.pairs[p];
+    const int ty = atom_type(a.atoms[pr.y]);
+    const double za = (double)a.z[pr.y];
+    const double z0h = 0.5 * ((double)a.slice_z[2 * pr.x] - za), zeh = 0.5 * ((double)a.slice_z[2 * pr.x + 1] - za);
+    const bool split = (z0h < 0.0) && (0.0 < zeh);
+    const double *P = a.par + ty * 12;
+    const double R2 = (double)a.tables[ty * 640 + k];
+    const double a1 = split ? -z0h : zeh - z0h, b1 = split ? z0h : zeh + z0h;
+    double V = 0.0, dV = 0.0;
+    for (int i = 0; i < 128; ++i) {
+        const double x = a.qx[i], w = a.qw[i];
+        double v, d;
+        vr_dvrir(a1 * x + b1, R2, P, a1 * w, v, d);
+        if (split) {
+            double v2, d2;
+            vr_dvrir(zeh * x + zeh, R2, P, zeh * w, v2, d2);
+            v += v2;
+            d += d2;
+        }
+        V += v;
+        dV += d;
+    }
+    dV = 0.5 * dV;
+    // apply_tapering (src/cgpu_fcns.cuh:1528-1539)
+    const double R2_tap = P[10], tap_cf = P[11];
+    if (R2_tap < R2) {
+        double sn, cs;
+        sincos(tap_cf * (R2 - R2_tap), &sn, &cs);
+        dV = dV * cs - tap_cf * V * sn;
+        V *= cs;
+    }
+    c0s[k] = V;
+    c1s[k] = dV;
+    __syncthreads();
+    R *out = a.coef + (size_t)p * 512;
+    out[k] = (R)V;
+    out[128 + k] = (R)dV;
+    double c2 = 0.0, c3 = 0.0;
+    if (k < 127) {
+        // cubic_poly_coef (src/cgpu_fcns.cuh:1544-1555)
+        const double idR = 1.0 / ((double)a.tables[ty * 640 + k + 1] - R2);
+        const double m = (c0s[k + 1] - V) * idR, n = dV + c1s[k + 1];
+        c2 = (3.0 * m - n - dV) * idR;
+        c3 = (n - 2.0 * m) * idR * idR;
+    }
+    out[256 + k] = (R)c2;
+    out[384 + k] = (R)c3;
 }
 
 // Transmission of stored potentials: T = exp(i w V) (or 1 + i w V), V in the
@@ -805,11 +894,16 @@ __global__ void k_slice_sum(const R *V, int n_slices, long long nxy, double *out
     out[i] = s;
 }
 
-// streaming copy, 16 B per thread per iteration (HBM bandwidth reference)
-__global__ __launch_bounds__(256) void k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n)
+// Streaming copy, the HBM bandwidth reference: one 16-byte element per
+// thread and one workgroup per 4 KiB, nontemporal loads and stores (the
+// fastest of the copy forms measured on MI355X: plain one-shot, several
+// elements per thread, grid-stride loops).
+typedef float stream_f4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(const stream_f4 *__restrict__ a, stream_f4 *__restrict__ b,
+                                                     long long n)
 {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
 }
 
 __global__ void k_m2_from_psi(const double *psi_full, int nx, int ny, int ix_0, int iy_0, int nx_o, int ny_o, double *m2)
@@ -1044,6 +1138,14 @@ hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s)
 }
 
 template <class R>
+hipError_t launch_subslice_coef(const SubArgsT<R> &a, hipStream_t s)
+{
+    if (a.n_pairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_subslice_coef<R>), dim3(a.n_pairs), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class R>
 hipError_t launch_trans_from_pot(const R *V, int n_slices, int nx, int ny, R w, int weak_phase, cplx_t<R> *trans,
                                  hipStream_t s)
 {
@@ -1140,8 +1242,9 @@ hipError_t launch_stream_copy(const void *a, void *b, size_t bytes, hipStream_t 
 {
     if (bytes % 16) return hipErrorInvalidValue;
     const long long n = (long long)(bytes / 16);
-    const long long blocks = std::min<long long>((n + 255) / 256, 256 * 64);
-    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)a, (float4 *)b, n);
+    const long long blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const stream_f4 *)a, (stream_f4 *)b, n);
     return hipGetLastError();
 }
 
@@ -1158,6 +1261,7 @@ hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_
     template hipError_t launch_row_pass<Z>(int, const RowArgsT<Z> &, hipStream_t);                                     \
     template hipError_t launch_col_pass<Z>(int, const ColArgsT<Z> &, bool, bool, hipStream_t);                         \
     template hipError_t launch_tf_build<R>(const TfArgsT<R> &, hipStream_t);                                           \
+    template hipError_t launch_subslice_coef<R>(const SubArgsT<R> &, hipStream_t);                                     \
     template hipError_t launch_trans_from_pot<R>(const R *, int, int, int, R, int, Z *, hipStream_t);                  \
     template hipError_t launch_prop_tables<Z>(int, int, const double *, int, double, int, double, double, double,      \
                                               double, Z *, Z *, hipStream_t);                                          \

@@ -142,8 +142,31 @@ struct TfArgsT {
     int weak_phase;               // transmission 1 + i w V instead of exp(i w V)
     cplx_t<R> *trans;             // out, shifted layout: [slice][nx*ny] (relative to slice0), may be null
     R *V;                         // out, natural layout: [slice][nx*ny], may be null
+    // sub-slicing: c0..c3 of each (slice, atom) pair, [pair][4][128], the
+    // pairs of slice s starting at pair_base[s] in atom order; null otherwise
+    const R *sub_coef;
+    const int *pair_base;
 };
 using TfArgs = TfArgsT<float>;
+
+// Sub-slice spline coefficients of (slice, atom) pairs: the slice's z range
+// integrated through the atom's 3D potential at each of its element's 128
+// knots in R^2, tapered, with cubic Hermite coefficients; computed in FP64
+// and cast to R, as the element tables are built in double and cast.
+template <class R>
+struct SubArgsT {
+    int n_pairs;
+    const int2 *pairs;            // (slice, atom) of each pair
+    const vec4_t<R> *atoms;       // x, y, occ, type of the configuration's atoms
+    const R *z;                   // z of the configuration's atoms
+    const R *slice_z;             // [slice][2]: z_0, z_e
+    const R *tables;              // [type][5][128]: the knots R2 first
+    const double *par;            // [type][12]: Vr terms a[5], k[5], R2_tap, tap_cf
+    const double *qx, *qw;        // 128-node rule on [-1, 1]
+    R *coef;                      // out: [pair][4][128]
+};
+template <class R>
+hipError_t launch_subslice_coef(const SubArgsT<R> &a, hipStream_t s);
 
 enum OutMode : int {
     OUT_SET_PSI = 1,       // psi_coh = O                     (single configuration)

@@ -278,6 +278,7 @@ typedef struct {
     int Z;
     double R_min_d, R_max_d;  /* as built (double) */
     real R2_max, R2_tap, tap_cf, R_max;
+    double R2_tap_d, tap_cf_d;   /* taper as built (double) */
     real R2[c_nR], c0[c_nR], c1[c_nR], c2[c_nR], c3[c_nR];
     real R[c_nR];
     int valid;
@@ -339,6 +340,8 @@ static void build_atom_tab(int Z, double Vrl, int nR, double R_min_i, atom_tab *
     t->R2_max = rsquare((real)R_max);
     t->R2_tap = rsquare((real)R_tap);
     t->tap_cf = (real)tap_cf;
+    t->R2_tap_d = R2_tap;
+    t->tap_cf_d = tap_cf;
     for (int i = 0; i < nR; i++) {
         t->R[i] = (real)R[i];
         t->R2[i] = (real)R2[i];
@@ -674,6 +677,10 @@ typedef struct {
     int n_thk; thick_t *thk;
 } sim_t;
 
+/* sub-slicing: dz_Sub on a multislice run (Input_Multislice::is_subslicing,
+   src/input_multislice.cuh:119-122) */
+static int is_subslicing(const sim_t *s) { return s->interaction_model == 1 && s->potential_slicing == 3; }
+
 static void atoms_alloc(atoms_t *a, int n)
 {
     memset(a, 0, sizeof *a);
@@ -912,6 +919,33 @@ static real *get_z_slice(int pot_sli, const real *zp, int np, const atoms_t *a, 
     zs[0] = zp[0] - 0.5 * get_spacing(0, zp, np, a->dz);
     for (int iz = 1; iz < n - 1; iz++) zs[iz] = 0.5 * (zp[iz] + zp[iz - 1]);
     zs[n - 1] = zs[n - 2] + get_spacing(np, zp, np, a->dz);
+    if (pot_sli == 3) {
+        /* dz_Sub: slices of the neighbouring spacing out to the interaction
+           range (src/slicing.hpp:220-247) */
+        real dz_b = get_spacing(1, zp, np, a->dz);
+        if (a->z_int_min < zs[0] - dz_b) {
+            int nt;
+            real *top = identify_planes_range(a->z_int_min, zs[0] - dz_b, get_spacing(2, zp, np, a->dz), &nt);
+            real *w = malloc(sizeof(real) * (size_t)(n + nt));
+            for (int i = 0; i < nt; i++) w[i] = top[i];
+            for (int i = 0; i < n; i++) w[nt + i] = zs[i];
+            free(zs); free(top);
+            zs = w; n += nt;
+        } else {
+            zs[0] = a->z_int_min;
+        }
+        dz_b = get_spacing(np - 1, zp, np, a->dz);
+        if (zs[n - 1] + dz_b < a->z_int_max) {
+            int nb;
+            real *bot = identify_planes_range(zs[n - 1] + dz_b, a->z_int_max, get_spacing(np - 2, zp, np, a->dz), &nb);
+            zs = realloc(zs, sizeof(real) * (size_t)(n + nb));
+            for (int i = 0; i < nb; i++) zs[n + i] = bot[i];
+            free(bot);
+            n += nb;
+        } else {
+            zs[n - 1] = a->z_int_max;
+        }
+    }
     *nz = n;
     return zs;
 }
@@ -970,7 +1004,9 @@ static void get_thick(sim_t *s, const atoms_t *ar)
         thick_t *t = &s->thk[ik];
         t->z = s->thick[ik];
         int islice = 0;
-        if (s->thick_type == 3) {
+        if (is_subslicing(s) && s->thick_type == 1) {
+            islice = nzs - 2;   /* whole specimen: the last slice (src/slicing.hpp:297-298) */
+        } else if (s->thick_type == 3) {
             for (int i = 0; i < nzs - 1; i++) if (fabs(zs[i + 1] - t->z) < 1e-4f) { islice = i; break; }
         } else {
             for (int i = 0; i < nzs - 1; i++) if ((zs[i] < t->z) && (t->z <= zs[i + 1])) { islice = i; break; }
@@ -1021,7 +1057,15 @@ static void get_slicing(sim_t *s, const atoms_t *a)
         sl->z_e = s->z_slice[is + 1];
         sl->z_int_0 = sl->z_0;
         sl->z_int_e = sl->z_e;
-        fd_by_z_range(a->z, a->n, sl->z_int_0, sl->z_int_e, &sl->iatom_0, &sl->iatom_e, 0);
+        int inc = 0;
+        if (s->potential_slicing == 3) {
+            /* atoms within R_int_max of the slice centre (src/slicing.hpp:376-386) */
+            real z_m = 0.5 * (sl->z_0 + sl->z_e);
+            sl->z_int_0 = fmin(z_m - s->atoms_u.R_int_max, sl->z_0);
+            sl->z_int_e = fmax(z_m + s->atoms_u.R_int_max, sl->z_e);
+            inc = 1;
+        }
+        fd_by_z_range(a->z, a->n, sl->z_int_0, sl->z_int_e, &sl->iatom_0, &sl->iatom_e, inc);
         sl->ithk = -1;
     }
     for (int ik = 0; ik < s->n_thk; ik++) s->slice[s->thk[ik].islice].ithk = ik;
@@ -1031,7 +1075,11 @@ static void get_slicing(sim_t *s, const atoms_t *a)
 static void slicing_calculate(sim_t *s)
 {
     free(s->z_slice);
-    s->z_slice = get_z_slice(s->potential_slicing, s->z_plane, s->n_plane, &s->atoms, &s->n_zslice);
+    /* with sub-slicing the boundaries are the undisplaced specimen's for every
+       configuration (the reference re-derives the outermost ones from the
+       displaced extremes, src/slicing.hpp:220-247) */
+    s->z_slice = get_z_slice(s->potential_slicing, s->z_plane, s->n_plane,
+                             is_subslicing(s) ? &s->atoms_u : &s->atoms, &s->n_zslice);
     get_thick(s, &s->atoms_u);
     get_slicing(s, &s->atoms);
 }
@@ -1444,8 +1492,8 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
         for (int i = 0; i < in->n_thick; i++) s->thick[i] = (real)in->thick[i];
     }
     s->potential_slicing = in->potential_slicing;
-    if (!(s->potential_slicing == 1 || s->potential_slicing == 2))
-        FAIL(MT_ERR_UNSUPPORTED, "potential_slicing 3/4 (sub-slicing)");
+    if (!(s->potential_slicing >= 1 && s->potential_slicing <= 3))
+        FAIL(MT_ERR_UNSUPPORTED, "potential_slicing 4 (auto: not implemented in the reference)");
 
     /* Grid_2d::set_input_data (src/types.cuh:2154-2178), pbc_xy = true */
     if (in->nx <= 0 || in->ny <= 0) FAIL(MT_ERR_INVALID_ARGUMENT, "nx, ny must be positive");
@@ -1578,7 +1626,10 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
         else if (s->tsi == 3) L->ti_npts = 1;
         lens_clamp_spreads(L);
     }
-    s->pn_dim_z = 0;  /* not sub-slicing */
+    /* z displacements only with sub-slicing; sub-slicing keeps only the
+       whole-specimen exit (src/input_multislice.cuh:697-705) */
+    if (!is_subslicing(s)) s->pn_dim_z = 0;
+    if (is_subslicing(s) && s->thick_type == 2) s->thick_type = 1;
     /* (weak) phase object: slicing by planes, through-slices becomes
        through-thickness, whole specimen slices by dz (src/input_multislice.cuh:686-728) */
     if (s->interaction_model != 1) {
@@ -1694,6 +1745,7 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
             s->interaction_model = 2;
             s->g.dz = s->atoms_u.s_z_int;
             s->atoms_u.dz = s->g.dz;
+            s->pn_dim_z = 0;
             if (s->thick_type == 3) s->thick_type = 2;
         }
         atoms_alloc(&s->atoms, s->atoms_u.n);
@@ -1728,6 +1780,7 @@ static void move_atoms(sim_t *s, int iconf)
             s->atoms.z[i] = s->atoms_u.z[i] + dzv;
         }
     }
+    if (fp && s->pn_dim_z) atoms_sort_by_z(&s->atoms);   /* src/spec.hpp:119-122 */
     atoms_statistic(s, &s->atoms, 1);
     slicing_calculate(s);
 }
@@ -1747,6 +1800,112 @@ static void get_bn(real R, real dR, real R_max, int *i0, int *n)
     *n = (iR_0 == iR_e) ? 0 : iR_e - iR_0 + 1;
 }
 
+/* Quadrature type 0 of the reference: tanh-sinh on [-1 + eps, 1 - eps],
+   eps = Epsilon<float>::eps, 128 nodes; limits in real, nodes and weights
+   in double (Quadrature::nw_TanhSinh, src/quadrature.hpp:134-151) */
+static void subslice_rule(double *x, double *w)
+{
+    const int n = c_nR;
+    const float eps = 10.0f * FLT_EPSILON;
+    const double x_min = (double)(real)(-1.0 + eps), x_max = (double)(real)(1.0 - eps);
+    const double t_min = asinh(atanh(x_min) / c_i2Pi), t_max = asinh(atanh(x_max) / c_i2Pi);
+    const double h = (t_max - t_min) / (n - 1);
+    for (int i = 0; i < n; i++) {
+        double ti = t_min + i * h, tti = c_i2Pi * sinh(ti);
+        x[i] = tanh(tti);
+        w[i] = h * c_i2Pi * cosh(ti) / pow(cosh(tti), 2);
+    }
+}
+
+/* Vr_dVrir_Lobato_neutral_0_12 with Vr_dVrir's factor f
+   (src/cgpu_fcns.cuh:1460-1475, 3026-3064), in double */
+static void vr_dvrir(double z, double R2, const double *a, const double *k, double f, double *Vr, double *dVrir)
+{
+    double r = sqrt(z * z + R2);
+    double kr[5], e[5];
+    for (int i = 0; i < 5; i++) { kr[i] = k[i] * r; e[i] = a[i] * exp(-kr[i]); }
+    double v = e[0] * (2 / kr[0] + 1) + e[1] * (2 / kr[1] + 1) + e[2] * (2 / kr[2] + 1) + e[3] * (2 / kr[3] + 1)
+               + e[4] * (2 / kr[4] + 1);
+    double d = -(v + e[0] * (kr[0] + 1) + e[1] * (kr[1] + 1) + e[2] * (kr[2] + 1) + e[3] * (kr[3] + 1)
+                 + e[4] * (kr[4] + 1)) / (r * r);
+    *Vr = v * f;
+    *dVrir = d * f;
+}
+
+/* Sub-slice spline of one atom (element table t, height za) over the slice
+   [z_0, z_e): at each knot R2_k, V = integral over z of Vr(sqrt(z^2 + R2_k))
+   and dV = dV/dR^2 on the rule mapped to [z_0 - za, z_e - za], split at
+   the atom when the slice holds it, nodes summed in order; the taper; the
+   cubic Hermite coefficients in R^2 (set_atom_Vp, linear_Vz, cubic_poly_coef,
+   src/projected_potential.cuh:253-264, src/gpu_fcns.cuh:175-285,
+   src/cgpu_fcns.cuh:1528-1555).  Computed in double and cast to real, as
+   the element tables are (the reference's FP32 path evaluates these in
+   float).  c[0..3][k]: c0..c3. */
+static void subslice_table(const atom_tab *t, int Z, real z_0, real z_e, real za, real c[4][c_nR])
+{
+    double qx[c_nR], qw[c_nR], a[5], k[5], c0[c_nR], c1[c_nR];
+    pp_coef Vr, VRc;
+    subslice_rule(qx, qw);
+    lobato_coef(Z, &Vr, &VRc);
+    for (int i = 0; i < 5; i++) { a[i] = Vr.cl[i]; k[i] = Vr.cnl[i]; }
+    const double z0h = 0.5 * ((double)z_0 - (double)za), zeh = 0.5 * ((double)z_e - (double)za);
+    const int split = (z0h < 0) && (0 < zeh);
+    const double a1 = split ? -z0h : zeh - z0h, b1 = split ? z0h : zeh + z0h;
+    for (int kk = 0; kk < c_nR; kk++) {
+        const double R2 = (double)t->R2[kk];
+        double V = 0, dV = 0;
+        for (int i = 0; i < c_nR; i++) {
+            double v, d;
+            vr_dvrir(a1 * qx[i] + b1, R2, a, k, a1 * qw[i], &v, &d);
+            if (split) {
+                double v2, d2;
+                vr_dvrir(zeh * qx[i] + zeh, R2, a, k, zeh * qw[i], &v2, &d2);
+                v += v2;
+                d += d2;
+            }
+            V += v;
+            dV += d;
+        }
+        dV = 0.5 * dV;
+        if (t->R2_tap_d < R2) {
+            double arg = t->tap_cf_d * (R2 - t->R2_tap_d);
+            double cs = cos(arg), sn = sin(arg);
+            dV = dV * cs - t->tap_cf_d * V * sn;
+            V *= cs;
+        }
+        c0[kk] = V;
+        c1[kk] = dV;
+    }
+    for (int kk = 0; kk < c_nR; kk++) {
+        double c2 = 0, c3 = 0;
+        if (kk < c_nR - 1) {
+            double idR = 1.0 / ((double)t->R2[kk + 1] - (double)t->R2[kk]);
+            double m = (c0[kk + 1] - c0[kk]) * idR, n = c1[kk] + c1[kk + 1];
+            c2 = (3.0 * m - n - c1[kk]) * idR;
+            c3 = (n - 2.0 * m) * idR * idR;
+        }
+        c[0][kk] = (real)c0[kk]; c[1][kk] = (real)c1[kk]; c[2][kk] = (real)c2; c[3][kk] = (real)c3;
+    }
+}
+
+/* Exported for the sub-slicing pins: the 128-node rule, and the sub-slice
+   spline of element Z (pixel size dR_min) over [z_0, z_e) for an atom at za */
+void orc_subslice_rule(double *x, double *w)
+{
+    subslice_rule(x, w);
+}
+int orc_subslice_table(int Z, double dR_min, double z_0, double z_e, double za, double *c0, double *c1,
+                       double *c2, double *c3)
+{
+    atom_tab t;
+    real c[4][c_nR];
+    if (Z < 1 || Z > 103) { set_err("Z out of range"); return MT_ERR_INVALID_ARGUMENT; }
+    build_atom_tab(Z, c_Vrl, c_nR, dR_min, &t);
+    subslice_table(&t, Z, (real)z_0, (real)z_e, (real)za, c);
+    for (int i = 0; i < c_nR; i++) { c0[i] = c[0][i]; c1[i] = c[1][i]; c2[i] = c[2][i]; c3[i] = c[3][i]; }
+    return MT_OK;
+}
+
 /* Projected potential of a slice, fft-shifted column-major layout:
    Projected_Potential::operator() (src/projected_potential.cuh:89-124)
    with host_detail::eval_cubic_poly (src/cpu_fcns.hpp:234-264).  V (nx*ny) */
@@ -1757,8 +1916,20 @@ static int slice_potential(sim_t *s, int islice, real *V)
     for (size_t i = 0; i < nxy; i++) V[i] = 0;
     if (islice < 0 || islice >= s->n_slice) return MT_OK;
     const slice_t *sl = &s->slice[islice];
+    const int sub = is_subslicing(s);
+    real (*sc)[c_nR] = sub ? malloc(sizeof(real) * 4 * c_nR) : NULL;
+    atom_tab *ts = sub ? malloc(sizeof(atom_tab)) : NULL;
     for (int ia = sl->iatom_0; ia <= sl->iatom_e; ia++) {
         const atom_tab *t = get_tab(s, s->atoms.Z[ia] % 1000);
+        if (sub) {
+            /* this atom's sub-slice spline on the element's knots */
+            *ts = *t;
+            subslice_table(t, s->atoms.Z[ia] % 1000, sl->z_0, sl->z_e, s->atoms.z[ia], sc);
+            for (int kk = 0; kk < c_nR; kk++) {
+                ts->c0[kk] = sc[0][kk]; ts->c1[kk] = sc[1][kk]; ts->c2[kk] = sc[2][kk]; ts->c3[kk] = sc[3][kk];
+            }
+            t = ts;
+        }
         real x = s->atoms.x[ia], y = s->atoms.y[ia], occ = s->atoms.occ[ia];
         int ix0, nxw, iy0, nyw;
         get_bn(x, g->dRx, t->R_max, &ix0, &nxw);
@@ -1776,6 +1947,7 @@ static int slice_potential(sim_t *s, int islice, real *V)
                     px = iRx_shift(g, px); py = iRy_shift(g, py);
                     if (px < 0 || px >= g->nx || py < 0 || py >= g->ny) {
                         set_err("potential index out of range");
+                        free(sc); free(ts);
                         return MT_ERR_INVALID_ARGUMENT;
                     }
                     V[(size_t)px * g->ny + py] += v;
@@ -1783,6 +1955,7 @@ static int slice_potential(sim_t *s, int islice, real *V)
             }
         }
     }
+    free(sc); free(ts);
     return MT_OK;
 }
 

@@ -39,6 +39,8 @@ def lib(precision="f64"):
         L.orc_displaced_atoms.argtypes = [inp, C.c_int, dp, C.c_int, P(C.c_int)]
         L.orc_atom_table.argtypes = [C.c_int, C.c_double] + [dp] * 7
         L.orc_lobato_VR.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+        L.orc_subslice_rule.argtypes = [dp, dp]
+        L.orc_subslice_table.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double] + [dp] * 4
         L.orc_bessel.argtypes = [C.c_int, C.c_double]
         L.orc_bessel.restype = C.c_double
         L.orc_lambda.argtypes = [C.c_double]
@@ -178,3 +180,21 @@ def lens_quadrature(params, which, precision="f64"):
     if n < 0:
         raise RuntimeError(L.orc_last_error().decode())
     return (x[:n], y[:n], w[:n]) if which == 1 else (x[:n], w[:n])
+
+
+def subslice_rule(precision="f64"):
+    """The 128-node rule on [-1, 1] the sub-slice integrals use."""
+    L = lib(precision)
+    x, w = np.zeros(128), np.zeros(128)
+    L.orc_subslice_rule(capi.dptr(x), capi.dptr(w))
+    return x, w
+
+
+def subslice_table(Z, dR_min, z_0, z_e, za, precision="f64"):
+    """c0..c3 (128 each) of the sub-slice spline of element Z over [z_0, z_e)
+    for an atom at height za, on the knots of atom_table(Z, dR_min)."""
+    L = lib(precision)
+    arrs = [np.zeros(128) for _ in range(4)]
+    _check(L, L.orc_subslice_table(int(Z), float(dR_min), float(z_0), float(z_e), float(za),
+                                   *[capi.dptr(a) for a in arrs]))
+    return arrs

@@ -108,7 +108,7 @@ def test_scan_grid_square_region():
     ("E_0", 0.0, 1),
     ("iw_type", 3, 1),                # user-defined wave without iw_psi
     ("interaction_model", 4, 1),
-    ("potential_slicing", 3, 2),      # dz_Sub: the reference leaves it unevaluated
+    ("potential_slicing", 4, 2),      # auto: no implementation in the reference
     ("theta", 95.0, 1),               # tilt beyond 90 degrees
 ])
 def test_errors_are_reported(field, value, status):
