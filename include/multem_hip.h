@@ -276,6 +276,22 @@ mt_status mt_hip_incident_wave(const mt_system_conf *sys, const mt_input_multisl
 mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice *in,
                            const double *psi_in, double z, int space, double *psi_out);
 
+/* mt_hip_propagate without device allocation: the same arguments, meaning
+   and result, bit for bit, with every device buffer of the call (staging of
+   psi_in / psi_out, the wave, twiddle and propagator tables) inside one
+   static device array of MT_NOALLOC_SCRATCH_BYTES per GPU, held from the
+   first call on that GPU until the process ends.  nx * ny must not exceed
+   MT_NOALLOC_MAX_NXY (1024^2), else MT_ERR_INVALID_ARGUMENT.  Work runs on
+   the null stream.  Calls on one GPU share the array; this library
+   serialises its own calls, and calls that overlap on one GPU from outside
+   it would race.  No counterpart in the reference: for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc). */
+#define MT_NOALLOC_MAX_NXY (1024 * 1024)
+#define MT_NOALLOC_SCRATCH_BYTES (2u * 16u * MT_NOALLOC_MAX_NXY + 262144u)
+mt_status mt_hip_propagate_noalloc(const mt_system_conf *sys, const mt_input_multislice *in,
+                                   const double *psi_in, double z, int space, double *psi_out);
+
 /* Projected-potential spline of element Z for pixel size dR_min (Angstrom),
    as the engine builds it: 128 knots each of R^2 and the cubic coefficients
    c0..c3, and scal = {R_min, R_max, R_tap, tap_cf}.  Host only.  Replaces

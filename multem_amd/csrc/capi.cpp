@@ -81,6 +81,7 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 } // namespace mtg
 
 static_assert(MT_PASS_VARIANTS == mtg::kPassVariants, "variant count of the C ABI");
+static_assert(MT_NOALLOC_MAX_NXY == 1024 * 1024, "allocation-free propagation limit of the C ABI");
 
 extern "C" {
 
@@ -213,6 +214,14 @@ mt_status mt_hip_propagate(const mt_system_conf *sys, const mt_input_multislice 
     if (!psi_in || !psi_out) return fail(MT_ERR_INVALID_ARGUMENT, "null wave");
     return guarded(sys, in,
                    [&](const auto &m) { mtg::propagate(m, sys->gpu_device, psi_in, z, space, psi_out); });
+}
+
+mt_status mt_hip_propagate_noalloc(const mt_system_conf *sys, const mt_input_multislice *in, const double *psi_in,
+                                   double z, int space, double *psi_out)
+{
+    if (!psi_in || !psi_out) return fail(MT_ERR_INVALID_ARGUMENT, "null wave");
+    return guarded(sys, in,
+                   [&](const auto &m) { mtg::propagate_noalloc(m, sys->gpu_device, psi_in, z, space, psi_out); });
 }
 
 mt_status mt_hip_atom_table(int Z, double dR_min, double *R2, double *c0, double *c1, double *c2, double *c3,

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <tuple>
 
@@ -1993,6 +1994,114 @@ void propagate(const ModelT<R> &m, int device, const double *psi_in, double z, i
     download_unshifted(S, w.p, psi_out);
 }
 
+// ---------------------------------------------------------------------------
+// Propagation without device allocation.  All working memory of the call --
+// the input/output staging area, the wave, the twiddle and propagator
+// tables -- lives in one static device array per GPU, `scratch`, whose
+// device address is looked up once per device (hipGetSymbolAddress) on the
+// first call there and kept; host code uses only that address.  The array
+// keeps the last call's contents; every region is fully written before it is
+// read, so nothing needs clearing.  Calls on one GPU share the array: the C
+// ABI serialises them behind its lock, and calls that overlap on one GPU
+// from outside that lock would race.
+constexpr size_t kNoAllocMaxNxy = size_t(1) << 20;   // nx * ny at most 1024^2 (FP32 and FP64)
+constexpr size_t kNoAllocScratchBytes = kNoAllocMaxNxy * 16 * 2 + (size_t(1) << 18);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+unsigned char *scratch_on(int device)
+{
+    static std::mutex mu;
+    static std::map<int, unsigned char *> ptr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptr.find(device);
+    if (it != ptr.end()) return it->second;
+    void *p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    ptr[device] = (unsigned char *)p;
+    return (unsigned char *)p;
+}
+
+} // namespace
+
+size_t propagate_noalloc_max_nxy() { return kNoAllocMaxNxy; }
+
+template <class R>
+void propagate_noalloc(const ModelT<R> &m, int device, const double *psi_in, double z, int space, double *psi_out)
+{
+    using Z = cplx_t<R>;
+    select_device(device);
+    if (space != 1 && space != 2) throw Error{MT_ERR_INVALID_ARGUMENT, "space must be 1 (real) or 2 (reciprocal)"};
+    const int nx = m.nx, ny = m.ny;
+    const size_t nxy = (size_t)nx * ny;
+    if (nxy > kNoAllocMaxNxy)
+        throw Error{MT_ERR_INVALID_ARGUMENT, "nx * ny = " + std::to_string(nxy) + " exceeds the " +
+                                                 std::to_string(kNoAllocMaxNxy) +
+                                                 " points of the allocation-free propagation"};
+    unsigned char *base = scratch_on(device);
+    auto carve = [&](size_t bytes) {
+        unsigned char *q = base;
+        base += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    double2 *stage = (double2 *)carve(nxy * sizeof(double2));
+    Z *w = (Z *)carve(nxy * sizeof(Z));
+    Z *tw_x = (Z *)carve(nx * sizeof(Z)), *tw_y = (Z *)carve(ny * sizeof(Z));
+    Z *px = (Z *)carve(nx * sizeof(Z)), *py = (Z *)carve(ny * sizeof(Z));
+    double *zd = (double *)carve(sizeof(double));
+    Member *mem = (Member *)carve(sizeof(Member));
+    if ((size_t)(base - scratch_on(device)) > kNoAllocScratchBytes)
+        throw Error{MT_ERR_INVALID_ARGUMENT, "grid exceeds the allocation-free scratch buffer"};
+
+    const hipStream_t s = nullptr;   // the null stream: creating a stream may allocate
+    const auto wx = twiddles<Z>(nx), wy = twiddles<Z>(ny);
+    const Member hm{0, -1, 0, 0, 1.0};
+    check(hipMemcpyAsync(stage, psi_in, nxy * sizeof(double2), hipMemcpyHostToDevice, s), "H2D psi");
+    check(hipMemcpyAsync(tw_x, wx.data(), nx * sizeof(Z), hipMemcpyHostToDevice, s), "H2D twiddles");
+    check(hipMemcpyAsync(tw_y, wy.data(), ny * sizeof(Z), hipMemcpyHostToDevice, s), "H2D twiddles");
+    check(hipMemcpyAsync(zd, &z, sizeof(double), hipMemcpyHostToDevice, s), "H2D z");
+    check(hipMemcpyAsync(mem, &hm, sizeof(Member), hipMemcpyHostToDevice, s), "H2D member");
+    check(launch_load_shifted(stage, w, nx, ny, s), "load shifted");
+    check(launch_prop_tables(nx, ny, zd, 1, m.lambda_d, m.mul_sign, m.lx_d, m.ly_d, (double)m.gx0_rot(0),
+                             (double)m.gy0_rot(0), px, py, s),
+          "prop tables");
+
+    // the passes of propagate(), on the scratch buffers
+    const bool zero = is_zero((R)z);
+    RowArgsT<Z> r{};
+    r.src = w; r.dst = w; r.member_stride = (long long)nxy; r.nrows = nx; r.n_members = 1;
+    r.tw = tw_y; r.members = mem;
+    bool transformed = false;
+    if (!zero || m.bwl || space == 2) {
+        r.flags = ROW_LOAD | ROW_FFT | ROW_STORE;
+        check(launch_row_pass(ny, r, s), "row pass");
+        ColArgsT<Z> c{};
+        c.src = w; c.dst = w; c.member_stride = (long long)nxy;
+        c.ncols = ny; c.n_members = 1;
+        c.tw = tw_x;
+        c.dgx_f = m.dgx; c.dgy_f = m.dgy;
+        c.nx = nx; c.ny = ny;
+        c.bwl_alpha = m.bwl_alpha; c.gl2_max = m.gl2_max;
+        c.inv_nxy = R(1) / (R)nxy;
+        c.p_scale = R(1);
+        if (!zero) c.flags = COL_LOAD | COL_FFT | COL_MUL_P | (m.bwl ? COL_BWL : 0) | COL_STORE;
+        else c.flags = COL_LOAD | COL_FFT | (m.bwl ? COL_BWL : 0) | COL_STORE;
+        if (zero && !m.bwl) c.flags = COL_LOAD | COL_FFT | COL_MUL_P | COL_STORE;
+        c.px = px; c.py = py;
+        if (space == 1) c.flags |= COL_IFFT;
+        check(launch_col_pass(nx, c, false, false, s), "col pass");
+        transformed = true;
+    }
+    if (transformed && space == 1) {
+        r.flags = ROW_LOAD | ROW_IFFT | ROW_STORE;
+        check(launch_row_pass(ny, r, s), "row pass");
+    }
+    check(launch_store_unshifted(w, stage, nx, ny, s), "store unshifted");
+    check(hipMemcpyAsync(psi_out, stage, nxy * sizeof(double2), hipMemcpyDeviceToHost, s), "D2H psi");
+    check(hipStreamSynchronize(s), "sync");
+}
+
 #define MTG_ENGINE_INSTANTIATE(R)                                                                                      \
     template void run_multislice<R>(const ModelT<R> &, int, mt_output_multislice *);                                   \
     template void run_multislice_group<R>(const std::vector<const ModelT<R> *> &, int,                                 \
@@ -2001,7 +2110,8 @@ void propagate(const ModelT<R> &m, int device, const double *psi_in, double z, i
     template void projected_potential_sum<R>(const ModelT<R> &, int, int, double *);                                   \
     template void transmission_function<R>(const ModelT<R> &, int, int, int, double *);                                \
     template void incident_wave<R>(const ModelT<R> &, int, double *);                                                  \
-    template void propagate<R>(const ModelT<R> &, int, const double *, double, int, double *);
+    template void propagate<R>(const ModelT<R> &, int, const double *, double, int, double *);                       \
+    template void propagate_noalloc<R>(const ModelT<R> &, int, const double *, double, int, double *);
 
 MTG_ENGINE_INSTANTIATE(float)
 MTG_ENGINE_INSTANTIATE(double)

@@ -66,6 +66,11 @@ template <class R>
 void incident_wave(const ModelT<R> &m, int device, double *psi0);
 template <class R>
 void propagate(const ModelT<R> &m, int device, const double *psi_in, double z, int space, double *psi_out);
+// propagate() with all device memory in a static per-GPU scratch array
+// (nx * ny at most propagate_noalloc_max_nxy()); no device allocation.
+template <class R>
+void propagate_noalloc(const ModelT<R> &m, int device, const double *psi_in, double z, int space, double *psi_out);
+size_t propagate_noalloc_max_nxy();
 
 // read + write bandwidth (GB/s) of a streaming copy of `bytes` bytes, reps times
 double stream_copy_gbs(int device, size_t bytes, int reps);

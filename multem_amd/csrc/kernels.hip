@@ -925,6 +925,30 @@ __global__ void k_shift_complex(const Z *src, Z *dst, int nx, int ny)
     dst[i] = src[(long long)shift_idx(ix, nx >> 1) * ny + shift_idx(iy, ny >> 1)];
 }
 
+// Natural-layout FP64 complex (ny x nx, interleaved) into the shifted layout
+// of the wave type, and back (the staging steps of a propagation whose input
+// and output are FP64 host arrays).
+template <class Z>
+__global__ void k_load_shifted(const double2 *src, Z *dst, int nx, int ny)
+{
+    using R = real_t<Z>;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nx * ny) return;
+    const int ix = (int)(i / ny), iy = (int)(i % ny);
+    const double2 v = src[(long long)shift_idx(ix, nx >> 1) * ny + shift_idx(iy, ny >> 1)];
+    dst[i] = cmake((R)v.x, (R)v.y);
+}
+
+template <class Z>
+__global__ void k_store_unshifted(const Z *src, double2 *dst, int nx, int ny)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nx * ny) return;
+    const int ix = (int)(i / ny), iy = (int)(i % ny);
+    const Z v = src[(long long)shift_idx(ix, nx >> 1) * ny + shift_idx(iy, ny >> 1)];
+    dst[i] = make_double2((double)v.x, (double)v.y);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -1256,6 +1280,24 @@ hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_
     return hipGetLastError();
 }
 
+template <class Z>
+hipError_t launch_load_shifted(const double2 *src, Z *dst, int nx, int ny, hipStream_t s)
+{
+    if ((nx & 1) || (ny & 1)) return hipErrorInvalidValue;
+    const long long n = (long long)nx * ny;
+    hipLaunchKernelGGL((k_load_shifted<Z>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, nx, ny);
+    return hipGetLastError();
+}
+
+template <class Z>
+hipError_t launch_store_unshifted(const Z *src, double2 *dst, int nx, int ny, hipStream_t s)
+{
+    if ((nx & 1) || (ny & 1)) return hipErrorInvalidValue;
+    const long long n = (long long)nx * ny;
+    hipLaunchKernelGGL((k_store_unshifted<Z>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, nx, ny);
+    return hipGetLastError();
+}
+
 // both precisions
 #define MTG_INSTANTIATE(R, Z)                                                                                          \
     template hipError_t launch_row_pass<Z>(int, const RowArgsT<Z> &, hipStream_t);                                     \
@@ -1273,7 +1315,9 @@ hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_
                                             const R *, int, R, R, double *, hipStream_t);                              \
     template hipError_t launch_ctf_table<R>(int, int, const CtfParamsT<R> &, Z *, hipStream_t);                        \
     template hipError_t launch_slice_sum<R>(const R *, int, long long, double *, hipStream_t);                         \
-    template hipError_t launch_shift_complex<Z>(const Z *, Z *, int, int, hipStream_t);
+    template hipError_t launch_shift_complex<Z>(const Z *, Z *, int, int, hipStream_t);                               \
+    template hipError_t launch_load_shifted<Z>(const double2 *, Z *, int, int, hipStream_t);                          \
+    template hipError_t launch_store_unshifted<Z>(const Z *, double2 *, int, int, hipStream_t);
 
 MTG_INSTANTIATE(float, float2)
 MTG_INSTANTIATE(double, double2)

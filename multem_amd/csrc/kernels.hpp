@@ -243,6 +243,9 @@ hipError_t launch_stream_copy(const void *a, void *b, size_t bytes, hipStream_t 
 template <class R>
 hipError_t launch_slice_sum(const R *V, int n_slices, long long nxy, double *out, hipStream_t s);
 template <class Z> hipError_t launch_shift_complex(const Z *src, Z *dst, int nx, int ny, hipStream_t s);
+// FP64 natural layout -> shifted wave (converted to Z), and back
+template <class Z> hipError_t launch_load_shifted(const double2 *src, Z *dst, int nx, int ny, hipStream_t s);
+template <class Z> hipError_t launch_store_unshifted(const Z *src, double2 *dst, int nx, int ny, hipStream_t s);
 
 // Launch variants of the slice passes.  Variant 0 is the default; the tuner
 // may pick another.  Variants differ in the radix schedule (different

@@ -44,6 +44,7 @@ def lib():
         L.mt_hip_projected_potential_sum.argtypes = [sysp, inp, C.c_int, dp]
         L.mt_hip_incident_wave.argtypes = [sysp, inp, dp]
         L.mt_hip_propagate.argtypes = [sysp, inp, dp, C.c_double, C.c_int, dp]
+        L.mt_hip_propagate_noalloc.argtypes = [sysp, inp, dp, C.c_double, C.c_int, dp]
         L.mt_hip_request_stop.argtypes = []
         L.mt_hip_progress.argtypes = [P(C.c_int), P(C.c_int)]
         L.mt_hip_set_memory_budget.argtypes = [C.c_size_t]
@@ -165,6 +166,21 @@ def propagate(params, psi, z, space=1):
     pout = np.zeros_like(pin)
     _check(lib().mt_hip_propagate(C.byref(sysc), C.byref(s), capi.dptr(pin), float(z), int(space), capi.dptr(pout)))
     return pout.view(np.complex128).reshape(params.nx, params.ny)
+
+
+def propagate_noalloc(params, psi, z, space=1):
+    """propagate() through mt_hip_propagate_noalloc: no device allocation,
+    nx * ny at most NOALLOC_MAX_NXY."""
+    s, keep = params.to_struct()
+    sysc = params.system_conf.as_struct()
+    pin = np.ascontiguousarray(psi, dtype=np.complex128).view(np.float64).ravel()
+    pout = np.zeros_like(pin)
+    _check(lib().mt_hip_propagate_noalloc(C.byref(sysc), C.byref(s), capi.dptr(pin), float(z), int(space),
+                                          capi.dptr(pout)))
+    return pout.view(np.complex128).reshape(params.nx, params.ny)
+
+
+NOALLOC_MAX_NXY = 1024 * 1024
 
 
 def synchronize(device=0):
