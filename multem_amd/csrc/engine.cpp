@@ -847,6 +847,10 @@ PassChoice choose_variants(int nx, int ny, int B, RowArgsT<Z> r, ColArgsT<Z> c, 
 // resident (transmission stack of the configuration, the batch's member and
 // beam tables) and pipeline(st) enqueues the item's whole device work on
 // stream st: probe generation, the slice loop, detector partials -> image.
+// The member and beam tables of every item are built once and kept on the
+// device; prepare(k) enqueues a device copy of item k's rows into the fixed
+// buffers the pipeline reads, so the host never waits for a batch to end
+// before enqueueing the next one.
 template <class R>
 class StemJob {
 public:
@@ -861,6 +865,8 @@ public:
     DBuf<Z> psi;
     DBuf<Member> members;
     DBuf<Beam> beams;
+    DBuf<Member> all_members;   // [n_t][nbatch][B]: every item's member table
+    DBuf<Beam> all_beams;       // [n_t][nbatch][B]
     DBuf<double> part, image;
     DBuf<Z> px, py;           // [n_slice + n_thk]: slice propagators, then back-propagations
     DBuf<Z> rx, ry, scratch;   // tilted illumination: output ramp and the output copy of the batch
@@ -877,7 +883,6 @@ public:
     ColArgsT<Z> c{};
     PassChoice choice;
     std::vector<Member> hm;
-    std::vector<Beam> hb;
     int cur_conf = -1;
     double ms_tf = 0;
     std::string batch_log;       // batch-width selection log
@@ -920,6 +925,7 @@ public:
             B = std::max(B, (2048 + nt - 1) / nt);
             B = std::min(B, B_cap);
             if (run_control().autotune != 0) B = choose_batch(B, std::min(B_cap, std::max(B, 256)));   // B0..2 B0
+            B = balanced(std::min(B, B_cap));
         }
         B = std::min(B, B_cap);
         nbatch = (nscan + B - 1) / B;
@@ -958,7 +964,22 @@ public:
         // addresses are fixed before any capture
         S.reserve(atoms);
         hm.resize(B);
-        hb.resize(B);
+        {
+            std::vector<Member> am((size_t)n_t * nbatch * B);
+            std::vector<Beam> ab(am.size());
+            for (int it = 0; it < n_t; ++it)
+                for (int is = 0; is < nbatch * B; ++is) {
+                    const bool valid = is < nscan;
+                    const int iq = valid ? is : nscan - 1;
+                    const size_t i = (size_t)it * nbatch * B + is;
+                    am[i] = Member{0, valid ? is : -1, is % B, 1, twq[it]};
+                    ab[i] = S.beam(m.sc_x[iq], m.sc_y[iq], tc10[it]);
+                }
+            all_members.alloc(am.size());
+            all_beams.alloc(ab.size());
+            all_members.upload(am.data(), am.size(), s);
+            all_beams.upload(ab.data(), ab.size(), s);
+        }
 
         // slice-step arguments shared by the tuner and the pipeline
         r.src = psi.p; r.dst = psi.p; r.member_stride = (long long)nxy; r.nrows = m.nx; r.n_members = B;
@@ -992,17 +1013,32 @@ public:
         r.variant = choice.row;
     }
 
-    // Batch width: B0, 1.5 B0 and 2 B0 (up to B_max) timed on the full slice
-    // step (variant 0 of both passes) per wave; the smallest within 2 % of
-    // the best wins.  Cached per (grid, B0, B_max).
+    // The narrowest batch that covers the scan in as many batches as width b
+    // does: the last batch then pads at most nbatch - 1 waves, instead of up
+    // to b - 1 (padding waves cost a full wave's work).
+    int balanced(int b) const
+    {
+        if (b <= 0 || nscan <= 0) return b;
+        const int nb = (nscan + b - 1) / b;
+        return (nscan + nb - 1) / nb;
+    }
+
+    // Batch width: B0, 1.5 B0 and 2 B0 (up to B_max, each balanced over the
+    // scan) timed on the full slice step (launch variants tuned for the
+    // first); the cost of a width is its time per wave times the waves it
+    // runs, padding included.  The smallest width within 2 % of the best
+    // wins.  Cached per (grid, B0, B_max, scan length).
     int choose_batch(int B0, int B_max)
     {
         std::vector<int> cand;
-        for (int b : {B0, B0 + B0 / 2, 2 * B0})
-            if (b <= B_max && (cand.empty() || b > cand.back())) cand.push_back(b);
+        for (int b : {B0, B0 + B0 / 2, 2 * B0}) {
+            if (b > B_max) continue;
+            b = balanced(b);
+            if (cand.empty() || b > cand.back()) cand.push_back(b);
+        }
         if (cand.size() < 2) return B0;
-        static std::map<std::tuple<int, int, int, int>, int> cache;   // one per precision
-        const auto key = std::make_tuple(m.nx, m.ny, B0, B_max);
+        static std::map<std::tuple<int, int, int, int, int>, int> cache;   // one per precision
+        const auto key = std::make_tuple(m.nx, m.ny, B0, B_max, nscan);
         const bool reuse = run_control().autotune == 1;
         if (reuse) {
             auto it = cache.find(key);
@@ -1051,7 +1087,8 @@ public:
             };
             const double t1 = time_variant(step, s), t2 = time_variant(step, s);
             const double t = (t1 > 0 && t2 > 0) ? std::min(t1, t2) : std::max(t1, t2);
-            per[i] = t / cand[i];
+            const int nb = (std::max(nscan, 1) + cand[i] - 1) / cand[i];
+            per[i] = t * nb;   // the whole scan at this width
             if (per[i] > 0 && (best < 0 || per[i] < best)) best = per[i];
             log << (i ? "," : "") << cand[i] << ":" << (int)std::lround(t * 1000.0) << "us";
         }
@@ -1089,11 +1126,18 @@ public:
         return o.str();
     }
 
-    // inputs of work item k resident on the device (returns after the uploads)
-    void prepare(int k)
+    // Inputs of work item k.  A new configuration is loaded and its
+    // transmission built on the job's stream after everything enqueued there
+    // (gate, if given, is an event that stream waits for first: the end of
+    // the work that still reads the previous configuration); the host
+    // returns once the build is done.  The batch's member and beam tables
+    // are copied on stream st, stream-ordered before the pipeline that reads
+    // them: no host wait.
+    void prepare(int k, hipStream_t st, const Event *gate)
     {
         const int iconf = m.fp_iconf_0 + k / (n_t * nbatch), it = (k / nbatch) % n_t, ib = k % nbatch;
         if (iconf != cur_conf) {
+            if (gate) check(hipStreamWaitEvent(s, gate->e, 0), "hipStreamWaitEvent");
             S.load(iconf, atoms, s);
             Event t0, t1;
             record(t0, s);
@@ -1103,15 +1147,11 @@ public:
             ms_tf += elapsed(t0, t1);
             cur_conf = iconf;
         }
-        for (int j = 0; j < B; ++j) {
-            const int is = ib * B + j;
-            const bool valid = is < nscan;
-            const int iq = valid ? is : nscan - 1;
-            hm[j] = Member{0, valid ? is : -1, j, 1, twq[it]};
-            hb[j] = S.beam(m.sc_x[iq], m.sc_y[iq], tc10[it]);
-        }
-        members.upload(hm.data(), B, s);
-        beams.upload(hb.data(), B, s);
+        const size_t row = ((size_t)it * nbatch + ib) * B;
+        check(hipMemcpyAsync(members.p, all_members.p + row, B * sizeof(Member), hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync D2D");
+        check(hipMemcpyAsync(beams.p, all_beams.p + row, B * sizeof(Beam), hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync D2D");
     }
 
     // the batch pipeline on stream st; ev (optional): 3 events per slice
@@ -1188,6 +1228,37 @@ public:
     }
 };
 
+// Begin/end events of the last two batches enqueued on one stream.  begin()
+// first waits for the batch that used the slot two batches ago and returns
+// its time (ms); drain() waits for both and returns their times.
+struct BatchRing {
+    Event b[2], e[2];
+    bool pending[2] = {false, false};
+    int k = 0;
+    double collect(int i)
+    {
+        if (!pending[i]) return 0.0;
+        pending[i] = false;
+        check(hipEventSynchronize(e[i].e), "hipEventSynchronize");
+        return events_enabled() ? (double)elapsed(b[i], e[i]) : 0.0;
+    }
+    double begin(hipStream_t s)
+    {
+        const double t = collect(k & 1);
+        record(b[k & 1], s);
+        return t;
+    }
+    void end(hipStream_t s)
+    {
+        check(hipEventRecord(e[k & 1].e, s), "hipEventRecord");
+        pending[k & 1] = true;
+        ++k;
+    }
+    // end event of the last batch enqueued, or null
+    const Event *last() const { return k > 0 && pending[(k - 1) & 1] ? &e[(k - 1) & 1] : nullptr; }
+    double drain() { return collect(k & 1) + collect((k + 1) & 1); }
+};
+
 struct GraphExec {
     hipGraph_t g = nullptr;
     hipGraphExec_t e = nullptr;
@@ -1224,25 +1295,28 @@ void run_stem_one(const ModelT<R> &m, mt_output_multislice *out, size_t budget, 
     }
 
     std::vector<Event> kev(run_control().time_kernels ? 3 * J.ns_run : 0);
-    Event t0, t1, sl0, sl1;
+    Event t0, t1;
+    // Two batches in flight: the host enqueues batch k while batch k-1 runs,
+    // and collects batch k-2's time (its events are reused for batch k).
+    BatchRing ring;
     double ms_slice = 0, ps = 0;
     record(t0, s);
     int last_conf = -1;
     for (int k = 0; k < J.n_work(); ++k) {
         stop_check();
-        J.prepare(k);
-        const int iconf = k / J.nbatch;
+        const int iconf = k / (J.n_t * J.nbatch);
+        if (iconf != last_conf) ms_slice += ring.drain();   // the configuration's build waits for the stream anyway
+        J.prepare(k, s, nullptr);
         // per-kernel events on the first batch of each configuration
         const bool timed = run_control().time_kernels && iconf != last_conf;
         last_conf = iconf;
-        record(sl0, s);
+        ms_slice += ring.begin(s);
         if (timed || !G.e) J.pipeline(s, timed ? &kev : nullptr);
         else check(hipGraphLaunch(G.e, s), "graph launch");
-        record(sl1, s);
-        wait(sl1, s);   // host tables are reused by the next batch
-        ms_slice += elapsed(sl0, sl1);
+        ring.end(s);
         ps += (double)J.batch_size(k) * J.ns_run;
         if (timed) {
+            ms_slice += ring.drain();
             for (int is = 0; is < J.ns_run; ++is) {
                 tm.ms_row += elapsed(kev[3 * is], kev[3 * is + 1]);
                 tm.ms_col += elapsed(kev[3 * is + 1], kev[3 * is + 2]);
@@ -1252,6 +1326,7 @@ void run_stem_one(const ModelT<R> &m, mt_output_multislice *out, size_t budget, 
         }
         progress_add(J.batch_size(k));
     }
+    ms_slice += ring.drain();
     record(t1, s);
     wait(t1, s);
     tm.ms_total = elapsed(t0, t1);
@@ -1371,7 +1446,8 @@ void run_stem_group(const std::vector<const ModelT<R> *> &ms, const std::vector<
         }
     };
 
-    Event t0, t1, sl0, sl1;
+    Event t0, t1;
+    BatchRing ring;
     double ms_slice = 0, ps = 0;
     record(t0, s);
     for (int k = 0; k < rounds; ++k) {
@@ -1379,10 +1455,11 @@ void run_stem_group(const std::vector<const ModelT<R> *> &ms, const std::vector<
         unsigned mask = 0;
         for (size_t i = 0; i < n; ++i)
             if (k < jobs[i]->n_work()) {
-                jobs[i]->prepare(k);
+                // a new configuration's build waits for the last round enqueued
+                jobs[i]->prepare(k, s, ring.last());
                 mask |= 1u << i;
             }
-        record(sl0, s);
+        ms_slice += ring.begin(s);
         if (run_control().use_graph) {
             auto it = graphs.find(mask);
             if (it == graphs.end()) {
@@ -1397,15 +1474,14 @@ void run_stem_group(const std::vector<const ModelT<R> *> &ms, const std::vector<
         } else {
             enqueue(mask);
         }
-        record(sl1, s);
-        wait(sl1, s);   // host tables are reused by the next round
-        ms_slice += elapsed(sl0, sl1);
+        ring.end(s);
         for (size_t i = 0; i < n; ++i)
             if (mask >> i & 1u) {
                 ps += (double)jobs[i]->batch_size(k) * jobs[i]->ns_run;
                 progress_add(jobs[i]->batch_size(k));
             }
     }
+    ms_slice += ring.drain();
     record(t1, s);
     wait(t1, s);
     tm.ms_total = elapsed(t0, t1);

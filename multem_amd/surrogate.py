@@ -156,19 +156,22 @@ def _rel_rms(pred, ref):
     return float(np.sqrt(np.mean((pred - ref) ** 2) / np.mean(ref ** 2)))
 
 
-def evaluate(model, n_specimens=4, seed=2024, ns=32, n_small=2, n_big=64):
+def evaluate(model, n_specimens=4, seed=2024, ns=32, n_small=2, n_big=64, n_ref=0, tilt_max=0.0):
     """Surrogate against plain multislice on held-out random specimens.
 
     GPU time: plain = the n_big-configuration run; surrogate = the
     n_small-configuration run + the potential-sum build + the network
     forward (device times from HIP events).  Errors are relative RMS
     against the n_big-configuration image, for the surrogate and for the
-    n_small-configuration image it starts from."""
+    n_small-configuration image it starts from; with n_ref > 0 also against
+    an n_ref-configuration image (closer to converged), for those two and
+    for the n_big-configuration image itself.  tilt_max: largest
+    illumination tilt of the specimens (degrees)."""
     from . import engine, surrogate_data
     rng = np.random.default_rng(seed)
     rows = []
     for _ in range(n_specimens):
-        p, meta = surrogate_data.random_specimen(rng, ns)
+        p, meta = surrogate_data.random_specimen(rng, ns, tilt_max=tilt_max)
         q = p.copy()
         q.pn_nconf = n_big
         big = engine.run_multislice(q).image_tot[0, 0].copy()
@@ -181,11 +184,22 @@ def evaluate(model, n_specimens=4, seed=2024, ns=32, n_small=2, n_big=64):
         t_v = engine.last_timing()["ms_total"]
         pred = model.predict(small, vmap)
         t_f = model.net.last_forward_ms()
-        rows.append(dict(desc=meta["desc"], thick=meta["thick"], ms_plain=t_big, ms_surrogate=t_small + t_v + t_f,
-                         ms_small=t_small, ms_vmap=t_v, ms_forward=t_f,
-                         rel_rms_surrogate=_rel_rms(pred, big), rel_rms_small=_rel_rms(small, big)))
+        row = dict(desc=meta["desc"], thick=meta["thick"], theta=p.theta, ms_plain=t_big,
+                   ms_surrogate=t_small + t_v + t_f, ms_small=t_small, ms_vmap=t_v, ms_forward=t_f,
+                   rel_rms_surrogate=_rel_rms(pred, big), rel_rms_small=_rel_rms(small, big))
+        if n_ref:
+            q = p.copy()
+            q.pn_nconf = n_ref
+            ref = engine.run_multislice(q).image_tot[0, 0].copy()
+            row.update(rel_rms_surrogate_ref=_rel_rms(pred, ref), rel_rms_small_ref=_rel_rms(small, ref),
+                       rel_rms_big_ref=_rel_rms(big, ref))
+        rows.append(row)
     ms_p = sum(r["ms_plain"] for r in rows)
     ms_s = sum(r["ms_surrogate"] for r in rows)
-    return dict(specimens=rows, gpu_time_ratio=ms_p / ms_s,
-                rel_rms_surrogate_mean=float(np.mean([r["rel_rms_surrogate"] for r in rows])),
-                rel_rms_small_mean=float(np.mean([r["rel_rms_small"] for r in rows])))
+    out = dict(specimens=rows, gpu_time_ratio=ms_p / ms_s,
+               rel_rms_surrogate_mean=float(np.mean([r["rel_rms_surrogate"] for r in rows])),
+               rel_rms_small_mean=float(np.mean([r["rel_rms_small"] for r in rows])))
+    if n_ref:
+        for k in ("surrogate", "small", "big"):
+            out["rel_rms_%s_ref_mean" % k] = float(np.mean([r["rel_rms_%s_ref" % k] for r in rows]))
+    return out

@@ -2,7 +2,8 @@
 
     python -m multem_amd.surrogate_train --data a.npz b.npz --n-eval 24 --steps 6000 --out weights.npz
 
-The last --n-eval pairs are held out.  Prints one JSON line with the
+The last --n-eval pairs (or the last --eval-per-file pairs of each file) are
+held out.  Prints one JSON line with the
 held-out relative RMS error of the network and of the 2-configuration image
 it starts from, both against the 64-configuration image:
 rel_rms = sqrt(mean((pred - ref)^2)) / sqrt(mean(ref^2)) per image.
@@ -17,14 +18,17 @@ import numpy as np
 from .surrogate import SurrogateNet, init_params, normalise
 
 
-def load_pairs(paths):
-    S, V, T = [], [], []
+def load_pairs(paths, eval_per_file=0):
+    """Training pairs of all files, then held-out pairs: the last
+    eval_per_file pairs of each file (none if 0)."""
+    S, V, T, Se, Ve, Te = [], [], [], [], [], []
     for p in paths:
         d = np.load(p, allow_pickle=False)
-        S.append(d["adf_small"])
-        V.append(d["vmap"])
-        T.append(d["adf_big"])
-    return np.concatenate(S), np.concatenate(V), np.concatenate(T)
+        k = len(d["adf_small"]) - eval_per_file
+        S.append(d["adf_small"][:k]); V.append(d["vmap"][:k]); T.append(d["adf_big"][:k])
+        Se.append(d["adf_small"][k:]); Ve.append(d["vmap"][k:]); Te.append(d["adf_big"][k:])
+    return (np.concatenate(S + Se), np.concatenate(V + Ve), np.concatenate(T + Te),
+            sum(len(x) for x in Se))
 
 
 def rel_rms(pred, ref):
@@ -35,7 +39,9 @@ def rel_rms(pred, ref):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--data", nargs="+", required=True)
-    ap.add_argument("--n-eval", type=int, default=24)
+    ap.add_argument("--n-eval", type=int, default=24, help="held-out pairs: the last n-eval of all files")
+    ap.add_argument("--eval-per-file", type=int, default=0,
+                    help="instead, hold out the last k pairs of each file")
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -46,9 +52,9 @@ def main():
     ap.add_argument("--eval-out", default=None, help="held-out pairs (.npz)")
     a = ap.parse_args()
 
-    S, V, T = load_pairs(a.data)
+    S, V, T, n_held = load_pairs(a.data, a.eval_per_file)
     n = len(S)
-    ntr = n - a.n_eval
+    ntr = n - (n_held if a.eval_per_file else a.n_eval)
     x, s = normalise(S, V)
     t = (T / s).astype(np.float32)
     net = SurrogateNet(a.blocks, a.channels)
