@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ns", type=int, default=16)
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-size group measurement")
+    ap.add_argument("--group-batch", type=int, default=256,
+                    help="waves per batch of each group member (0: each member's planner and tuner)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path measurement")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -87,9 +89,10 @@ def cpu_baseline(args):
                       "FP32 C oracle (own radix-2 FFT, FFTW absent), %.1f s" % (args.cpu_ns, args.cpu_ns, nsl, ps, dt)}
 
 
-def group_measurement():
+def group_measurement(group_batch=256):
     """SrTiO3 [001] at 256^2 / 512^2 / 1024^2 (21 / 51 / 101 slices), 16x16
-    probes and 4 phonon configurations each, 256 probes per batch: one group
+    probes and 4 phonon configurations each, group_batch probes per batch
+    (0: each member's planner and tuner pick its width): one group
     run (one hipGraph launch per round) against the three runs one after the
     other, and each grid's own rate and column-pass roofline fraction from
     its solo run (HIP events around the passes of each configuration's
@@ -99,7 +102,7 @@ def group_measurement():
     nsl = [len(engine.spec_slicing(p)) for p in ps]
     work_i = [16 * 16 * 4 * n for n in nsl]
     work = sum(work_i)
-    engine.set_batch(256, True)
+    engine.set_batch(group_batch, True)
     per_grid = []
     try:
         engine.run_multislice_group(ps)          # warm-up (tuning, graphs)
@@ -137,7 +140,8 @@ def group_measurement():
     finally:
         engine.set_batch(0, True)
     return {"workload": "sto_group: SrTiO3[001] 256^2/512^2/1024^2, %s slices, 16x16 probes x 4 configs each, "
-                        "256 probes per batch" % "/".join(str(n) for n in nsl),
+                        "%s" % ("/".join(str(n) for n in nsl),
+                                "%d probes per batch" % group_batch if group_batch > 0 else "planner batch widths"),
             "probe_slices": work, "value": work / t_group, "unit": "probe·slices/s",
             "sequential_value": work / t_seq, "plan": plan, "per_grid": per_grid}
 
@@ -329,7 +333,7 @@ def main():
     out["roofline"]["frac_of_copy_ceiling"] = (achieved / out["roofline"]["copy_ceiling_GBs"]
                                                if achieved else None)
     if not args.no_group:
-        out["uneven_group"] = group_measurement()
+        out["uneven_group"] = group_measurement(args.group_batch)
     if not args.no_fp64:
         out["fp64"] = fp64_measurement()
     if not args.no_surrogate and os.path.exists(SURROGATE_WEIGHTS):

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmultem_hip.so")
-SOURCES = ["kernels.hip", "surrogate.hip", "engine.cpp", "host_model.cpp", "quadrature.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "slice_passes_m3.hip", "slice_passes_m5.hip", "surrogate.hip", "engine.cpp", "host_model.cpp", "quadrature.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code.

@@ -174,11 +174,72 @@ __device__ __forceinline__ void dft16(C (&v)[E], int o)
     }
 }
 
+// odd radices: constants cos / sin (2 pi k / R) in the precision they are used in
+template <class C> struct OddConst;
+template <> struct OddConst<float2> {
+    static constexpr float s3 = 0.86602540378443864676f;
+    static constexpr float c5a = 0.30901699437494742410f, c5b = -0.80901699437494742410f;
+    static constexpr float s5a = 0.95105651629515357212f, s5b = 0.58778525229247312917f;
+};
+template <> struct OddConst<double2> {
+    static constexpr double s3 = 0.86602540378443864676;
+    static constexpr double c5a = 0.30901699437494742410, c5b = -0.80901699437494742410;
+    static constexpr double s5a = 0.95105651629515357212, s5b = 0.58778525229247312917;
+};
+
+// 3-point DFT: X0 = a0 + s, X1,2 = (a0 - s/2) -/+ i sin(2pi/3) d (forward;
+// signs of the i terms swapped for the inverse), s = a1 + a2, d = a1 - a2.
+// Every output component is fixed to one rounding pattern (explicit FMAs).
+template <bool INV, int S, int E, class C>
+__device__ __forceinline__ void dft3(C (&v)[E], int o)
+{
+#pragma clang fp contract(off)
+    using R = real_t<C>;
+    const R h = OddConst<C>::s3;
+    const C a0 = v[o], a1 = v[o + S], a2 = v[o + 2 * S];
+    const C s = cadd(a1, a2), d = csub(a1, a2);
+    const C m = cmake(fmah(s.x, R(-0.5), a0.x), fmah(s.y, R(-0.5), a0.y));
+    // -i d (forward) or +i d (inverse), scaled by sin(2pi/3) inside the FMA
+    const C e = mul_mi<INV>(d);
+    v[o] = cadd(a0, s);
+    v[o + S] = cmake(fmah(e.x, h, m.x), fmah(e.y, h, m.y));
+    v[o + 2 * S] = cmake(fmah(-e.x, h, m.x), fmah(-e.y, h, m.y));
+}
+
+// 5-point DFT with s14 = a1 + a4, d14 = a1 - a4, s23 = a2 + a3, d23 = a2 - a3:
+//   X0 = a0 + s14 + s23
+//   X1,4 = m1 -/+ i n1,  m1 = a0 + c1 s14 + c2 s23,  n1 = s1 d14 + s2 d23
+//   X2,3 = m2 -/+ i n2,  m2 = a0 + c2 s14 + c1 s23,  n2 = s2 d14 - s1 d23
+// (c_k = cos(2 pi k / 5), s_k = sin(2 pi k / 5); forward signs, swapped for
+// the inverse).  Explicit FMAs fix the rounding.
+template <bool INV, int S, int E, class C>
+__device__ __forceinline__ void dft5(C (&v)[E], int o)
+{
+#pragma clang fp contract(off)
+    using R = real_t<C>;
+    const R c1 = OddConst<C>::c5a, c2 = OddConst<C>::c5b, s1 = OddConst<C>::s5a, s2 = OddConst<C>::s5b;
+    const C a0 = v[o], a1 = v[o + S], a2 = v[o + 2 * S], a3 = v[o + 3 * S], a4 = v[o + 4 * S];
+    const C s14 = cadd(a1, a4), d14 = csub(a1, a4), s23 = cadd(a2, a3), d23 = csub(a2, a3);
+    const C m1 = cmake(fmah(s23.x, c2, fmah(s14.x, c1, a0.x)), fmah(s23.y, c2, fmah(s14.y, c1, a0.y)));
+    const C m2 = cmake(fmah(s23.x, c1, fmah(s14.x, c2, a0.x)), fmah(s23.y, c1, fmah(s14.y, c2, a0.y)));
+    const C n1 = cmake(fmah(d23.x, s2, d14.x * s1), fmah(d23.y, s2, d14.y * s1));
+    const C n2 = cmake(fmah(d23.x, -s1, d14.x * s2), fmah(d23.y, -s1, d14.y * s2));
+    const C e1 = mul_mi<INV>(n1), e2 = mul_mi<INV>(n2);   // -i n (forward), +i n (inverse)
+    v[o] = cadd(cadd(a0, s14), s23);
+    v[o + S] = cadd(m1, e1);
+    v[o + 4 * S] = csub(m1, e1);
+    v[o + 2 * S] = cadd(m2, e2);
+    v[o + 3 * S] = csub(m2, e2);
+}
+
 template <int R, bool INV, int S, int E, class C>
 __device__ __forceinline__ void dftR(C (&v)[E], int o)
 {
+    static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16, "radix");
     if constexpr (R == 2) dft2<INV>(v[o], v[o + S]);
+    else if constexpr (R == 3) dft3<INV, S, E>(v, o);
     else if constexpr (R == 4) dft4<INV>(v[o], v[o + S], v[o + 2 * S], v[o + 3 * S]);
+    else if constexpr (R == 5) dft5<INV, S, E>(v, o);
     else if constexpr (R == 8) dft8<INV, S, E>(v, o);
     else if constexpr (R == 16) dft16<INV, S, E>(v, o);
 }
@@ -196,6 +257,7 @@ struct Radix {
 };
 
 // Schedules per length: 0 is the default, 1 the alternative the tuner may pick.
+// Lengths 2^k, 3 x 2^k and 5 x 2^k in [64, 2048]; the odd radix runs last.
 template <int N, int SC> struct Sched;
 template <> struct Sched<64, 0>   : Radix<64, 8, 8, 8> {};
 template <> struct Sched<64, 1>   : Radix<64, 16, 16, 4> {};
@@ -209,6 +271,26 @@ template <> struct Sched<1024, 0> : Radix<1024, 16, 16, 8, 8> {};
 template <> struct Sched<1024, 1> : Radix<1024, 8, 8, 8, 8, 2> {};
 template <> struct Sched<2048, 0> : Radix<2048, 16, 16, 16, 8> {};
 template <> struct Sched<2048, 1> : Radix<2048, 8, 8, 8, 8, 4> {};
+template <> struct Sched<96, 0>   : Radix<96, 12, 4, 4, 2, 3> {};
+template <> struct Sched<96, 1>   : Radix<96, 24, 8, 4, 3> {};
+template <> struct Sched<192, 0>  : Radix<192, 12, 4, 4, 4, 3> {};
+template <> struct Sched<192, 1>  : Radix<192, 24, 8, 8, 3> {};
+template <> struct Sched<384, 0>  : Radix<384, 24, 8, 4, 4, 3> {};
+template <> struct Sched<384, 1>  : Radix<384, 48, 16, 8, 3> {};
+template <> struct Sched<768, 0>  : Radix<768, 24, 8, 8, 4, 3> {};
+template <> struct Sched<768, 1>  : Radix<768, 48, 16, 16, 3> {};
+template <> struct Sched<1536, 0> : Radix<1536, 24, 8, 8, 8, 3> {};
+template <> struct Sched<1536, 1> : Radix<1536, 48, 16, 16, 2, 3> {};
+template <> struct Sched<80, 0>   : Radix<80, 20, 4, 4, 5> {};
+template <> struct Sched<80, 1>   : Radix<80, 40, 8, 2, 5> {};
+template <> struct Sched<160, 0>  : Radix<160, 20, 4, 4, 2, 5> {};
+template <> struct Sched<160, 1>  : Radix<160, 40, 8, 4, 5> {};
+template <> struct Sched<320, 0>  : Radix<320, 20, 4, 4, 4, 5> {};
+template <> struct Sched<320, 1>  : Radix<320, 40, 8, 8, 5> {};
+template <> struct Sched<640, 0>  : Radix<640, 40, 8, 4, 4, 5> {};
+template <> struct Sched<640, 1>  : Radix<640, 40, 8, 8, 2, 5> {};
+template <> struct Sched<1280, 0> : Radix<1280, 40, 8, 8, 4, 5> {};
+template <> struct Sched<1280, 1> : Radix<1280, 40, 4, 8, 8, 5> {};
 
 template <class S, int ST>
 struct StageNS {
@@ -245,8 +327,8 @@ template <class S, int ST>
 __device__ __forceinline__ int stage_out_index(int t, int q, int r)
 {
     constexpr int R = S::R[ST], NS = StageNS<S, ST>::value;
-    const int j = t + S::T * q;
-    return (j / NS) * NS * R + (j & (NS - 1)) + r * NS;
+    const unsigned j = t + S::T * q;
+    return (j / NS) * NS * R + (j % NS) + r * NS;
 }
 
 // Per-thread twiddles of all stages with NS > 1, in stage order.
@@ -274,7 +356,7 @@ struct TwReg {
             constexpr int Q = S::E / R, off = TwOffset<S, ST>::value;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const int k = (t + S::T * q) & (NS - 1);
+                const int k = (unsigned)(t + S::T * q) % NS;
 #pragma unroll
                 for (int r = 1; r < R; ++r) w[off + q * (R - 1) + r - 1] = tw[r * k * (S::N / (NS * R))];
             }
@@ -302,7 +384,7 @@ struct TwTable {
     __device__ __forceinline__ C get(int q, int r) const
     {
         constexpr int R = S::R[ST], NS = StageNS<S, ST>::value;
-        return tab[r * ((t + S::T * q) & (NS - 1)) * (S::N / (NS * R))];
+        return tab[r * ((unsigned)(t + S::T * q) % NS) * (S::N / (NS * R))];
     }
 };
 

@@ -439,9 +439,15 @@ ModelT<R>::ModelT(const mt_input_multislice &in)
 
     // grid (Grid_2d::set_input_data, src/types.cuh:2154-2178), periodic
     nx = in.nx; ny = in.ny;
-    auto pow2 = [](int n) { return n >= 64 && n <= 2048 && (n & (n - 1)) == 0; };
+    // transform lengths of the slice passes: 2^k, 3 x 2^k, 5 x 2^k in [64, 2048]
+    auto supported = [](int n) {
+        if (n < 64 || n > 2048) return false;
+        const int m = n % 3 == 0 ? 3 : n % 5 == 0 ? 5 : 1, p = n / m;
+        return (p & (p - 1)) == 0 && p % 16 == 0;
+    };
     if (nx <= 0 || ny <= 0) fail(MT_ERR_INVALID_ARGUMENT, "nx, ny must be positive");
-    if (!pow2(nx) || !pow2(ny)) fail(MT_ERR_UNSUPPORTED, "nx, ny must be powers of two in [64, 2048]");
+    if (!supported(nx) || !supported(ny))
+        fail(MT_ERR_UNSUPPORTED, "nx, ny must be 2^k, 3 x 2^k or 5 x 2^k in [64, 2048]");
     nxh = nx / 2; nyh = ny / 2;
     lx = (R)in.spec_lx; ly = (R)in.spec_ly;
     lx_d = in.spec_lx; ly_d = in.spec_ly;

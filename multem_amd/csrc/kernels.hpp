@@ -261,16 +261,41 @@ struct PassVariant {
 };
 constexpr int kPassVariants = 11;
 
+// Transform lengths of the slice passes: 2^k, 3 x 2^k and 5 x 2^k in
+// [64, 2048] (every one a multiple of 16).  Powers of two have every launch
+// variant; the others run variant 0 only.
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+constexpr bool fft_length_supported(int n)
+{
+    if (n < 64 || n > 2048) return false;
+    const int m = n % 3 == 0 ? 3 : n % 5 == 0 ? 5 : 1;
+    return is_pow2(n / m) && (n / m) % 16 == 0;
+}
+constexpr int pass_variants(int n) { return is_pow2(n) ? kPassVariants : 1; }
+
 // elements per thread of schedule sc at length n (Sched<n, sc>::E)
 constexpr int sched_elems(int n, int sc)
 {
+    if (n % 3 == 0) return sc == 0 ? (n <= 192 ? 12 : 24) : (n <= 192 ? 24 : 48);
+    if (n % 5 == 0) return sc == 0 ? (n <= 320 ? 20 : 40) : 40;
     return sc == 0 ? (n >= 1024 ? 16 : 8) : (n >= 1024 ? 8 : 16);
 }
-constexpr int row_tile0(int n) { return n == 64 ? 32 : n == 128 ? 16 : n == 256 ? 8 : n <= 1024 ? 4 : 2; }
-constexpr int col_tile0(int n) { return n >= 2048 ? 8 : 16; }
+constexpr int kLdsBytes = 160 * 1024;
+// rows of length n per row-pass workgroup: 256 threads, at most 16 rows (so
+// the count divides every supported length), LDS rows (n + n/8 elements of
+// 16 bytes, either precision) within the 160 KiB of a CU
+constexpr int row_tile0(int n)
+{
+    if (is_pow2(n)) return n <= 128 ? 16 : n == 256 ? 8 : n <= 1024 ? 4 : 2;
+    int r = 256 / (n / sched_elems(n, 0));
+    r = r > 16 ? 16 : r;
+    while (r > 1 && r * (n + n / 8) * 16 > kLdsBytes) r /= 2;
+    return r;
+}
+constexpr int col_tile0(int n) { return n > 1024 ? 8 : 16; }
 // FP64 column pass: columns per workgroup that keep the double LDS tile
 // (N x C x 16 bytes) at or below 128 KiB
-constexpr int col_tile0_fp64(int n) { return n >= 2048 ? 4 : n >= 1024 ? 8 : 16; }
+constexpr int col_tile0_fp64(int n) { return n * 16 * 16 <= 128 * 1024 ? 16 : n * 8 * 16 <= 128 * 1024 ? 8 : 4; }
 
 // Row pass, v: 0 default; 1 table twiddles; 2 twice the rows; 3 alternative
 // schedule at the same thread count; 4 = 3 with table twiddles; 5 half the
@@ -292,7 +317,7 @@ constexpr PassVariant row_variant(int n, int v)
     case 4: return PassVariant{1, alt, 1, 0};
     case 5: return PassVariant{0, r > 1 ? r / 2 : 1, 0, 0};
     case 6: return PassVariant{0, 2 * r, 1, 0};
-    default: return PassVariant{0, r, 0, 0};
+    default: return PassVariant{0, r, is_pow2(n) ? 0 : 1, 0};   // 3 x 2^k, 5 x 2^k: table twiddles
     }
 }
 // Column pass, v: 0 default; 1 table twiddles; 2 half the columns;
@@ -319,7 +344,7 @@ constexpr PassVariant col_variant(int n, int v)
     case 4: return PassVariant{1, c1, 1, 0};
     case 5: return PassVariant{0, c / 4, 0, 0};
     case 6: return PassVariant{0, c / 2, 1, 0};
-    default: return PassVariant{0, c, 0, 0};
+    default: return PassVariant{0, c, is_pow2(n) ? 0 : 1, 0};   // 3 x 2^k, 5 x 2^k: table twiddles
     }
 }
 

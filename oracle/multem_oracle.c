@@ -509,38 +509,57 @@ void orc_normal(uint64_t seed, int n, double *out)
 }
 
 /* ------------------------------------------------------------------ */
-/* FFT: iterative radix-2, unnormalised both ways, on real precision.   */
+/* FFT: unnormalised both ways, on real precision.  Lengths 2^k run an   */
+/* iterative radix-2 transform; lengths m x 2^k (m = 3 or 5) split into  */
+/* m interleaved radix-2 transforms and one direct radix-m combination   */
+/* X[k] = sum_r W_n^(r k) Y_r[k mod 2^k] (decimation in time).           */
 /* 2D layout column-major: element (ix, iy) at ix*ny + iy.              */
 /* ------------------------------------------------------------------ */
-typedef struct { int n, logn; cplx *w; int *rev; } fft1;
+typedef struct { int n, m, L, logn; cplx *w; int *rev; cplx *wn; } fft1;
 
 static int ilog2i(int n) { int l = 0; while ((1 << l) < n) l++; return ((1 << l) == n) ? l : -1; }
 
 static int fft1_init(fft1 *p, int n)
 {
     p->n = n;
-    p->logn = ilog2i(n);
-    if (p->logn < 0) return -1;
-    p->w = malloc(sizeof(cplx) * (size_t)(n / 2 + 1));
-    p->rev = malloc(sizeof(int) * (size_t)n);
-    for (int k = 0; k < n / 2 + 1; k++) {
-        double a = -c_2Pi * (double)k / (double)n;
+    p->m = 1;
+    if (ilog2i(n) < 0) {
+        if (n % 3 == 0 && ilog2i(n / 3) >= 0) p->m = 3;
+        else if (n % 5 == 0 && ilog2i(n / 5) >= 0) p->m = 5;
+        else return -1;
+    }
+    const int L = n / p->m;
+    p->L = L;
+    p->logn = ilog2i(L);
+    p->w = malloc(sizeof(cplx) * (size_t)(L / 2 + 1));
+    p->rev = malloc(sizeof(int) * (size_t)L);
+    for (int k = 0; k < L / 2 + 1; k++) {
+        double a = -c_2Pi * (double)k / (double)L;
         p->w[k].re = (real)cos(a);
         p->w[k].im = (real)sin(a);
     }
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < L; i++) {
         int r = 0;
         for (int b = 0; b < p->logn; b++) if (i & (1 << b)) r |= 1 << (p->logn - 1 - b);
         p->rev[i] = r;
     }
+    p->wn = NULL;
+    if (p->m > 1) {
+        p->wn = malloc(sizeof(cplx) * (size_t)n);
+        for (int k = 0; k < n; k++) {
+            double a = -c_2Pi * (double)k / (double)n;
+            p->wn[k].re = (real)cos(a);
+            p->wn[k].im = (real)sin(a);
+        }
+    }
     return 0;
 }
-static void fft1_free(fft1 *p) { free(p->w); free(p->rev); }
+static void fft1_free(fft1 *p) { free(p->w); free(p->rev); free(p->wn); }
 
-/* transform nb interleaved sequences: element k of sequence b at x[k*nb + b] */
-static void fft1_exec_multi(const fft1 *p, cplx *x, int nb, int inverse)
+/* radix-2 transform of length p->L on nb interleaved sequences */
+static void fft1_radix2(const fft1 *p, cplx *x, int nb, int inverse)
 {
-    const int n = p->n;
+    const int n = p->L;
     for (int i = 0; i < n; i++) {
         int r = p->rev[i];
         if (r > i) {
@@ -564,6 +583,33 @@ static void fft1_exec_multi(const fft1 *p, cplx *x, int nb, int inverse)
             }
         }
     }
+}
+
+/* transform nb interleaved sequences: element k of sequence b at x[k*nb + b] */
+static void fft1_exec_multi(const fft1 *p, cplx *x, int nb, int inverse)
+{
+    if (p->m == 1) { fft1_radix2(p, x, nb, inverse); return; }
+    const int n = p->n, m = p->m, L = p->L;
+    cplx *y = malloc(sizeof(cplx) * (size_t)n * nb);
+    /* y_r[j] = x[m j + r] */
+    for (int r = 0; r < m; r++)
+        for (int j = 0; j < L; j++)
+            memcpy(y + ((size_t)r * L + j) * nb, x + ((size_t)m * j + r) * nb, sizeof(cplx) * (size_t)nb);
+    for (int r = 0; r < m; r++) fft1_radix2(p, y + (size_t)r * L * nb, nb, inverse);
+    for (int k = 0; k < n; k++) {
+        cplx *o = x + (size_t)k * nb;
+        for (int q = 0; q < nb; q++) { o[q].re = 0; o[q].im = 0; }
+        for (int r = 0; r < m; r++) {
+            cplx w = p->wn[(size_t)r * k % n];
+            if (inverse) w.im = -w.im;
+            const cplx *a = y + ((size_t)r * L + k % L) * nb;
+            for (int q = 0; q < nb; q++) {
+                o[q].re += a[q].re * w.re - a[q].im * w.im;
+                o[q].im += a[q].re * w.im + a[q].im * w.re;
+            }
+        }
+    }
+    free(y);
 }
 
 typedef struct { int nx, ny; fft1 fx, fy; } fft2;
@@ -595,7 +641,7 @@ static void fft2_exec(const fft2 *p, cplx *a, int inverse, cplx *work)
 int orc_fft2(int nx, int ny, double *data, int inverse)
 {
     fft2 p;
-    if (fft2_init(&p, nx, ny)) { set_err("fft sizes must be powers of two"); return MT_ERR_UNSUPPORTED; }
+    if (fft2_init(&p, nx, ny)) { set_err("fft sizes must be 2^k, 3 x 2^k or 5 x 2^k"); return MT_ERR_UNSUPPORTED; }
     size_t n = (size_t)nx * ny;
     cplx *a = malloc(sizeof(cplx) * n), *w = malloc(sizeof(cplx) * (size_t)nx * 16);
     for (size_t i = 0; i < n; i++) { a[i].re = (real)data[2 * i]; a[i].im = (real)data[2 * i + 1]; }
@@ -1497,7 +1543,13 @@ static sim_t *sim_create(const mt_input_multislice *in, int *status)
 
     /* Grid_2d::set_input_data (src/types.cuh:2154-2178), pbc_xy = true */
     if (in->nx <= 0 || in->ny <= 0) FAIL(MT_ERR_INVALID_ARGUMENT, "nx, ny must be positive");
-    if (ilog2i(in->nx) < 0 || ilog2i(in->ny) < 0) FAIL(MT_ERR_UNSUPPORTED, "nx, ny must be powers of two");
+    {
+        fft1 tx, ty;
+        const int okx = fft1_init(&tx, in->nx) == 0, oky = okx && fft1_init(&ty, in->ny) == 0;
+        if (okx) fft1_free(&tx);
+        if (oky) fft1_free(&ty);
+        if (!okx || !oky) FAIL(MT_ERR_UNSUPPORTED, "nx, ny must be 2^k, 3 x 2^k or 5 x 2^k");
+    }
     {
         grid_t *g = &s->g;
         g->nx = in->nx; g->ny = in->ny; g->nxh = g->nx / 2; g->nyh = g->ny / 2;

@@ -103,7 +103,9 @@ def test_scan_grid_square_region():
     ("simulation_type", 61, 2),       # EELS: not provided
     ("simulation_type", 7, 2),        # not a simulation type
     ("potential_type", 4, 2),         # Kirkland tables: not provided
-    ("nx", 300, 2),                   # not a power of two
+    ("nx", 300, 2),                   # 3 x 100: not 2^k, 3 x 2^k or 5 x 2^k
+    ("nx", 112, 2),                   # 7 x 16: prime factor 7
+    ("ny", 4096, 2),                  # beyond 2048
     ("nx", 0, 1),
     ("E_0", 0.0, 1),
     ("iw_type", 3, 1),                # user-defined wave without iw_psi

@@ -145,3 +145,58 @@ def test_group_fp64_matches_solo():
     assert engine.last_plan().count("stem fp64") == 2
     for a, o in zip(solo, grp):
         assert np.array_equal(a, o.image_tot)
+
+
+def _run_forced(p, row, col, col_det):
+    engine.set_pass_variants(row, col, col_det)
+    try:
+        return engine.run_multislice(p).image_tot.copy()
+    finally:
+        engine.set_pass_variants(-1, -1, -1)
+
+
+# FP64 launch variants: the FP32 set without the three that differ only in
+# where twiddles live (1, 4, 6: the FP64 passes always read the table);
+# those keeping the default radix schedule are bit-identical to variant 0.
+FP64_TWIDDLE_TWINS = (1, 4, 6)
+FP64_SAME_SCHEDULE = (2, 5, 8, 10)
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024])
+def test_fp64_pass_variant_sweep(n):
+    """Each launch variant of the FP64 row and column slice passes, one pass
+    at a time: same-schedule variants bit-identical to variant 0, the
+    alternative schedule within 1e-12 of it, and (256^2, 512^2) within the
+    FP64 run tolerance of the oracle.  Variants whose double LDS tile would
+    not fit are refused."""
+    p = small_stem(n=n, ns=3, nconf=1)
+    base = _run_forced(p, 0, 0, 0)
+    if n <= 512:
+        e = rel_l2(base, oracle.run(p, "f64").image_tot)
+        print("n=%d variant 0: rel L2 vs FP64 oracle %.2e" % (n, e))
+        assert e < TOL_RUN
+    ran = 0
+    for v in range(1, engine.PASS_VARIANTS):
+        for which, (rv, cv) in (("row", (v, 0)), ("col", (0, v))):
+            try:
+                img = _run_forced(p, rv, cv, 0)
+            except engine.EngineError as err:
+                assert err.status == 1, err
+                print("n=%d %s variant %d: refused" % (n, which, v))
+                continue
+            assert v not in FP64_TWIDDLE_TWINS, (which, v)
+            ran += 1
+            e = rel_l2(img, base)
+            print("n=%d %s variant %d: rel L2 vs variant 0 %.2e" % (n, which, v, e))
+            if v in FP64_SAME_SCHEDULE:
+                assert np.array_equal(img, base), (which, v)
+            else:
+                assert e < 1e-12, (which, v)
+    assert ran >= 8
+
+
+def test_fp64_detector_pass_has_one_variant():
+    p = small_stem(n=256, ns=2, nconf=1)
+    with pytest.raises(engine.EngineError) as ei:
+        _run_forced(p, 0, 0, 2)
+    assert ei.value.status == 1

@@ -65,11 +65,18 @@ def test_progress_rises_to_niter():
 
 def test_request_stop_from_second_thread():
     p = _stem(ns=16, nconf=8)
+    niter = 16 * 16 * 8
     engine.set_batch(4, True)                  # 64 batches x 8 configurations: many stop points
     try:
         th, res = _start(p)
         t0 = time.time()
-        while th.is_alive() and engine.progress()[0] == 0 and time.time() - t0 < 60:
+
+        def started():
+            # this run's counters (an earlier run's stay until this one resets them)
+            it, nit = engine.progress()
+            return nit == niter and it > 0
+
+        while th.is_alive() and not started() and time.time() - t0 < 60:
             time.sleep(0.0005)
         engine.request_stop()
         th.join()

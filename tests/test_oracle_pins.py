@@ -28,7 +28,7 @@ def test_bessel_against_scipy():
         assert err < tol, (which, err)
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (128, 32), (256, 512)])
+@pytest.mark.parametrize("shape", [(64, 64), (128, 32), (256, 512), (96, 80), (1536, 64), (160, 1280), (384, 640)])
 def test_fft_against_numpy(shape):
     rng = np.random.default_rng(3)
     a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
