@@ -102,7 +102,7 @@ typedef struct {
 
     int potential_slicing;     /* 1 planes, 2 dz_Proj, 3 dz_Sub (4 auto: refused) */
 
-    int nx, ny;
+    int nx, ny;                /* 2^k, 3 x 2^k or 5 x 2^k in [64, 2048] (others: MT_ERR_UNSUPPORTED) */
     int bwl;
 
     int iw_type;               /* 1 plane, 2 convergent, 3 user defined, 4 auto */
