@@ -38,11 +38,13 @@ def parse():
                          "512: 12x12x25, 1024: 24x24x50) instead of Au[001] (for profiling)")
     ap.add_argument("--batch", type=int, default=0, help="waves per batch (0: planner)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--wave-pad", type=int, default=-1,
+                    help="row pitch pad of the wave batches in elements (-1: the engine's default)")
     ap.add_argument("--variants", default="", help="row,col,col_det launch variants to force (default: tuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ns", type=int, default=16)
     ap.add_argument("--no-group", action="store_true", help="skip the uneven-size group measurement")
-    ap.add_argument("--group-batch", type=int, default=256,
+    ap.add_argument("--group-batch", type=int, default=0,
                     help="waves per batch of each group member (0: each member's planner and tuner)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path measurement")
@@ -89,7 +91,7 @@ def cpu_baseline(args):
                       "FP32 C oracle (own radix-2 FFT, FFTW absent), %.1f s" % (args.cpu_ns, args.cpu_ns, nsl, ps, dt)}
 
 
-def group_measurement(group_batch=256):
+def group_measurement(group_batch=0):
     """SrTiO3 [001] at 256^2 / 512^2 / 1024^2 (21 / 51 / 101 slices), 16x16
     probes and 4 phonon configurations each, group_batch probes per batch
     (0: each member's planner and tuner pick its width): one group
@@ -178,12 +180,18 @@ def surrogate_measurement():
     256^2 grid, illumination tilt up to 1 degree): GPU time of plain
     64-configuration multislice over that of the surrogate path
     (2-configuration run + potential map + network), and relative RMS errors
-    against the 64-configuration image and against a 256-configuration image
-    (closer to converged; the 64-configuration image's own error is listed
-    beside)."""
+    against the 64-configuration image and against an independent
+    256-configuration image (another phonon seed; the 64-configuration
+    image's own error is listed beside), with the errors against the
+    converged image estimated by removing the 256-configuration image's own
+    noise (surrogate.evaluate); then one specimen at a 128x128 scan (4 x 4
+    unit cells, the training's 32 points per cell) for the time ratio and
+    the error against its 64-configuration image."""
     from multem_amd import surrogate
     model = surrogate.Surrogate.load(SURROGATE_WEIGHTS)
     ev = surrogate.evaluate(model, n_specimens=4, seed=2024, n_ref=256, tilt_max=1.0)
+    e128 = surrogate.evaluate(model, n_specimens=1, seed=2025, ns=128, tilt_max=1.0)
+    r128 = e128["specimens"][0]
     return {"metric": "surrogate ADF: plain 64-config GPU time / surrogate GPU time",
             "gpu_time_ratio": ev["gpu_time_ratio"],
             "rel_rms_surrogate_vs_64conf": ev["rel_rms_surrogate_mean"],
@@ -191,9 +199,18 @@ def surrogate_measurement():
             "rel_rms_surrogate_vs_256conf": ev["rel_rms_surrogate_ref_mean"],
             "rel_rms_2conf_vs_256conf": ev["rel_rms_small_ref_mean"],
             "rel_rms_64conf_vs_256conf": ev["rel_rms_big_ref_mean"],
+            "rel_rms_vs_converged_est": {"surrogate": ev["rel_rms_surrogate_conv_est"],
+                                         "2conf": ev["rel_rms_small_conv_est"],
+                                         "64conf": ev["rel_rms_big_conv_est"]},
             "ms_plain": sum(r["ms_plain"] for r in ev["specimens"]),
             "ms_surrogate": sum(r["ms_surrogate"] for r in ev["specimens"]),
-            "specimens": ["%s %.0f A tilt %.2f deg" % (r["desc"], r["thick"], r["theta"]) for r in ev["specimens"]]}
+            "specimens": ["%s %.0f A tilt %.2f deg" % (r["desc"], r["thick"], r["theta"]) for r in ev["specimens"]],
+            "scan_128": {"gpu_time_ratio": e128["gpu_time_ratio"],
+                         "rel_rms_surrogate_vs_64conf": e128["rel_rms_surrogate_mean"],
+                         "rel_rms_2conf_vs_64conf": e128["rel_rms_small_mean"],
+                         "ms_plain": r128["ms_plain"], "ms_surrogate": r128["ms_surrogate"],
+                         "specimen": "%s %.0f A tilt %.2f deg, 128x128 scan over 4x4 cells"
+                                     % (r128["desc"], r128["thick"], r128["theta"])}}
 
 
 def copy_calibration():
@@ -250,6 +267,7 @@ def main():
     else:
         p = configs.au_stem_512(ns=args.ns, nconf=args.nconf, n=args.n)
     engine.set_batch(args.batch, not args.no_graph)
+    engine.set_wave_pad(args.wave_pad)
     if args.variants:
         engine.set_pass_variants(*[int(v) for v in args.variants.split(",")])
     nsl = len(engine.spec_slicing(p))

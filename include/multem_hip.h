@@ -338,6 +338,12 @@ mt_status mt_hip_synchronize(int device);
    loop of a batch is replayed from a captured hipGraph. */
 void mt_hip_set_batch(int batch, int use_graph);
 
+/* Row pitch of the STEM wave batches: ny + pad complex elements (pad < 0:
+   the default, 0).  A padded pitch moves the column pass's strided row
+   segments to other HBM channels; results do not depend on it.  No
+   counterpart in the reference (a layout choice of this engine). */
+void mt_hip_set_wave_pad(int pad);
+
 /* Per-kernel device time of the last run, from HIP events around the row and
    column passes of the first batch of each configuration: summed ms and
    launch counts; ms_tf is the transmission-function build time. */

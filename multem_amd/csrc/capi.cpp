@@ -287,6 +287,11 @@ void mt_hip_set_batch(int batch, int use_graph)
     mtg::run_control().use_graph = use_graph != 0;
 }
 
+void mt_hip_set_wave_pad(int pad)
+{
+    mtg::run_control().wave_pad = pad < 0 ? -1 : pad;
+}
+
 // Slice-step launch variants: mode 0 fixed (variant 0 or the forced ones),
 // 1 tuned once per shape and cached (default), 2 tuned on every run.
 void mt_hip_set_autotune(int mode)

@@ -62,6 +62,7 @@ struct RowArgsT {
     const Z *src;
     Z *dst;
     long long member_stride;      // complex elements between members
+    long long pitch;              // complex elements between rows of src / dst (0: the row length)
     int nrows;                    // rows per member (= nx)
     int n_members;
     int flags;
@@ -97,7 +98,8 @@ struct ColArgsT {
     const Z *src;
     Z *dst;
     long long member_stride;
-    int ncols;                    // = ny (row length, element stride of x)
+    long long pitch;              // element stride of x in src / dst (0: ncols)
+    int ncols;                    // = ny (row length, element stride of x in the other tables)
     int n_members;
     int flags;
     const Z *tw;             // twiddles W_nx

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(RW *S::T) void k_row_pass(RowArgsT<Z> a)
     auto idx = [](int i) { return i + (i >> 3); };
     auto sync = [] { __syncthreads(); };
 
-    const long long off = (long long)member * a.member_stride + (long long)row * N;
+    const long long off = (long long)member * a.member_stride + (long long)row * (a.pitch ? a.pitch : N);
     Z v[E], tv[E];
     // issue the wave and transmission loads together, then the twiddles
     if (flags & ROW_LOAD) {
@@ -173,13 +173,14 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
     auto idx = [c](int i) { return i * C + c; };
     auto sync = [] { __syncthreads(); };
     const long long base = (long long)member * a.member_stride + col;
+    const long long pitch = a.pitch ? a.pitch : a.ncols;   // x stride of the wave buffers
     const int igy = signed_freq(col, a.ny);
 
     Z v[E];
     if constexpr (!PROBE) {
         if (flags & COL_LOAD) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = a.src[base + (long long)(t + T * e) * a.ncols];
+            for (int e = 0; e < E; ++e) v[e] = a.src[base + (long long)(t + T * e) * pitch];
         }
     }
     TW tw;
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_pass(ColArgsT<Z> a)
     if (flags & COL_IFFT) fft_run<S, true>(v, t, tw, lds, idx, sync);
     if (flags & COL_STORE) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * a.ncols] = v[e];
+        for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * pitch] = v[e];
     }
 }
 
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(RW *S::T) void k_row_slice_pp(RowArgsT<Z> a)
         int rg, member;
         map_rows(item, G, a.n_members, rg, member);
         const int row = rg * RW + r;
-        noff = (long long)member * a.member_stride + (long long)row * N;
+        noff = (long long)member * a.member_stride + (long long)row * (a.pitch ? a.pitch : N);
         const Z *T_s = a.trans + (long long)a.members[member].tslot * a.tslot_stride +
                        (long long)a.slice * a.nrows * N + (long long)row * N;
 #pragma unroll
@@ -402,6 +403,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_slice_pp(ColArgsT<Z> a)
     const int tid = threadIdx.x;
     const int c = tid % C, t = tid / C;
     const int ntile = a.ncols / C, n_items = ntile * a.n_members;
+    const long long pitch = a.pitch ? a.pitch : a.ncols;   // x stride of the wave buffers
     auto idx = [c](int i) { return i * C + c; };
     auto sync = [] { __syncthreads(); };
     Z nv[E], npy;
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_slice_pp(ColArgsT<Z> a)
         const int col = (item % ntile) * C + c;
         nbase = (long long)(item / ntile) * a.member_stride + col;
 #pragma unroll
-        for (int e = 0; e < E; ++e) nv[e] = a.src[nbase + (long long)(t + T * e) * a.ncols];
+        for (int e = 0; e < E; ++e) nv[e] = a.src[nbase + (long long)(t + T * e) * pitch];
         npy = a.py[col];
     };
 
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(C *S::T) void k_col_slice_pp(ColArgsT<Z> a)
         for (int e = 0; e < E; ++e) v[e] = cmul(cmul(pxv[e], pyv), v[e]);
         fft_run<S, true>(v, t, tw, lds, idx, sync);
 #pragma unroll
-        for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * a.ncols] = v[e];
+        for (int e = 0; e < E; ++e) a.dst[base + (long long)(t + T * e) * pitch] = v[e];
     }
 }
 

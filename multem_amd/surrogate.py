@@ -163,10 +163,19 @@ def evaluate(model, n_specimens=4, seed=2024, ns=32, n_small=2, n_big=64, n_ref=
     n_small-configuration run + the potential-sum build + the network
     forward (device times from HIP events).  Errors are relative RMS
     against the n_big-configuration image, for the surrogate and for the
-    n_small-configuration image it starts from; with n_ref > 0 also against
-    an n_ref-configuration image (closer to converged), for those two and
-    for the n_big-configuration image itself.  tilt_max: largest
-    illumination tilt of the specimens (degrees)."""
+    n_small-configuration image it starts from.
+
+    With n_ref > 0 they are also measured against an n_ref-configuration
+    image drawn with another phonon seed, so that its configurations are
+    independent of the other two runs', for those two and for the
+    n_big-configuration image itself.  The reference's own phonon noise
+    then adds to every squared error alike: E|x - ref|^2 = E|x - conv|^2 +
+    v_ref for anything x independent of it, with conv the converged image;
+    since the n_big and n_ref images are both configuration means,
+    v_ref = E|big - ref|^2 n_big / (n_big + n_ref).  The *_conv_est means
+    are sqrt(mean(e_ref^2) - v_ref) over the specimens: the errors against
+    the converged image, estimated.  tilt_max: largest illumination tilt of
+    the specimens (degrees)."""
     from . import engine, surrogate_data
     rng = np.random.default_rng(seed)
     rows = []
@@ -190,6 +199,7 @@ def evaluate(model, n_specimens=4, seed=2024, ns=32, n_small=2, n_big=64, n_ref=
         if n_ref:
             q = p.copy()
             q.pn_nconf = n_ref
+            q.pn_seed = (p.pn_seed + 104729) % (2 ** 31 - 1) or 1   # independent configurations
             ref = engine.run_multislice(q).image_tot[0, 0].copy()
             row.update(rel_rms_surrogate_ref=_rel_rms(pred, ref), rel_rms_small_ref=_rel_rms(small, ref),
                        rel_rms_big_ref=_rel_rms(big, ref))
@@ -200,6 +210,9 @@ def evaluate(model, n_specimens=4, seed=2024, ns=32, n_small=2, n_big=64, n_ref=
                rel_rms_surrogate_mean=float(np.mean([r["rel_rms_surrogate"] for r in rows])),
                rel_rms_small_mean=float(np.mean([r["rel_rms_small"] for r in rows])))
     if n_ref:
+        ms2 = {k: float(np.mean([r["rel_rms_%s_ref" % k] ** 2 for r in rows])) for k in ("surrogate", "small", "big")}
+        v_ref = ms2["big"] * n_big / (n_big + n_ref)
         for k in ("surrogate", "small", "big"):
             out["rel_rms_%s_ref_mean" % k] = float(np.mean([r["rel_rms_%s_ref" % k] for r in rows]))
+            out["rel_rms_%s_conv_est" % k] = float(np.sqrt(max(ms2[k] - v_ref, 0.0)))
     return out

@@ -3,7 +3,8 @@
 One pair is a seeded random specimen with
   input  channel 0: the 2-configuration frozen-phonon ADF image,
          channel 1: the still-atom projected potential summed over all
-                    slices, sampled at the scan points,
+                    slices along the (possibly tilted) beam path, sampled
+                    at the scan points,
   target          : the 64-configuration ADF image of the same specimen
 (configurations 1..2 of the 64-configuration run are the 2-configuration
 run's, since configuration i is seeded by (pn_seed, i)).
@@ -11,7 +12,9 @@ run's, since configuration i is seeded by (pn_seed, i)).
 Specimens: fcc, diamond-cubic or perovskite [001], a in [3.5, 5.5] A, Z in
 6..82 (perovskite X site O), rms displacement 0.05-0.12 A, thickness
 20-200 A; 300 kV, 20-25 mrad probe, zero defocus, Cs = 0, 60-200 mrad ring;
-ns x ns periodic scan over one unit cell; the lateral box is the largest
+ns x ns periodic scan over ns/32 unit cells per side (one cell at the
+training size ns = 32, so every scan samples a cell by 32 points; a scan
+larger than the box repeats it, the specimen being periodic); the lateral box is the largest
 number of cells that keeps the 256^2 grid's Nyquist frequency beyond the
 200 mrad ring.  Illumination tilt theta is uniform in [0, tilt_max] degrees
 with a uniform azimuth (tilt_max = 0 by default: untilted).
@@ -69,8 +72,9 @@ def random_specimen(rng, ns=32, nconf=64, tilt_max=0.0):
     p.scanning_square_pxs = 1
     p.scanning_ns = ns
     c0 = nl // 2
+    k = max(1, ns // 32)   # cells scanned per side: 32 scan points per cell
     p.scanning_x0 = p.scanning_y0 = c0 * a
-    p.scanning_xe = p.scanning_ye = (c0 + 1) * a
+    p.scanning_xe = p.scanning_ye = (c0 + k) * a
     p.detector_type = 1
     p.detector_cir = [(60.0, 200.0)]
     if tilt_max > 0:
@@ -82,10 +86,25 @@ def random_specimen(rng, ns=32, nconf=64, tilt_max=0.0):
 
 def potential_map(p):
     """Still-atom projected potential summed over all slices, sampled (bilinear,
-    periodic) at the scan points: [ns, ns] with [ix, iy] as the ADF image."""
+    periodic) at the scan points: [ns, ns] with [ix, iy] as the ADF image.
+
+    Under an illumination tilt (theta, phi) the wave drifts laterally by
+    -z sin(theta) (cos phi, sin phi) at depth z below the entrance surface
+    (the tilted propagator, |g - g0|^2), so the probe entering at r meets
+    the atoms near r - z sin(theta) u at depth z.  The map is then the
+    projection along the tilted path: every atom is moved by
+    +z sin(theta) u (periodically wrapped) before projecting."""
     q = p.copy()
     q.pn_model = 1
     q.pn_nconf = 1
+    if p.theta:
+        at = np.array(q.spec_atoms, dtype=np.float64, copy=True)
+        t, f = np.radians(p.theta), np.radians(p.phi)
+        zr = at[:, 3] - at[:, 3].min()
+        at[:, 1] = np.mod(at[:, 1] + zr * np.sin(t) * np.cos(f), p.spec_lx)
+        at[:, 2] = np.mod(at[:, 2] + zr * np.sin(t) * np.sin(f), p.spec_ly)
+        q.spec_atoms = at
+        q.theta = 0.0
     V = engine.projected_potential_sum(q, 1)   # [nx, ny], pixel ix at x = ix*dx
     n = p.scanning_ns
     xs = p.scanning_x0 + (p.scanning_xe - p.scanning_x0) * np.arange(n) / n

@@ -495,3 +495,22 @@ def test_tuner_cache_separates_generic_and_specialised_passes():
                 assert "col_det=v0(generic)" in engine.tune_log() or "cached" in engine.tune_log()
     finally:
         engine.set_batch(0, True)
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_wave_pitch_does_not_change_the_image(n):
+    """STEM wave batches stored at row pitch ny (the default), ny + 16 and
+    ny + 40: bit-identical images (launch variants are fixed so that all
+    three runs use the same kernels)."""
+    p = small_stem(n=n, ns=3, nconf=2)
+    imgs = []
+    engine.set_pass_variants(0, 3, 0)
+    try:
+        for pad in (0, 16, 40):
+            engine.set_wave_pad(pad)
+            imgs.append(engine.run_multislice(p).image_tot.copy())
+    finally:
+        engine.set_wave_pad(-1)
+        engine.set_pass_variants(-1, -1, -1)
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
+    assert rel_l2(imgs[1], oracle.run(p, "f64").image_tot) < 1e-4
