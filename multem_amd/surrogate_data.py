@@ -80,7 +80,7 @@ def random_specimen(rng, ns=32, nconf=64, tilt_max=0.0):
     if tilt_max > 0:
         p.theta = float(rng.uniform(0.0, tilt_max))
         p.phi = float(rng.uniform(0.0, 360.0))
-    meta = dict(kind=kind, a=a, sigma=sigma, thick=nc * a, cells=nl, desc=desc)
+    meta = dict(kind=kind, a=a, sigma=sigma, thick=nc * a, cells=nl, desc=desc, theta=p.theta, phi=p.phi)
     return p, meta
 
 
@@ -169,7 +169,8 @@ def main():
     np.savez_compressed(a.out, adf_small=S.astype(np.float32), vmap=Vm.astype(np.float32),
                         adf_big=T.astype(np.float32), a=np.array([m["a"] for m in metas]),
                         thick=np.array([m["thick"] for m in metas]), kind=np.array([m["kind"] for m in metas]),
-                        seed=a.seed)
+                        theta=np.array([m["theta"] for m in metas]), phi=np.array([m["phi"] for m in metas]),
+                        seed=a.seed, tilt_max=a.tilt_max)
     print("wrote %d pairs to %s" % (len(S), a.out), file=sys.stderr)
 
 

@@ -20,15 +20,21 @@ from .surrogate import SurrogateNet, init_params, normalise
 
 def load_pairs(paths, eval_per_file=0):
     """Training pairs of all files, then held-out pairs: the last
-    eval_per_file pairs of each file (none if 0)."""
-    S, V, T, Se, Ve, Te = [], [], [], [], [], []
+    eval_per_file pairs of each file (none if 0).  Also returns, for the
+    held-out pairs, the generation seed of their file and whether the pair
+    was tilted (-1 where a file does not record it)."""
+    S, V, T, Se, Ve, Te, src, tilt = [], [], [], [], [], [], [], []
     for p in paths:
         d = np.load(p, allow_pickle=False)
         k = len(d["adf_small"]) - eval_per_file
         S.append(d["adf_small"][:k]); V.append(d["vmap"][:k]); T.append(d["adf_big"][:k])
         Se.append(d["adf_small"][k:]); Ve.append(d["vmap"][k:]); Te.append(d["adf_big"][k:])
+        n_e = len(d["adf_small"]) - k
+        src.append(np.full(n_e, int(d["seed"]) if "seed" in d.files else -1, dtype=np.int64))
+        tilt.append((d["theta"][k:] > 0).astype(np.int8) if "theta" in d.files else np.full(n_e, -1, np.int8))
+    held = dict(source_seed=np.concatenate(src), tilted=np.concatenate(tilt))
     return (np.concatenate(S + Se), np.concatenate(V + Ve), np.concatenate(T + Te),
-            sum(len(x) for x in Se))
+            sum(len(x) for x in Se), held)
 
 
 def rel_rms(pred, ref):
@@ -52,7 +58,7 @@ def main():
     ap.add_argument("--eval-out", default=None, help="held-out pairs (.npz)")
     a = ap.parse_args()
 
-    S, V, T, n_held = load_pairs(a.data, a.eval_per_file)
+    S, V, T, n_held, held = load_pairs(a.data, a.eval_per_file)
     n = len(S)
     ntr = n - (n_held if a.eval_per_file else a.n_eval)
     x, s = normalise(S, V)
@@ -68,7 +74,8 @@ def main():
     e_base = rel_rms(S[ntr:], T[ntr:])
     np.savez(a.out, theta=theta, n_blocks=a.blocks, channels=a.channels, losses=losses)
     if a.eval_out:
-        np.savez_compressed(a.eval_out, adf_small=S[ntr:], vmap=V[ntr:], adf_big=T[ntr:])
+        extra = held if a.eval_per_file else {}
+        np.savez_compressed(a.eval_out, adf_small=S[ntr:], vmap=V[ntr:], adf_big=T[ntr:], **extra)
     print(json.dumps({"pairs_train": int(ntr), "pairs_eval": int(n - ntr), "steps": a.steps, "batch": a.batch,
                       "train_s": t_train, "loss_first": float(losses[:20].mean()),
                       "loss_last": float(losses[-50:].mean()),

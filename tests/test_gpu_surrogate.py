@@ -108,22 +108,33 @@ def _rel_rms(a, b):
 
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained surrogate weights")
 def test_trained_surrogate_gain():
-    """Held-out pairs (never seen in training): the network's error against
-    the 64-configuration image, next to that of the 2-configuration image it
-    starts from."""
+    """Held-out pairs (never seen in training; the last 40 of each of the five
+    data files: 80 untilted, 120 with illumination tilt up to 1 degree): the
+    network's error against the 64-configuration image, next to that of the
+    2-configuration image it starts from.
+
+    Measured with the committed weights (the CPU oracle's bf16 forward pass
+    gives the same figures): untilted mean ratio 0.51, network better on 79
+    of 80; tilted mean ratio 0.73, better on 99 of 120.  The bars below sit
+    above those figures as regression guards.  The 64-configuration image is
+    not the converged one (its own phonon noise is about 1.5 %), so the
+    network is measured against a converged estimate in
+    surrogate.evaluate and in the bench."""
     from multem_amd.surrogate import Surrogate
     model = Surrogate.load(WEIGHTS)
     d = np.load(EVAL)
     pred = model.predict(d["adf_small"], d["vmap"])
     e_net = _rel_rms(pred, d["adf_big"])
     e_base = _rel_rms(d["adf_small"], d["adf_big"])
-    print("held-out pairs %d: rel RMS vs 64 configs: network %.4f (max %.4f), 2 configs %.4f (max %.4f); "
-          "network better on %d" % (len(e_net), e_net.mean(), e_net.max(), e_base.mean(), e_base.max(),
-                                    int(np.sum(e_net < e_base))))
-    # measured: mean ratio 0.53, better on 39 of 40 pairs (the CPU oracle's
-    # forward pass gives the same figures)
-    assert e_net.mean() < 0.6 * e_base.mean()
-    assert np.mean(e_net < e_base) >= 0.95
+    tilted = d["tilted"].astype(bool)
+    for name, sel, ratio, better in (("untilted", ~tilted, 0.6, 0.95), ("tilted", tilted, 0.8, 0.75)):
+        en, eb = e_net[sel], e_base[sel]
+        print("held-out %s pairs %d: rel RMS vs 64 configs: network %.4f (max %.4f), 2 configs %.4f (max %.4f); "
+              "ratio %.3f, network better on %d" % (name, len(en), en.mean(), en.max(), eb.mean(), eb.max(),
+                                                    en.mean() / eb.mean(), int(np.sum(en < eb))))
+        assert len(en) > 0
+        assert en.mean() < ratio * eb.mean(), name
+        assert np.mean(en < eb) >= better, name
 
 
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained surrogate weights")

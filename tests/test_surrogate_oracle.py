@@ -130,3 +130,26 @@ def test_identity_at_zero_head(rounding):
     x = np.random.default_rng(5).standard_normal((1, 4, 4, 2))
     y, _ = so.forward(theta, x, 2, 4, rounding)
     assert np.array_equal(y, x[..., 0])
+
+
+def test_committed_weights_gain_with_the_oracle_forward():
+    """The committed surrogate weights on the committed held-out pairs,
+    evaluated by the CPU oracle's forward pass with the GPU's bf16 rounding
+    points: the figures the GPU test asserts (untilted mean ratio to the
+    2-configuration error about 0.51, tilted about 0.73)."""
+    import os
+    from multem_amd.surrogate import normalise
+    data = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "multem_amd", "data")
+    w = np.load(os.path.join(data, "surrogate_6x32.npz"))
+    d = np.load(os.path.join(data, "surrogate_eval.npz"))
+    x, s = normalise(d["adf_small"], d["vmap"])
+    y, _ = so.forward(w["theta"], x, int(w["n_blocks"]), int(w["channels"]), rounding="bf16")
+
+    def rel_rms(a, b):
+        return np.sqrt(np.mean((a - b) ** 2, axis=(-2, -1)) / np.mean(b ** 2, axis=(-2, -1)))
+
+    e_net = rel_rms(y * s, d["adf_big"])
+    e_base = rel_rms(d["adf_small"], d["adf_big"])
+    tilted = d["tilted"].astype(bool)
+    for sel, ratio in ((~tilted, 0.6), (tilted, 0.8)):
+        assert e_net[sel].mean() < ratio * e_base[sel].mean()
