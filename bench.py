@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--group-batch", type=int, default=0,
                     help="waves per batch of each group member (0: each member's planner and tuner)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
+    ap.add_argument("--surrogate-scan128", action="store_true",
+                    help="also time the surrogate on one 128x128-scan specimen (about a minute of GPU time)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM counters")
@@ -175,7 +177,7 @@ def fp64_measurement():
 SURROGATE_WEIGHTS = os.path.join(ROOT, "multem_amd", "data", "surrogate_6x32.npz")
 
 
-def surrogate_measurement():
+def surrogate_measurement(scan128=False):
     """Frozen-phonon surrogate on 4 held-out random specimens (32x32 scans,
     256^2 grid, illumination tilt up to 1 degree): GPU time of plain
     64-configuration multislice over that of the surrogate path
@@ -184,15 +186,13 @@ def surrogate_measurement():
     256-configuration image (another phonon seed; the 64-configuration
     image's own error is listed beside), with the errors against the
     converged image estimated by removing the 256-configuration image's own
-    noise (surrogate.evaluate); then one specimen at a 128x128 scan (4 x 4
-    unit cells, the training's 32 points per cell) for the time ratio and
-    the error against its 64-configuration image."""
+    noise (surrogate.evaluate); with scan128, also one specimen at a 128x128
+    scan (4 x 4 unit cells, the training's 32 points per cell) for the time
+    ratio and the error against its 64-configuration image."""
     from multem_amd import surrogate
     model = surrogate.Surrogate.load(SURROGATE_WEIGHTS)
     ev = surrogate.evaluate(model, n_specimens=4, seed=2024, n_ref=256, tilt_max=1.0)
-    e128 = surrogate.evaluate(model, n_specimens=1, seed=2025, ns=128, tilt_max=1.0)
-    r128 = e128["specimens"][0]
-    return {"metric": "surrogate ADF: plain 64-config GPU time / surrogate GPU time",
+    out = {"metric": "surrogate ADF: plain 64-config GPU time / surrogate GPU time",
             "gpu_time_ratio": ev["gpu_time_ratio"],
             "rel_rms_surrogate_vs_64conf": ev["rel_rms_surrogate_mean"],
             "rel_rms_2conf_vs_64conf": ev["rel_rms_small_mean"],
@@ -204,13 +204,17 @@ def surrogate_measurement():
                                          "64conf": ev["rel_rms_big_conv_est"]},
             "ms_plain": sum(r["ms_plain"] for r in ev["specimens"]),
             "ms_surrogate": sum(r["ms_surrogate"] for r in ev["specimens"]),
-            "specimens": ["%s %.0f A tilt %.2f deg" % (r["desc"], r["thick"], r["theta"]) for r in ev["specimens"]],
-            "scan_128": {"gpu_time_ratio": e128["gpu_time_ratio"],
-                         "rel_rms_surrogate_vs_64conf": e128["rel_rms_surrogate_mean"],
-                         "rel_rms_2conf_vs_64conf": e128["rel_rms_small_mean"],
-                         "ms_plain": r128["ms_plain"], "ms_surrogate": r128["ms_surrogate"],
-                         "specimen": "%s %.0f A tilt %.2f deg, 128x128 scan over 4x4 cells"
-                                     % (r128["desc"], r128["thick"], r128["theta"])}}
+            "specimens": ["%s %.0f A tilt %.2f deg" % (r["desc"], r["thick"], r["theta"]) for r in ev["specimens"]]}
+    if scan128:
+        e128 = surrogate.evaluate(model, n_specimens=1, seed=2025, ns=128, tilt_max=1.0)
+        r128 = e128["specimens"][0]
+        out["scan_128"] = {"gpu_time_ratio": e128["gpu_time_ratio"],
+                           "rel_rms_surrogate_vs_64conf": e128["rel_rms_surrogate_mean"],
+                           "rel_rms_2conf_vs_64conf": e128["rel_rms_small_mean"],
+                           "ms_plain": r128["ms_plain"], "ms_surrogate": r128["ms_surrogate"],
+                           "specimen": "%s %.0f A tilt %.2f deg, 128x128 scan over 4x4 cells"
+                                       % (r128["desc"], r128["thick"], r128["theta"])}
+    return out
 
 
 def copy_calibration():
@@ -355,7 +359,7 @@ def main():
     if not args.no_fp64:
         out["fp64"] = fp64_measurement()
     if not args.no_surrogate and os.path.exists(SURROGATE_WEIGHTS):
-        out["surrogate"] = surrogate_measurement()
+        out["surrogate"] = surrogate_measurement(args.surrogate_scan128)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))

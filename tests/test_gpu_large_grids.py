@@ -21,8 +21,9 @@ Workloads:
       test test_boundary.py::test_fp32_slicing_quirk_srtio3;
   Au [001] 2048^2, 16x16 cells, 2 cells thick (5 slices);
   Au [001] 512^2, 8x8x20 cells (41 slices), 11x11 scan, 2 configurations,
-      batch 120 and launch variants 3/3/3 -- the specimen, grid, batch and
-      variants of the 64x64-scan, 16-configuration benchmark;
+      batch 118 and launch variants 8/3/3 (row, column, detector) -- the
+      specimen, grid, batch and variants of the 64x64-scan,
+      16-configuration benchmark;
   Pt [110] CBED 1024^2, 200 kV, 10 mrad, -100 A defocus, 20 configurations,
       full depth (thicknesses every 50 A to 450 A, 325 slices): FP32 wave
       against the FP64 wave path, per thickness; the FP64 path is itself
@@ -134,11 +135,13 @@ def test_stem_2048_variant_sweep():
 
 def test_benchmarked_stem_plan():
     """The benchmark's specimen and plan (Au [001] 8x8x20 cells, 41 slices,
-    512^2, batch 120, variants 3/3/3) on an 11x11 scan over one cell with 2
-    phonon configurations: 121 probes = one full batch and a tail of one."""
+    512^2, batch 118, variants 8/3/3: persistent row pass, alternative
+    schedule column and detector passes) on an 11x11 scan over one cell
+    with 2 phonon configurations: 121 probes = one full batch and a tail of
+    three."""
     p = configs.au_stem_512(ns=11, nconf=2)
-    out, plan = _forced(p, 3, 3, 3, batch=120)
-    assert "batch=120 " in plan and "row_variant=3 col_variant=3 col_det_variant=3" in plan, plan
+    out, plan = _forced(p, 8, 3, 3, batch=118)
+    assert "batch=118 " in plan and "row_variant=8 col_variant=3 col_det_variant=3" in plan, plan
     ref = oracle.run(p, "f64").image_tot
     e = rel_l2(out.image_tot, ref)
     print("benchmarked plan (%s) on 11x11 probes x 2 configs: rel L2 vs FP64 %.3e" % (plan, e))
