@@ -622,12 +622,28 @@ static int fft2_init(fft2 *p, int nx, int ny)
 }
 static void fft2_free(fft2 *p) { fft1_free(&p->fx); fft1_free(&p->fy); }
 
-/* 2D: along y (contiguous) for each ix, then along x for blocks of 16 columns */
+/* Work buffer of fft2_exec: 16 interleaved sequences of the longer side. */
+static size_t fft2_work_elems(int nx, int ny) { return (size_t)(nx > ny ? nx : ny) * 16; }
+
+/* 2D: along y for blocks of 16 rows, then along x for blocks of 16 columns,
+   each block gathered into 16 interleaved sequences (work[k*16 + b]) so the
+   butterflies run over 16 independent sequences at once. */
 static void fft2_exec(const fft2 *p, cplx *a, int inverse, cplx *work)
 {
     const int nx = p->nx, ny = p->ny;
-    for (int ix = 0; ix < nx; ix++) fft1_exec_multi(&p->fy, a + (size_t)ix * ny, 1, inverse);
     const int B = 16;
+    for (int ix0 = 0; ix0 < nx; ix0 += B) {
+        int nb = (nx - ix0 < B) ? nx - ix0 : B;
+        for (int b = 0; b < nb; b++) {
+            const cplx *row = a + (size_t)(ix0 + b) * ny;
+            for (int k = 0; k < ny; k++) work[(size_t)k * nb + b] = row[k];
+        }
+        fft1_exec_multi(&p->fy, work, nb, inverse);
+        for (int b = 0; b < nb; b++) {
+            cplx *row = a + (size_t)(ix0 + b) * ny;
+            for (int k = 0; k < ny; k++) row[k] = work[(size_t)k * nb + b];
+        }
+    }
     for (int iy0 = 0; iy0 < ny; iy0 += B) {
         int nb = (ny - iy0 < B) ? ny - iy0 : B;
         for (int ix = 0; ix < nx; ix++)
@@ -643,7 +659,7 @@ int orc_fft2(int nx, int ny, double *data, int inverse)
     fft2 p;
     if (fft2_init(&p, nx, ny)) { set_err("fft sizes must be 2^k, 3 x 2^k or 5 x 2^k"); return MT_ERR_UNSUPPORTED; }
     size_t n = (size_t)nx * ny;
-    cplx *a = malloc(sizeof(cplx) * n), *w = malloc(sizeof(cplx) * (size_t)nx * 16);
+    cplx *a = malloc(sizeof(cplx) * n), *w = malloc(sizeof(cplx) * fft2_work_elems(nx, ny));
     for (size_t i = 0; i < n; i++) { a[i].re = (real)data[2 * i]; a[i].im = (real)data[2 * i + 1]; }
     fft2_exec(&p, a, inverse, w);
     for (size_t i = 0; i < n; i++) { data[2 * i] = a[i].re; data[2 * i + 1] = a[i].im; }
@@ -2255,7 +2271,7 @@ static void *stem_worker(void *arg)
     fft2 F;
     fft2_init(&F, g->nx, g->ny);
     cplx *psi = malloc(sizeof(cplx) * nxy), *fpsi = malloc(sizeof(cplx) * nxy);
-    cplx *po = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    cplx *po = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * fft2_work_elems(g->nx, g->ny));
     for (;;) {
         pthread_mutex_lock(&J->lock);
         int it = J->next++;
@@ -2315,7 +2331,7 @@ static void *trans_worker(void *arg)
     fft2 F;
     fft2_init(&F, g->nx, g->ny);
     real *V = malloc(sizeof(real) * nxy);
-    cplx *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    cplx *work = malloc(sizeof(cplx) * fft2_work_elems(g->nx, g->ny));
     J->st = (V && work) ? MT_OK : MT_ERR_OUT_OF_MEMORY;
     for (int is = J->t; is < s->n_slice && J->st == MT_OK; is += J->nt) {
         J->st = slice_potential(s, is, V);
@@ -2563,7 +2579,7 @@ static int run_ew_conf(sim_t *s, const wave_set *ws, cplx **po_thk)
     cplx *T = malloc(sizeof(cplx) * nxy);
     real *V = malloc(sizeof(real) * nxy);
     cplx *psi = malloc(sizeof(cplx) * nxy * (size_t)nw), *fpsi = malloc(sizeof(cplx) * nxy);
-    cplx *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    cplx *work = malloc(sizeof(cplx) * fft2_work_elems(g->nx, g->ny));
     int st = (T && V && psi && fpsi && work) ? MT_OK : MT_ERR_OUT_OF_MEMORY;
     if (st == MT_OK) {
         real c_10_0 = s->cond.c_10;
@@ -2818,7 +2834,7 @@ static int run_ew(sim_t *s, const mt_input_multislice *in, mt_output_multislice 
     fft2_init(&A.F, g->nx, g->ny);
     A.tmp = malloc(sizeof(cplx) * nxy);
     A.tmp2 = malloc(sizeof(cplx) * nxy);
-    A.work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    A.work = malloc(sizeof(cplx) * fft2_work_elems(g->nx, g->ny));
     wave_set ws;
     wave_set_build(s, &ws);
     int st = MT_OK;
@@ -2982,7 +2998,7 @@ int orc_transmission_function(const mt_system_conf *sys, const mt_input_multisli
     if (iconf >= 1 && is_frozen(s)) move_atoms(s, iconf);
     size_t nxy = (size_t)s->g.nx * s->g.ny;
     real *v = malloc(sizeof(real) * nxy);
-    cplx *t = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * (size_t)s->g.nx * 16);
+    cplx *t = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * fft2_work_elems(s->g.nx, s->g.ny));
     fft2 F;
     fft2_init(&F, s->g.nx, s->g.ny);
     st = slice_potential(s, islice, v);
@@ -3001,7 +3017,7 @@ int orc_incident_wave(const mt_system_conf *sys, const mt_input_multislice *in, 
     if (!s) return st;
     size_t nxy = (size_t)s->g.nx * s->g.ny;
     cplx *p = malloc(sizeof(cplx) * nxy), *f = malloc(sizeof(cplx) * nxy);
-    cplx *work = malloc(sizeof(cplx) * (size_t)s->g.nx * 16);
+    cplx *work = malloc(sizeof(cplx) * fft2_work_elems(s->g.nx, s->g.ny));
     fft2 F;
     fft2_init(&F, s->g.nx, s->g.ny);
     /* the stand-alone incident wave evaluates chi at g + g0 (Incident_Wave::operator(),
@@ -3027,7 +3043,7 @@ int orc_propagate(const mt_system_conf *sys, const mt_input_multislice *in, cons
     if (!s) return st;
     const grid_t *g = &s->g;
     size_t nxy = (size_t)g->nx * g->ny;
-    cplx *p = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * (size_t)g->nx * 16);
+    cplx *p = malloc(sizeof(cplx) * nxy), *work = malloc(sizeof(cplx) * fft2_work_elems(g->nx, g->ny));
     for (int ix = 0; ix < g->nx; ix++)
         for (int iy = 0; iy < g->ny; iy++) {
             size_t src = (size_t)ix * g->ny + iy;

@@ -47,6 +47,18 @@ def test_noalloc_tilt_and_bandwidth_limit():
             assert np.array_equal(engine.propagate_noalloc(p, psi, z, space), engine.propagate(p, psi, z, space))
 
 
+@pytest.mark.parametrize("nx,ny", [(768, 640), (96, 1280)])
+def test_noalloc_mixed_lengths(nx, ny):
+    """Grids of 3 x 2^k and 5 x 2^k points through the allocation-free entry."""
+    p = configs.si_ewrs_256()
+    p.nx, p.ny = nx, ny
+    psi = _wave(p, 7)
+    for space in (1, 2):
+        a = engine.propagate_noalloc(p, psi, 12.5, space)
+        assert np.array_equal(a, engine.propagate(p, psi, 12.5, space)), space
+        assert rel_l2(a, oracle.propagate(p, psi, 12.5, space, precision="f64")) < 1e-5
+
+
 def test_noalloc_refuses_grid_above_limit():
     p = configs.si_ewrs_256(n=2048)
     assert p.nx * p.ny > engine.NOALLOC_MAX_NXY
