@@ -344,6 +344,15 @@ void mt_hip_set_batch(int batch, int use_graph);
    counterpart in the reference (a layout choice of this engine). */
 void mt_hip_set_wave_pad(int pad);
 
+/* Concurrent chains of a STEM batch: the waves of a batch are split into
+   that many groups, each running its slice loop on its own stream inside
+   the batch's graph, so one group's passes fill the CUs while the other's
+   drain (0: automatic, two chains for grids of 512^2 and more whose batch
+   holds at least 128 MiB of waves, with the transmission stack resident;
+   1: one chain; at most 4).  Results do not depend on it.  No counterpart
+   in the reference. */
+void mt_hip_set_batch_chains(int chains);
+
 /* Per-kernel device time of the last run, from HIP events around the row and
    column passes of the first batch of each configuration: summed ms and
    launch counts; ms_tf is the transmission-function build time. */

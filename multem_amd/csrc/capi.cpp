@@ -292,6 +292,11 @@ void mt_hip_set_wave_pad(int pad)
     mtg::run_control().wave_pad = pad < 0 ? -1 : pad;
 }
 
+void mt_hip_set_batch_chains(int chains)
+{
+    mtg::run_control().chains = chains < 0 ? 0 : chains;
+}
+
 // Slice-step launch variants: mode 0 fixed (variant 0 or the forced ones),
 // 1 tuned once per shape and cached (default), 2 tuned on every run.
 void mt_hip_set_autotune(int mode)

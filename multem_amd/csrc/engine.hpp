@@ -17,6 +17,7 @@ struct RunControl {
     size_t memory_budget = 0;       // 0: derived from free device memory
     int batch_override = 0;         // 0: planner's choice
     int wave_pad = -1;              // STEM wave-batch row pitch = ny + pad; < 0: the default (0)
+    int chains = 0;                 // STEM batch split into concurrent chains; 0: automatic
     bool use_graph = true;
     bool time_kernels = true;       // per-kernel HIP-event timing of one batch per configuration
     int autotune = 1;               // 0: default launch variants; 1: time the candidates once per

@@ -52,6 +52,7 @@ def lib():
         L.mt_hip_last_timing.argtypes = [dp, dp, dp]
         L.mt_hip_set_batch.argtypes = [C.c_int, C.c_int]
         L.mt_hip_set_wave_pad.argtypes = [C.c_int]
+        L.mt_hip_set_batch_chains.argtypes = [C.c_int]
         L.mt_hip_kernel_timing.argtypes = [dp, P(C.c_int), dp, P(C.c_int), dp]
         L.mt_hip_synchronize.argtypes = [C.c_int]
         L.mt_hip_atom_table.argtypes = [C.c_int, C.c_double] + [dp] * 6
@@ -198,6 +199,12 @@ def atom_table(Z, dR_min):
 
 def set_batch(batch=0, use_graph=True):
     lib().mt_hip_set_batch(int(batch), int(bool(use_graph)))
+
+
+def set_chains(chains=0):
+    """Concurrent chains a STEM batch is split into (0: automatic, 1: one;
+    at most 4).  Results do not depend on it."""
+    lib().mt_hip_set_batch_chains(int(chains))
 
 
 def set_wave_pad(pad=-1):

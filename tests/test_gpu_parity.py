@@ -514,3 +514,33 @@ def test_wave_pitch_does_not_change_the_image(n):
         engine.set_pass_variants(-1, -1, -1)
     assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
     assert rel_l2(imgs[1], oracle.run(p, "f64").image_tot) < 1e-4
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(theta=0.4, phi=20.0, pn_coh_contrib=1)])
+def test_batch_chains_do_not_change_the_image(kw):
+    """A batch split into 1, 2 or 3 concurrent chains (each on its own
+    stream in the batch's graph): bit-identical images, incoherent and, with
+    tilted illumination, coherent (the output-ramp scratch and the coherent
+    sums are indexed per chain)."""
+    p = small_stem(n=256, ns=6, nconf=2)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    outs, plans = [], []
+    engine.set_pass_variants(0, 3, 0)
+    engine.set_batch(18, True)
+    try:
+        for chains in (1, 2, 3):
+            engine.set_chains(chains)
+            o = engine.run_multislice(p)
+            outs.append([a.copy() for a in (o.image_tot, o.image_coh) if a is not None])
+            plans.append(engine.last_plan())
+    finally:
+        engine.set_chains(0)
+        engine.set_batch(0, True)
+        engine.set_pass_variants(-1, -1, -1)
+    assert "chains=2" in plans[1] and "chains=3" in plans[2], plans
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+    ref = oracle.run(p, "f64")
+    assert rel_l2(outs[0][0], ref.image_tot) < 1e-4
