@@ -28,6 +28,8 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
     __shared__ R tab[kMaxTypes * 5 * 128];
     __shared__ R2v rmx[kMaxTypes];
     __shared__ int list[kTfThreads];
+    __shared__ int4 lwin[kTfThreads];     // the listed atoms' pixel windows [ix0, ixe] x [iy0, iye]
+    __shared__ A4 latom[kTfThreads];      // and their records
     __shared__ int wave_cnt[kTfThreads / 64];
 
     const int tid = threadIdx.x;
@@ -51,11 +53,13 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
         // cull this chunk of atoms against the tile (all periodic images)
         const int ia = c0 + tid;
         bool hit = false;
+        A4 at{};
+        int ix0 = 0, ixe = 0, iy0 = 0, iye = 0;
         if (ia <= rng.y) {
-            const A4 at = a.atoms[ia];
+            at = a.atoms[ia];
             const R R_max = rmx[atom_type(at)].x;
-            int ix0 = (int)floor((at.x - R_max) / a.dRx), ixe = (int)ceil((at.x + R_max) / a.dRx);
-            int iy0 = (int)floor((at.y - R_max) / a.dRy), iye = (int)ceil((at.y + R_max) / a.dRy);
+            ix0 = (int)floor((at.x - R_max) / a.dRx); ixe = (int)ceil((at.x + R_max) / a.dRx);
+            iy0 = (int)floor((at.y - R_max) / a.dRy); iye = (int)ceil((at.y + R_max) / a.dRy);
             if (ix0 != ixe && iy0 != iye) {
                 // smallest image of the tile's last column/row not left of the window start
                 auto fdiv = [](int p, int q) { return p >= 0 ? p / q : -((-p + q - 1) / q); };
@@ -73,28 +77,38 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
             if (w < wave) base += wave_cnt[w];
             total += wave_cnt[w];
         }
-        if (hit) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = ia;
+        if (hit) {
+            const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+            list[pos] = ia;
+            lwin[pos] = make_int4(ix0, ixe, iy0, iye);
+            latom[pos] = at;
+        }
         __syncthreads();
         for (int li = 0; li < total; ++li) {
-            const A4 at = a.atoms[list[li]];
+            const A4 at = latom[li];
+            const int4 wn = lwin[li];
             const int ty_ = atom_type(at);
             const R *tb = tab + ty_ * 5 * 128;
             // the knots are the element's; the coefficients c0..c3 the
             // element's, or those of this atom's sub-slice
             const R *cf = a.sub_coef ? a.sub_coef + (size_t)(a.pair_base[is] + (list[li] - rng.x)) * 512
                                      : tb + 128;
-            const R R_max = rmx[ty_].x, R2_max = rmx[ty_].y;
-            const int ix0 = (int)floor((at.x - R_max) / a.dRx), ixe = (int)ceil((at.x + R_max) / a.dRx);
-            const int iy0 = (int)floor((at.y - R_max) / a.dRy), iye = (int)ceil((at.y + R_max) / a.dRy);
+            const R R2_max = rmx[ty_].y;
+            // images iy = py + k*ny inside [iy0, iye]: the first is the smallest
+            // not below iy0
+            int iy_first = py;
+            while (iy_first < wn.z) iy_first += a.ny;
+            while (iy_first - a.ny >= wn.z) iy_first -= a.ny;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int px = px0 + tx + 4 * h;
-                // images ix = px + k*nx inside [ix0, ixe]
-                int kx = (ix0 - px >= 0) ? (ix0 - px + a.nx - 1) / a.nx : -((px - ix0) / a.nx);
-                for (int ix = px + kx * a.nx; ix <= ixe; ix += a.nx) {
+                // images ix = px + k*nx inside [ix0, ixe], in increasing order
+                int ix = px;
+                while (ix < wn.x) ix += a.nx;
+                while (ix - a.nx >= wn.x) ix -= a.nx;
+                for (; ix <= wn.y; ix += a.nx) {
                     const R rx = (R)ix * a.dRx - at.x;
-                    int ky = (iy0 - py >= 0) ? (iy0 - py + a.ny - 1) / a.ny : -((py - iy0) / a.ny);
-                    for (int iy = py + ky * a.ny; iy <= iye; iy += a.ny) {
+                    for (int iy = iy_first; iy <= wn.w; iy += a.ny) {
                         const R ry = (R)iy * a.dRy - at.y;
                         const R R2 = rx * rx + ry * ry;
                         if (R2 < R2_max) {
