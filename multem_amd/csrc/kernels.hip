@@ -15,7 +15,8 @@ namespace mtg {
 // src/cgpu_fcns.cuh:1496-1566), restricted to each atom's window
 // [floor((x-R_max)/dR), ceil((x+R_max)/dR)] (get_bn, src/cpu_fcns.hpp:1669-1684)
 // with periodic images; T = exp(i w V) written fft-shifted.
-constexpr int kTfTy = 64, kTfTx = 8, kTfThreads = 256;
+constexpr int kTfTy = 64, kTfTx = 16, kTfThreads = 256;
+constexpr int kTfPx = kTfTx * kTfTy / kTfThreads;   // pixels per thread, kTfThreads / kTfTy apart in x
 
 template <class R>
 __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
@@ -42,10 +43,12 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
     for (int i = tid; i < a.n_types * 5 * 128; i += kTfThreads) tab[i] = a.tables[i];
     if (tid < a.n_types) rmx[tid] = a.rmax[tid];
 
-    const int ty = tid % kTfTy, tx = tid / kTfTy;   // pixels (px0+tx, py) and (px0+tx+4, py)
+    const int ty = tid % kTfTy, tx = tid / kTfTy;   // pixels (px0 + tx + 4 h, py), h < kTfPx
     const int py = py0 + ty;
     const bool py_ok = py < a.ny;
-    R vacc[2] = {R(0), R(0)};
+    R vacc[kTfPx];
+#pragma unroll
+    for (int h = 0; h < kTfPx; ++h) vacc[h] = R(0);
 
     const int2 rng = a.slice_range[is];
     __syncthreads();
@@ -100,8 +103,8 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
             while (iy_first < wn.z) iy_first += a.ny;
             while (iy_first - a.ny >= wn.z) iy_first -= a.ny;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int px = px0 + tx + 4 * h;
+            for (int h = 0; h < kTfPx; ++h) {
+                const int px = px0 + tx + (kTfThreads / kTfTy) * h;
                 // images ix = px + k*nx inside [ix0, ixe], in increasing order
                 int ix = px;
                 while (ix < wn.x) ix += a.nx;
@@ -133,8 +136,8 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
     if (!py_ok) return;
     const long long nxy = (long long)a.nx * a.ny;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int px = px0 + tx + 4 * h;
+    for (int h = 0; h < kTfPx; ++h) {
+        const int px = px0 + tx + (kTfThreads / kTfTy) * h;
         const long long sl = is - a.slice0;
         if (a.V) a.V[sl * nxy + (long long)px * a.ny + py] = vacc[h];
         if (a.trans) {
