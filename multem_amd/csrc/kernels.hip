@@ -26,7 +26,11 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
 #pragma clang fp contract(off)
     using R2v = cplx_t<R>;
     using A4 = vec4_t<R>;
-    __shared__ R tab[kMaxTypes * 5 * 128];
+    // spline tables of the run's atom types only (dynamic LDS, n_types x 5 x
+    // 128 values): the LDS per workgroup, not registers, would otherwise cap
+    // the resident workgroups (16 types: three per CU in FP32, one in FP64)
+    extern __shared__ __align__(16) unsigned char tf_lds[];
+    R *tab = reinterpret_cast<R *>(tf_lds);
     __shared__ R2v rmx[kMaxTypes];
     __shared__ int list[kTfThreads];
     __shared__ int4 lwin[kTfThreads];     // the listed atoms' pixel windows [ix0, ixe] x [iy0, iye]
@@ -583,7 +587,15 @@ hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s)
 {
     if (a.nx % kTfTx || a.n_types > kMaxTypes || a.n_slices <= 0) return hipErrorInvalidValue;
     const int tiles = (a.nx / kTfTx) * ((a.ny + kTfTy - 1) / kTfTy);
-    hipLaunchKernelGGL((k_tf_build<R>), dim3(tiles, a.n_slices), dim3(kTfThreads), 0, s, a);
+    const size_t lds = (size_t)a.n_types * 5 * 128 * sizeof(R);
+    if (lds > 48 * 1024) {
+        // allow the dynamic LDS of the largest table set (kMaxTypes types) once
+        static const hipError_t allowed = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&k_tf_build<R>), hipFuncAttributeMaxDynamicSharedMemorySize,
+            (int)(kMaxTypes * 5 * 128 * sizeof(R)));
+        if (allowed != hipSuccess) return allowed;
+    }
+    hipLaunchKernelGGL((k_tf_build<R>), dim3(tiles, a.n_slices), dim3(kTfThreads), lds, s, a);
     return hipGetLastError();
 }
 

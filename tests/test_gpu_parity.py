@@ -544,3 +544,23 @@ def test_batch_chains_do_not_change_the_image(kw):
             assert np.array_equal(a, b)
     ref = oracle.run(p, "f64")
     assert rel_l2(outs[0][0], ref.image_tot) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_projected_potential_many_atom_types(precision):
+    """Ten elements in one specimen: the spline tables of every type share
+    the potential kernel's LDS (51 KB of FP64 tables); V is bit-identical to
+    the FP32 restatement and within 1e-12 of the FP64 oracle."""
+    p = configs.si_ewrs_256(nc=3)
+    Zs = np.array([6, 8, 14, 22, 26, 29, 38, 47, 56, 79], dtype=float)
+    p.spec_atoms = p.spec_atoms.copy()
+    p.spec_atoms[:, 0] = Zs[np.arange(len(p.spec_atoms)) % len(Zs)]
+    if precision == "f64":
+        p.system_conf.precision = 2
+    for islice in (0, 3, 7):
+        V = engine.projected_potential(p, islice)
+        ref = oracle.projected_potential(p, islice, precision=precision)
+        if precision == "f32":
+            assert np.array_equal(V, ref), islice
+        else:
+            assert rel_l2(V, ref) < 1e-12, islice
