@@ -261,8 +261,9 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not provided (see DESIGN.md for how the "
-              "path shards)" % args.gpus, file=sys.stderr)
+        print("bench.py: the engine runs on one GPU; --gpus %d is not provided (several GPUs would each run "
+              "a share of the phonon configurations or probe tiles and reduce the detector images)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
     from multem_amd import configs, engine
     if args.sto:
