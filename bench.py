@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--surrogate-scan128", action="store_true",
                     help="also time the surrogate on one 128x128-scan specimen (about a minute of GPU time)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path measurement")
+    ap.add_argument("--c4", action="store_true",
+                    help="only time the C4 CBED run (Pt[110] 1024^2, 20 configurations, 325 slices) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate HBM counters")
     return ap.parse_args()
@@ -148,6 +150,27 @@ def group_measurement(group_batch=0):
                                 "%d probes per batch" % group_batch if group_batch > 0 else "planner batch widths"),
             "probe_slices": work, "value": work / t_group, "unit": "probe·slices/s",
             "sequential_value": work / t_seq, "plan": plan, "per_grid": per_grid}
+
+
+def c4_measurement():
+    """C4: Pt[110] CBED at 1024^2, 20 phonon configurations, full depth (325
+    slices, thicknesses every 50 A to 450 A), FP32 wave path: wall time of one
+    run after a warm-up (tuning, tables), and the transmission build's
+    share (HIP events).  One JSON line; not the headline metric."""
+    from multem_amd import configs, engine
+    p = configs.pt_cbed_1024(nconf=20, n=1024)
+    engine.run_multislice(p)
+    sync()
+    t0 = time.perf_counter()
+    engine.run_multislice(p)
+    sync()
+    dt = time.perf_counter() - t0
+    t = engine.last_timing()
+    print(json.dumps({"metric": "C4 CBED run time", "value": dt, "unit": "s", "higher_is_better": False,
+                      "workload": "pt_cbed_1024: Pt[110] 10x14 cells, 1024^2, 200 kV, 10 mrad, -100 A defocus, "
+                                  "20 configurations, 325 slices, 9 thicknesses",
+                      "ms_total_device": t["ms_total"], "plan": engine.last_plan()}))
+    sys.stdout.flush()
 
 
 def fp64_measurement():
@@ -259,6 +282,9 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.c4:
+        c4_measurement()
         return
     if args.gpus != 1:
         print("bench.py: the engine runs on one GPU; --gpus %d is not provided (several GPUs would each run "
