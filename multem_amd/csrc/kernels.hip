@@ -17,8 +17,31 @@ namespace mtg {
 // with periodic images; T = exp(i w V) written fft-shifted.
 constexpr int kTfTy = 64, kTfTx = 16, kTfThreads = 256;
 constexpr int kTfPx = kTfTx * kTfTy / kTfThreads;   // pixels per thread, kTfThreads / kTfTy apart in x
+constexpr int kKnots = 128;
 
+// Knot interval of R2 in a table of kKnots increasing knots: the largest
+// i in [0, kKnots - 2] with tb[i] <= R2, or 0 -- the result of the
+// reference's 7-step binary search (eval_cubic_poly, src/cgpu_fcns.cuh:1496-1566).
+// The knots are log-spaced in R (R_min e^(i dlnr)), so log2(R2) is linear in
+// i: the guess from the hardware log2 and the table's end knots (lg.x =
+// log2 tb[0], lg.y = (kKnots - 1) / (log2 tb[kKnots - 1] - lg.x)) lands
+// within one interval of the answer, and one comparison on each side
+// settles it.
 template <class R>
+__device__ __forceinline__ int knot_interval(const R *tb, R R2, float2 lg)
+{
+    float f = (__builtin_amdgcn_logf((float)R2) - lg.x) * lg.y;
+    f = fminf(fmaxf(f, 0.0f), (float)(kKnots - 2));   // also maps log2(0) = -inf to 0
+    int g = (int)f;
+    const R lo = tb[g], hi = tb[g + 1];
+    if (g > 0 && R2 < lo) --g;
+    else if (g < kKnots - 2 && R2 >= hi) ++g;
+    return g;
+}
+
+// SUB: the spline coefficients are per (slice, atom) pair in global memory
+// (sub-slicing) instead of the element's, held in LDS.
+template <class R, bool SUB>
 __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
 {
     // Every product and sum rounds separately, as on the reference's host
@@ -32,8 +55,11 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
     extern __shared__ __align__(16) unsigned char tf_lds[];
     R *tab = reinterpret_cast<R *>(tf_lds);
     __shared__ R2v rmx[kMaxTypes];
+    __shared__ float2 klog[kMaxTypes];    // knot_interval's constants per type
     __shared__ int list[kTfThreads];
     __shared__ int4 lwin[kTfThreads];     // the listed atoms' pixel windows [ix0, ixe] x [iy0, iye]
+    __shared__ int2 lbase[kTfThreads];    // and image offsets: the tile's last column / row + offset
+                                          // is the smallest image not before the window start
     __shared__ A4 latom[kTfThreads];      // and their records
     __shared__ int wave_cnt[kTfThreads / 64];
 
@@ -45,7 +71,12 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
     const int is = a.slice0 + blockIdx.y;
 
     for (int i = tid; i < a.n_types * 5 * 128; i += kTfThreads) tab[i] = a.tables[i];
-    if (tid < a.n_types) rmx[tid] = a.rmax[tid];
+    if (tid < a.n_types) {
+        rmx[tid] = a.rmax[tid];
+        const float l0 = __builtin_amdgcn_logf((float)a.tables[tid * 5 * kKnots]);
+        const float l1 = __builtin_amdgcn_logf((float)a.tables[tid * 5 * kKnots + kKnots - 1]);
+        klog[tid] = make_float2(l0, (float)(kKnots - 1) / (l1 - l0));
+    }
 
     const int ty = tid % kTfTy, tx = tid / kTfTy;   // pixels (px0 + tx + 4 h, py), h < kTfPx
     const int py = py0 + ty;
@@ -61,7 +92,7 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
         const int ia = c0 + tid;
         bool hit = false;
         A4 at{};
-        int ix0 = 0, ixe = 0, iy0 = 0, iye = 0;
+        int ix0 = 0, ixe = 0, iy0 = 0, iye = 0, kx = 0, ky = 0;
         if (ia <= rng.y) {
             at = a.atoms[ia];
             const R R_max = rmx[atom_type(at)].x;
@@ -70,8 +101,8 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
             if (ix0 != ixe && iy0 != iye) {
                 // smallest image of the tile's last column/row not left of the window start
                 auto fdiv = [](int p, int q) { return p >= 0 ? p / q : -((-p + q - 1) / q); };
-                int kx = -fdiv(-(ix0 - (px0 + kTfTx - 1)), a.nx);   // ceil((ix0 - px_last)/nx)
-                int ky = -fdiv(-(iy0 - (py0 + kTfTy - 1)), a.ny);
+                kx = -fdiv(-(ix0 - (px0 + kTfTx - 1)), a.nx);   // ceil((ix0 - px_last)/nx)
+                ky = -fdiv(-(iy0 - (py0 + kTfTy - 1)), a.ny);
                 hit = (px0 + kx * a.nx <= ixe) && (py0 + ky * a.ny <= iye);
             }
         }
@@ -88,46 +119,42 @@ __global__ __launch_bounds__(kTfThreads) void k_tf_build(TfArgsT<R> a)
             const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
             list[pos] = ia;
             lwin[pos] = make_int4(ix0, ixe, iy0, iye);
+            lbase[pos] = make_int2(kx * a.nx, ky * a.ny);
             latom[pos] = at;
         }
         __syncthreads();
         for (int li = 0; li < total; ++li) {
             const A4 at = latom[li];
             const int4 wn = lwin[li];
+            const int2 lb = lbase[li];
             const int ty_ = atom_type(at);
-            const R *tb = tab + ty_ * 5 * 128;
+            const R *tb = tab + ty_ * 5 * kKnots;
+            const float2 lg = klog[ty_];
             // the knots are the element's; the coefficients c0..c3 the
             // element's, or those of this atom's sub-slice
-            const R *cf = a.sub_coef ? a.sub_coef + (size_t)(a.pair_base[is] + (list[li] - rng.x)) * 512
-                                     : tb + 128;
+            const R *cf = SUB ? a.sub_coef + (size_t)(a.pair_base[is] + (list[li] - rng.x)) * 512 : tb + kKnots;
             const R R2_max = rmx[ty_].y;
             // images iy = py + k*ny inside [iy0, iye]: the first is the smallest
-            // not below iy0
-            int iy_first = py;
-            while (iy_first < wn.z) iy_first += a.ny;
-            while (iy_first - a.ny >= wn.z) iy_first -= a.ny;
+            // not below iy0 (py is at most 63 rows above the tile's last row,
+            // and ny >= 64, so it is lb.y or lb.y + ny above py)
+            int iy_first = py + lb.y;
+            if (iy_first < wn.z) iy_first += a.ny;
 #pragma unroll
             for (int h = 0; h < kTfPx; ++h) {
                 const int px = px0 + tx + (kTfThreads / kTfTy) * h;
-                // images ix = px + k*nx inside [ix0, ixe], in increasing order
-                int ix = px;
-                while (ix < wn.x) ix += a.nx;
-                while (ix - a.nx >= wn.x) ix -= a.nx;
+                // images ix = px + k*nx inside [ix0, ixe], in increasing order,
+                // from the smallest not below ix0 (likewise lb.x or lb.x + nx)
+                int ix = px + lb.x;
+                if (ix < wn.x) ix += a.nx;
                 for (; ix <= wn.y; ix += a.nx) {
                     const R rx = (R)ix * a.dRx - at.x;
                     for (int iy = iy_first; iy <= wn.w; iy += a.ny) {
                         const R ry = (R)iy * a.dRy - at.y;
                         const R R2 = rx * rx + ry * ry;
                         if (R2 < R2_max) {
-                            // 7-step binary search over the 128 knots, then the cubic
-                            int i0 = 0, ie = 127;
-#pragma unroll
-                            for (int k = 0; k < 7; ++k) {
-                                const int im = (i0 + ie) >> 1;
-                                if (R2 < tb[im]) ie = im; else i0 = im;
-                            }
+                            const int i0 = knot_interval(tb, R2, lg);
                             const R dx = R2 - tb[i0];
-                            const R p = ((cf[3 * 128 + i0] * dx + cf[2 * 128 + i0]) * dx + cf[1 * 128 + i0]) * dx
+                            const R p = ((cf[3 * kKnots + i0] * dx + cf[2 * kKnots + i0]) * dx + cf[1 * kKnots + i0]) * dx
                                             + cf[i0];
                             vacc[h] = vacc[h] + at.z * p;
                         }
@@ -582,21 +609,29 @@ hipError_t launch_col_pass(int n, const ColArgsT<Z> &a, bool probe, bool det, hi
     return hipErrorInvalidValue;
 }
 
-template <class R>
-hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s)
+template <class R, bool SUB>
+hipError_t tf_build_go(const TfArgsT<R> &a, int tiles, size_t lds, hipStream_t s)
 {
-    if (a.nx % kTfTx || a.n_types > kMaxTypes || a.n_slices <= 0) return hipErrorInvalidValue;
-    const int tiles = (a.nx / kTfTx) * ((a.ny + kTfTy - 1) / kTfTy);
-    const size_t lds = (size_t)a.n_types * 5 * 128 * sizeof(R);
     if (lds > 48 * 1024) {
         // allow the dynamic LDS of the largest table set (kMaxTypes types) once
         static const hipError_t allowed = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&k_tf_build<R>), hipFuncAttributeMaxDynamicSharedMemorySize,
-            (int)(kMaxTypes * 5 * 128 * sizeof(R)));
+            reinterpret_cast<const void *>(&k_tf_build<R, SUB>), hipFuncAttributeMaxDynamicSharedMemorySize,
+            (int)(kMaxTypes * 5 * kKnots * sizeof(R)));
         if (allowed != hipSuccess) return allowed;
     }
-    hipLaunchKernelGGL((k_tf_build<R>), dim3(tiles, a.n_slices), dim3(kTfThreads), lds, s, a);
+    hipLaunchKernelGGL((k_tf_build<R, SUB>), dim3(tiles, a.n_slices), dim3(kTfThreads), lds, s, a);
     return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_tf_build(const TfArgsT<R> &a, hipStream_t s)
+{
+    // the pixel tile's rows and columns lie within one grid period (the
+    // image offsets of k_tf_build rely on it)
+    if (a.nx % kTfTx || a.ny < kTfTy || a.n_types > kMaxTypes || a.n_slices <= 0) return hipErrorInvalidValue;
+    const int tiles = (a.nx / kTfTx) * ((a.ny + kTfTy - 1) / kTfTy);
+    const size_t lds = (size_t)a.n_types * 5 * kKnots * sizeof(R);
+    return a.sub_coef ? tf_build_go<R, true>(a, tiles, lds, s) : tf_build_go<R, false>(a, tiles, lds, s);
 }
 
 template <class R>
