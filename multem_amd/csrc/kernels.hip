@@ -32,11 +32,11 @@ __device__ __forceinline__ int knot_interval(const R *tb, R R2, float2 lg)
 {
     float f = (__builtin_amdgcn_logf((float)R2) - lg.x) * lg.y;
     f = fminf(fmaxf(f, 0.0f), (float)(kKnots - 2));   // also maps log2(0) = -inf to 0
-    int g = (int)f;
-    const R lo = tb[g], hi = tb[g + 1];
-    if (g > 0 && R2 < lo) --g;
-    else if (g < kKnots - 2 && R2 >= hi) ++g;
-    return g;
+    const int g = (int)f;
+    const R lo = tb[g], hi = tb[g + 1];   // both loads issued at once, no branches
+    const int dn = (int)(R2 < lo) & (int)(g > 0);
+    const int up = (int)(R2 >= hi) & (int)(g < kKnots - 2) & (dn ^ 1);
+    return g - dn + up;
 }
 
 // SUB: the spline coefficients are per (slice, atom) pair in global memory
