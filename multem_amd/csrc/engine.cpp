@@ -62,21 +62,102 @@ void check(hipError_t e, const char *what)
     }
 }
 
+// Device blocks kept between runs.  A simulation allocates the same buffers
+// each time it is called (the transmission stack of a 1024^2 CBED with 20
+// configurations x 325 slices is 52 GB), and returning them to the runtime
+// and asking again costs host time that varies from run to run.  A released
+// block is kept in a per-device list keyed by its byte size; an allocation
+// takes a block of exactly its size from that list, else calls hipMalloc,
+// and when hipMalloc runs out of memory every kept block of the device is
+// freed and the call retried.  A release first waits for the device, as
+// hipFree does, so that a block is never handed on while kernels still use
+// it.  The lists are never destroyed (their blocks live until the process
+// ends).
+struct BlockCache {
+    std::mutex mu;
+    std::multimap<std::pair<int, size_t>, void *> blocks;   // (device, bytes) -> block
+    std::map<int, size_t> kept;                             // kept bytes per device
+};
+
+BlockCache &block_cache()
+{
+    static BlockCache *c = new BlockCache;
+    return *c;
+}
+
+void trim_blocks(int dev)
+{
+    BlockCache &c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (auto it = c.blocks.begin(); it != c.blocks.end();) {
+        if (it->first.first == dev) {
+            (void)hipFree(it->second);
+            it = c.blocks.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    c.kept[dev] = 0;
+}
+
+size_t kept_bytes(int dev)
+{
+    BlockCache &c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.kept.find(dev);
+    return it == c.kept.end() ? 0 : it->second;
+}
+
+void *dev_alloc(size_t bytes, int &dev)
+{
+    check(hipGetDevice(&dev), "hipGetDevice");
+    BlockCache &c = block_cache();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.blocks.find({dev, bytes});
+        if (it != c.blocks.end()) {
+            void *p = it->second;
+            c.blocks.erase(it);
+            c.kept[dev] -= bytes;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        trim_blocks(dev);
+        e = hipMalloc(&p, bytes);
+    }
+    check(e, "hipMalloc");
+    return p;
+}
+
+void dev_release(void *p, size_t bytes, int dev)
+{
+    (void)hipDeviceSynchronize();
+    BlockCache &c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.blocks.insert({{dev, bytes}, p});
+    c.kept[dev] += bytes;
+}
+
 template <class T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    int dev = 0;   // the device the block belongs to
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     void alloc(size_t count)
     {
         release();
+        if (count) p = (T *)dev_alloc(count * sizeof(T), dev);
         n = count;
-        if (count) check(hipMalloc((void **)&p, count * sizeof(T)), "hipMalloc");
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) dev_release(p, n * sizeof(T), dev);
         p = nullptr;
         n = 0;
     }
@@ -673,7 +754,9 @@ size_t free_device_memory()
 {
     size_t fr = 0, tot = 0;
     check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
-    return fr;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    return fr + kept_bytes(dev);   // blocks kept from earlier runs are free for this one
 }
 
 // Row pitch of a STEM wave batch: ny plus a pad (mt_hip_set_wave_pad;
