@@ -331,6 +331,18 @@ mt_status mt_hip_last_timing(double *ms_slice_kernels, double *ms_total, double 
    the reference, whose mex entry synchronises its stream before returning. */
 mt_status mt_hip_synchronize(int device);
 
+/* Device memory kept between runs.  Device blocks a run releases are kept
+   for the next run that asks for blocks of the same sizes (a simulation
+   called again allocates the same buffers), and are counted as free memory
+   by the memory planner; when an allocation runs out of memory the device's
+   kept blocks are freed first.  mt_hip_kept_device_bytes reports the bytes
+   kept on `device`; mt_hip_release_device_memory waits for the device and
+   returns its kept blocks to the HIP runtime (for callers that share the GPU
+   with other work between runs).  No counterpart in the reference, which
+   allocates and frees its buffers in every call. */
+mt_status mt_hip_kept_device_bytes(int device, size_t *bytes);
+mt_status mt_hip_release_device_memory(int device);
+
 /* ---- engine tuning hooks (no counterpart in the reference; for the
         benchmark and tests) ------------------------------------------------ */
 

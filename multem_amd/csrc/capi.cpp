@@ -252,6 +252,32 @@ mt_status mt_hip_synchronize(int device)
     return MT_OK;
 }
 
+mt_status mt_hip_kept_device_bytes(int device, size_t *bytes)
+{
+    if (!bytes) return fail(MT_ERR_INVALID_ARGUMENT, "null argument");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(MT_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(MT_ERR_INVALID_ARGUMENT, "device out of range");
+    std::lock_guard<std::mutex> lock(g_engine_mutex);
+    *bytes = mtg::kept_device_bytes(device);
+    return MT_OK;
+}
+
+mt_status mt_hip_release_device_memory(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(MT_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(MT_ERR_INVALID_ARGUMENT, "device out of range");
+    std::lock_guard<std::mutex> lock(g_engine_mutex);
+    try {
+        mtg::release_kept_blocks(device);
+    } catch (const mtg::Error &e) {
+        return fail(e.code, e.msg);
+    }
+    g_last_error.clear();
+    return MT_OK;
+}
+
 void mt_hip_request_stop(void) { mtg::run_control().stop = true; }
 
 void mt_hip_progress(int *iter, int *niter)

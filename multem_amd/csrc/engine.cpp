@@ -2258,6 +2258,15 @@ unsigned char *scratch_on(int device)
 
 size_t propagate_noalloc_max_nxy() { return kNoAllocMaxNxy; }
 
+size_t kept_device_bytes(int device) { return kept_bytes(device); }
+
+void release_kept_blocks(int device)
+{
+    check(hipSetDevice(device), "hipSetDevice");
+    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    trim_blocks(device);
+}
+
 template <class R>
 void propagate_noalloc(const ModelT<R> &m, int device, const double *psi_in, double z, int space, double *psi_out)
 {

@@ -73,6 +73,10 @@ void propagate(const ModelT<R> &m, int device, const double *psi_in, double z, i
 template <class R>
 void propagate_noalloc(const ModelT<R> &m, int device, const double *psi_in, double z, int space, double *psi_out);
 size_t propagate_noalloc_max_nxy();
+// device blocks kept between runs: bytes kept on a device; wait for the
+// device and free its kept blocks
+size_t kept_device_bytes(int device);
+void release_kept_blocks(int device);
 
 // read + write bandwidth (GB/s) of a streaming copy of `bytes` bytes, reps times
 double stream_copy_gbs(int device, size_t bytes, int reps);

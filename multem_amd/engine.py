@@ -55,6 +55,8 @@ def lib():
         L.mt_hip_set_batch_chains.argtypes = [C.c_int]
         L.mt_hip_kernel_timing.argtypes = [dp, P(C.c_int), dp, P(C.c_int), dp]
         L.mt_hip_synchronize.argtypes = [C.c_int]
+        L.mt_hip_kept_device_bytes.argtypes = [C.c_int, C.POINTER(C.c_size_t)]
+        L.mt_hip_release_device_memory.argtypes = [C.c_int]
         L.mt_hip_atom_table.argtypes = [C.c_int, C.c_double] + [dp] * 6
         L.mt_hip_set_autotune.argtypes = [C.c_int]
         L.mt_hip_stream_copy_bandwidth.argtypes = [C.c_int, C.c_size_t, C.c_int, P(C.c_double)]
@@ -187,6 +189,18 @@ NOALLOC_MAX_NXY = 1024 * 1024
 
 def synchronize(device=0):
     _check(lib().mt_hip_synchronize(int(device)))
+
+
+def kept_device_bytes(device=0):
+    """Bytes of device memory the library keeps on `device` for later runs."""
+    b = C.c_size_t(0)
+    _check(lib().mt_hip_kept_device_bytes(int(device), C.byref(b)))
+    return b.value
+
+
+def release_device_memory(device=0):
+    """Returns the device blocks kept between runs to the HIP runtime."""
+    _check(lib().mt_hip_release_device_memory(int(device)))
 
 
 def atom_table(Z, dR_min):
