@@ -133,8 +133,22 @@ void *dev_alloc(size_t bytes, int &dev)
     return p;
 }
 
+// MULTEM_HIP_KEEP_BLOCKS=0 turns the kept blocks off (every release frees)
+bool keep_blocks()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("MULTEM_HIP_KEEP_BLOCKS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 void dev_release(void *p, size_t bytes, int dev)
 {
+    if (!keep_blocks()) {
+        (void)hipFree(p);
+        return;
+    }
     (void)hipDeviceSynchronize();
     BlockCache &c = block_cache();
     std::lock_guard<std::mutex> lk(c.mu);
